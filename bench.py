@@ -1,0 +1,157 @@
+#!/usr/bin/env python3
+"""Benchmark: D3Q19 TRT (Lambda = 1/4) body-force flow through a 256^3
+periodic porous medium (overlapping spheres, radii U[6,12], 30 % solid,
+seed 2024), tau = 0.9, g = 1e-6 along x, FP32 storage and compute.
+
+One step = one fused stream-collide-boundary kernel over the whole lattice,
+populations resident in HBM.  Prints one JSON line:
+  value      MLUPS = all nodes x steps / time / 1e6 (the reference's convention,
+             T/Resources/CGPerformance.xlsx sheet3 B6-B7)
+  roofline   algorithmic bytes per launch / average launch time (HIP events on
+             the solver's stream) against 8 TB/s
+  cpu_baseline  the FP64 CPU oracle on the same lattice, a few steps
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(n_nodes, n_fluid, q, sbytes):
+    """Per launch: every fluid node reads and writes each population once,
+    plus one byte of node class per node."""
+    return n_fluid * 2 * q * sbytes + n_nodes * 1
+
+
+def traffic_from_profiles(name):
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as fh:
+            return json.load(fh).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(cls, bb, n, seconds_target=15.0):
+    """FP64 oracle (a C restatement of the reference step, OpenMP over the
+    host cores) on the same lattice; steps until ~seconds_target elapse."""
+    from oracle import oracle as O
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = min(threads, 16)
+    s = O.Sim(19, (n, n, n), cls, bb, collision=O.TRT, omega=1 / 0.9, trt_magic=0.25, force_model=O.FORCE_GUO,
+              g=(1e-6, 0, 0), periodic=(1, 1, 1), nthreads=threads)
+    s.init(0)
+    s.step(1)   # warm-up (first touch)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        s.step(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > seconds_target or steps >= 50:
+            break
+    mlups = n ** 3 * steps / el / 1e6
+    return {"value": round(mlups, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle, D3Q19 TRT on the same {n}^3 lattice, {steps} steps after 1 warm-up step "
+                      f"({el:.1f} s, OpenMP {threads} threads)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--storage", default="f32", choices=["f64", "f32", "f16", "bf16"])
+    ap.add_argument("--collision", default="trt", choices=["bgk", "trt", "mrt"])
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--variants", action="store_true", help="also time f16/bf16 storage and MRT")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        print("bench.py: this build runs on one GPU; --gpus above 1 is not supported "
+              "(the multi-GPU slab decomposition is a design note only).", file=sys.stderr)
+        sys.exit(2)
+
+    from thesis_amd import lattices as LT
+    from thesis_amd import lbm as L
+
+    n = args.size
+    cls, bb, info = LT.porous3d(n, seed=args.seed)
+    n_nodes = n ** 3
+    n_fluid = info["n_fluid"]
+    coll = {"bgk": L.BGK, "trt": L.TRT, "mrt": L.MRT}[args.collision]
+    sbytes = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2}[args.storage]
+
+    def make(storage, collision):
+        s = L.Solver(19, (n, n, n), cls, bb, collision=collision, omega=1 / 0.9, trt_magic=0.25,
+                     force_model=L.FORCE_GUO, g=(1e-6, 0.0, 0.0), periodic=(1, 1, 1), storage=storage)
+        s.init(0)
+        return s
+
+    solver = make(args.storage, coll)
+    solver.step(args.warmup)
+    t0 = time.perf_counter()
+    ms = solver.time_steps(args.steps)         # HIP events on the solver's stream
+    wall = time.perf_counter() - t0
+    avg_ms = ms / args.steps
+    mlups = n_nodes * args.steps / (ms / 1e3) / 1e6
+    bytes_launch = algorithmic_bytes(n_nodes, n_fluid, 19, sbytes)
+    achieved = bytes_launch / (avg_ms / 1e3) / 1e9
+    red = solver.reduce()
+    solver.close()
+
+    variants = {}
+    if args.variants:
+        for st, co in (("f16", L.TRT), ("bf16", L.TRT), ("f32", L.MRT), ("f32", L.BGK)):
+            s = make(st, co)
+            s.step(args.warmup)
+            vms = s.time_steps(args.steps)
+            vb = algorithmic_bytes(n_nodes, n_fluid, 19, {"f32": 4, "f16": 2, "bf16": 2}[st])
+            key = f"{st}_{['bgk', 'trt', 'mrt'][co]}"
+            variants[key] = {"mlups": round(n_nodes * args.steps / (vms / 1e3) / 1e6, 1),
+                             "gbs": round(vb / (vms / args.steps / 1e3) / 1e9, 1)}
+            s.close()
+
+    out = {
+        "metric": "MLUPS & % HBM roofline (D3Q19 256^3)",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(avg_ms, 5),
+        "wall_s": round(wall, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f64": "f64", "f32": "f32"}.get(args.storage, args.storage + "-storage/f32-compute"),
+        "data": "synthetic",
+        "config": {"workload": f"D3Q19 {args.collision.upper()} Lambda=1/4 body-force flow, {n}^3 periodic porous "
+                               f"medium (spheres r~U[6,12], seed {args.seed}, solid {info['solid_fraction']:.4f}), "
+                               f"tau=0.9, g=1e-6 x, {args.storage} storage",
+                   "nodes": n_nodes, "fluid_nodes": n_fluid, "lattice": "D3Q19", "storage": args.storage},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles(f"traffic_d3q19_{args.collision}_{args.storage}_{n}.json")},
+        "mflups": round(n_fluid * args.steps / (ms / 1e3) / 1e6, 1),
+        "mass": red["mass"],
+    }
+    if variants:
+        out["variants"] = variants
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cls, bb, n)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,156 @@
+/*
+ * lbm.h -- C ABI of the MI355X lattice-Boltzmann time step.
+ *
+ * Replaces the kernel-level surface of ZPaulo/Thesis `lbm-solver_final`:
+ * the __global__ kernels over raw device pointers with parameters in
+ * __constant__ symbols (S/include/GpuFunctions.h:63-472,
+ * S/include/GpuConstants.h:43-118) and the driver entry points
+ * Iterate2D/Iterate3D (S/include/Iterate.h:57-65).  Prefix S/ stands for
+ * `Source code/lbm-solver_final/` of the reference tree.
+ *
+ * Plain C types only.  A context owns one HIP stream and every device
+ * buffer; host pointers passed in are borrowed for the duration of the call.
+ * Errors are status codes (the library never exits); lbm_last_error()
+ * returns the message.  A context is not re-entrant; distinct contexts may
+ * be used from distinct host threads.
+ */
+#ifndef LBM_H
+#define LBM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_ABI_VERSION 1
+
+typedef struct lbm_ctx lbm_ctx;
+
+/* Status codes.  LBM_DIVERGED / LBM_EIO keep the reference's driver return
+ * values 1 and 2 (S/include/Iterate.h:62). */
+typedef enum {
+    LBM_OK = 0,
+    LBM_DIVERGED = 1,
+    LBM_EIO = 2,
+    LBM_EINVAL = 3,
+    LBM_EDEVICE = 4,
+    LBM_ENOMEM = 5,
+    LBM_EUNSUPPORTED = 6
+} lbm_status;
+
+/* node classes: one byte per node (latticegen2 fluid flag + boundary kind) */
+enum { LBM_SOLID = 0, LBM_FLUID = 1, LBM_VEL_W = 2, LBM_PRES_E = 3 };
+/* collision models (S/include/Arguments.h: BGKW, TRT, MRT) */
+enum { LBM_BGK = 0, LBM_TRT = 1, LBM_MRT = 2 };
+/* population storage; compute is FP64 for F64 storage and FP32 otherwise */
+enum { LBM_STORE_F64 = 0, LBM_STORE_F32 = 1, LBM_STORE_F16 = 2, LBM_STORE_BF16 = 3 };
+/* body force: NONE, the reference's stored-velocity shift u += g/omega
+ * (S/GpuUpdateMacro.cu:207), or Guo forcing */
+enum { LBM_FORCE_NONE = 0, LBM_FORCE_SHIFT = 1, LBM_FORCE_GUO = 2 };
+/* walls: half-way bounce-back, or the reference's on-node
+ * f[opp] = f[dir] (S/GpuBoundaries.cu:435-437, :482-484) */
+enum { LBM_WALL_HALFWAY = 0, LBM_WALL_ONNODE = 1 };
+/* velocity inlet: textbook Zou-He, or the reference's 2D west form
+ * without transverse terms (S/GpuBoundaries.cu:45-64) */
+enum { LBM_INLET_ZOUHE = 0, LBM_INLET_REF2D = 1 };
+/* reference quirks reproduced on request */
+#define LBM_PARITY_MRT_RHO2   0x1u /* 2D MRT m_eq with rho^2 (S/GpuCollision.cu:987-988) */
+#define LBM_PARITY_MRT_NO_F18 0x2u /* 3D MRT leaves f18 unwritten (S/GpuCollision.cu:1161) */
+#define LBM_PARITY_OUTLET_V0  0x8u /* 2D v := 0 on outlet nodes (S/GpuUpdateMacro.cu:30) */
+
+/* Solver parameters; mirrors the numerical part of the reference's
+ * Arguments struct (S/include/Arguments.h:86-135) plus the new knobs. */
+typedef struct {
+    uint32_t abi_version;   /* LBM_ABI_VERSION */
+    int32_t q;              /* 9 (D2Q9) or 19 (D3Q19) */
+    int32_t nx, ny, nz;     /* n, m, h of the reference; nz = 1 for D2Q9 */
+    int32_t periodic[3];    /* per axis, modular wrap */
+    int32_t collision;
+    double omega;           /* 1/tau = 1/(3 nu + 1/2) (S/GpuInit.cu:102) */
+    double trt_magic;       /* Lambda; 0 selects 3/16, the reference's omegaA (S/GpuInit.cu:103) */
+    double mrt_s[19];       /* diag(S); all zero selects S/CellFunctions.cu:44-52 / :122-140 */
+    int32_t force_model;
+    double force[3];        /* acceleration g */
+    int32_t storage;
+    int32_t wall_model;
+    int32_t inlet_model;
+    uint32_t parity_flags;
+    double rho_out;         /* Zou-He pressure outlet density */
+} lbm_params;
+
+/* Lattice description; node_class is required, the rest may be NULL.
+ * bb_links[i] bit d (1 <= d < q): the link from node i in direction d
+ * crosses a wall (the _BC.dat line "i j k d WALL").
+ * u_bc/v_bc/w_bc: prescribed velocity on LBM_VEL_W nodes. */
+typedef struct {
+    const uint8_t* node_class;
+    const uint32_t* bb_links;
+    const double* u_bc;
+    const double* v_bc;
+    const double* w_bc;
+} lbm_lattice;
+
+int lbm_abi_version(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+int lbm_device_count(void);
+
+/* Allocate a solver; replaces the allocation/initialisation part of
+ * Iterate2D/Iterate3D (S/Iterate.cu:23-330, S/Iterate3D.cu:30-490). */
+lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out);
+void lbm_destroy(lbm_ctx* ctx);
+const char* lbm_last_error(const lbm_ctx* ctx);
+
+/* Initial state.  mode 0: f = f_eq(rho, u); mode 1: f = 0 with the given
+ * macroscopic fields, the reference's start (S/Iterate.cu:244-245,
+ * S/Iterate3D.cu:301-302).  NULL fields mean rho = 1, u = 0. */
+lbm_status lbm_init(lbm_ctx* ctx, int mode, const double* rho, const double* u,
+                    const double* v, const double* w);
+
+/* Advance nsteps fused steps (collide, stream, boundaries, moments: the
+ * loop body of S/Iterate.cu:385-520 / S/Iterate3D.cu:493-653).  Enqueues
+ * everything on the context stream and waits once at the end. */
+lbm_status lbm_step(lbm_ctx* ctx, int64_t nsteps);
+
+/* Steps in blocks of check_every until the relative L2 change of the
+ * velocity over one block falls below tol (FP64 reduction), or max_steps.
+ * Returns LBM_DIVERGED on a non-finite field (S/Iterate.cu:552-564). */
+lbm_status lbm_run_to_residual(lbm_ctx* ctx, double tol, int64_t check_every, int64_t max_steps,
+                               int64_t* steps_done, double* final_residual);
+
+/* Reductions over non-solid nodes, FP64: out[0] = sum |u - u_prev|^2,
+ * out[1] = sum |u|^2, out[2] = sum rho (mass), out[3] = non-finite count.
+ * u_prev is the field at the previous call (zero before the first). */
+lbm_status lbm_reduce(lbm_ctx* ctx, double out[4]);
+
+/* Macroscopic fields of the current state (N doubles each; any may be NULL).
+ * The velocity is the stored velocity of the force model in use. */
+lbm_status lbm_get_macros(lbm_ctx* ctx, double* rho, double* u, double* v, double* w);
+/* Post-boundary populations f[q*N + i] of the current state (FP64). */
+lbm_status lbm_get_f(lbm_ctx* ctx, double* f);
+/* Replace the state by f (q*N doubles); moments are recomputed from f. */
+lbm_status lbm_set_f(lbm_ctx* ctx, const double* f);
+
+/* Timing for benchmarks: runs nsteps steps bracketed by HIP events on the
+ * context stream; *ms is the elapsed device time of those steps. */
+lbm_status lbm_time_steps(lbm_ctx* ctx, int64_t nsteps, double* ms);
+
+/* Bytes of device memory held by the context. */
+int64_t lbm_device_bytes(const lbm_ctx* ctx);
+
+/* ---- lattice files (host only; no device needed) ------------------------ */
+/* latticegen2 reader: <name>_Node.dat ("i j [k] x y [z] fluid") and
+ * <name>_BC.dat ("i j [k] dir type x y [z] id"), S/FilesReading.cu:270-347,
+ * L/main.cpp:59-66, L/FluidTracking.cpp:229-236.  First call with
+ * node_class == NULL returns dims (nx, ny, nz) and the link count; the second
+ * fills node_class (0/1 from the fluid column), bb_links (WALL links) and
+ * type_counts[6] (links per type 0..5).  Returns LBM_EIO on a missing or
+ * malformed file. */
+lbm_status lbm_read_lattice(const char* node_path, const char* bc_path, int32_t dims[3],
+                            int64_t* nlinks, uint8_t* node_class, uint32_t* bb_links,
+                            int64_t type_counts[6], int64_t dir_counts[19]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,694 @@
+/*
+ * FP64 CPU oracle -- TEST INFRASTRUCTURE ONLY (see lbm_oracle.h).
+ *
+ * Restates the reference's per-node arithmetic; each function names the
+ * reference lines it follows.  Written for clarity, parallelised with OpenMP
+ * over nodes so that it can also serve as the timed CPU baseline.
+ */
+#include "lbm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* lattice tables: reference numbering (S/GpuInit.cu:94-99, :822-842)        */
+static const int CX9[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+static const int CY9[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+static const int CX19[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+static const int CY19[19] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
+static const int CZ19[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+
+int orc_lattice(int q, int* cx, int* cy, int* cz, int* opp, double* w)
+{
+    if (q != 9 && q != 19) return -1;
+    for (int i = 0; i < q; ++i) {
+        cx[i] = q == 9 ? CX9[i] : CX19[i];
+        cy[i] = q == 9 ? CY9[i] : CY19[i];
+        cz[i] = q == 9 ? 0 : CZ19[i];
+    }
+    for (int i = 0; i < q; ++i) {
+        int c2 = cx[i] * cx[i] + cy[i] * cy[i] + cz[i] * cz[i];
+        if (q == 9) w[i] = c2 == 0 ? 4.0 / 9.0 : (c2 == 1 ? 1.0 / 9.0 : 1.0 / 36.0);
+        else w[i] = c2 == 0 ? 1.0 / 3.0 : (c2 == 1 ? 1.0 / 18.0 : 1.0 / 36.0);
+        for (int j = 0; j < q; ++j)
+            if (cx[j] == -cx[i] && cy[j] == -cy[i] && cz[j] == -cz[i]) opp[i] = j;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* MRT: rows of M as moment polynomials of c.  The reference tabulates the
+ * same matrices (S/CellFunctions.cu:15-36 for D2Q9, :73-114 for D3Q19);
+ * rows are orthogonal, so Minv = M^T diag(1/|row|^2).                       */
+static double mrt_row(int q, int r, int cx, int cy, int cz)
+{
+    double c2 = cx * cx + cy * cy + cz * cz;
+    if (q == 9) {
+        switch (r) {
+        case 0: return 1;
+        case 1: return 3 * c2 - 4;
+        case 2: return (9 * c2 * c2 - 21 * c2 + 8) / 2;
+        case 3: return cx;
+        case 4: return (3 * c2 - 5) * cx;
+        case 5: return cy;
+        case 6: return (3 * c2 - 5) * cy;
+        case 7: return cx * cx - cy * cy;
+        default: return cx * cy;
+        }
+    }
+    switch (r) {
+    case 0: return 1;
+    case 1: return 19 * c2 - 30;
+    case 2: return (21 * c2 * c2 - 53 * c2 + 24) / 2;
+    case 3: return cx;
+    case 4: return (5 * c2 - 9) * cx;
+    case 5: return cy;
+    case 6: return (5 * c2 - 9) * cy;
+    case 7: return cz;
+    case 8: return (5 * c2 - 9) * cz;
+    case 9: return 3 * cx * cx - c2;
+    case 10: return (3 * c2 - 5) * (3 * cx * cx - c2);
+    case 11: return cy * cy - cz * cz;
+    case 12: return (3 * c2 - 5) * (cy * cy - cz * cz);
+    case 13: return cx * cy;
+    case 14: return cy * cz;
+    case 15: return cx * cz;
+    case 16: return (cy * cy - cz * cz) * cx;
+    case 17: return (cz * cz - cx * cx) * cy;
+    default: return (cx * cx - cy * cy) * cz;
+    }
+}
+
+int orc_mrt_matrices(int q, double omega, double* M, double* Minv, double* s)
+{
+    int cx[19], cy[19], cz[19], opp[19];
+    double w[19];
+    if (orc_lattice(q, cx, cy, cz, opp, w)) return -1;
+    for (int r = 0; r < q; ++r) {
+        double n2 = 0;
+        for (int j = 0; j < q; ++j) {
+            M[r * q + j] = mrt_row(q, r, cx[j], cy[j], cz[j]);
+            n2 += M[r * q + j] * M[r * q + j];
+        }
+        for (int j = 0; j < q; ++j) Minv[j * q + r] = M[r * q + j] / n2;
+    }
+    if (q == 9) { /* S/CellFunctions.cu:44-52 */
+        const double s9[9] = {1.0, 1.4, 1.4, 1.0, 1.2, 1.0, 1.2, omega, omega};
+        memcpy(s, s9, sizeof s9);
+    } else {      /* S/CellFunctions.cu:122-140 */
+        const double s19[19] = {0.0, 1.19, 1.4, 0.0, 1.2, 0.0, 1.2, 0.0, 1.2, omega,
+                                1.4, omega, 1.4, omega, omega, omega, 1.98, 1.98, 1.98};
+        memcpy(s, s19, sizeof s19);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+struct orc_sim {
+    orc_params p;
+    long n;
+    int Q;
+    int cx[19], cy[19], cz[19], opp[19];
+    double w[19];
+    double omega_m;                 /* TRT antisymmetric rate */
+    double M[361], Minv[361], s[19];
+    uint8_t* cls;
+    uint32_t* bb;
+    double *ubc, *vbc, *wbc;
+    double *f, *fcoll, *rho, *u, *v, *w3;
+};
+
+/* f_eq = rho w (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u.u)  (S/GpuCollision.cu:21-32) */
+static void feq_all(const orc_sim* s, double rho, const double* uvw, double* feq)
+{
+    double uu = uvw[0] * uvw[0] + uvw[1] * uvw[1] + uvw[2] * uvw[2];
+    for (int i = 0; i < s->Q; ++i) {
+        double cu = s->cx[i] * uvw[0] + s->cy[i] * uvw[1] + s->cz[i] * uvw[2];
+        feq[i] = rho * s->w[i] * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
+    }
+}
+
+void orc_equilibrium(int q, double rho, const double* uvw, double* feq)
+{
+    orc_sim tmp;
+    memset(&tmp, 0, sizeof tmp);
+    tmp.Q = q;
+    orc_lattice(q, tmp.cx, tmp.cy, tmp.cz, tmp.opp, tmp.w);
+    feq_all(&tmp, rho, uvw, feq);
+}
+
+/* Guo source without its relaxation prefactor:
+ * S_i = w_i [3 (c_i - u) + 9 (c_i.u) c_i] . F, with F = rho g. */
+static void guo_source(const orc_sim* s, double rho, const double* uvw, double* S)
+{
+    const double* g = s->p.g;
+    double F[3] = {rho * g[0], rho * g[1], rho * g[2]};
+    for (int i = 0; i < s->Q; ++i) {
+        double c[3] = {s->cx[i], s->cy[i], s->cz[i]};
+        double cu = c[0] * uvw[0] + c[1] * uvw[1] + c[2] * uvw[2];
+        double acc = 0;
+        for (int d = 0; d < 3; ++d) acc += (3.0 * (c[d] - uvw[d]) + 9.0 * cu * c[d]) * F[d];
+        S[i] = s->w[i] * acc;
+    }
+}
+
+/* MRT equilibrium moments.
+ * D2Q9: S/GpuCollision.cu:986-994 (rho^2 in e, eps under ORC_PARITY_MRT_RHO2).
+ * D3Q19: S/GpuCollision.cu:1065-1083 (j = rho u; incompressible j^2 forms). */
+static void mrt_meq(const orc_sim* s, double r, const double* uvw, double* meq)
+{
+    if (s->Q == 9) {
+        double u = uvw[0], v = uvw[1], uu = u * u + v * v;
+        double k = (s->p.parity & ORC_PARITY_MRT_RHO2) ? r : 1.0;
+        meq[0] = r;
+        meq[1] = r * (-2.0 + 3.0 * k * uu);
+        meq[2] = r * (1.0 - 3.0 * k * uu);
+        meq[3] = r * u;
+        meq[4] = -r * u;
+        meq[5] = r * v;
+        meq[6] = -r * v;
+        meq[7] = r * (u * u - v * v);
+        meq[8] = r * u * v;
+        return;
+    }
+    double jx = r * uvw[0], jy = r * uvw[1], jz = r * uvw[2];
+    double j2 = jx * jx + jy * jy + jz * jz;
+    meq[0] = r;
+    meq[1] = -11.0 * r + 19.0 * j2;
+    meq[2] = -475.0 / 63.0 * j2;
+    meq[3] = jx;
+    meq[4] = -2.0 / 3.0 * jx;
+    meq[5] = jy;
+    meq[6] = -2.0 / 3.0 * jy;
+    meq[7] = jz;
+    meq[8] = -2.0 / 3.0 * jz;
+    meq[9] = 2.0 * jx * jx - (jy * jy + jz * jz);
+    meq[10] = 0.0;
+    meq[11] = jy * jy - jz * jz;
+    meq[12] = 0.0;
+    meq[13] = jx * jy;
+    meq[14] = jy * jz;
+    meq[15] = jx * jz;
+    meq[16] = meq[17] = meq[18] = 0.0;
+}
+
+/* Collision of one node from its populations f and stored macros (rho, u).
+ * BGK: S/GpuCollision.cu:528-550, :882-916.  TRT: :917-969 with a general
+ * magic parameter.  MRT: :971-1037, :1039-1170.  Forcing: the reference
+ * shifts the stored velocity (S/GpuUpdateMacro.cu:207), which enters here
+ * through u; GUO adds the source term with the (1 - s/2) prefactors. */
+void orc_collide_node(const orc_sim* s, const double* f, double rho, const double* uvw, double* out)
+{
+    const int Q = s->Q;
+    double feq[19], S[19];
+    const int guo = s->p.force_model == ORC_FORCE_GUO;
+    if (guo) guo_source(s, rho, uvw, S);
+    if (s->p.collision == ORC_BGK) {
+        const double om = s->p.omega;
+        feq_all(s, rho, uvw, feq);
+        for (int i = 0; i < Q; ++i) {
+            out[i] = om * feq[i] + (1.0 - om) * f[i];
+            if (guo) out[i] += (1.0 - 0.5 * om) * S[i];
+        }
+    } else if (s->p.collision == ORC_TRT) {
+        const double op = s->p.omega, om = s->omega_m;
+        feq_all(s, rho, uvw, feq);
+        for (int i = 0; i < Q; ++i) {
+            int o = s->opp[i];
+            double fp = 0.5 * (f[i] + f[o]), fm = 0.5 * (f[i] - f[o]);
+            double ep = 0.5 * (feq[i] + feq[o]), em = 0.5 * (feq[i] - feq[o]);
+            out[i] = f[i] - op * (fp - ep) - om * (fm - em);
+            if (guo) {
+                double sp = 0.5 * (S[i] + S[o]), sm = 0.5 * (S[i] - S[o]);
+                out[i] += (1.0 - 0.5 * op) * sp + (1.0 - 0.5 * om) * sm;
+            }
+        }
+    } else {
+        double m[19], meq[19], dm[19];
+        mrt_meq(s, rho, uvw, meq);
+        for (int r = 0; r < Q; ++r) {
+            double acc = 0;
+            for (int j = 0; j < Q; ++j) acc += s->M[r * Q + j] * f[j];
+            m[r] = acc;
+            dm[r] = s->s[r] * (m[r] - meq[r]);
+        }
+        double sm[19];
+        if (guo) {
+            for (int r = 0; r < Q; ++r) {
+                double acc = 0;
+                for (int j = 0; j < Q; ++j) acc += s->M[r * Q + j] * S[j];
+                sm[r] = (1.0 - 0.5 * s->s[r]) * acc;
+            }
+        }
+        for (int i = 0; i < Q; ++i) {
+            double acc = 0;
+            for (int r = 0; r < Q; ++r) acc += s->Minv[i * Q + r] * dm[r];
+            out[i] = f[i] - acc;
+            if (guo) {
+                double a2 = 0;
+                for (int r = 0; r < Q; ++r) a2 += s->Minv[i * Q + r] * sm[r];
+                out[i] += a2;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+static void set_threads(int nt)
+{
+#ifdef _OPENMP
+    if (nt > 0) omp_set_num_threads(nt);
+#else
+    (void)nt;
+#endif
+}
+
+orc_sim* orc_create(const orc_params* p, const uint8_t* node_class, const uint32_t* bb,
+                    const double* ubc, const double* vbc, const double* wbc)
+{
+    if (p->q != 9 && p->q != 19) return NULL;
+    orc_sim* s = (orc_sim*)calloc(1, sizeof *s);
+    s->p = *p;
+    if (s->p.nz < 1) s->p.nz = 1;
+    s->Q = p->q;
+    s->n = (long)p->nx * p->ny * s->p.nz;
+    orc_lattice(s->Q, s->cx, s->cy, s->cz, s->opp, s->w);
+    {
+        /* Lambda = (1/w+ - 1/2)(1/w- - 1/2) */
+        double lam = p->trt_magic > 0 ? p->trt_magic : 3.0 / 16.0;
+        s->omega_m = 1.0 / (0.5 + lam / (1.0 / p->omega - 0.5));
+    }
+    orc_mrt_matrices(s->Q, p->omega, s->M, s->Minv, s->s);
+    {
+        int any = 0;
+        for (int i = 0; i < s->Q; ++i) any |= p->mrt_s[i] != 0.0;
+        if (any) memcpy(s->s, p->mrt_s, sizeof(double) * s->Q);
+    }
+    long n = s->n;
+    s->cls = (uint8_t*)malloc(n);
+    s->bb = (uint32_t*)malloc(n * sizeof(uint32_t));
+    memcpy(s->cls, node_class, n);
+    if (bb) memcpy(s->bb, bb, n * sizeof(uint32_t));
+    else memset(s->bb, 0, n * sizeof(uint32_t));
+    s->ubc = (double*)calloc(n, sizeof(double));
+    s->vbc = (double*)calloc(n, sizeof(double));
+    s->wbc = (double*)calloc(n, sizeof(double));
+    if (ubc) memcpy(s->ubc, ubc, n * sizeof(double));
+    if (vbc) memcpy(s->vbc, vbc, n * sizeof(double));
+    if (wbc) memcpy(s->wbc, wbc, n * sizeof(double));
+    s->f = (double*)calloc((size_t)n * s->Q, sizeof(double));
+    s->fcoll = (double*)calloc((size_t)n * s->Q, sizeof(double));
+    s->rho = (double*)calloc(n, sizeof(double));
+    s->u = (double*)calloc(n, sizeof(double));
+    s->v = (double*)calloc(n, sizeof(double));
+    s->w3 = (double*)calloc(n, sizeof(double));
+    return s;
+}
+
+void orc_destroy(orc_sim* s)
+{
+    if (!s) return;
+    free(s->cls); free(s->bb); free(s->ubc); free(s->vbc); free(s->wbc);
+    free(s->f); free(s->fcoll); free(s->rho); free(s->u); free(s->v); free(s->w3);
+    free(s);
+}
+
+/* macroscopic update of one node (S/GpuUpdateMacro.cu:6-41, :137-213) */
+static void macro_node(orc_sim* s, long ind)
+{
+    const int Q = s->Q;
+    const long n = s->n;
+    uint8_t c = s->cls[ind];
+    if (c == ORC_SOLID) return;
+    if ((s->p.parity & ORC_PARITY_INLET_FROZEN) && c == ORC_VEL_W) return;
+    double r = 0, jx = 0, jy = 0, jz = 0;
+    for (int i = 0; i < Q; ++i) {
+        double fi = s->f[(size_t)i * n + ind];
+        r += fi;
+        jx += s->cx[i] * fi;
+        jy += s->cy[i] * fi;
+        jz += s->cz[i] * fi;
+    }
+    double u = jx / r, v = jy / r, w = jz / r;
+    if (s->p.force_model == ORC_FORCE_SHIFT) {
+        u += s->p.g[0] / s->p.omega;
+        v += s->p.g[1] / s->p.omega;
+        w += s->p.g[2] / s->p.omega;
+    } else if (s->p.force_model == ORC_FORCE_GUO) {
+        u += 0.5 * s->p.g[0];
+        v += 0.5 * s->p.g[1];
+        w += 0.5 * s->p.g[2];
+    }
+    if ((s->p.parity & ORC_PARITY_OUTLET_V0) && c == ORC_PRES_E) v = 0.0;
+    s->rho[ind] = r;
+    s->u[ind] = u;
+    s->v[ind] = v;
+    s->w3[ind] = w;
+}
+
+void orc_init(orc_sim* s, int mode, const double* rho, const double* u, const double* v, const double* w)
+{
+    const long n = s->n;
+    const int Q = s->Q;
+    set_threads(s->p.nthreads);
+#pragma omp parallel for
+    for (long ind = 0; ind < n; ++ind) {
+        double r = rho ? rho[ind] : 1.0;
+        double uvw[3] = {u ? u[ind] : 0.0, v ? v[ind] : 0.0, w ? w[ind] : 0.0};
+        s->rho[ind] = r;
+        s->u[ind] = uvw[0];
+        s->v[ind] = uvw[1];
+        s->w3[ind] = uvw[2];
+        double feq[19];
+        feq_all(s, r, uvw, feq);
+        for (int i = 0; i < Q; ++i) {
+            s->f[(size_t)i * n + ind] = (mode == 0 && s->cls[ind] != ORC_SOLID) ? feq[i] : 0.0;
+            s->fcoll[(size_t)i * n + ind] = 0.0;
+        }
+    }
+}
+
+static inline long wrap_src(const orc_sim* s, int x, int y, int z, int i, int* outside)
+{
+    int sx = x - s->cx[i], sy = y - s->cy[i], sz = z - s->cz[i];
+    const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
+    *outside = 0;
+    if (sx < 0 || sx >= nx) { if (s->p.periodic[0]) sx = (sx + nx) % nx; else *outside = 1; }
+    if (sy < 0 || sy >= ny) { if (s->p.periodic[1]) sy = (sy + ny) % ny; else *outside = 1; }
+    if (sz < 0 || sz >= nz) { if (s->p.periodic[2]) sz = (sz + nz) % nz; else *outside = 1; }
+    return ((long)sz * ny + sy) * nx + sx;
+}
+
+/* Zou-He boundaries on D2Q9 faces.
+ * Velocity inlet, west face: textbook (Zou & He 1997) or the reference's
+ * simplified form without the transverse terms (S/GpuBoundaries.cu:45-64).
+ * Pressure outlet, east face: closed form of
+ * T/LBM/Tom/Lecture Script/LBMchannel.m:254-270 (absent from the solver). */
+static void zouhe_node(const orc_sim* s, long ind, double* f)
+{
+    uint8_t c = s->cls[ind];
+    if (s->Q != 9) return;
+    if (c == ORC_VEL_W) {
+        double u0 = s->ubc[ind], v0 = s->vbc[ind];
+        if (s->p.inlet_model == ORC_INLET_REF2D) {
+            if (s->bb[ind]) return;   /* corner nodes are skipped (BC_CORNER) */
+            double r = (f[0] + f[2] + f[4] + 2.0 * (f[3] + f[6] + f[7])) / (1.0 - u0);
+            f[1] = f[3] + 2.0 * r * u0 / 3.0;
+            f[5] = f[7] + r * u0 / 6.0;
+            f[8] = f[6] + r * u0 / 6.0;
+        } else {
+            double r = (f[0] + f[2] + f[4] + 2.0 * (f[3] + f[6] + f[7])) / (1.0 - u0);
+            f[1] = f[3] + 2.0 / 3.0 * r * u0;
+            f[5] = f[7] - 0.5 * (f[2] - f[4]) + r * u0 / 6.0 + 0.5 * r * v0;
+            f[8] = f[6] + 0.5 * (f[2] - f[4]) + r * u0 / 6.0 - 0.5 * r * v0;
+        }
+    } else if (c == ORC_PRES_E) {
+        double ro = s->p.rho_out;
+        double ux = -1.0 + (f[0] + f[2] + f[4] + 2.0 * (f[1] + f[5] + f[8])) / ro;
+        f[3] = f[1] - 2.0 / 3.0 * ro * ux;
+        f[7] = f[5] + 0.5 * (f[2] - f[4]) - ro * ux / 6.0;
+        f[6] = f[8] - 0.5 * (f[2] - f[4]) - ro * ux / 6.0;
+    }
+}
+
+void orc_step(orc_sim* s, int nsteps)
+{
+    const long n = s->n;
+    const int Q = s->Q;
+    const int nx = s->p.nx, ny = s->p.ny;
+    set_threads(s->p.nthreads);
+    for (int it = 0; it < nsteps; ++it) {
+        /* collide (reads stored macros) */
+#pragma omp parallel for
+        for (long ind = 0; ind < n; ++ind) {
+            if (s->cls[ind] == ORC_SOLID) continue;
+            double fin[19], out[19];
+            for (int i = 0; i < Q; ++i) fin[i] = s->f[(size_t)i * n + ind];
+            double uvw[3] = {s->u[ind], s->v[ind], s->w3[ind]};
+            orc_collide_node(s, fin, s->rho[ind], uvw, out);
+            int nw = (Q == 19 && s->p.collision == ORC_MRT && (s->p.parity & ORC_PARITY_MRT_NO_F18)) ? 18 : Q;
+            for (int i = 0; i < nw; ++i) s->fcoll[(size_t)i * n + ind] = out[i];
+        }
+        /* stream (pull, S/GpuStream.cu:8-28, :160-194) + walls + open boundaries */
+#pragma omp parallel for
+        for (long ind = 0; ind < n; ++ind) {
+            if (s->cls[ind] == ORC_SOLID) continue;
+            int x = (int)(ind % nx), y = (int)((ind / nx) % ny), z = (int)(ind / ((long)nx * ny));
+            uint32_t bb = s->bb[ind];
+            double f[19];
+            for (int i = 0; i < Q; ++i) {
+                int o = s->opp[i];
+                if (i > 0 && (bb >> o) & 1u) {
+                    /* the source node lies beyond a wall */
+                    f[i] = s->p.wall_model == ORC_WALL_HALFWAY ? s->fcoll[(size_t)o * n + ind]
+                                                               : s->fcoll[(size_t)i * n + ind];
+                    continue;
+                }
+                int outside;
+                long src = wrap_src(s, x, y, z, i, &outside);
+                f[i] = outside ? s->fcoll[(size_t)i * n + ind] : s->fcoll[(size_t)i * n + src];
+            }
+            if (s->cls[ind] == ORC_VEL_W) zouhe_node(s, ind, f);
+            if (s->p.wall_model == ORC_WALL_ONNODE && bb) {
+                /* on-node bounce-back, S/GpuBoundaries.cu:400-443, :445-490 */
+                for (int d = 1; d < Q; ++d)
+                    if ((bb >> d) & 1u) f[s->opp[d]] = f[d];
+            }
+            if (s->cls[ind] == ORC_PRES_E) zouhe_node(s, ind, f);
+            for (int i = 0; i < Q; ++i) s->f[(size_t)i * n + ind] = f[i];
+        }
+        /* macroscopic update */
+#pragma omp parallel for
+        for (long ind = 0; ind < n; ++ind) macro_node(s, ind);
+    }
+}
+
+void orc_get_f(const orc_sim* s, double* f)
+{
+    memcpy(f, s->f, sizeof(double) * (size_t)s->n * s->Q);
+}
+
+void orc_set_f(orc_sim* s, const double* f)
+{
+    memcpy(s->f, f, sizeof(double) * (size_t)s->n * s->Q);
+    for (long ind = 0; ind < s->n; ++ind) macro_node(s, ind);
+}
+
+void orc_get_macros(const orc_sim* s, double* rho, double* u, double* v, double* w)
+{
+    size_t b = sizeof(double) * (size_t)s->n;
+    if (rho) memcpy(rho, s->rho, b);
+    if (u) memcpy(u, s->u, b);
+    if (v) memcpy(v, s->v, b);
+    if (w) memcpy(w, s->w3, b);
+}
+
+double orc_mass(const orc_sim* s)
+{
+    long double acc = 0;
+    for (long ind = 0; ind < s->n; ++ind) {
+        if (s->cls[ind] == ORC_SOLID) continue;
+        for (int i = 0; i < s->Q; ++i) acc += s->f[(size_t)i * s->n + ind];
+    }
+    return (double)acc;
+}
+
+/* ======================================================================== */
+/* colour-gradient two-phase model, D2Q9, periodic box.
+ * Follows the GPU formulation gpuCollBgkwGC2D (S/GpuCollision.cu:552-622),
+ * gradient calculateColorGradient / calculateHOColorGradient case 0
+ * (S/GpuCollision.cu:112-281), streaming gpuStreaming2DCG case 0
+ * (S/GpuStream.cu:30-158) and moments gpuUpdateMacro2DCG
+ * (S/GpuUpdateMacro.cu:43-135), with periodic wrap instead of the reference's
+ * edge-orientation stencils.                                                */
+struct orc_cg {
+    orc_cg_params p;
+    long n;
+    double *r, *b, *rc, *bc, *f;          /* colour populations, post-collision, total */
+    double *rr, *br, *rho, *u, *v;        /* stored macros */
+    int hx[25], hy[25];
+    double hw[25];
+    int nst;
+};
+
+/* CG constants (S/GpuInit.cu:150-186) */
+static const double CG_PHI[9] = {0.0, 0.2, 0.2, 0.2, 0.2, 0.05, 0.05, 0.05, 0.05};
+static const double CG_TETA[9] = {1.0, -0.2, -0.2, -0.2, -0.2, -0.05, -0.05, -0.05, -0.05};
+static const double CG_BPERT[9] = {-4.0 / 27.0, 2.0 / 27.0, 2.0 / 27.0, 2.0 / 27.0, 2.0 / 27.0,
+                                   5.0 / 108.0, 5.0 / 108.0, 5.0 / 108.0, 5.0 / 108.0};
+static const double W9[9] = {4.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0,
+                             1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+
+orc_cg* orc_cg_create(const orc_cg_params* p)
+{
+    orc_cg* s = (orc_cg*)calloc(1, sizeof *s);
+    s->p = *p;
+    s->n = (long)p->nx * p->ny;
+    size_t n9 = (size_t)s->n * 9;
+    s->r = (double*)calloc(n9, 8); s->b = (double*)calloc(n9, 8);
+    s->rc = (double*)calloc(n9, 8); s->bc = (double*)calloc(n9, 8);
+    s->f = (double*)calloc(n9, 8);
+    s->rr = (double*)calloc(s->n, 8); s->br = (double*)calloc(s->n, 8);
+    s->rho = (double*)calloc(s->n, 8); s->u = (double*)calloc(s->n, 8); s->v = (double*)calloc(s->n, 8);
+    if (p->high_order) {
+        /* 24-neighbour isotropic stencil; weights by |c|^2 over 5040
+         * (S/GpuInit.cu:186-199): 1 -> 960, 2 -> 448, 4 -> 84, 5 -> 32, 8 -> 1 */
+        int k = 0;
+        for (int dy = -2; dy <= 2; ++dy)
+            for (int dx = -2; dx <= 2; ++dx) {
+                int c2 = dx * dx + dy * dy;
+                if (c2 == 0) continue;
+                double wt = c2 == 1 ? 960 : c2 == 2 ? 448 : c2 == 4 ? 84 : c2 == 5 ? 32 : 1;
+                s->hx[k] = dx; s->hy[k] = dy; s->hw[k] = wt / 5040.0; ++k;
+            }
+        s->nst = k;
+    } else {
+        for (int i = 1; i < 9; ++i) {
+            s->hx[i - 1] = CX9[i]; s->hy[i - 1] = CY9[i];
+            s->hw[i - 1] = (i < 5) ? 4.0 / 12.0 : 1.0 / 12.0;   /* cg_w, S/GpuInit.cu:186 */
+        }
+        s->nst = 8;
+    }
+    return s;
+}
+
+void orc_cg_destroy(orc_cg* s)
+{
+    if (!s) return;
+    free(s->r); free(s->b); free(s->rc); free(s->bc); free(s->f);
+    free(s->rr); free(s->br); free(s->rho); free(s->u); free(s->v);
+    free(s);
+}
+
+void orc_cg_init(orc_cg* s, const double* r_rho, const double* b_rho)
+{
+    const long n = s->n;
+    for (long ind = 0; ind < n; ++ind) {
+        double rr = r_rho[ind], br = b_rho[ind];
+        for (int i = 0; i < 9; ++i) {
+            s->r[i * n + ind] = rr * (CG_PHI[i] + CG_TETA[i] * s->p.r_alpha);
+            s->b[i * n + ind] = br * (CG_PHI[i] + CG_TETA[i] * s->p.b_alpha);
+            s->f[i * n + ind] = s->r[i * n + ind] + s->b[i * n + ind];
+        }
+        s->rr[ind] = rr; s->br[ind] = br; s->rho[ind] = rr + br;
+        s->u[ind] = 0; s->v[ind] = 0;
+    }
+}
+
+static void cg_grad_node(const orc_cg* s, int x, int y, double* gx, double* gy)
+{
+    const int nx = s->p.nx, ny = s->p.ny;
+    double ax = 0, ay = 0;
+    for (int k = 0; k < s->nst; ++k) {
+        int xx = (x + s->hx[k] + nx) % nx, yy = (y + s->hy[k] + ny) % ny;
+        long j = (long)yy * nx + xx;
+        double psi = s->hw[k] * (s->rr[j] - s->br[j]) / s->rho[j];
+        ax += psi * s->hx[k];
+        ay += psi * s->hy[k];
+    }
+    *gx = ax; *gy = ay;
+}
+
+void orc_cg_gradient(const orc_cg* s, double* cgx, double* cgy)
+{
+    for (long ind = 0; ind < s->n; ++ind)
+        cg_grad_node(s, (int)(ind % s->p.nx), (int)(ind / s->p.nx), &cgx[ind], &cgy[ind]);
+}
+
+void orc_cg_step(orc_cg* s, int nsteps)
+{
+    const long n = s->n;
+    const int nx = s->p.nx, ny = s->p.ny;
+    const orc_cg_params* P = &s->p;
+    set_threads(P->nthreads);
+    for (int it = 0; it < nsteps; ++it) {
+#pragma omp parallel for
+        for (long ind = 0; ind < n; ++ind) {
+            int x = (int)(ind % nx), y = (int)(ind / nx);
+            double cgx, cgy;
+            cg_grad_node(s, x, y, &cgx, &cgy);
+            double rr = s->rr[ind], br = s->br[ind], r = s->rho[ind];
+            double u = s->u[ind], v = s->v[ind];
+            double mean_nu = 1.0 / (rr / (r * P->r_nu) + br / (r * P->b_nu));
+            double om = 1.0 / (3.0 * mean_nu + 0.5);
+            double alpha = P->r_alpha * rr / r + P->b_alpha * br / r;
+            double gn = sqrt(cgx * cgx + cgy * cgy);
+            double kr = rr / r, kb = br / r, kk = P->beta * rr * br / r;
+            double uu = u * u + v * v;
+            for (int k = 0; k < 9; ++k) {
+                double cs = 0, pert = 0;
+                if (gn > P->g_limit) {
+                    double cg = CX9[k] * cgx + CY9[k] * cgy;
+                    if (k) cs = cg / (gn * sqrt((double)(CX9[k] * CX9[k] + CY9[k] * CY9[k])));
+                    pert = P->A * gn * (W9[k] * cg * cg / (gn * gn) - CG_BPERT[k]);
+                }
+                double cu = u * CX9[k] + v * CY9[k];
+                double rest = CG_PHI[k] + CG_TETA[k] * alpha;
+                double feq = r * (rest + W9[k] * (3.0 * cu + 4.5 * cu * cu - 1.5 * uu));
+                double fs = om * feq + (1.0 - om) * s->f[k * n + ind] + pert;
+                s->rc[k * n + ind] = kr * fs + kk * cs * rest;
+                s->bc[k * n + ind] = kb * fs - kk * cs * rest;
+            }
+        }
+#pragma omp parallel for
+        for (long ind = 0; ind < n; ++ind) {
+            int x = (int)(ind % nx), y = (int)(ind / nx);
+            double rr0 = s->rr[ind], br0 = s->br[ind], r0 = s->rho[ind];
+            double R = 0, B = 0, jx = 0, jy = 0;
+            for (int k = 0; k < 9; ++k) {
+                int sx = (x - CX9[k] + nx) % nx, sy = (y - CY9[k] + ny) % ny;
+                long src = (long)sy * nx + sx;
+                double rv = s->rc[k * n + src], bv = s->bc[k * n + src];
+                s->r[k * n + ind] = rv;
+                s->b[k * n + ind] = bv;
+                s->f[k * n + ind] = rv + bv;
+                R += rv; B += bv;
+                jx += CX9[k] * (rv + bv);
+                jy += CY9[k] * (rv + bv);
+            }
+            /* omega_eff from the previous densities (S/GpuUpdateMacro.cu:62-64) */
+            double mean_nu = 1.0 / (rr0 / (r0 * P->r_nu) + br0 / (r0 * P->b_nu));
+            double om = 1.0 / (3.0 * mean_nu + 0.5);
+            double r = R + B;
+            s->u[ind] = jx / r + (P->external_force ? P->g / (r * om) : 0.0);
+            s->v[ind] = jy / r + (P->external_force ? 0.0 : P->g / om);
+        }
+#pragma omp parallel for
+        for (long ind = 0; ind < n; ++ind) {
+            double R = 0, B = 0;
+            for (int k = 0; k < 9; ++k) { R += s->r[k * n + ind]; B += s->b[k * n + ind]; }
+            s->rr[ind] = R; s->br[ind] = B; s->rho[ind] = R + B;
+        }
+    }
+}
+
+void orc_cg_get(const orc_cg* s, double* r_f, double* b_f, double* r_rho, double* b_rho,
+                double* rho, double* u, double* v)
+{
+    size_t n = (size_t)s->n, b9 = n * 9 * 8, b1 = n * 8;
+    if (r_f) memcpy(r_f, s->r, b9);
+    if (b_f) memcpy(b_f, s->b, b9);
+    if (r_rho) memcpy(r_rho, s->rr, b1);
+    if (b_rho) memcpy(b_rho, s->br, b1);
+    if (rho) memcpy(rho, s->rho, b1);
+    if (u) memcpy(u, s->u, b1);
+    if (v) memcpy(v, s->v, b1);
+}
+
+/* Laplace sampling masks (S/GpuUpdateMacro.cu:124-133) and max |u|. */
+void orc_cg_laplace(const orc_cg* s, double* out)
+{
+    double pin = 0, nin = 0, pout = 0, nout = 0, umax = 0;
+    for (long ind = 0; ind < s->n; ++ind) {
+        double chi = (s->rr[ind] - s->br[ind]) / s->rho[ind];
+        if (chi >= s->p.kappa) { pin += s->rr[ind]; nin += 1; }
+        else if (chi <= -s->p.kappa) { pout += s->br[ind]; nout += 1; }
+        double um = sqrt(s->u[ind] * s->u[ind] + s->v[ind] * s->v[ind]);
+        if (um > umax) umax = um;
+    }
+    out[0] = pin; out[1] = nin; out[2] = pout; out[3] = nout; out[4] = umax;
+}

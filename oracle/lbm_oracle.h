@@ -1,0 +1,121 @@
+/*
+ * FP64 CPU oracle for the lattice-Boltzmann time step of ZPaulo/Thesis
+ * (`Source code/lbm-solver_final`).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This restatement is the checker that the HIP
+ * product path is compared against; it is linked by the tests, by
+ * __graft_entry__.smoke() and by bench.py's CPU-baseline leg, and by nothing
+ * that ships.  The product never calls into it.
+ *
+ * It keeps the reference's data flow: populations f and post-collision
+ * populations fColl are separate SoA arrays f[q*N + ind] (S/include/Iterate.h:10-35),
+ * macroscopic fields rho,u,v,w are stored and the collision reads them
+ * (S/GpuCollision.cu:528-550), and one step is
+ *   collide -> stream -> boundaries -> macroscopic update
+ * (S/Iterate.cu:385-520, S/Iterate3D.cu:493-653).
+ *
+ * Every reference behaviour that the north-star configurations replace by a
+ * textbook one has a switch here (wall model, inlet model, forcing, MRT
+ * quirks), so the same oracle pins both the reference-parity and the
+ * textbook modes of the product.
+ */
+#ifndef LBM_ORACLE_H
+#define LBM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* node classes (shared with the product's lattice description) */
+enum { ORC_SOLID = 0, ORC_FLUID = 1, ORC_VEL_W = 2, ORC_PRES_E = 3 };
+enum { ORC_BGK = 0, ORC_TRT = 1, ORC_MRT = 2 };
+enum { ORC_FORCE_NONE = 0, ORC_FORCE_SHIFT = 1, ORC_FORCE_GUO = 2 };
+enum { ORC_WALL_HALFWAY = 0, ORC_WALL_ONNODE = 1 };
+enum { ORC_INLET_ZOUHE = 0, ORC_INLET_REF2D = 1 };
+/* parity flags: reference quirks reproduced on request */
+#define ORC_PARITY_MRT_RHO2      0x1u /* 2D MRT m_eq uses rho^2 (S/GpuCollision.cu:987-988) */
+#define ORC_PARITY_MRT_NO_F18    0x2u /* 3D MRT never writes f18 (S/GpuCollision.cu:1161) */
+#define ORC_PARITY_INLET_FROZEN  0x4u /* 3D macros skip inlet nodes (S/GpuUpdateMacro.cu:150-151) */
+#define ORC_PARITY_OUTLET_V0     0x8u /* 2D v := 0 on east-outlet nodes (S/GpuUpdateMacro.cu:30) */
+
+typedef struct {
+    int q;              /* 9 (D2Q9) or 19 (D3Q19) */
+    int nx, ny, nz;     /* n, m, h in the reference's naming; nz = 1 for 2D */
+    int periodic[3];
+    int collision;
+    double omega;       /* 1/tau = 1/(3 nu + 1/2) (S/GpuInit.cu:102) */
+    double trt_magic;   /* Lambda; 3/16 reproduces omegaA = 8(2-w)/(8-w) (S/GpuInit.cu:103) */
+    double mrt_s[19];   /* diag(S); all zero selects the reference's tables */
+    int force_model;
+    double g[3];
+    int wall_model;
+    int inlet_model;
+    unsigned parity;
+    double rho_out;     /* Zou-He pressure outlet density */
+    int nthreads;       /* OpenMP threads; <= 0 uses the runtime default */
+} orc_params;
+
+typedef struct orc_sim orc_sim;
+
+/* Lattice tables (reference numbering, S/GpuInit.cu:94-99 and :822-842). */
+int orc_lattice(int q, int* cx, int* cy, int* cz, int* opp, double* w);
+
+/* MRT matrices built from the moment polynomials; M and Minv are Q*Q row-major.
+ * s receives the reference's diag(S) for the given omega
+ * (S/CellFunctions.cu:44-52 for D2Q9, :122-140 for D3Q19). */
+int orc_mrt_matrices(int q, double omega, double* M, double* Minv, double* s);
+
+/* Create a simulation.  node_class[N], bb[N] (bit d: the link from the node
+ * in direction d crosses a wall), ubc/vbc/wbc: prescribed velocity on
+ * velocity-inlet nodes (may be NULL). */
+orc_sim* orc_create(const orc_params* p, const uint8_t* node_class, const uint32_t* bb,
+                    const double* ubc, const double* vbc, const double* wbc);
+void orc_destroy(orc_sim* s);
+
+/* mode 0: f = f_eq(rho,u); mode 1: f = 0 (the reference's start, S/Iterate3D.cu:301-302).
+ * rho/u/v/w may be NULL (1 and 0). Stored macros are set to the given fields. */
+void orc_init(orc_sim* s, int mode, const double* rho, const double* u, const double* v, const double* w);
+void orc_step(orc_sim* s, int nsteps);
+
+void orc_get_f(const orc_sim* s, double* f);         /* post-boundary f (Q*N) */
+void orc_set_f(orc_sim* s, const double* f);         /* also recomputes the stored macros */
+void orc_get_macros(const orc_sim* s, double* rho, double* u, double* v, double* w);
+double orc_mass(const orc_sim* s);                   /* sum of f over non-solid nodes */
+
+/* Kernels exposed for closed-form tests. */
+void orc_collide_node(const orc_sim* s, const double* f, double rho, const double* uvw, double* out);
+void orc_equilibrium(int q, double rho, const double* uvw, double* feq);
+
+/* ---- colour-gradient two-phase model (D2Q9) ----------------------------- */
+typedef struct {
+    int nx, ny;
+    double r_alpha, b_alpha;     /* rest fractions alpha_k */
+    double r_nu, b_nu;           /* kinematic viscosities */
+    double beta;                 /* recolouring parameter */
+    double A;                    /* surface-tension parameter */
+    double g_limit;              /* gradient threshold */
+    int high_order;              /* 0: 8-neighbour stencil, 1: 24-neighbour stencil */
+    double g; int external_force;/* force (S/GpuUpdateMacro.cu:118-119) */
+    double kappa;                /* Laplace sampling threshold (control_param) */
+    int nthreads;
+} orc_cg_params;
+
+typedef struct orc_cg orc_cg;
+orc_cg* orc_cg_create(const orc_cg_params* p);
+void orc_cg_destroy(orc_cg* s);
+/* rest-state colours from the given density fields (S/GpuInit.cu:323-352) */
+void orc_cg_init(orc_cg* s, const double* r_rho, const double* b_rho);
+void orc_cg_step(orc_cg* s, int nsteps);
+void orc_cg_get(const orc_cg* s, double* r_f, double* b_f, double* r_rho, double* b_rho,
+                double* rho, double* u, double* v);
+/* out[0]=sum p_in, out[1]=n_in, out[2]=sum p_out, out[3]=n_out, out[4]=max|u| */
+void orc_cg_laplace(const orc_cg* s, double* out);
+/* gradient field (cg_x, cg_y) for tests of the stencil */
+void orc_cg_gradient(const orc_cg* s, double* cgx, double* cgy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

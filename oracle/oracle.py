@@ -1,0 +1,237 @@
+"""ctypes binding of the FP64 CPU oracle.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker.  The product path never imports
+this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblbm_oracle.so")
+REF_MRT_PATH = os.path.join(HERE, "_ref", "libref_cellfunctions.so")
+
+SOLID, FLUID, VEL_W, PRES_E = 0, 1, 2, 3
+BGK, TRT, MRT = 0, 1, 2
+FORCE_NONE, FORCE_SHIFT, FORCE_GUO = 0, 1, 2
+WALL_HALFWAY, WALL_ONNODE = 0, 1
+INLET_ZOUHE, INLET_REF2D = 0, 1
+PARITY_MRT_RHO2, PARITY_MRT_NO_F18, PARITY_INLET_FROZEN, PARITY_OUTLET_V0 = 1, 2, 4, 8
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("q", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("periodic", C.c_int * 3), ("collision", C.c_int), ("omega", C.c_double),
+        ("trt_magic", C.c_double), ("mrt_s", C.c_double * 19), ("force_model", C.c_int),
+        ("g", C.c_double * 3), ("wall_model", C.c_int), ("inlet_model", C.c_int),
+        ("parity", C.c_uint), ("rho_out", C.c_double), ("nthreads", C.c_int),
+    ]
+
+
+class OrcCgParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("r_alpha", C.c_double), ("b_alpha", C.c_double),
+        ("r_nu", C.c_double), ("b_nu", C.c_double), ("beta", C.c_double), ("A", C.c_double),
+        ("g_limit", C.c_double), ("high_order", C.c_int), ("g", C.c_double),
+        ("external_force", C.c_int), ("kappa", C.c_double), ("nthreads", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        dp = P(C.c_double)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [P(OrcParams), P(C.c_uint8), P(C.c_uint32), dp, dp, dp]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_init.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, dp]
+        L.orc_step.argtypes = [C.c_void_p, C.c_int]
+        L.orc_get_f.argtypes = [C.c_void_p, dp]
+        L.orc_set_f.argtypes = [C.c_void_p, dp]
+        L.orc_get_macros.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.orc_mass.argtypes = [C.c_void_p]
+        L.orc_mass.restype = C.c_double
+        L.orc_collide_node.argtypes = [C.c_void_p, dp, C.c_double, dp, dp]
+        L.orc_equilibrium.argtypes = [C.c_int, C.c_double, dp, dp]
+        L.orc_lattice.argtypes = [C.c_int, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), dp]
+        L.orc_mrt_matrices.argtypes = [C.c_int, C.c_double, dp, dp, dp]
+        L.orc_cg_create.restype = C.c_void_p
+        L.orc_cg_create.argtypes = [P(OrcCgParams)]
+        L.orc_cg_destroy.argtypes = [C.c_void_p]
+        L.orc_cg_init.argtypes = [C.c_void_p, dp, dp]
+        L.orc_cg_step.argtypes = [C.c_void_p, C.c_int]
+        L.orc_cg_get.argtypes = [C.c_void_p] + [dp] * 7
+        L.orc_cg_laplace.argtypes = [C.c_void_p, dp]
+        L.orc_cg_gradient.argtypes = [C.c_void_p, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel()
+
+
+def lattice(q):
+    cx = (C.c_int * q)(); cy = (C.c_int * q)(); cz = (C.c_int * q)(); opp = (C.c_int * q)()
+    w = (C.c_double * q)()
+    lib().orc_lattice(q, cx, cy, cz, opp, w)
+    return (np.array(cx[:]), np.array(cy[:]), np.array(cz[:]), np.array(opp[:]), np.array(w[:]))
+
+
+def mrt_matrices(q, omega):
+    M = np.zeros(q * q); Mi = np.zeros(q * q); s = np.zeros(q)
+    lib().orc_mrt_matrices(q, omega, _dp(M), _dp(Mi), _dp(s))
+    return M.reshape(q, q), Mi.reshape(q, q), s
+
+
+def equilibrium(q, rho, uvw):
+    out = np.zeros(q)
+    u = np.array(list(uvw) + [0.0] * (3 - len(uvw)), dtype=np.float64)
+    lib().orc_equilibrium(q, rho, _dp(u), _dp(out))
+    return out
+
+
+class Sim:
+    """One oracle simulation; arrays are (Q, nz, ny, nx) / (nz, ny, nx)."""
+
+    def __init__(self, q, shape, node_class, bb=None, *, collision=BGK, omega=1.0, trt_magic=0.0,
+                 mrt_s=None, force_model=FORCE_NONE, g=(0.0, 0.0, 0.0), wall_model=WALL_HALFWAY,
+                 inlet_model=INLET_ZOUHE, parity=0, rho_out=1.0, periodic=(0, 0, 0), ubc=None,
+                 vbc=None, wbc=None, nthreads=0):
+        nz, ny, nx = (1,) + tuple(shape) if len(shape) == 2 else tuple(shape)
+        self.q, self.shape3 = q, (nz, ny, nx)
+        self.n = nx * ny * nz
+        p = OrcParams()
+        p.q, p.nx, p.ny, p.nz = q, nx, ny, nz
+        for i in range(3):
+            p.periodic[i] = int(periodic[i])
+            p.g[i] = float(g[i]) if i < len(g) else 0.0
+        p.collision, p.omega, p.trt_magic = collision, omega, trt_magic
+        if mrt_s is not None:
+            for i, v in enumerate(mrt_s):
+                p.mrt_s[i] = v
+        p.force_model, p.wall_model, p.inlet_model = force_model, wall_model, inlet_model
+        p.parity, p.rho_out, p.nthreads = parity, rho_out, nthreads
+        self.params = p
+        cls = np.ascontiguousarray(node_class, dtype=np.uint8).ravel()
+        assert cls.size == self.n
+        bbv = np.zeros(self.n, np.uint32) if bb is None else np.ascontiguousarray(bb, dtype=np.uint32).ravel()
+        self._keep = [_f64(ubc), _f64(vbc), _f64(wbc)]
+        self.h = lib().orc_create(C.byref(p), cls.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                  bbv.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                  *[_dp(a) for a in self._keep])
+        if not self.h:
+            raise ValueError("oracle: bad parameters")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def init(self, mode=0, rho=None, u=None, v=None, w=None):
+        arrs = [_f64(a) for a in (rho, u, v, w)]
+        lib().orc_init(self.h, mode, *[_dp(a) for a in arrs])
+
+    def step(self, n=1):
+        lib().orc_step(self.h, int(n))
+
+    def get_f(self):
+        f = np.zeros(self.q * self.n)
+        lib().orc_get_f(self.h, _dp(f))
+        return f.reshape((self.q,) + self.shape3)
+
+    def set_f(self, f):
+        f = _f64(f)
+        lib().orc_set_f(self.h, _dp(f))
+
+    def macros(self):
+        out = [np.zeros(self.n) for _ in range(4)]
+        lib().orc_get_macros(self.h, *[_dp(a) for a in out])
+        return [a.reshape(self.shape3) for a in out]
+
+    def mass(self):
+        return lib().orc_mass(self.h)
+
+    def collide_node(self, f, rho, uvw):
+        f = _f64(f); u = np.array(list(uvw) + [0.0] * (3 - len(uvw)))
+        out = np.zeros(self.q)
+        lib().orc_collide_node(self.h, _dp(f), rho, _dp(u), _dp(out))
+        return out
+
+
+class CgSim:
+    """Colour-gradient D2Q9 oracle on a periodic nx x ny box; arrays are (ny, nx)."""
+
+    def __init__(self, nx, ny, *, r_alpha=4 / 9, b_alpha=4 / 9, r_nu=1 / 6, b_nu=1 / 6, beta=0.7,
+                 A=0.01125, g_limit=0.0, high_order=0, g=0.0, external_force=0, kappa=0.9,
+                 nthreads=0):
+        p = OrcCgParams()
+        p.nx, p.ny, p.r_alpha, p.b_alpha, p.r_nu, p.b_nu = nx, ny, r_alpha, b_alpha, r_nu, b_nu
+        p.beta, p.A, p.g_limit, p.high_order, p.g = beta, A, g_limit, high_order, g
+        p.external_force, p.kappa, p.nthreads = external_force, kappa, nthreads
+        self.params, self.nx, self.ny, self.n = p, nx, ny, nx * ny
+        self.h = lib().orc_cg_create(C.byref(p))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_cg_destroy(self.h)
+            self.h = None
+
+    def init(self, r_rho, b_rho):
+        r = _f64(r_rho); b = _f64(b_rho)
+        lib().orc_cg_init(self.h, _dp(r), _dp(b))
+
+    def step(self, n=1):
+        lib().orc_cg_step(self.h, int(n))
+
+    def get(self):
+        n = self.n
+        arrs = [np.zeros(9 * n), np.zeros(9 * n)] + [np.zeros(n) for _ in range(5)]
+        lib().orc_cg_get(self.h, *[_dp(a) for a in arrs])
+        sh9, sh = (9, self.ny, self.nx), (self.ny, self.nx)
+        keys = ["r_f", "b_f", "r_rho", "b_rho", "rho", "u", "v"]
+        return {k: a.reshape(sh9 if i < 2 else sh) for i, (k, a) in enumerate(zip(keys, arrs))}
+
+    def laplace(self):
+        out = np.zeros(5)
+        lib().orc_cg_laplace(self.h, _dp(out))
+        return out
+
+    def gradient(self):
+        gx = np.zeros(self.n); gy = np.zeros(self.n)
+        lib().orc_cg_gradient(self.h, _dp(gx), _dp(gy))
+        return gx.reshape(self.ny, self.nx), gy.reshape(self.ny, self.nx)
+
+
+def ref_mrt_tables(q, omega):
+    """M and M^-1 S from the reference's own MRTInitializer2D/3D, when
+    oracle/_ref was built (S/CellFunctions.cu:7-154)."""
+    if not os.path.exists(REF_MRT_PATH):
+        return None
+    L = C.CDLL(REF_MRT_PATH)
+    fn = L["_Z16MRTInitializer2DPdS_d" if q == 9 else "_Z16MRTInitializer3DPdS_d"]
+    fn.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double]
+    M = np.zeros(q * q); MS = np.zeros(q * q)
+    fn(_dp(M), _dp(MS), omega)
+    return M.reshape(q, q), MS.reshape(q, q)

@@ -1,0 +1,63 @@
+"""The C-ABI library builds, loads without a GPU and exports every entry
+point include/lbm.h declares; host-only entry points work on the CPU."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from thesis_amd import lbm as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "lbm.h")).read()
+    return sorted(set(re.findall(r"\b(lbm_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_header_declarations_match_binding_list():
+    assert declared_symbols() == sorted(L.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.load()
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+    assert lib.lbm_abi_version() == L.ABI_VERSION
+    assert lib.lbm_device_count() >= 0
+
+
+def test_create_rejects_bad_parameters():
+    cls = np.ones((4, 4), np.uint8)
+    with pytest.raises(L.LbmError) as ei:
+        L.Solver(9, (4, 4), cls, omega=2.5)          # tau < 1/2
+    assert ei.value.status == 3
+    with pytest.raises(L.LbmError):
+        L.Solver(7, (4, 4), cls)                     # unknown lattice
+    bad = cls.copy()
+    bad[1, 1] = L.VEL_W                              # inlet off the west face
+    with pytest.raises(L.LbmError):
+        L.Solver(9, (4, 4), bad, ubc=np.zeros(16))
+
+
+def test_read_lattice_missing_file_is_eio(tmp_path):
+    with pytest.raises(L.LbmError) as ei:
+        L.read_lattice(str(tmp_path / "nope_Node.dat"), None)
+    assert ei.value.status == 2
+
+
+def test_read_lattice_2d_and_crlf(tmp_path):
+    node = tmp_path / "t_Node.dat"
+    bc = tmp_path / "t_BC.dat"
+    lines = []
+    for j in range(3):
+        for i in range(4):
+            lines.append(f"{i} {j} {i * 0.5:.6f} {j * 0.5:.6f} {0 if (i, j) == (2, 1) else 1}")
+    node.write_bytes(("\r\n".join(lines) + "\r\n").encode())
+    bc.write_text("0 0 4 1 0.0 -0.25 3\n0 0 3 2 -0.25 0.0 1\n3 2 2 1 1.5 1.25 3\n")
+    r = L.read_lattice(str(node), str(bc))
+    assert r["dims"] == (4, 3, 1)
+    assert r["node_class"].sum() == 11 and r["node_class"][0, 1, 2] == 0
+    assert r["nlinks"] == 3 and list(r["type_counts"][:3]) == [0, 2, 1]
+    assert r["bb_links"][0, 0, 0] == (1 << 4) and r["bb_links"][0, 2, 3] == (1 << 2)

@@ -1,0 +1,241 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 oracle.
+
+FP64 storage: max |gpu - oracle| / max |oracle| <= 1e-12 for rho, u and f,
+the bar the north star sets for FP64 mode.  Reduced storage is checked
+against its own stated error bounds.  Every case prints its name first so a
+failing GPU run names it."""
+import numpy as np
+import pytest
+
+from thesis_amd import lattices as LT
+from thesis_amd import lbm as L
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+
+
+def rel(a, b):
+    a = np.asarray(a, float)
+    b = np.asarray(b, float)
+    scale = max(np.abs(b).max(), 1e-300)
+    return float(np.abs(a - b).max() / scale)
+
+
+def pair(oracle, q, shape, cls, bb=None, storage=L.STORE_F64, **kw):
+    print(f"case q={q} shape={shape} storage={storage} {kw.get('collision')} {kw.get('force_model')}", flush=True)
+    g = L.Solver(q, shape, cls, bb, storage=storage, **kw)
+    o = oracle.Sim(q, shape, cls, bb, **kw)
+    return g, o
+
+
+# FP64 resolution of a moment of Q populations of magnitude <= 1/3: errors below
+# this absolute floor are rounding, whatever the size of the field.
+ABS_FLOOR = 1e-15
+
+
+def compare(g, o, steps, tol=TOL64, check_f=True):
+    """Density error relative to max rho; velocity error as the largest vector
+    difference relative to the largest speed; f relative to max |f|."""
+    g.step(steps)
+    o.step(steps)
+    sq = lambda a: np.asarray(a, float).reshape(-1)
+    gr, gu, gv, gw = (sq(a) for a in g.macros())
+    orr, ou, ov, ow = (sq(a) for a in o.macros())
+    dr = np.abs(gr - orr).max()
+    du = np.sqrt((gu - ou) ** 2 + (gv - ov) ** 2 + (gw - ow) ** 2).max()
+    speed = np.sqrt(ou ** 2 + ov ** 2 + ow ** 2).max()
+    errs = {"rho": dr / np.abs(orr).max(), "u": du / max(speed, 1e-300), "u_abs": du, "speed": speed}
+    print("errors", errs, flush=True)
+    assert dr <= max(tol * np.abs(orr).max(), ABS_FLOOR), errs
+    assert du <= max(tol * speed, ABS_FLOOR), errs
+    if check_f:
+        gf, of = sq(g.get_f()), sq(o.get_f())
+        df = np.abs(gf - of).max()
+        assert df <= max(tol * np.abs(of).max(), ABS_FLOOR), df
+    return errs
+
+
+def channel(nx=32, ny=16, obstacles=((10, 7, 3),)):
+    return LT.channel2d(nx, ny, 0.05, obstacles=obstacles)
+
+
+@pytest.mark.parametrize("coll", [L.BGK, L.TRT, L.MRT])
+def test_d2q9_channel_zouhe_halfway_f64(oracle, coll):
+    ch = channel()
+    g, o = pair(oracle, 9, (16, 32), ch["node_class"], ch["bb"], collision=coll, omega=1 / 0.8,
+                ubc=ch["ubc"], vbc=ch["vbc"], rho_out=1.0)
+    g.init(0)
+    o.init(0)
+    compare(g, o, 200)
+
+
+def test_d2q9_reference_parity_modes_f64(oracle):
+    """On-node bounce-back, the reference's simplified west inlet with corner
+    skip, v = 0 on outlet nodes, MRT with the rho^2 equilibrium and the f = 0
+    start (S/GpuBoundaries.cu:45-64, :400-443; S/GpuUpdateMacro.cu:30;
+    S/GpuCollision.cu:987-988; S/Iterate.cu:244-245)."""
+    ch = channel(obstacles=())
+    kw = dict(collision=L.MRT, omega=1 / 0.8, ubc=ch["ubc"], vbc=ch["vbc"], wall_model=L.WALL_ONNODE,
+              inlet_model=L.INLET_REF2D, parity=L.PARITY_MRT_RHO2 | L.PARITY_OUTLET_V0)
+    g, o = pair(oracle, 9, (16, 32), ch["node_class"], ch["bb"], **kw)
+    rho = np.ones((16, 32))
+    g.init(1, rho=rho, u=ch["ubc"])
+    o.init(1, rho=rho, u=ch["ubc"])
+    compare(g, o, 150)
+
+
+@pytest.mark.parametrize("coll", [L.BGK, L.TRT, L.MRT])
+@pytest.mark.parametrize("force", [L.FORCE_SHIFT, L.FORCE_GUO])
+def test_d3q19_porous_periodic_f64(oracle, coll, force):
+    cls, bb, _ = LT.porous3d(20, seed=11, rmin=2.5, rmax=4.0)
+    g, o = pair(oracle, 19, (20, 20, 20), cls, bb, collision=coll, omega=1 / 0.9, trt_magic=0.25,
+                force_model=force, g=(1e-4, 0, 0), periodic=(1, 1, 1))
+    g.init(0)
+    o.init(0)
+    compare(g, o, 60)
+
+
+def test_d3q19_mrt_reference_f18_quirk_f64(oracle):
+    cls, bb, _ = LT.porous3d(12, seed=3, rmin=2.0, rmax=3.0)
+    g, o = pair(oracle, 19, (12, 12, 12), cls, bb, collision=L.MRT, omega=1 / 0.7,
+                force_model=L.FORCE_SHIFT, g=(1e-4, 0, 0), periodic=(1, 1, 1), parity=L.PARITY_MRT_NO_F18)
+    g.init(0)
+    o.init(0)
+    compare(g, o, 30)
+
+
+def test_d3q19_onnode_walls_nonperiodic_f64(oracle):
+    """Channel in z with on-node walls given as WALL links on the outermost
+    planes, as latticegen2 writes them (L/MeshGen.cpp:47,67-73)."""
+    n = 10
+    solid = np.zeros((n, n, n), bool)
+    bb = LT.bb_from_solid(solid, 19, periodic=(True, True, False), outside_is_wall=(False, False, True))
+    cls = np.ones((n, n, n), np.uint8)
+    g, o = pair(oracle, 19, (n, n, n), cls, bb, collision=L.BGK, omega=1 / 0.75, wall_model=L.WALL_ONNODE,
+                force_model=L.FORCE_SHIFT, g=(1e-4, 0, 0), periodic=(1, 1, 0))
+    g.init(0)
+    o.init(0)
+    compare(g, o, 40)
+
+
+def test_set_f_then_step_matches_oracle(oracle):
+    ch = channel()
+    rng = np.random.default_rng(5)
+    g, o = pair(oracle, 9, (16, 32), ch["node_class"], ch["bb"], collision=L.TRT, omega=1 / 0.7,
+                ubc=ch["ubc"], vbc=ch["vbc"])
+    g.init(0)
+    o.init(0)
+    f0 = o.get_f() * (1 + 0.01 * rng.random((9, 1, 16, 32)))
+    f0[:, 0][:, ch["node_class"] == 0] = 0.0
+    g.set_f(f0[:, 0])
+    o.set_f(f0)
+    compare(g, o, 20)
+
+
+@pytest.mark.parametrize("coll", [L.BGK, L.TRT, L.MRT])
+def test_d3q19_f32_storage_close_to_f64(oracle, coll):
+    cls, bb, _ = LT.porous3d(20, seed=11, rmin=2.5, rmax=4.0)
+    g, o = pair(oracle, 19, (20, 20, 20), cls, bb, storage=L.STORE_F32, collision=coll, omega=1 / 0.9,
+                trt_magic=0.25, force_model=L.FORCE_GUO, g=(1e-4, 0, 0), periodic=(1, 1, 1))
+    g.init(0)
+    o.init(0)
+    g.step(200)
+    o.step(200)
+    gr, gu, gv, gw = g.macros()
+    orr, ou, ov, ow = o.macros()
+    fl = cls.astype(bool)
+    e = rel(gu[fl], ou[fl])
+    print(f"f32 collision {coll}: relative velocity error {e:.3e}", flush=True)
+    # FP32 arithmetic on deviations: ~1e-7 per step, a few 1e-5 after 200 steps
+    assert e < 5e-5
+    assert np.abs(gr[fl] - orr[fl]).max() < 1e-6
+
+
+@pytest.mark.parametrize("storage,tol", [(L.STORE_F16, 2e-3), (L.STORE_BF16, 2e-2)])
+def test_d3q19_16bit_storage_error_bounded(oracle, storage, tol):
+    cls, bb, _ = LT.porous3d(20, seed=11, rmin=2.5, rmax=4.0)
+    g, o = pair(oracle, 19, (20, 20, 20), cls, bb, storage=storage, collision=L.TRT, omega=1 / 0.9,
+                trt_magic=0.25, force_model=L.FORCE_GUO, g=(1e-4, 0, 0), periodic=(1, 1, 1))
+    g.init(0)
+    o.init(0)
+    g.step(200)
+    o.step(200)
+    gu = g.macros()[1]
+    ou = o.macros()[1]
+    fl = cls.astype(bool)
+    e = rel(gu[fl], ou[fl])
+    print(f"storage {storage}: relative velocity error {e:.3e}")
+    assert e < tol
+
+
+def test_deterministic_bitwise():
+    cls, bb, _ = LT.porous3d(24, seed=5, rmin=2.5, rmax=4.0)
+    outs = []
+    for _ in range(2):
+        g = L.Solver(19, (24, 24, 24), cls, bb, storage=L.STORE_F32, collision=L.TRT, omega=1 / 0.9,
+                     trt_magic=0.25, force_model=L.FORCE_GUO, g=(1e-5, 0, 0), periodic=(1, 1, 1))
+        g.init(0)
+        g.step(50)
+        outs.append(g.get_f())
+        g.close()
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_run_to_residual_and_divergence_flag(oracle):
+    # Zou-He channel to the 1e-6 residual of the batched-channel config
+    ch = channel(nx=48, ny=12, obstacles=())
+    g = L.Solver(9, (12, 48), ch["node_class"], ch["bb"], collision=L.TRT, omega=1 / 0.8, trt_magic=3 / 16,
+                 ubc=ch["ubc"], vbc=ch["vbc"])
+    g.init(0)
+    steps, res = g.run_to_residual(1e-6, every=100, max_steps=40000)
+    assert res < 1e-6 and steps < 40000
+    # body-force channel (periodic in x) converges to round-off
+    H = 12
+    bb = LT.bb_from_solid(np.zeros((H, 4), bool), 9, periodic=(True, False, True), outside_is_wall=(False, True, False))
+    p = L.Solver(9, (H, 4), np.ones((H, 4), np.uint8), bb, collision=L.TRT, omega=1 / 0.9, trt_magic=3 / 16,
+                 force_model=L.FORCE_GUO, g=(1e-5, 0, 0), periodic=(1, 0, 0))
+    p.init(0)
+    steps2, res2 = p.run_to_residual(1e-11, every=100, max_steps=50000)
+    assert res2 < 1e-11
+    nu = (0.9 - 0.5) / 3
+    y = np.arange(H) + 0.5
+    ua = 1e-5 / (2 * nu) * y * (H - y)
+    assert np.abs(p.macros()[1][:, 2] - ua).max() / ua.max() < 1e-9
+    # a non-finite population is reported as divergence (S/Iterate.cu:552-564)
+    f = g.get_f()
+    f[3, 5, 20] = np.nan
+    g.set_f(f)
+    with pytest.raises(L.LbmError) as ei:
+        g.run_to_residual(1e-8, every=10, max_steps=100)
+    assert ei.value.status == 1
+
+
+def test_c1_poiseuille_channel_256x64_f64(oracle):
+    """Config: D2Q9 BGK 256x64, FP64, tau 0.8, Zou-He parabolic inlet
+    u_max 0.05, Zou-He pressure outlet rho 1, half-way walls, rest start,
+    20 000 steps: FP64 parity with the oracle and the analytic parabola."""
+    ch = LT.channel2d(256, 64, 0.05)
+    g, o = pair(oracle, 9, (64, 256), ch["node_class"], ch["bb"], collision=L.BGK, omega=1 / 0.8,
+                ubc=ch["ubc"], vbc=ch["vbc"], rho_out=1.0)
+    g.init(0)
+    o.init(0)
+    compare(g, o, 20000, check_f=False)
+    u = g.macros()[1]
+    prof = ch["profile"]
+    # mid-channel section against the inlet parabola (BGK wall slip and
+    # compressibility keep this at the per-cent level)
+    assert rel(u[:, 128], prof) < 0.02
+
+
+@pytest.mark.slow
+def test_c2_full_size_256_trt_f64_two_steps(oracle):
+    """Full-size config lattice (256^3 porous, 30 % solid, seed 2024): FP64
+    TRT Lambda = 1/4 with Guo forcing, two steps, against the oracle."""
+    cls, bb, info = LT.porous3d(256, seed=2024)
+    assert info["solid_fraction"] >= 0.30
+    g, o = pair(oracle, 19, (256, 256, 256), cls, bb, collision=L.TRT, omega=1 / 0.9, trt_magic=0.25,
+                force_model=L.FORCE_GUO, g=(1e-6, 0, 0), periodic=(1, 1, 1))
+    g.init(0)
+    o.init(0)
+    compare(g, o, 2, check_f=False)

@@ -1,0 +1,130 @@
+// latticegen2 lattice files: <name>_Node.dat and <name>_BC.dat.
+//
+// Formats (whitespace separated):
+//   node, 2D: i j x y fluid                  (S/FilesReading.cu:290-291)
+//   node, 3D: i j k x y z fluid              (L/main.cpp:59-66)
+//   BC,   2D: i j dir type x y id            (S/FilesReading.cu:331-333)
+//   BC,   3D: i j k dir type x y z id        (L/FluidTracking.cpp:229-236)
+// The node-file column count decides 2D/3D (S/main.cu:109-140); n, m, h are
+// the largest index + 1 (S/FilesReading.cu:469-471).  BC types: 1 WALL,
+// 2 INLE, 3 OUTL, 4 CYCL, 5 SYMP (L/GeomRead.cpp:85-99).  Lines are split on
+// '\n' only, so CRLF files count correctly (unlike S/FilesReading.cu:27-31).
+#include "lbm.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+bool slurp(const char* path, std::string& out) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return false;
+    std::fseek(f, 0, SEEK_END);
+    long sz = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    out.resize(sz > 0 ? (size_t)sz : 0);
+    size_t got = sz > 0 ? std::fread(&out[0], 1, (size_t)sz, f) : 0;
+    std::fclose(f);
+    return got == out.size();
+}
+
+// Parses up to maxc numbers of one line; returns the count, advances p.
+int parse_line(const char*& p, const char* end, double* vals, int maxc) {
+    int k = 0;
+    while (p < end && *p != '\n') {
+        while (p < end && (*p == ' ' || *p == '\t' || *p == '\r')) ++p;
+        if (p >= end || *p == '\n') break;
+        char* q = nullptr;
+        double v = std::strtod(p, &q);
+        if (q == p) {   // not a number: skip the token
+            while (p < end && *p != ' ' && *p != '\t' && *p != '\n' && *p != '\r') ++p;
+            continue;
+        }
+        if (k < maxc) vals[k] = v;
+        ++k;
+        p = q;
+    }
+    if (p < end && *p == '\n') ++p;
+    return k;
+}
+
+}  // namespace
+
+extern "C" lbm_status lbm_read_lattice(const char* node_path, const char* bc_path, int32_t dims[3],
+                                       int64_t* nlinks, uint8_t* node_class, uint32_t* bb_links,
+                                       int64_t type_counts[6], int64_t dir_counts[19]) {
+    if (!node_path || !dims) return LBM_EINVAL;
+    std::string txt;
+    if (!slurp(node_path, txt)) return LBM_EIO;
+    const char* p = txt.data();
+    const char* end = p + txt.size();
+    double v[16];
+    int cols = 0;
+    int64_t mx[3] = {-1, -1, -1};
+    struct Rec { int64_t i, j, k; int fluid; };
+    std::vector<Rec> recs;
+    while (p < end) {
+        int c = parse_line(p, end, v, 16);
+        if (c == 0) continue;
+        if (cols == 0) cols = c;
+        if (c != cols || (cols != 5 && cols != 7)) return LBM_EIO;
+        Rec r;
+        r.i = (int64_t)v[0]; r.j = (int64_t)v[1]; r.k = cols == 7 ? (int64_t)v[2] : 0;
+        r.fluid = (int)v[cols - 1];
+        if (r.i < 0 || r.j < 0 || r.k < 0) return LBM_EIO;
+        if (r.i > mx[0]) mx[0] = r.i;
+        if (r.j > mx[1]) mx[1] = r.j;
+        if (r.k > mx[2]) mx[2] = r.k;
+        recs.push_back(r);
+    }
+    if (recs.empty()) return LBM_EIO;
+    dims[0] = (int32_t)(mx[0] + 1); dims[1] = (int32_t)(mx[1] + 1); dims[2] = (int32_t)(mx[2] + 1);
+    const int64_t nx = dims[0], ny = dims[1];
+    const int64_t n = nx * ny * dims[2];
+    const bool three_d = cols == 7;
+
+    std::string bct;
+    const bool have_bc = bc_path && bc_path[0];
+    if (have_bc && !slurp(bc_path, bct)) return LBM_EIO;
+
+    if (!node_class) {   // first pass: sizes only
+        if (nlinks) {
+            int64_t cnt = 0;
+            const char* b = bct.data();
+            const char* be = b + bct.size();
+            while (b < be) cnt += parse_line(b, be, v, 16) > 0;
+            *nlinks = have_bc ? cnt : 0;
+        }
+        return LBM_OK;
+    }
+    std::memset(node_class, 0, (size_t)n);
+    if (bb_links) std::memset(bb_links, 0, sizeof(uint32_t) * (size_t)n);
+    if (type_counts) std::memset(type_counts, 0, sizeof(int64_t) * 6);
+    if (dir_counts) std::memset(dir_counts, 0, sizeof(int64_t) * 19);
+    for (const Rec& r : recs) node_class[r.i + r.j * nx + r.k * nx * ny] = r.fluid ? LBM_FLUID : LBM_SOLID;
+    int64_t cnt = 0;
+    if (have_bc) {
+        const char* b = bct.data();
+        const char* be = b + bct.size();
+        const int want = three_d ? 9 : 7;
+        const int qmax = three_d ? 19 : 9;
+        while (b < be) {
+            int c = parse_line(b, be, v, 16);
+            if (c == 0) continue;
+            if (c != want) return LBM_EIO;
+            const int64_t i = (int64_t)v[0], j = (int64_t)v[1], k = three_d ? (int64_t)v[2] : 0;
+            const int dir = (int)v[three_d ? 3 : 2], type = (int)v[three_d ? 4 : 3];
+            if (i < 0 || i >= nx || j < 0 || j >= ny || k < 0 || k >= dims[2]) return LBM_EIO;
+            if (dir < 1 || dir >= qmax || type < 0 || type > 5) return LBM_EIO;
+            if (type_counts) type_counts[type]++;
+            if (dir_counts) dir_counts[dir]++;
+            if (type == 1 && bb_links) bb_links[i + j * nx + k * nx * ny] |= 1u << dir;
+            ++cnt;
+        }
+    }
+    if (nlinks) *nlinks = cnt;
+    return LBM_OK;
+}

@@ -1,0 +1,384 @@
+// C ABI of the lattice-Boltzmann step (declared in include/lbm.h).
+//
+// Replaces the allocation, initialisation and time loop of the reference
+// driver (S/Iterate.cu:23-625, S/Iterate3D.cu:30-839): one context owns one
+// HIP stream and all device buffers; the loop enqueues one fused kernel per
+// step and synchronises the host once per call, not once per phase.
+#include "lbm.h"
+#include "lbm_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace lbm;
+
+struct lbm_ctx {
+    lbm_params p{};
+    KParams kp{};
+    int q = 0;
+    size_t n = 0;
+    size_t raw_bytes = 0;     // bytes per stored population
+    bool open = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void* P[2] = {nullptr, nullptr};
+    int cur = 0;              // buffer holding the newest post-collision state
+    int64_t steps = -1;       // steps since the state was set; -1: no state
+    uint32_t* node = nullptr;
+    double *rho = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
+    double *up = nullptr, *vp = nullptr, *wp = nullptr;
+    double *ubc = nullptr, *vbc = nullptr, *wbc = nullptr;
+    double* fout = nullptr;
+    double* partials = nullptr;
+    double* out4 = nullptr;
+    int64_t bytes = 0;
+    std::string err;
+};
+
+namespace {
+
+lbm_status fail(lbm_ctx* c, lbm_status s, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return s;
+}
+
+#define HIPCHK(c, expr)                                                                     \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail((c), LBM_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_));          \
+    } while (0)
+
+lbm_status dalloc(lbm_ctx* c, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(c, LBM_ENOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    }
+    c->bytes += (int64_t)bytes;
+    return LBM_OK;
+}
+
+Fields fields(lbm_ctx* c) {
+    Fields f;
+    f.rho = c->rho; f.u = c->u; f.v = c->v; f.w = c->w;
+    f.ubc = c->ubc; f.vbc = c->vbc; f.wbc = c->wbc;
+    f.fout = c->fout;
+    return f;
+}
+
+size_t storage_bytes(int s) { return s == LBM_STORE_F64 ? 8 : s == LBM_STORE_F32 ? 4 : 2; }
+
+// Population plane stride: rounded to 256 B, plus an offset of one 4 KiB page
+// and 256 B so that the Q planes of one node start on different HBM channels.
+size_t plane_stride(size_t n, size_t eb) {
+    size_t pad_bytes = 4096 + 256;
+    if (const char* e = std::getenv("LBM_QPAD_BYTES")) pad_bytes = (size_t)std::strtoull(e, nullptr, 10);
+    size_t per = 256 / eb;
+    size_t s = (n + per - 1) / per * per;
+    return s + (pad_bytes + eb - 1) / eb;
+}
+
+lbm_status ensure_fout(lbm_ctx* c) {
+    if (c->fout) return LBM_OK;
+    return dalloc(c, (void**)&c->fout, sizeof(double) * c->n * c->q);
+}
+
+lbm_status run_steps(lbm_ctx* c, int64_t nsteps) {
+    for (int64_t k = 0; k < nsteps; ++k) {
+        const bool wm = (k == nsteps - 1);
+        hipError_t e = launch_step(c->q, c->p.storage, c->p.collision, c->open, c->kp, c->P[c->cur],
+                                   c->P[c->cur ^ 1], c->node, fields(c), wm, c->stream);
+        if (e != hipSuccess) return fail(c, LBM_EDEVICE, "step launch: %s", hipGetErrorString(e));
+        c->cur ^= 1;
+    }
+    c->steps += nsteps;
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_abi_version(void) { return LBM_ABI_VERSION; }
+
+int lbm_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+const char* lbm_last_error(const lbm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int64_t lbm_device_bytes(const lbm_ctx* ctx) { return ctx ? ctx->bytes : 0; }
+
+static thread_local std::string g_create_err;
+
+lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out) {
+    if (!out) return LBM_EINVAL;
+    *out = nullptr;
+    auto reject = [](const char* msg) { g_create_err = msg; std::fprintf(stderr, "lbm_create: %s\n", msg); return LBM_EINVAL; };
+    if (!p || !lat || !lat->node_class) return reject("null parameters or lattice");
+    if (p->abi_version != LBM_ABI_VERSION) return reject("ABI version mismatch");
+    if (p->q != 9 && p->q != 19) return reject("q must be 9 or 19");
+    const int nz = p->q == 9 ? 1 : p->nz;
+    if (p->nx < 1 || p->ny < 1 || nz < 1) return reject("empty lattice");
+    if (p->q == 9 && p->nz > 1) return reject("D2Q9 needs nz == 1");
+    if (!(p->omega > 0.0 && p->omega < 2.0)) return reject("omega must lie in (0, 2) (tau > 1/2)");
+    if (p->collision < 0 || p->collision > 2) return reject("bad collision model");
+    if (p->storage < 0 || p->storage > 3) return reject("bad storage");
+    if (p->force_model < 0 || p->force_model > 2) return reject("bad force model");
+    if (p->wall_model < 0 || p->wall_model > 1) return reject("bad wall model");
+    if (p->inlet_model < 0 || p->inlet_model > 1) return reject("bad inlet model");
+    const size_t n = (size_t)p->nx * p->ny * nz;
+    if (n > (size_t)1 << 33) return reject("lattice too large");
+
+    // node words and open-boundary checks
+    std::vector<uint32_t> node(n);
+    bool any_open_cls = false;
+    const uint32_t mask = p->q == 9 ? 0x1FEu : 0x7FFFEu;
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t cls = lat->node_class[i];
+        if (cls > LBM_PRES_E) return reject("unknown node class");
+        if (cls == LBM_VEL_W || cls == LBM_PRES_E) {
+            if (p->q != 9) return reject("velocity/pressure faces are D2Q9 only");
+            const int x = (int)(i % p->nx);
+            if (cls == LBM_VEL_W && x != 0) return reject("velocity inlet nodes must lie on x = 0");
+            if (cls == LBM_PRES_E && x != p->nx - 1) return reject("pressure outlet nodes must lie on x = nx-1");
+            if (p->periodic[0]) return reject("open x faces need a non-periodic x axis");
+            any_open_cls = true;
+        }
+        const uint32_t bb = lat->bb_links ? (lat->bb_links[i] & mask) : 0u;
+        node[i] = (cls << kClsShift) | bb;
+    }
+    if (any_open_cls && !lat->u_bc) {
+        bool need = false;
+        for (size_t i = 0; i < n; ++i) need |= lat->node_class[i] == LBM_VEL_W;
+        if (need) return reject("velocity inlet nodes need u_bc");
+    }
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return reject("no HIP device");
+
+    lbm_ctx* c = new lbm_ctx();
+    c->p = *p;
+    c->p.nz = nz;
+    c->q = p->q;
+    c->n = n;
+    c->raw_bytes = storage_bytes(p->storage);
+    c->open = !(p->periodic[0] && p->periodic[1] && (p->q == 9 || p->periodic[2]));
+    KParams& k = c->kp;
+    k.nx = p->nx; k.ny = p->ny; k.nz = nz;
+    for (int a = 0; a < 3; ++a) k.per[a] = p->periodic[a] ? 1 : 0;
+    if (p->q == 9) k.per[2] = 1;
+    k.qstride = plane_stride(n, c->raw_bytes);
+    k.omega = p->omega;
+    const double lam = p->trt_magic > 0 ? p->trt_magic : 3.0 / 16.0;
+    k.omega_m = 1.0 / (0.5 + lam / (1.0 / p->omega - 0.5));
+    // reference diag(S) (S/CellFunctions.cu:44-52, :122-140) unless given
+    bool any_s = false;
+    for (int i = 0; i < p->q; ++i) any_s |= p->mrt_s[i] != 0.0;
+    if (any_s) {
+        for (int i = 0; i < p->q; ++i) k.s[i] = p->mrt_s[i];
+    } else if (p->q == 9) {
+        const double s9[9] = {1.0, 1.4, 1.4, 1.0, 1.2, 1.0, 1.2, p->omega, p->omega};
+        for (int i = 0; i < 9; ++i) k.s[i] = s9[i];
+    } else {
+        const double s19[19] = {0.0, 1.19, 1.4, 0.0, 1.2, 0.0, 1.2, 0.0, 1.2, p->omega,
+                                1.4, p->omega, 1.4, p->omega, p->omega, p->omega, 1.98, 1.98, 1.98};
+        for (int i = 0; i < 19; ++i) k.s[i] = s19[i];
+    }
+    for (int a = 0; a < 3; ++a) k.g[a] = p->force[a];
+    k.force = p->force_model;
+    k.wall = p->wall_model;
+    k.inlet = p->inlet_model;
+    k.parity = p->parity_flags;
+    k.rho_out = p->rho_out > 0 ? p->rho_out : 1.0;
+
+    lbm_status st = LBM_OK;
+    auto cleanup = [&](lbm_status s) { lbm_destroy(c); return s; };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        g_create_err = "stream/event creation failed";
+        return cleanup(LBM_EDEVICE);
+    }
+    const size_t pbytes = c->raw_bytes * k.qstride * c->q;
+    for (int b = 0; b < 2 && st == LBM_OK; ++b) st = dalloc(c, &c->P[b], pbytes);
+    if (st == LBM_OK) st = dalloc(c, (void**)&c->node, sizeof(uint32_t) * n);
+    double** fl[] = {&c->rho, &c->u, &c->v, &c->w, &c->up, &c->vp, &c->wp};
+    for (double** a : fl)
+        if (st == LBM_OK) st = dalloc(c, (void**)a, sizeof(double) * n);
+    if (st == LBM_OK && any_open_cls) {
+        st = dalloc(c, (void**)&c->ubc, sizeof(double) * n);
+        if (st == LBM_OK) st = dalloc(c, (void**)&c->vbc, sizeof(double) * n);
+        if (st == LBM_OK) st = dalloc(c, (void**)&c->wbc, sizeof(double) * n);
+    }
+    if (st == LBM_OK) st = dalloc(c, (void**)&c->partials, sizeof(double) * 4 * reduce_blocks());
+    if (st == LBM_OK) st = dalloc(c, (void**)&c->out4, sizeof(double) * 4);
+    if (st != LBM_OK) {
+        g_create_err = c->err;
+        return cleanup(st);
+    }
+    auto h2d = [&](void* d, const void* h, size_t b) { return hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, c->stream); };
+    if (h2d(c->node, node.data(), sizeof(uint32_t) * n) != hipSuccess) return cleanup(LBM_EDEVICE);
+    if (c->ubc) {
+        std::vector<double> zero(n, 0.0);
+        if (h2d(c->ubc, lat->u_bc ? lat->u_bc : zero.data(), 8 * n) != hipSuccess ||
+            h2d(c->vbc, lat->v_bc ? lat->v_bc : zero.data(), 8 * n) != hipSuccess ||
+            h2d(c->wbc, lat->w_bc ? lat->w_bc : zero.data(), 8 * n) != hipSuccess)
+            return cleanup(LBM_EDEVICE);
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(LBM_EDEVICE);
+    }
+    double* zs[] = {c->up, c->vp, c->wp};
+    for (double* a : zs)
+        if (hipMemsetAsync(a, 0, 8 * n, c->stream) != hipSuccess) return cleanup(LBM_EDEVICE);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(LBM_EDEVICE);
+    *out = c;
+    return LBM_OK;
+}
+
+void lbm_destroy(lbm_ctx* c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    void* bufs[] = {c->P[0], c->P[1], c->node, c->rho, c->u, c->v, c->w, c->up, c->vp, c->wp,
+                    c->ubc, c->vbc, c->wbc, c->fout, c->partials, c->out4};
+    for (void* b : bufs)
+        if (b) hipFree(b);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+lbm_status lbm_init(lbm_ctx* c, int mode, const double* rho, const double* u, const double* v, const double* w) {
+    if (!c) return LBM_EINVAL;
+    if (mode != 0 && mode != 1) return fail(c, LBM_EINVAL, "init mode must be 0 (equilibrium) or 1 (zero f)");
+    const size_t n = c->n, b = 8 * n;
+    std::vector<double> ones;
+    if (!rho) ones.assign(n, 1.0);
+    HIPCHK(c, hipMemcpyAsync(c->rho, rho ? rho : ones.data(), b, hipMemcpyHostToDevice, c->stream));
+    const double* src[3] = {u, v, w};
+    double* dst[3] = {c->u, c->v, c->w};
+    for (int a = 0; a < 3; ++a) {
+        if (src[a]) HIPCHK(c, hipMemcpyAsync(dst[a], src[a], b, hipMemcpyHostToDevice, c->stream));
+        else HIPCHK(c, hipMemsetAsync(dst[a], 0, b, c->stream));
+    }
+    HIPCHK(c, launch_init(c->q, c->p.storage, c->p.collision, mode == 0 ? INIT_EQ : INIT_ZERO, c->kp,
+                          c->P[0], c->node, fields(c), c->stream));
+    c->cur = 0;
+    c->steps = 0;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+lbm_status lbm_step(lbm_ctx* c, int64_t nsteps) {
+    if (!c) return LBM_EINVAL;
+    if (c->steps < 0) return fail(c, LBM_EINVAL, "state not initialised (call lbm_init or lbm_set_f)");
+    if (nsteps < 0) return fail(c, LBM_EINVAL, "negative step count");
+    lbm_status s = run_steps(c, nsteps);
+    if (s != LBM_OK) return s;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+lbm_status lbm_reduce(lbm_ctx* c, double out[4]) {
+    if (!c || !out) return LBM_EINVAL;
+    if (c->steps < 0) return fail(c, LBM_EINVAL, "state not initialised");
+    HIPCHK(c, launch_reduce(c->kp, c->node, fields(c), c->up, c->vp, c->wp, c->partials, reduce_blocks(),
+                            c->out4, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out, c->out4, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+lbm_status lbm_run_to_residual(lbm_ctx* c, double tol, int64_t every, int64_t max_steps, int64_t* steps_done,
+                               double* final_res) {
+    if (!c) return LBM_EINVAL;
+    if (every < 1 || max_steps < 0) return fail(c, LBM_EINVAL, "bad check interval or step limit");
+    double r4[4];
+    lbm_status s = lbm_reduce(c, r4);   // seeds u_prev with the current field
+    if (s != LBM_OK) return s;
+    int64_t done = 0;
+    double res = INFINITY;
+    while (done < max_steps) {
+        const int64_t k = (max_steps - done) < every ? (max_steps - done) : every;
+        if ((s = run_steps(c, k)) != LBM_OK) return s;
+        done += k;
+        if ((s = lbm_reduce(c, r4)) != LBM_OK) return s;
+        if (r4[3] > 0) {
+            if (steps_done) *steps_done = done;
+            if (final_res) *final_res = NAN;
+            return fail(c, LBM_DIVERGED, "non-finite field after %lld steps", (long long)done);
+        }
+        res = r4[1] > 0 ? std::sqrt(r4[0] / r4[1]) : std::sqrt(r4[0]);
+        if (res < tol) break;
+    }
+    if (steps_done) *steps_done = done;
+    if (final_res) *final_res = res;
+    return LBM_OK;
+}
+
+lbm_status lbm_get_macros(lbm_ctx* c, double* rho, double* u, double* v, double* w) {
+    if (!c) return LBM_EINVAL;
+    if (c->steps < 0) return fail(c, LBM_EINVAL, "state not initialised");
+    const size_t b = 8 * c->n;
+    const double* srcs[4] = {c->rho, c->u, c->v, c->w};
+    double* dsts[4] = {rho, u, v, w};
+    for (int a = 0; a < 4; ++a)
+        if (dsts[a]) HIPCHK(c, hipMemcpyAsync(dsts[a], srcs[a], b, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+lbm_status lbm_get_f(lbm_ctx* c, double* f) {
+    if (!c || !f) return LBM_EINVAL;
+    if (c->steps < 1) return fail(c, LBM_EINVAL, "post-boundary populations exist after the first step");
+    lbm_status s = ensure_fout(c);
+    if (s != LBM_OK) return s;
+    HIPCHK(c, launch_recon(c->q, c->p.storage, c->open, c->kp, c->P[c->cur ^ 1], c->node, fields(c), c->stream));
+    HIPCHK(c, hipMemcpyAsync(f, c->fout, 8 * c->n * c->q, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+lbm_status lbm_set_f(lbm_ctx* c, const double* f) {
+    if (!c || !f) return LBM_EINVAL;
+    lbm_status s = ensure_fout(c);
+    if (s != LBM_OK) return s;
+    HIPCHK(c, hipMemcpyAsync(c->fout, f, 8 * c->n * c->q, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_init(c->q, c->p.storage, c->p.collision, INIT_FROM_F, c->kp, c->P[0], c->node, fields(c),
+                          c->stream));
+    c->cur = 0;
+    c->steps = 0;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+lbm_status lbm_time_steps(lbm_ctx* c, int64_t nsteps, double* ms) {
+    if (!c || !ms || nsteps < 1) return LBM_EINVAL;
+    if (c->steps < 0) return fail(c, LBM_EINVAL, "state not initialised");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    lbm_status s = run_steps(c, nsteps);
+    if (s != LBM_OK) return s;
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float t = 0;
+    HIPCHK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
+    *ms = t;
+    return LBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,51 @@
+// Internal interface between the C ABI (lbm_api.cpp) and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace lbm {
+
+enum Cls : uint32_t { SOLID = 0, FLUID = 1, VEL_W = 2, PRES_E = 3 };
+enum Mode : int { INIT_EQ = 0, INIT_ZERO = 1, INIT_FROM_F = 2 };
+
+// Per-node word: bits 0..18 bounce-back links, bits 24..31 node class.
+constexpr uint32_t kBBMask = 0x7FFFFu;
+constexpr int kClsShift = 24;
+
+// Everything a kernel needs, passed by value as a kernel argument.
+struct KParams {
+    int nx, ny, nz;
+    int per[3];
+    size_t qstride;          // elements between population planes
+    double omega, omega_m;   // BGK/TRT rates
+    double s[19];            // MRT diag(S)
+    double g[3];
+    int force;               // 0 none, 1 shift, 2 guo
+    int wall;                // 0 halfway, 1 on-node
+    int inlet;               // 0 zou-he, 1 reference 2D
+    uint32_t parity;
+    double rho_out;
+};
+
+struct Fields {              // FP64 side arrays (N each)
+    double *rho, *u, *v, *w; // macroscopic output / input
+    const double *ubc, *vbc, *wbc;
+    double* fout;            // Q*N full populations (reconstruct) or input (INIT_FROM_F)
+};
+
+// Launchers; q in {9,19}; storage 0..3; collision 0..2.
+hipError_t launch_step(int q, int storage, int collision, bool open, const KParams& kp,
+                       const void* src, void* dst, const uint32_t* node, Fields fl,
+                       bool write_macros, hipStream_t st);
+hipError_t launch_recon(int q, int storage, bool open, const KParams& kp, const void* src,
+                        const uint32_t* node, Fields fl, hipStream_t st);
+hipError_t launch_init(int q, int storage, int collision, int mode, const KParams& kp, void* dst,
+                       const uint32_t* node, Fields fl, hipStream_t st);
+// FP64 reductions over non-solid nodes: out4 (device) = {sum du^2, sum u^2, sum rho, nonfinite}.
+hipError_t launch_reduce(const KParams& kp, const uint32_t* node, Fields fl, double* uprev,
+                         double* vprev, double* wprev, double* partials, int nblocks, double* out4,
+                         hipStream_t st);
+int reduce_blocks();
+
+}  // namespace lbm

@@ -1,0 +1,642 @@
+// Fused lattice-Boltzmann step kernels for CDNA4 (gfx950).
+//
+// One thread per lattice node, x fastest; a workgroup covers a run of x at a
+// fixed (y, z), so the (y, z) neighbour rows of the pull are wave-uniform.
+// Populations live in two SoA buffers P[q*qstride + node] (A/B).  A step
+// reads the post-collision populations of the previous step from A, pulls
+// them along the links (stream), applies walls and open boundaries, takes the
+// moments, collides and writes B: the reference's collide -> stream -> BC ->
+// macro sequence (S/Iterate.cu:385-520, S/Iterate3D.cu:493-653) rotated so
+// that one node's whole update stays in registers.
+//
+// Populations are held as deviations h_i = f_i - w_i (rest density 1) both
+// in storage and in the arithmetic, so FP32 and 16-bit storage keep the
+// small flow signal instead of rounding it against w_i.
+#include "lbm_internal.h"
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace lbm {
+
+// ---------------------------------------------------------------------------
+// lattices: reference numbering (S/GpuInit.cu:94-99, :822-842)
+struct D2Q9 {
+    static constexpr int Q = 9;
+    __host__ __device__ static constexpr int cx(int i) { return i == 1 || i == 5 || i == 8 ? 1 : (i == 3 || i == 6 || i == 7 ? -1 : 0); }
+    __host__ __device__ static constexpr int cy(int i) { return i == 2 || i == 5 || i == 6 ? 1 : (i == 4 || i == 7 || i == 8 ? -1 : 0); }
+    __host__ __device__ static constexpr int cz(int) { return 0; }
+    __host__ __device__ static constexpr int opp(int i) { return i == 0 ? 0 : (i <= 4 ? (i <= 2 ? i + 2 : i - 2) : (i <= 6 ? i + 2 : i - 2)); }
+    __host__ __device__ static constexpr double w(int i) { return i == 0 ? 4.0 / 9.0 : (i <= 4 ? 1.0 / 9.0 : 1.0 / 36.0); }
+};
+
+struct D3Q19 {
+    static constexpr int Q = 19;
+    __host__ __device__ static constexpr int cx(int i) {
+        return (i == 1 || i == 7 || i == 9 || i == 11 || i == 13) ? 1 : ((i == 2 || i == 8 || i == 10 || i == 12 || i == 14) ? -1 : 0);
+    }
+    __host__ __device__ static constexpr int cy(int i) {
+        return (i == 3 || i == 7 || i == 8 || i == 15 || i == 17) ? 1 : ((i == 4 || i == 9 || i == 10 || i == 16 || i == 18) ? -1 : 0);
+    }
+    __host__ __device__ static constexpr int cz(int i) {
+        return (i == 5 || i == 11 || i == 12 || i == 15 || i == 16) ? 1 : ((i == 6 || i == 13 || i == 14 || i == 17 || i == 18) ? -1 : 0);
+    }
+    // pairs (1,2)(3,4)(5,6)(7,10)(8,9)(11,14)(12,13)(15,18)(16,17)
+    __host__ __device__ static constexpr int opp(int i) {
+        return i == 0 ? 0 : i <= 6 ? ((i & 1) ? i + 1 : i - 1)
+             : i == 7 ? 10 : i == 10 ? 7 : i == 8 ? 9 : i == 9 ? 8
+             : i == 11 ? 14 : i == 14 ? 11 : i == 12 ? 13 : i == 13 ? 12
+             : i == 15 ? 18 : i == 18 ? 15 : i == 16 ? 17 : 16;
+    }
+    __host__ __device__ static constexpr double w(int i) { return i == 0 ? 1.0 / 3.0 : (i <= 6 ? 1.0 / 18.0 : 1.0 / 36.0); }
+};
+
+static_assert(D2Q9::opp(1) == 3 && D2Q9::opp(5) == 7 && D2Q9::opp(8) == 6, "D2Q9 opp");
+static_assert(D3Q19::opp(7) == 10 && D3Q19::opp(16) == 17 && D3Q19::opp(2) == 1, "D3Q19 opp");
+
+// MRT moment polynomials (the rows of M tabulated in S/CellFunctions.cu:15-36
+// for D2Q9 and :73-114 for D3Q19).  Rows are orthogonal: Minv = M^T / |row|^2.
+template <class L>
+__host__ __device__ constexpr double mrow(int r, int i) {
+    const double x = L::cx(i), y = L::cy(i), z = L::cz(i), c2 = x * x + y * y + z * z;
+    if (L::Q == 9) {
+        return r == 0 ? 1 : r == 1 ? 3 * c2 - 4 : r == 2 ? (9 * c2 * c2 - 21 * c2 + 8) / 2
+             : r == 3 ? x : r == 4 ? (3 * c2 - 5) * x : r == 5 ? y : r == 6 ? (3 * c2 - 5) * y
+             : r == 7 ? x * x - y * y : x * y;
+    }
+    return r == 0 ? 1 : r == 1 ? 19 * c2 - 30 : r == 2 ? (21 * c2 * c2 - 53 * c2 + 24) / 2
+         : r == 3 ? x : r == 4 ? (5 * c2 - 9) * x : r == 5 ? y : r == 6 ? (5 * c2 - 9) * y
+         : r == 7 ? z : r == 8 ? (5 * c2 - 9) * z : r == 9 ? 3 * x * x - c2
+         : r == 10 ? (3 * c2 - 5) * (3 * x * x - c2) : r == 11 ? y * y - z * z
+         : r == 12 ? (3 * c2 - 5) * (y * y - z * z) : r == 13 ? x * y : r == 14 ? y * z
+         : r == 15 ? x * z : r == 16 ? (y * y - z * z) * x : r == 17 ? (z * z - x * x) * y
+         : (x * x - y * y) * z;
+}
+template <class L>
+__host__ __device__ constexpr double mnorm2(int r) {
+    double s = 0;
+    for (int i = 0; i < L::Q; ++i) s += mrow<L>(r, i) * mrow<L>(r, i);
+    return s;
+}
+template <class L>
+__host__ __device__ constexpr double mrest(int r) {   // (M w)_r: moments of the rest state
+    double s = 0;
+    for (int i = 0; i < L::Q; ++i) s += mrow<L>(r, i) * L::w(i);
+    return s;
+}
+static_assert(mrest<D2Q9>(1) > -2.0000001 && mrest<D2Q9>(1) < -1.9999999, "D2Q9 rest e");
+static_assert(mrest<D2Q9>(2) > 0.9999999 && mrest<D2Q9>(2) < 1.0000001, "D2Q9 rest eps");
+static_assert(mrest<D3Q19>(1) > -11.0000001 && mrest<D3Q19>(1) < -10.9999999, "D3Q19 rest e");
+static_assert(mrest<D3Q19>(2) > 2.9999999 && mrest<D3Q19>(2) < 3.0000001, "D3Q19 rest eps");
+
+// ---------------------------------------------------------------------------
+// storage: deviations h_i, raw in F64/F32; scaled by 1/(w_i a) in 16 bits
+// FP16: stored value = h / w_i * 2^10 keeps typical deviations in the normal range.
+template <class L> __device__ inline float f16_a(int i) { return (float)(L::w(i) / 1024.0); }
+template <class L> __device__ inline float f16_ainv(int i) { return (float)(1024.0 / L::w(i)); }
+template <class L> __device__ inline float bf16_a(int i) { return (float)L::w(i); }
+template <class L> __device__ inline float bf16_ainv(int i) { return (float)(1.0 / L::w(i)); }
+
+__device__ inline float bf16_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ inline uint16_t f32_to_bf16(float f) {   // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+template <class L, int S> struct Io;
+template <class L> struct Io<L, 0> {
+    using raw = double; using T = double;
+    __device__ static T ld(const raw* p, size_t k, int) { return p[k]; }
+    __device__ static void st(raw* p, size_t k, T v, int) { __builtin_nontemporal_store(v, p + k); }
+};
+template <class L> struct Io<L, 1> {
+    using raw = float; using T = float;
+    __device__ static T ld(const raw* p, size_t k, int) { return p[k]; }
+    __device__ static void st(raw* p, size_t k, T v, int) { __builtin_nontemporal_store(v, p + k); }
+};
+template <class L> struct Io<L, 2> {
+    using raw = uint16_t; using T = float;
+    __device__ static T ld(const raw* p, size_t k, int i) {
+        _Float16 h = __builtin_bit_cast(_Float16, p[k]);
+        return (float)h * f16_a<L>(i);
+    }
+    __device__ static void st(raw* p, size_t k, T v, int i) {
+        _Float16 h = (_Float16)(v * f16_ainv<L>(i));
+        __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, h), p + k);
+    }
+};
+template <class L> struct Io<L, 3> {
+    using raw = uint16_t; using T = float;
+    __device__ static T ld(const raw* p, size_t k, int i) { return bf16_to_f32(p[k]) * bf16_a<L>(i); }
+    __device__ static void st(raw* p, size_t k, T v, int i) {
+        __builtin_nontemporal_store(f32_to_bf16(v * bf16_ainv<L>(i)), p + k);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// node pipeline pieces (T = double for F64 storage, float otherwise)
+
+// Pull stream with walls (S/GpuStream.cu:8-28, :160-194; half-way or on-node
+// bounce-back per S/GpuBoundaries.cu:400-490) and periodic wrap.
+template <class L, int S, bool OPEN>
+__device__ inline void gather(const typename Io<L, S>::raw* __restrict__ src, const KParams& kp,
+                              int x, int y, int z, size_t idx, uint32_t nd,
+                              typename Io<L, S>::T* h) {
+    const int nx = kp.nx, ny = kp.ny, nz = kp.nz;
+    const size_t qs = kp.qstride;
+    const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+    const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
+    const int zm = z == 0 ? nz - 1 : z - 1, zp = z == nz - 1 ? 0 : z + 1;
+    bool xm_o = false, xp_o = false, ym_o = false, yp_o = false, zm_o = false, zp_o = false;
+    if (OPEN) {
+        xm_o = !kp.per[0] && x == 0;  xp_o = !kp.per[0] && x == nx - 1;
+        ym_o = !kp.per[1] && y == 0;  yp_o = !kp.per[1] && y == ny - 1;
+        zm_o = !kp.per[2] && z == 0;  zp_o = !kp.per[2] && z == nz - 1;
+    }
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) {
+        const int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
+        const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
+        const int sy = cy == 0 ? y : (cy > 0 ? ym : yp);
+        const int sz = cz == 0 ? z : (cz > 0 ? zm : zp);
+        size_t a = (size_t)i * qs + ((size_t)sz * ny + sy) * nx + sx;
+        if (OPEN) {
+            const bool out = (cx > 0 && xm_o) || (cx < 0 && xp_o) || (cy > 0 && ym_o) ||
+                             (cy < 0 && yp_o) || (cz > 0 && zm_o) || (cz < 0 && zp_o);
+            if (out) a = (size_t)i * qs + idx;   // not streamed: keeps its own value
+        }
+        if (i != 0) {
+            const int o = L::opp(i);
+            if ((nd >> o) & 1u) a = (size_t)(kp.wall == 0 ? o : i) * qs + idx;
+        }
+        h[i] = Io<L, S>::ld(src, a, i);
+    }
+}
+
+// Open boundaries on D2Q9 faces, in full populations.
+// Velocity inlet (west): Zou-He, or the reference's form (S/GpuBoundaries.cu:45-64).
+// Pressure outlet (east): T/LBM/Tom/Lecture Script/LBMchannel.m:254-270.
+template <class L, class T>
+__device__ inline void zou_inlet(const KParams& kp, const Fields& fl, size_t idx, uint32_t nd, T* h) {
+    if (L::Q != 9) return;
+    if (kp.inlet == 1 && (nd & kBBMask)) return;       // reference skips corner nodes
+    T f[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f[i] = h[i] + (T)L::w(i);
+    const T u0 = (T)fl.ubc[idx], v0 = (T)fl.vbc[idx];
+    const T r = (f[0] + f[2] + f[4] + (T)2 * (f[3] + f[6] + f[7])) / ((T)1 - u0);
+    f[1] = f[3] + (T)(2.0 / 3.0) * r * u0;
+    if (kp.inlet == 1) {
+        f[5] = f[7] + r * u0 / (T)6;
+        f[8] = f[6] + r * u0 / (T)6;
+    } else {
+        f[5] = f[7] - (T)0.5 * (f[2] - f[4]) + r * u0 / (T)6 + (T)0.5 * r * v0;
+        f[8] = f[6] + (T)0.5 * (f[2] - f[4]) + r * u0 / (T)6 - (T)0.5 * r * v0;
+    }
+    h[1] = f[1] - (T)L::w(1); h[5] = f[5] - (T)L::w(5); h[8] = f[8] - (T)L::w(8);
+}
+template <class L, class T>
+__device__ inline void zou_outlet(const KParams& kp, T* h) {
+    if (L::Q != 9) return;
+    T f[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f[i] = h[i] + (T)L::w(i);
+    const T ro = (T)kp.rho_out;
+    const T ux = (T)-1 + (f[0] + f[2] + f[4] + (T)2 * (f[1] + f[5] + f[8])) / ro;
+    f[3] = f[1] - (T)(2.0 / 3.0) * ro * ux;
+    f[7] = f[5] + (T)0.5 * (f[2] - f[4]) - ro * ux / (T)6;
+    f[6] = f[8] - (T)0.5 * (f[2] - f[4]) - ro * ux / (T)6;
+    h[3] = f[3] - (T)L::w(3); h[6] = f[6] - (T)L::w(6); h[7] = f[7] - (T)L::w(7);
+}
+template <class L, class T>
+__device__ inline void onnode_walls(uint32_t nd, T* h) {
+#pragma unroll
+    for (int d = 1; d < L::Q; ++d)
+        if ((nd >> d) & 1u) h[L::opp(d)] = h[d];
+}
+
+// Moments of h and the stored-velocity semantics of the force model
+// (S/GpuUpdateMacro.cu:6-41, :137-213).
+template <class L, class T>
+__device__ inline void moments(const KParams& kp, uint32_t cls, const T* h, T& drho, T& rho, T* u) {
+    T d = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) {
+        d += h[i];
+        if (L::cx(i)) jx += L::cx(i) > 0 ? h[i] : -h[i];
+        if (L::cy(i)) jy += L::cy(i) > 0 ? h[i] : -h[i];
+        if (L::cz(i)) jz += L::cz(i) > 0 ? h[i] : -h[i];
+    }
+    drho = d;
+    rho = (T)1 + d;
+    const T ir = (T)1 / rho;
+    u[0] = jx * ir; u[1] = jy * ir; u[2] = jz * ir;
+    if (kp.force == 1) {
+        const T io = (T)(1.0 / kp.omega);
+        u[0] += (T)kp.g[0] * io; u[1] += (T)kp.g[1] * io; u[2] += (T)kp.g[2] * io;
+    } else if (kp.force == 2) {
+        u[0] += (T)(0.5 * kp.g[0]); u[1] += (T)(0.5 * kp.g[1]); u[2] += (T)(0.5 * kp.g[2]);
+    }
+    if ((kp.parity & 0x8u) && cls == PRES_E) u[1] = 0;
+}
+
+template <class L, class T>
+__device__ inline void guo_terms(const KParams& kp, T rho, const T* u, T* S) {
+    const T Fx = rho * (T)kp.g[0], Fy = rho * (T)kp.g[1], Fz = rho * (T)kp.g[2];
+    const T uF = u[0] * Fx + u[1] * Fy + u[2] * Fz;
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) {
+        const T cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
+        const T cF = cx * Fx + cy * Fy + cz * Fz;
+        const T cu = cx * u[0] + cy * u[1] + cz * u[2];
+        S[i] = (T)L::w(i) * ((T)3 * (cF - uF) + (T)9 * cu * cF);
+    }
+}
+
+// Deviations of the MRT equilibrium moments from the rest moments (M w).
+// D2Q9: S/GpuCollision.cu:986-994; D3Q19: :1065-1083.
+template <class L, class T>
+__device__ inline void mrt_dmeq(const KParams& kp, T drho, T rho, const T* u, T* dm) {
+    if (L::Q == 9) {
+        const T uu = u[0] * u[0] + u[1] * u[1];
+        const T k = (kp.parity & 0x1u) ? rho : (T)1;
+        dm[0] = drho;
+        dm[1] = (T)-2 * drho + (T)3 * k * rho * uu;
+        dm[2] = drho - (T)3 * k * rho * uu;
+        dm[3] = rho * u[0]; dm[4] = -rho * u[0];
+        dm[5] = rho * u[1]; dm[6] = -rho * u[1];
+        dm[7] = rho * (u[0] * u[0] - u[1] * u[1]);
+        dm[8] = rho * u[0] * u[1];
+    } else {
+        const T jx = rho * u[0], jy = rho * u[1], jz = rho * u[2];
+        const T j2 = jx * jx + jy * jy + jz * jz;
+        dm[0] = drho;
+        dm[1] = (T)-11 * drho + (T)19 * j2;
+        dm[2] = (T)(-475.0 / 63.0) * j2 - (T)mrest<D3Q19>(2);
+        dm[3] = jx; dm[4] = (T)(-2.0 / 3.0) * jx;
+        dm[5] = jy; dm[6] = (T)(-2.0 / 3.0) * jy;
+        dm[7] = jz; dm[8] = (T)(-2.0 / 3.0) * jz;
+        dm[9] = (T)2 * jx * jx - (jy * jy + jz * jz);
+        dm[10] = 0;
+        dm[11] = jy * jy - jz * jz;
+        dm[12] = 0;
+        dm[13] = jx * jy; dm[14] = jy * jz; dm[15] = jx * jz;
+        dm[16] = dm[17] = dm[18] = 0;
+    }
+}
+
+// Collision in deviation form.  BGK: S/GpuCollision.cu:528-550, :882-916;
+// TRT: :917-969 (general Lambda); MRT: :971-1037, :1039-1170.
+template <class L, class T, int C>
+__device__ inline void collide(const KParams& kp, const T* h, T drho, T rho, const T* u, T* out) {
+    T S[L::Q];
+    const bool guo = kp.force == 2;
+    if (guo) guo_terms<L, T>(kp, rho, u, S);
+    if (C == 2) {
+        T m[L::Q], dm[L::Q];
+        mrt_dmeq<L, T>(kp, drho, rho, u, dm);
+#pragma unroll
+        for (int r = 0; r < L::Q; ++r) {
+            T acc = 0, gs = 0;
+#pragma unroll
+            for (int i = 0; i < L::Q; ++i) {
+                const double c = mrow<L>(r, i);
+                if (c != 0.0) { acc += (T)c * h[i]; if (guo) gs += (T)c * S[i]; }
+            }
+            const T s = (T)kp.s[r];
+            m[r] = s * (acc - dm[r]);
+            if (guo) m[r] -= ((T)1 - (T)0.5 * s) * gs;
+        }
+        const bool nof18 = L::Q == 19 && (kp.parity & 0x2u);
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i) {
+            T acc = 0;
+#pragma unroll
+            for (int r = 0; r < L::Q; ++r) {
+                const double c = mrow<L>(r, i);
+                if (c != 0.0) acc += (T)(c / mnorm2<L>(r)) * m[r];
+            }
+            out[i] = h[i] - acc;
+        }
+        if (nof18) out[L::Q - 1] = -(T)L::w(L::Q - 1);   // fColl_18 stays at its zero initial value
+        return;
+    }
+    const T uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    T e[L::Q];
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) {
+        const T cu = (T)L::cx(i) * u[0] + (T)L::cy(i) * u[1] + (T)L::cz(i) * u[2];
+        e[i] = (T)L::w(i) * (drho + rho * ((T)3 * cu + (T)4.5 * cu * cu - (T)1.5 * uu));
+    }
+    const T op = (T)kp.omega;
+    if (C == 0) {
+        const T gf = (T)1 - (T)0.5 * op;
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i) {
+            out[i] = h[i] - op * (h[i] - e[i]);
+            if (guo) out[i] += gf * S[i];
+        }
+        return;
+    }
+    const T om = (T)kp.omega_m;
+    const T gp = (T)1 - (T)0.5 * op, gm = (T)1 - (T)0.5 * om;
+    out[0] = h[0] - op * (h[0] - e[0]);
+    if (guo) out[0] += gp * S[0];
+#pragma unroll
+    for (int i = 1; i < L::Q; ++i) {
+        const int o = L::opp(i);
+        if (i < o) {
+            const T dp = op * (T)0.5 * ((h[i] + h[o]) - (e[i] + e[o]));
+            const T dn = om * (T)0.5 * ((h[i] - h[o]) - (e[i] - e[o]));
+            out[i] = h[i] - dp - dn;
+            out[o] = h[o] - dp + dn;
+            if (guo) {
+                const T sp = gp * (T)0.5 * (S[i] + S[o]), sn = gm * (T)0.5 * (S[i] - S[o]);
+                out[i] += sp + sn;
+                out[o] += sp - sn;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+template <class L, int S, int C, bool OPEN>
+__global__ __launch_bounds__(256) void k_step(const typename Io<L, S>::raw* __restrict__ src,
+                                              typename Io<L, S>::raw* __restrict__ dst,
+                                              const uint32_t* __restrict__ node, KParams kp,
+                                              Fields fl, int write_macros) {
+    using T = typename Io<L, S>::T;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= kp.nx) return;
+    const size_t idx = ((size_t)z * kp.ny + y) * kp.nx + x;
+    const uint32_t nd = node[idx];
+    const uint32_t cls = nd >> kClsShift;
+    if (cls == SOLID) return;
+    T h[L::Q];
+    gather<L, S, OPEN>(src, kp, x, y, z, idx, nd, h);
+    if (OPEN && cls == VEL_W) zou_inlet<L, T>(kp, fl, idx, nd, h);
+    if (kp.wall == 1 && (nd & kBBMask)) onnode_walls<L, T>(nd, h);
+    if (OPEN && cls == PRES_E) zou_outlet<L, T>(kp, h);
+    T drho, rho, u[3];
+    moments<L, T>(kp, cls, h, drho, rho, u);
+    if (write_macros) {
+        fl.rho[idx] = (double)rho; fl.u[idx] = (double)u[0];
+        fl.v[idx] = (double)u[1]; fl.w[idx] = (double)u[2];
+    }
+    T out[L::Q];
+    collide<L, T, C>(kp, h, drho, rho, u, out);
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) Io<L, S>::st(dst, (size_t)i * kp.qstride + idx, out[i], i);
+}
+
+// Post-boundary populations and moments of the state held in src.
+template <class L, int S, bool OPEN>
+__global__ __launch_bounds__(256) void k_recon(const typename Io<L, S>::raw* __restrict__ src,
+                                               const uint32_t* __restrict__ node, KParams kp, Fields fl) {
+    using T = typename Io<L, S>::T;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= kp.nx) return;
+    const size_t n = (size_t)kp.nx * kp.ny * kp.nz;
+    const size_t idx = ((size_t)z * kp.ny + y) * kp.nx + x;
+    const uint32_t nd = node[idx];
+    const uint32_t cls = nd >> kClsShift;
+    if (cls == SOLID) {
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i) fl.fout[i * n + idx] = 0.0;
+        return;
+    }
+    T h[L::Q];
+    gather<L, S, OPEN>(src, kp, x, y, z, idx, nd, h);
+    if (OPEN && cls == VEL_W) zou_inlet<L, T>(kp, fl, idx, nd, h);
+    if (kp.wall == 1 && (nd & kBBMask)) onnode_walls<L, T>(nd, h);
+    if (OPEN && cls == PRES_E) zou_outlet<L, T>(kp, h);
+    T drho, rho, u[3];
+    moments<L, T>(kp, cls, h, drho, rho, u);
+    fl.rho[idx] = (double)rho; fl.u[idx] = (double)u[0];
+    fl.v[idx] = (double)u[1]; fl.w[idx] = (double)u[2];
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) fl.fout[i * n + idx] = (double)h[i] + L::w(i);
+}
+
+// First post-collision state from an initial f and macroscopic fields.
+template <class L, int S, int C>
+__global__ __launch_bounds__(256) void k_init(typename Io<L, S>::raw* __restrict__ dst,
+                                              const uint32_t* __restrict__ node, KParams kp,
+                                              Fields fl, int mode) {
+    using T = typename Io<L, S>::T;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= kp.nx) return;
+    const size_t n = (size_t)kp.nx * kp.ny * kp.nz;
+    const size_t idx = ((size_t)z * kp.ny + y) * kp.nx + x;
+    const uint32_t nd = node[idx];
+    const uint32_t cls = nd >> kClsShift;
+    if (cls == SOLID) {
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i) Io<L, S>::st(dst, (size_t)i * kp.qstride + idx, (T)0, i);
+        return;
+    }
+    T h[L::Q], drho, rho, u[3];
+    if (mode == INIT_FROM_F) {
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i) h[i] = (T)(fl.fout[i * n + idx] - L::w(i));
+        moments<L, T>(kp, cls, h, drho, rho, u);
+        fl.rho[idx] = (double)rho; fl.u[idx] = (double)u[0];
+        fl.v[idx] = (double)u[1]; fl.w[idx] = (double)u[2];
+    } else {
+        const double r = fl.rho[idx];
+        drho = (T)(r - 1.0);
+        rho = (T)r;
+        u[0] = (T)fl.u[idx]; u[1] = (T)fl.v[idx]; u[2] = (T)fl.w[idx];
+        const T uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i) {
+            if (mode == INIT_ZERO) {
+                h[i] = -(T)L::w(i);
+            } else {
+                const T cu = (T)L::cx(i) * u[0] + (T)L::cy(i) * u[1] + (T)L::cz(i) * u[2];
+                h[i] = (T)L::w(i) * (drho + rho * ((T)3 * cu + (T)4.5 * cu * cu - (T)1.5 * uu));
+            }
+        }
+    }
+    T out[L::Q];
+    collide<L, T, C>(kp, h, drho, rho, u, out);
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) Io<L, S>::st(dst, (size_t)i * kp.qstride + idx, out[i], i);
+}
+
+// FP64 reduction: per-block partials (wave shuffles, then LDS), then one block
+// sums the partials in a fixed order, so results are deterministic.
+constexpr int kRedThreads = 256;
+constexpr int kRedBlocks = 1024;
+int reduce_blocks() { return kRedBlocks; }
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_reduce_partial(const uint32_t* __restrict__ node,
+                                                                 Fields fl, double* uprev, double* vprev,
+                                                                 double* wprev, size_t n, double* partials) {
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if ((node[i] >> kClsShift) == SOLID) continue;
+        const double u = fl.u[i], v = fl.v[i], w = fl.w[i], r = fl.rho[i];
+        const double du = u - uprev[i], dv = v - vprev[i], dw = w - wprev[i];
+        const double uu = u * u + v * v + w * w;
+        a0 += du * du + dv * dv + dw * dw;
+        a1 += uu;
+        a2 += r;
+        if (!(uu == uu) || !(r == r) || isinf(uu) || isinf(r)) a3 += 1.0;
+        uprev[i] = u; vprev[i] = v; wprev[i] = w;
+    }
+    __shared__ double sh[4][kRedThreads / 64];
+    a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2); a3 = wave_sum(a3);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { sh[0][wv] = a0; sh[1][wv] = a1; sh[2][wv] = a2; sh[3][wv] = a3; }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double s = 0;
+        for (int k = 0; k < kRedThreads / 64; ++k) s += sh[threadIdx.x][k];
+        partials[(size_t)blockIdx.x * 4 + threadIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_final(const double* __restrict__ partials, int nb, double* out4) {
+    __shared__ double sh[4][256];
+    for (int k = 0; k < 4; ++k) {
+        double s = 0;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partials[(size_t)b * 4 + k];
+        sh[k][threadIdx.x] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double s = 0;
+        for (int t = 0; t < 256; ++t) s += sh[threadIdx.x][t];
+        out4[threadIdx.x] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+static dim3 node_block(const KParams& kp) {
+    int bx = kp.nx >= 256 ? 256 : ((kp.nx + 63) / 64) * 64;
+    return dim3(bx, 1, 1);
+}
+static dim3 node_grid(const KParams& kp) {
+    dim3 b = node_block(kp);
+    return dim3((kp.nx + b.x - 1) / b.x, kp.ny, kp.nz);
+}
+
+template <class L, int S, int C, bool OPEN>
+static hipError_t step_t(const KParams& kp, const void* src, void* dst, const uint32_t* node, Fields fl,
+                         bool wm, hipStream_t st) {
+    using raw = typename Io<L, S>::raw;
+    hipLaunchKernelGGL((k_step<L, S, C, OPEN>), node_grid(kp), node_block(kp), 0, st,
+                       (const raw*)src, (raw*)dst, node, kp, fl, wm ? 1 : 0);
+    return hipGetLastError();
+}
+template <class L, int S, int C>
+static hipError_t step_o(bool open, const KParams& kp, const void* src, void* dst, const uint32_t* node,
+                         Fields fl, bool wm, hipStream_t st) {
+    return open ? step_t<L, S, C, true>(kp, src, dst, node, fl, wm, st)
+                : step_t<L, S, C, false>(kp, src, dst, node, fl, wm, st);
+}
+template <class L, int S>
+static hipError_t step_c(int c, bool open, const KParams& kp, const void* src, void* dst,
+                         const uint32_t* node, Fields fl, bool wm, hipStream_t st) {
+    switch (c) {
+    case 0: return step_o<L, S, 0>(open, kp, src, dst, node, fl, wm, st);
+    case 1: return step_o<L, S, 1>(open, kp, src, dst, node, fl, wm, st);
+    default: return step_o<L, S, 2>(open, kp, src, dst, node, fl, wm, st);
+    }
+}
+template <class L>
+static hipError_t step_s(int s, int c, bool open, const KParams& kp, const void* src, void* dst,
+                         const uint32_t* node, Fields fl, bool wm, hipStream_t st) {
+    switch (s) {
+    case 0: return step_c<L, 0>(c, open, kp, src, dst, node, fl, wm, st);
+    case 1: return step_c<L, 1>(c, open, kp, src, dst, node, fl, wm, st);
+    case 2: return step_c<L, 2>(c, open, kp, src, dst, node, fl, wm, st);
+    default: return step_c<L, 3>(c, open, kp, src, dst, node, fl, wm, st);
+    }
+}
+hipError_t launch_step(int q, int storage, int collision, bool open, const KParams& kp, const void* src,
+                       void* dst, const uint32_t* node, Fields fl, bool wm, hipStream_t st) {
+    return q == 9 ? step_s<D2Q9>(storage, collision, open, kp, src, dst, node, fl, wm, st)
+                  : step_s<D3Q19>(storage, collision, open, kp, src, dst, node, fl, wm, st);
+}
+
+template <class L, int S>
+static hipError_t recon_t(bool open, const KParams& kp, const void* src, const uint32_t* node, Fields fl,
+                          hipStream_t st) {
+    using raw = typename Io<L, S>::raw;
+    if (open)
+        hipLaunchKernelGGL((k_recon<L, S, true>), node_grid(kp), node_block(kp), 0, st, (const raw*)src, node, kp, fl);
+    else
+        hipLaunchKernelGGL((k_recon<L, S, false>), node_grid(kp), node_block(kp), 0, st, (const raw*)src, node, kp, fl);
+    return hipGetLastError();
+}
+template <class L>
+static hipError_t recon_s(int s, bool open, const KParams& kp, const void* src, const uint32_t* node,
+                          Fields fl, hipStream_t st) {
+    switch (s) {
+    case 0: return recon_t<L, 0>(open, kp, src, node, fl, st);
+    case 1: return recon_t<L, 1>(open, kp, src, node, fl, st);
+    case 2: return recon_t<L, 2>(open, kp, src, node, fl, st);
+    default: return recon_t<L, 3>(open, kp, src, node, fl, st);
+    }
+}
+hipError_t launch_recon(int q, int storage, bool open, const KParams& kp, const void* src, const uint32_t* node,
+                        Fields fl, hipStream_t st) {
+    return q == 9 ? recon_s<D2Q9>(storage, open, kp, src, node, fl, st)
+                  : recon_s<D3Q19>(storage, open, kp, src, node, fl, st);
+}
+
+template <class L, int S, int C>
+static hipError_t init_t(int mode, const KParams& kp, void* dst, const uint32_t* node, Fields fl, hipStream_t st) {
+    using raw = typename Io<L, S>::raw;
+    hipLaunchKernelGGL((k_init<L, S, C>), node_grid(kp), node_block(kp), 0, st, (raw*)dst, node, kp, fl, mode);
+    return hipGetLastError();
+}
+template <class L, int S>
+static hipError_t init_c(int c, int mode, const KParams& kp, void* dst, const uint32_t* node, Fields fl,
+                         hipStream_t st) {
+    switch (c) {
+    case 0: return init_t<L, S, 0>(mode, kp, dst, node, fl, st);
+    case 1: return init_t<L, S, 1>(mode, kp, dst, node, fl, st);
+    default: return init_t<L, S, 2>(mode, kp, dst, node, fl, st);
+    }
+}
+template <class L>
+static hipError_t init_s(int s, int c, int mode, const KParams& kp, void* dst, const uint32_t* node, Fields fl,
+                         hipStream_t st) {
+    switch (s) {
+    case 0: return init_c<L, 0>(c, mode, kp, dst, node, fl, st);
+    case 1: return init_c<L, 1>(c, mode, kp, dst, node, fl, st);
+    case 2: return init_c<L, 2>(c, mode, kp, dst, node, fl, st);
+    default: return init_c<L, 3>(c, mode, kp, dst, node, fl, st);
+    }
+}
+hipError_t launch_init(int q, int storage, int collision, int mode, const KParams& kp, void* dst,
+                       const uint32_t* node, Fields fl, hipStream_t st) {
+    return q == 9 ? init_s<D2Q9>(storage, collision, mode, kp, dst, node, fl, st)
+                  : init_s<D3Q19>(storage, collision, mode, kp, dst, node, fl, st);
+}
+
+hipError_t launch_reduce(const KParams& kp, const uint32_t* node, Fields fl, double* uprev, double* vprev,
+                         double* wprev, double* partials, int nblocks, double* out4, hipStream_t st) {
+    const size_t n = (size_t)kp.nx * kp.ny * kp.nz;
+    hipLaunchKernelGGL(k_reduce_partial, dim3(nblocks), dim3(kRedThreads), 0, st, node, fl, uprev, vprev, wprev,
+                       n, partials);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, (const double*)partials, nblocks, out4);
+    return hipGetLastError();
+}
+
+}  // namespace lbm

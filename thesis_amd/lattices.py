@@ -1,0 +1,131 @@
+"""Lattice builders in the solver's node description.
+
+A lattice is `node_class` (uint8 per node: SOLID, FLUID, VEL_W, PRES_E) plus
+`bb` (uint32 per node, bit d set when the link from the node in direction d
+crosses a wall -- the content of a latticegen2 _BC.dat WALL line,
+L/FluidTracking.cpp:129-156), in the reference's node order
+ind = i + j*n + k*n*m (S/include/Iterate.h:10-35).
+"""
+import numpy as np
+
+from .lbm import SOLID, FLUID, VEL_W, PRES_E
+
+# reference direction numbering (S/GpuInit.cu:94-99, :822-842)
+D2Q9_C = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, -1, 0],
+                   [1, 1, 0], [-1, 1, 0], [-1, -1, 0], [1, -1, 0]])
+D3Q19_C = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1],
+                    [1, 1, 0], [-1, 1, 0], [1, -1, 0], [-1, -1, 0], [1, 0, 1], [-1, 0, 1], [1, 0, -1],
+                    [-1, 0, -1], [0, 1, 1], [0, -1, 1], [0, 1, -1], [0, -1, -1]])
+D2Q9_W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+D3Q19_W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+
+
+def velocities(q):
+    return D2Q9_C if q == 9 else D3Q19_C
+
+
+def weights(q):
+    return D2Q9_W if q == 9 else D3Q19_W
+
+
+def opposite(q):
+    c = velocities(q)
+    return np.array([int(np.where((c == -c[i]).all(axis=1))[0][0]) for i in range(q)])
+
+
+def bb_from_solid(solid, q, periodic=(True, True, True), outside_is_wall=(False, False, False)):
+    """Bounce-back link masks of the non-solid nodes of a (nz, ny, nx) or (ny, nx)
+    boolean solid field: bit d set when node + c_d is solid, or lies outside
+    the box on an axis flagged in outside_is_wall."""
+    s3 = solid[None] if solid.ndim == 2 else solid
+    nz, ny, nx = s3.shape
+    bb = np.zeros(s3.shape, np.uint32)
+    c = velocities(q)
+    for d in range(1, q):
+        dx, dy, dz = (int(v) for v in c[d])
+        nb = s3
+        # neighbour at +c_d: roll by -c_d
+        nb = np.roll(nb, shift=(-dz, -dy, -dx), axis=(0, 1, 2))
+        hit = nb.copy()
+        for ax, (dd, n) in enumerate(((dx, nx), (dy, ny), (dz, nz))):
+            if dd == 0:
+                continue
+            axis = 2 - ax
+            edge = [slice(None)] * 3
+            edge[axis] = slice(n - 1, n) if dd > 0 else slice(0, 1)
+            if not periodic[ax]:
+                hit[tuple(edge)] = bool(outside_is_wall[ax])
+            elif outside_is_wall[ax]:
+                hit[tuple(edge)] = True
+        bb |= (hit & ~s3).astype(np.uint32) << np.uint32(d)
+    return bb[0] if solid.ndim == 2 else bb
+
+
+def channel2d(nx, ny, u_max, obstacles=(), rho_out=1.0):
+    """D2Q9 channel: half-way walls at y = -1/2 and y = ny - 1/2, Zou-He
+    parabolic velocity inlet on x = 0 (u = 4 u_max (y+1/2)(ny-y-1/2)/ny^2),
+    Zou-He pressure outlet on x = nx-1, optional circular obstacles
+    (cx, cy, r): nodes with |x - c| <= r are solid."""
+    solid = np.zeros((ny, nx), bool)
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    for (ox, oy, r) in obstacles:
+        solid |= (xx - ox) ** 2 + (yy - oy) ** 2 <= r * r
+    cls = np.where(solid, SOLID, FLUID).astype(np.uint8)
+    cls[:, 0] = np.where(solid[:, 0], SOLID, VEL_W)
+    cls[:, -1] = np.where(solid[:, -1], SOLID, PRES_E)
+    bb = bb_from_solid(solid, 9, periodic=(False, False, True), outside_is_wall=(False, True, False))
+    y = np.arange(ny) + 0.5
+    prof = 4.0 * u_max * y * (ny - y) / ny ** 2
+    ubc = np.zeros((ny, nx))
+    ubc[:, 0] = prof
+    return dict(node_class=cls, bb=bb, ubc=ubc, vbc=np.zeros((ny, nx)), solid=solid, profile=prof,
+                periodic=(0, 0, 0), rho_out=rho_out)
+
+
+def porous3d(n, seed=2024, rmin=6.0, rmax=12.0, solid_fraction=0.30, ny=None, nz=None):
+    """Periodic overlapping-sphere porous medium on an n x ny x nz box.
+
+    Sphere centres ~ U[0, n)^3 and radii ~ U[rmin, rmax] from
+    numpy.random.Generator(PCG64(seed)); a node is solid when its periodic
+    minimum-image distance to a centre is <= r.  Spheres are added until the
+    solid fraction reaches `solid_fraction`.  Returns (node_class, bb, info)."""
+    ny = ny or n
+    nz = nz or n
+    rng = np.random.Generator(np.random.PCG64(seed))
+    solid = np.zeros((nz, ny, nx := n), bool)
+    total = nx * ny * nz
+    count = 0
+    spheres = 0
+    while count < solid_fraction * total:
+        c = rng.uniform(0.0, 1.0, 3) * np.array([nx, ny, nz])
+        r = rng.uniform(rmin, rmax)
+        ri = int(np.ceil(r))
+        ix = np.arange(int(np.floor(c[0])) - ri, int(np.floor(c[0])) + ri + 2)
+        iy = np.arange(int(np.floor(c[1])) - ri, int(np.floor(c[1])) + ri + 2)
+        iz = np.arange(int(np.floor(c[2])) - ri, int(np.floor(c[2])) + ri + 2)
+        dz = (iz - c[2])[:, None, None]
+        dy = (iy - c[1])[None, :, None]
+        dx = (ix - c[0])[None, None, :]
+        inside = dx * dx + dy * dy + dz * dz <= r * r
+        sub = np.ix_(iz % nz, iy % ny, ix % nx)
+        before = solid[sub]
+        # duplicated wrapped indices only occur when 2r+2 > n; handled by |=
+        solid[sub] = before | inside
+        count = int(solid.sum()) if (2 * ri + 2 > min(nx, ny, nz)) else count + int((inside & ~before).sum())
+        spheres += 1
+    count = int(solid.sum())
+    cls = np.where(solid, SOLID, FLUID).astype(np.uint8)
+    bb = bb_from_solid(solid, 19, periodic=(True, True, True))
+    info = dict(seed=seed, spheres=spheres, solid_fraction=count / total, n_fluid=total - count)
+    return cls, bb, info
+
+
+def droplet2d(nx, ny, radius, cx=None, cy=None):
+    """Colour densities of a disc of radius `radius` (red inside, blue
+    outside), centred in an nx x ny periodic box."""
+    cx = (nx - 1) / 2.0 if cx is None else cx
+    cy = (ny - 1) / 2.0 if cy is None else cy
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    inside = (xx - cx) ** 2 + (yy - cy) ** 2 <= radius ** 2
+    r = inside.astype(np.float64)
+    return r, 1.0 - r

@@ -1,0 +1,242 @@
+"""Host-side binding of the C ABI (include/lbm.h) through ctypes.
+
+Mirrors the solver interface of the reference (S/include/Arguments.h:86-135,
+S/include/Iterate.h:57-65): a `Solver` is one configured lattice with its
+populations on the GPU.  The HIP library is mandatory: importing works without
+it, but creating a solver raises when thesis_amd/liblbm.so is missing or no
+HIP device is visible -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblbm.so")
+ABI_VERSION = 1
+
+SOLID, FLUID, VEL_W, PRES_E = 0, 1, 2, 3
+BGK, TRT, MRT = 0, 1, 2
+STORE_F64, STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2, 3
+FORCE_NONE, FORCE_SHIFT, FORCE_GUO = 0, 1, 2
+WALL_HALFWAY, WALL_ONNODE = 0, 1
+INLET_ZOUHE, INLET_REF2D = 0, 1
+PARITY_MRT_RHO2, PARITY_MRT_NO_F18, PARITY_OUTLET_V0 = 0x1, 0x2, 0x8
+
+STATUS = {0: "OK", 1: "DIVERGED", 2: "EIO", 3: "EINVAL", 4: "EDEVICE", 5: "ENOMEM", 6: "EUNSUPPORTED"}
+STORAGE_NAMES = {"f64": STORE_F64, "f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16}
+
+EXPORTED = [
+    "lbm_abi_version", "lbm_device_count", "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_init",
+    "lbm_step", "lbm_run_to_residual", "lbm_reduce", "lbm_get_macros", "lbm_get_f", "lbm_set_f",
+    "lbm_time_steps", "lbm_device_bytes", "lbm_read_lattice",
+]
+
+
+class LbmError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_uint32), ("q", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32),
+        ("nz", C.c_int32), ("periodic", C.c_int32 * 3), ("collision", C.c_int32),
+        ("omega", C.c_double), ("trt_magic", C.c_double), ("mrt_s", C.c_double * 19),
+        ("force_model", C.c_int32), ("force", C.c_double * 3), ("storage", C.c_int32),
+        ("wall_model", C.c_int32), ("inlet_model", C.c_int32), ("parity_flags", C.c_uint32),
+        ("rho_out", C.c_double),
+    ]
+
+
+class Lattice(C.Structure):
+    _fields_ = [
+        ("node_class", C.POINTER(C.c_uint8)), ("bb_links", C.POINTER(C.c_uint32)),
+        ("u_bc", C.POINTER(C.c_double)), ("v_bc", C.POINTER(C.c_double)), ("w_bc", C.POINTER(C.c_double)),
+    ]
+
+
+_lib = None
+
+
+def load():
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LbmError(4, f"{LIB_PATH} is missing: run `python -m thesis_amd.build` first")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    dp = P(C.c_double)
+    vp = C.c_void_p
+    L.lbm_abi_version.restype = C.c_int
+    L.lbm_device_count.restype = C.c_int
+    L.lbm_create.argtypes = [P(Params), P(Lattice), P(vp)]
+    L.lbm_destroy.argtypes = [vp]
+    L.lbm_destroy.restype = None
+    L.lbm_last_error.argtypes = [vp]
+    L.lbm_last_error.restype = C.c_char_p
+    L.lbm_init.argtypes = [vp, C.c_int, dp, dp, dp, dp]
+    L.lbm_step.argtypes = [vp, C.c_int64]
+    L.lbm_run_to_residual.argtypes = [vp, C.c_double, C.c_int64, C.c_int64, P(C.c_int64), dp]
+    L.lbm_reduce.argtypes = [vp, dp]
+    L.lbm_get_macros.argtypes = [vp, dp, dp, dp, dp]
+    L.lbm_get_f.argtypes = [vp, dp]
+    L.lbm_set_f.argtypes = [vp, dp]
+    L.lbm_time_steps.argtypes = [vp, C.c_int64, dp]
+    L.lbm_device_bytes.argtypes = [vp]
+    L.lbm_device_bytes.restype = C.c_int64
+    L.lbm_read_lattice.argtypes = [C.c_char_p, C.c_char_p, P(C.c_int32), P(C.c_int64), P(C.c_uint8),
+                                   P(C.c_uint32), P(C.c_int64), P(C.c_int64)]
+    for name in EXPORTED:
+        fn = getattr(L, name)
+        if fn.restype is C.c_int and name not in ("lbm_abi_version", "lbm_device_count"):
+            fn.restype = C.c_int
+    if L.lbm_abi_version() != ABI_VERSION:
+        raise LbmError(3, "liblbm.so ABI version mismatch")
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f64(a, n=None):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def device_count():
+    return load().lbm_device_count()
+
+
+def read_lattice(node_path, bc_path=None):
+    """Read a latticegen2 pair (S/FilesReading.cu:270-347).
+
+    Returns dict(dims=(nx, ny, nz), node_class, bb_links, nlinks, type_counts, dir_counts)
+    with arrays shaped (nz, ny, nx)."""
+    L = load()
+    dims = (C.c_int32 * 3)()
+    nl = C.c_int64()
+    nb = bc_path.encode() if bc_path else None
+    st = L.lbm_read_lattice(node_path.encode(), nb, dims, C.byref(nl), None, None, None, None)
+    if st:
+        raise LbmError(st, f"cannot read {node_path} / {bc_path}")
+    nx, ny, nz = dims[:]
+    n = nx * ny * nz
+    cls = np.zeros(n, np.uint8)
+    bb = np.zeros(n, np.uint32)
+    tc = np.zeros(6, np.int64)
+    dc = np.zeros(19, np.int64)
+    st = L.lbm_read_lattice(node_path.encode(), nb, dims, C.byref(nl),
+                            cls.ctypes.data_as(C.POINTER(C.c_uint8)), bb.ctypes.data_as(C.POINTER(C.c_uint32)),
+                            tc.ctypes.data_as(C.POINTER(C.c_int64)), dc.ctypes.data_as(C.POINTER(C.c_int64)))
+    if st:
+        raise LbmError(st, f"cannot read {node_path} / {bc_path}")
+    return dict(dims=(nx, ny, nz), node_class=cls.reshape(nz, ny, nx), bb_links=bb.reshape(nz, ny, nx),
+                nlinks=nl.value, type_counts=tc, dir_counts=dc)
+
+
+class Solver:
+    """One lattice on the GPU.  Arrays are (Q, nz, ny, nx) / (nz, ny, nx) in
+    3D and (Q, ny, nx) / (ny, nx) in 2D, x fastest as in the reference
+    (ind = i + j*n + k*n*m)."""
+
+    def __init__(self, q, shape, node_class, bb=None, *, collision=BGK, omega=1.0, trt_magic=0.0,
+                 mrt_s=None, force_model=FORCE_NONE, g=(0.0, 0.0, 0.0), wall_model=WALL_HALFWAY,
+                 inlet_model=INLET_ZOUHE, parity=0, rho_out=1.0, periodic=(0, 0, 0), ubc=None,
+                 vbc=None, wbc=None, storage=STORE_F64):
+        L = load()
+        if isinstance(storage, str):
+            storage = STORAGE_NAMES[storage]
+        self.two_d = len(shape) == 2
+        nz, ny, nx = (1,) + tuple(shape) if self.two_d else tuple(shape)
+        self.q, self.shape3, self.n = q, (nz, ny, nx), nx * ny * nz
+        self.shape = tuple(shape)
+        p = Params()
+        p.abi_version, p.q, p.nx, p.ny, p.nz = ABI_VERSION, q, nx, ny, nz
+        for i in range(3):
+            p.periodic[i] = int(periodic[i]) if i < len(periodic) else 0
+            p.force[i] = float(g[i]) if i < len(g) else 0.0
+        p.collision, p.omega, p.trt_magic = collision, float(omega), float(trt_magic)
+        if mrt_s is not None:
+            for i, s in enumerate(mrt_s):
+                p.mrt_s[i] = s
+        p.force_model, p.storage, p.wall_model, p.inlet_model = force_model, storage, wall_model, inlet_model
+        p.parity_flags, p.rho_out = parity, float(rho_out)
+        self.params = p
+        cls = np.ascontiguousarray(node_class, dtype=np.uint8).ravel()
+        if cls.size != self.n:
+            raise ValueError("node_class size does not match the lattice shape")
+        bbv = None if bb is None else np.ascontiguousarray(bb, dtype=np.uint32).ravel()
+        self._keep = [cls, bbv, _f64(ubc, self.n), _f64(vbc, self.n), _f64(wbc, self.n)]
+        lat = Lattice()
+        lat.node_class = cls.ctypes.data_as(C.POINTER(C.c_uint8))
+        lat.bb_links = bbv.ctypes.data_as(C.POINTER(C.c_uint32)) if bbv is not None else None
+        lat.u_bc, lat.v_bc, lat.w_bc = (_dp(a) for a in self._keep[2:])
+        h = C.c_void_p()
+        st = L.lbm_create(C.byref(p), C.byref(lat), C.byref(h))
+        if st:
+            raise LbmError(st, "lbm_create failed (see stderr)")
+        self.h = h
+        self._L = L
+
+    def _check(self, st):
+        if st:
+            raise LbmError(st, self._L.lbm_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.lbm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def init(self, mode=0, rho=None, u=None, v=None, w=None):
+        arrs = [_f64(a, self.n) for a in (rho, u, v, w)]
+        self._check(self._L.lbm_init(self.h, mode, *[_dp(a) for a in arrs]))
+
+    def step(self, n=1):
+        self._check(self._L.lbm_step(self.h, int(n)))
+
+    def run_to_residual(self, tol, every=100, max_steps=10 ** 6):
+        done = C.c_int64()
+        res = C.c_double()
+        self._check(self._L.lbm_run_to_residual(self.h, tol, every, max_steps, C.byref(done), C.byref(res)))
+        return done.value, res.value
+
+    def reduce(self):
+        out = np.zeros(4)
+        self._check(self._L.lbm_reduce(self.h, _dp(out)))
+        return dict(du2=out[0], u2=out[1], mass=out[2], nonfinite=out[3])
+
+    def macros(self):
+        out = [np.zeros(self.n) for _ in range(4)]
+        self._check(self._L.lbm_get_macros(self.h, *[_dp(a) for a in out]))
+        sh = self.shape if self.two_d else self.shape3
+        return [a.reshape(sh) for a in out]
+
+    def get_f(self):
+        f = np.zeros(self.q * self.n)
+        self._check(self._L.lbm_get_f(self.h, _dp(f)))
+        return f.reshape((self.q,) + (self.shape if self.two_d else self.shape3))
+
+    def set_f(self, f):
+        f = _f64(f, self.q * self.n)
+        self._check(self._L.lbm_set_f(self.h, _dp(f)))
+
+    def time_steps(self, n):
+        ms = C.c_double()
+        self._check(self._L.lbm_time_steps(self.h, int(n), C.byref(ms)))
+        return ms.value
+
+    def device_bytes(self):
+        return self._L.lbm_device_bytes(self.h)
