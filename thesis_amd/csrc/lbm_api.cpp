@@ -28,10 +28,15 @@ struct lbm_ctx {
     bool open = false;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    void* P[2] = {nullptr, nullptr};
+    void* P[2] = {nullptr, nullptr};      // population buffers (256 B into their allocations)
+    void* Palloc[2] = {nullptr, nullptr};
     int cur = 0;              // buffer holding the newest post-collision state
     int64_t steps = -1;       // steps since the state was set; -1: no state
     uint32_t* node = nullptr;
+    uint32_t* cidx = nullptr;     // dense node -> compact population index (all ones: solid)
+    uint32_t* didx = nullptr;     // compact -> dense
+    uint32_t* nodew = nullptr;    // compact node words
+    size_t nf = 0;                // non-solid nodes
     double *rho = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
     double *up = nullptr, *vp = nullptr, *wp = nullptr;
     double *ubc = nullptr, *vbc = nullptr, *wbc = nullptr;
@@ -100,7 +105,7 @@ lbm_status run_steps(lbm_ctx* c, int64_t nsteps) {
     for (int64_t k = 0; k < nsteps; ++k) {
         const bool wm = (k == nsteps - 1);
         hipError_t e = launch_step(c->q, c->p.storage, c->p.collision, c->open, c->kp, c->P[c->cur],
-                                   c->P[c->cur ^ 1], c->node, fields(c), wm, c->stream);
+                                   c->P[c->cur ^ 1], c->node, c->cidx, fields(c), wm, c->stream);
         if (e != hipSuccess) return fail(c, LBM_EDEVICE, "step launch: %s", hipGetErrorString(e));
         c->cur ^= 1;
     }
@@ -163,6 +168,50 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
         const uint32_t bb = lat->bb_links ? (lat->bb_links[i] & mask) : 0u;
         node[i] = (cls << kClsShift) | bb;
     }
+    // A link into a solid node is a wall: complete the bounce-back bits from the
+    // node classes (periodic axes wrap; links leaving a non-periodic box are
+    // left to the lattice's own WALL links or the open boundaries).
+    {
+        static const int C9[9][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {-1, 0, 0}, {0, -1, 0},
+                                     {1, 1, 0}, {-1, 1, 0}, {-1, -1, 0}, {1, -1, 0}};
+        static const int C19[19][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1},
+                                       {0, 0, -1}, {1, 1, 0}, {-1, 1, 0}, {1, -1, 0}, {-1, -1, 0}, {1, 0, 1},
+                                       {-1, 0, 1}, {1, 0, -1}, {-1, 0, -1}, {0, 1, 1}, {0, -1, 1}, {0, 1, -1},
+                                       {0, -1, -1}};
+        const int nx = p->nx, ny = p->ny;
+        const int per[3] = {p->periodic[0] != 0, p->periodic[1] != 0, p->q == 9 ? 1 : p->periodic[2] != 0};
+        for (size_t i = 0; i < n; ++i) {
+            if (lat->node_class[i] == LBM_SOLID) continue;
+            const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((size_t)nx * ny));
+            for (int d = 1; d < p->q; ++d) {
+                const int* c = p->q == 9 ? C9[d] : C19[d];
+                int xyz[3] = {x + c[0], y + c[1], z + c[2]};
+                const int dims[3] = {nx, ny, nz};
+                bool out = false;
+                for (int a = 0; a < 3; ++a) {
+                    if (xyz[a] < 0 || xyz[a] >= dims[a]) {
+                        if (per[a]) xyz[a] = (xyz[a] + dims[a]) % dims[a];
+                        else out = true;
+                    }
+                }
+                if (out) continue;
+                const size_t j = ((size_t)xyz[2] * ny + xyz[1]) * nx + xyz[0];
+                if (lat->node_class[j] == LBM_SOLID) node[i] |= 1u << d;
+            }
+        }
+    }
+    if (n >= 0xFFFFFFFFu) return reject("lattice too large for 32-bit node indices");
+    std::vector<uint32_t> cidx(n), didx, nodew;
+    size_t nf = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (lat->node_class[i] == LBM_SOLID) {
+            cidx[i] = 0xFFFFFFFFu;
+        } else {
+            cidx[i] = (uint32_t)nf++;
+            didx.push_back((uint32_t)i);
+            nodew.push_back(node[i]);
+        }
+    }
     if (any_open_cls && !lat->u_bc) {
         bool need = false;
         for (size_t i = 0; i < n; ++i) need |= lat->node_class[i] == LBM_VEL_W;
@@ -177,13 +226,24 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     c->p.nz = nz;
     c->q = p->q;
     c->n = n;
+    c->nf = nf;
     c->raw_bytes = storage_bytes(p->storage);
     c->open = !(p->periodic[0] && p->periodic[1] && (p->q == 9 || p->periodic[2]));
     KParams& k = c->kp;
     k.nx = p->nx; k.ny = p->ny; k.nz = nz;
     for (int a = 0; a < 3; ++a) k.per[a] = p->periodic[a] ? 1 : 0;
     if (p->q == 9) k.per[2] = 1;
-    k.qstride = plane_stride(n, c->raw_bytes);
+    k.qstride = plane_stride(nf > 0 ? nf : 1, c->raw_bytes);
+    k.nf = (uint32_t)nf;
+    auto magic = [](uint32_t d, uint64_t& m, uint32_t& s) {
+        s = 0;
+        while ((1ull << s) < d) ++s;
+        m = ((1ull << 32) * ((1ull << s) - d)) / d + 1;
+    };
+    magic((uint32_t)p->nx, k.mx, k.sx);
+    magic((uint32_t)p->ny, k.my, k.sy);
+    k.nxcd = 8;   // MI355X: 8 XCDs
+    if (const char* e = std::getenv("LBM_XCD")) k.nxcd = (uint32_t)std::strtoul(e, nullptr, 10);
     k.omega = p->omega;
     const double lam = p->trt_magic > 0 ? p->trt_magic : 3.0 / 16.0;
     k.omega_m = 1.0 / (0.5 + lam / (1.0 / p->omega - 0.5));
@@ -214,9 +274,17 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
         g_create_err = "stream/event creation failed";
         return cleanup(LBM_EDEVICE);
     }
-    const size_t pbytes = c->raw_bytes * k.qstride * c->q;
-    for (int b = 0; b < 2 && st == LBM_OK; ++b) st = dalloc(c, &c->P[b], pbytes);
+    // 256 B guard before and after the planes: the shifted vector loads of the
+    // first and last rows may touch one element outside the lattice (discarded).
+    const size_t pbytes = c->raw_bytes * k.qstride * c->q + 512;
+    for (int b = 0; b < 2 && st == LBM_OK; ++b) {
+        st = dalloc(c, &c->Palloc[b], pbytes);
+        if (st == LBM_OK) c->P[b] = (char*)c->Palloc[b] + 256;
+    }
     if (st == LBM_OK) st = dalloc(c, (void**)&c->node, sizeof(uint32_t) * n);
+    if (st == LBM_OK) st = dalloc(c, (void**)&c->cidx, sizeof(uint32_t) * n);
+    if (st == LBM_OK) st = dalloc(c, (void**)&c->didx, sizeof(uint32_t) * (nf > 0 ? nf : 1));
+    if (st == LBM_OK) st = dalloc(c, (void**)&c->nodew, sizeof(uint32_t) * (nf > 0 ? nf : 1));
     double** fl[] = {&c->rho, &c->u, &c->v, &c->w, &c->up, &c->vp, &c->wp};
     for (double** a : fl)
         if (st == LBM_OK) st = dalloc(c, (void**)a, sizeof(double) * n);
@@ -231,8 +299,15 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
         g_create_err = c->err;
         return cleanup(st);
     }
+    k.didx = c->didx;
+    k.nodew = c->nodew;
     auto h2d = [&](void* d, const void* h, size_t b) { return hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, c->stream); };
-    if (h2d(c->node, node.data(), sizeof(uint32_t) * n) != hipSuccess) return cleanup(LBM_EDEVICE);
+    if (h2d(c->node, node.data(), sizeof(uint32_t) * n) != hipSuccess ||
+        h2d(c->cidx, cidx.data(), sizeof(uint32_t) * n) != hipSuccess ||
+        (nf > 0 && h2d(c->didx, didx.data(), sizeof(uint32_t) * nf) != hipSuccess) ||
+        (nf > 0 && h2d(c->nodew, nodew.data(), sizeof(uint32_t) * nf) != hipSuccess))
+        return cleanup(LBM_EDEVICE);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(LBM_EDEVICE);
     if (c->ubc) {
         std::vector<double> zero(n, 0.0);
         if (h2d(c->ubc, lat->u_bc ? lat->u_bc : zero.data(), 8 * n) != hipSuccess ||
@@ -252,7 +327,7 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
 void lbm_destroy(lbm_ctx* c) {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
-    void* bufs[] = {c->P[0], c->P[1], c->node, c->rho, c->u, c->v, c->w, c->up, c->vp, c->wp,
+    void* bufs[] = {c->Palloc[0], c->Palloc[1], c->node, c->cidx, c->didx, c->nodew, c->rho, c->u, c->v, c->w, c->up, c->vp, c->wp,
                     c->ubc, c->vbc, c->wbc, c->fout, c->partials, c->out4};
     for (void* b : bufs)
         if (b) hipFree(b);
@@ -276,7 +351,7 @@ lbm_status lbm_init(lbm_ctx* c, int mode, const double* rho, const double* u, co
         else HIPCHK(c, hipMemsetAsync(dst[a], 0, b, c->stream));
     }
     HIPCHK(c, launch_init(c->q, c->p.storage, c->p.collision, mode == 0 ? INIT_EQ : INIT_ZERO, c->kp,
-                          c->P[0], c->node, fields(c), c->stream));
+                          c->P[0], c->node, c->cidx, fields(c), c->stream));
     c->cur = 0;
     c->steps = 0;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -347,7 +422,7 @@ lbm_status lbm_get_f(lbm_ctx* c, double* f) {
     if (c->steps < 1) return fail(c, LBM_EINVAL, "post-boundary populations exist after the first step");
     lbm_status s = ensure_fout(c);
     if (s != LBM_OK) return s;
-    HIPCHK(c, launch_recon(c->q, c->p.storage, c->open, c->kp, c->P[c->cur ^ 1], c->node, fields(c), c->stream));
+    HIPCHK(c, launch_recon(c->q, c->p.storage, c->open, c->kp, c->P[c->cur ^ 1], c->node, c->cidx, fields(c), c->stream));
     HIPCHK(c, hipMemcpyAsync(f, c->fout, 8 * c->n * c->q, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return LBM_OK;
@@ -358,7 +433,7 @@ lbm_status lbm_set_f(lbm_ctx* c, const double* f) {
     lbm_status s = ensure_fout(c);
     if (s != LBM_OK) return s;
     HIPCHK(c, hipMemcpyAsync(c->fout, f, 8 * c->n * c->q, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, launch_init(c->q, c->p.storage, c->p.collision, INIT_FROM_F, c->kp, c->P[0], c->node, fields(c),
+    HIPCHK(c, launch_init(c->q, c->p.storage, c->p.collision, INIT_FROM_F, c->kp, c->P[0], c->node, c->cidx, fields(c),
                           c->stream));
     c->cur = 0;
     c->steps = 0;

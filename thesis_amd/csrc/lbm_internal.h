@@ -26,6 +26,12 @@ struct KParams {
     int inlet;               // 0 zou-he, 1 reference 2D
     uint32_t parity;
     double rho_out;
+    uint32_t nf;             // non-solid nodes (compact population count)
+    uint64_t mx, my;         // invariant-divisor multipliers for nx, ny
+    uint32_t sx, sy;
+    uint32_t nxcd;           // XCDs the workgroups are dealt over (1: no renumbering)
+    const uint32_t* didx;    // compact -> dense node index
+    const uint32_t* nodew;   // compact node words
 };
 
 struct Fields {              // FP64 side arrays (N each)
@@ -36,12 +42,12 @@ struct Fields {              // FP64 side arrays (N each)
 
 // Launchers; q in {9,19}; storage 0..3; collision 0..2.
 hipError_t launch_step(int q, int storage, int collision, bool open, const KParams& kp,
-                       const void* src, void* dst, const uint32_t* node, Fields fl,
+                       const void* src, void* dst, const uint32_t* node, const uint32_t* cidx, Fields fl,
                        bool write_macros, hipStream_t st);
 hipError_t launch_recon(int q, int storage, bool open, const KParams& kp, const void* src,
-                        const uint32_t* node, Fields fl, hipStream_t st);
+                        const uint32_t* node, const uint32_t* cidx, Fields fl, hipStream_t st);
 hipError_t launch_init(int q, int storage, int collision, int mode, const KParams& kp, void* dst,
-                       const uint32_t* node, Fields fl, hipStream_t st);
+                       const uint32_t* node, const uint32_t* cidx, Fields fl, hipStream_t st);
 // FP64 reductions over non-solid nodes: out4 (device) = {sum du^2, sum u^2, sum rho, nonfinite}.
 hipError_t launch_reduce(const KParams& kp, const uint32_t* node, Fields fl, double* uprev,
                          double* vprev, double* wprev, double* partials, int nblocks, double* out4,

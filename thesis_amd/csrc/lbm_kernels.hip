@@ -2,12 +2,17 @@
 //
 // One thread per lattice node, x fastest; a workgroup covers a run of x at a
 // fixed (y, z), so the (y, z) neighbour rows of the pull are wave-uniform.
-// Populations live in two SoA buffers P[q*qstride + node] (A/B).  A step
-// reads the post-collision populations of the previous step from A, pulls
-// them along the links (stream), applies walls and open boundaries, takes the
-// moments, collides and writes B: the reference's collide -> stream -> BC ->
-// macro sequence (S/Iterate.cu:385-520, S/Iterate3D.cu:493-653) rotated so
-// that one node's whole update stays in registers.
+//
+// Populations of the non-solid nodes only are stored, in lattice order, as SoA
+// planes P[q*qstride + j] (j = compact index of the node), in two buffers
+// (A/B).  A dense map cidx[node] -> j (all-ones for solid nodes, 4 B per node,
+// small enough to stay in the last-level cache) turns the pull into a gather.
+// A step reads the post-collision populations of the previous step from A,
+// pulls them along the links (stream), applies walls and open boundaries, takes
+// the moments, collides and writes B: the reference's collide -> stream -> BC
+// -> macro sequence (S/Iterate.cu:385-520, S/Iterate3D.cu:493-653) rotated so
+// that one node's whole update stays in registers.  Solid nodes cost no
+// population traffic.
 //
 // Populations are held as deviations h_i = f_i - w_i (rest density 1) both
 // in storage and in the arithmetic, so FP32 and 16-bit storage keep the
@@ -16,6 +21,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <utility>
 
 namespace lbm {
 
@@ -89,6 +95,33 @@ static_assert(mrest<D2Q9>(2) > 0.9999999 && mrest<D2Q9>(2) < 1.0000001, "D2Q9 re
 static_assert(mrest<D3Q19>(1) > -11.0000001 && mrest<D3Q19>(1) < -10.9999999, "D3Q19 rest e");
 static_assert(mrest<D3Q19>(2) > 2.9999999 && mrest<D3Q19>(2) < 3.0000001, "D3Q19 rest eps");
 
+// Compile-time coefficients of M and Minv; the products below are expanded by
+// fold expressions so every zero drops out and every +-1 becomes an add.
+template <class L, int R, int I> struct Mc {
+    static constexpr double m = mrow<L>(R, I);
+    static constexpr double mi = mrow<L>(R, I) / mnorm2<L>(R);
+};
+template <class L, class T, int R, int... I>
+__device__ inline T m_row_dot(const T* h, std::integer_sequence<int, I...>) {
+    T acc = 0;
+    ((Mc<L, R, I>::m != 0.0 ? (void)(acc += (T)Mc<L, R, I>::m * h[I]) : (void)0), ...);
+    return acc;
+}
+template <class L, class T, int... R>
+__device__ inline void m_apply(const T* h, T* m, std::integer_sequence<int, R...>) {
+    ((m[R] = m_row_dot<L, T, R>(h, std::make_integer_sequence<int, L::Q>{})), ...);
+}
+template <class L, class T, int I, int... R>
+__device__ inline T mi_col_dot(const T* m, std::integer_sequence<int, R...>) {
+    T acc = 0;
+    ((Mc<L, R, I>::mi != 0.0 ? (void)(acc += (T)Mc<L, R, I>::mi * m[R]) : (void)0), ...);
+    return acc;
+}
+template <class L, class T, int... I>
+__device__ inline void mi_apply(const T* m, T* out, std::integer_sequence<int, I...>) {
+    ((out[I] = mi_col_dot<L, T, I>(m, std::make_integer_sequence<int, L::Q>{})), ...);
+}
+
 // ---------------------------------------------------------------------------
 // storage: deviations h_i, raw in F64/F32; scaled by 1/(w_i a) in 16 bits
 // FP16: stored value = h / w_i * 2^10 keeps typical deviations in the normal range.
@@ -107,16 +140,22 @@ __device__ inline uint16_t f32_to_bf16(float f) {   // round to nearest even
 template <class L, int S> struct Io;
 template <class L> struct Io<L, 0> {
     using raw = double; using T = double;
+    __device__ static T cv(raw r, int) { return r; }
+    __device__ static raw pk(T v, int) { return v; }
     __device__ static T ld(const raw* p, size_t k, int) { return p[k]; }
     __device__ static void st(raw* p, size_t k, T v, int) { __builtin_nontemporal_store(v, p + k); }
 };
 template <class L> struct Io<L, 1> {
     using raw = float; using T = float;
+    __device__ static T cv(raw r, int) { return r; }
+    __device__ static raw pk(T v, int) { return v; }
     __device__ static T ld(const raw* p, size_t k, int) { return p[k]; }
     __device__ static void st(raw* p, size_t k, T v, int) { __builtin_nontemporal_store(v, p + k); }
 };
 template <class L> struct Io<L, 2> {
     using raw = uint16_t; using T = float;
+    __device__ static T cv(raw r, int i) { return (float)__builtin_bit_cast(_Float16, r) * f16_a<L>(i); }
+    __device__ static raw pk(T v, int i) { return __builtin_bit_cast(uint16_t, (_Float16)(v * f16_ainv<L>(i))); }
     __device__ static T ld(const raw* p, size_t k, int i) {
         _Float16 h = __builtin_bit_cast(_Float16, p[k]);
         return (float)h * f16_a<L>(i);
@@ -128,6 +167,8 @@ template <class L> struct Io<L, 2> {
 };
 template <class L> struct Io<L, 3> {
     using raw = uint16_t; using T = float;
+    __device__ static T cv(raw r, int i) { return bf16_to_f32(r) * bf16_a<L>(i); }
+    __device__ static raw pk(T v, int i) { return f32_to_bf16(v * bf16_ainv<L>(i)); }
     __device__ static T ld(const raw* p, size_t k, int i) { return bf16_to_f32(p[k]) * bf16_a<L>(i); }
     __device__ static void st(raw* p, size_t k, T v, int i) {
         __builtin_nontemporal_store(f32_to_bf16(v * bf16_ainv<L>(i)), p + k);
@@ -137,14 +178,15 @@ template <class L> struct Io<L, 3> {
 // ---------------------------------------------------------------------------
 // node pipeline pieces (T = double for F64 storage, float otherwise)
 
-// Pull stream with walls (S/GpuStream.cu:8-28, :160-194; half-way or on-node
-// bounce-back per S/GpuBoundaries.cu:400-490) and periodic wrap.
-template <class L, int S, bool OPEN>
-__device__ inline void gather(const typename Io<L, S>::raw* __restrict__ src, const KParams& kp,
-                              int x, int y, int z, size_t idx, uint32_t nd,
-                              typename Io<L, S>::T* h) {
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// Stage 1 of the pull: compact indices of the source node of every direction
+// (periodic wrap; kNone marks a source outside a non-periodic face).  These
+// loads do not depend on the node's own word, so they issue together.
+template <class L, bool OPEN>
+__device__ inline void source_index(const uint32_t* __restrict__ cidx, const KParams& kp, int x, int y, int z,
+                                    uint32_t* cn) {
     const int nx = kp.nx, ny = kp.ny, nz = kp.nz;
-    const size_t qs = kp.qstride;
     const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
     const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
     const int zm = z == 0 ? nz - 1 : z - 1, zp = z == nz - 1 ? 0 : z + 1;
@@ -155,22 +197,40 @@ __device__ inline void gather(const typename Io<L, S>::raw* __restrict__ src, co
         zm_o = !kp.per[2] && z == 0;  zp_o = !kp.per[2] && z == nz - 1;
     }
 #pragma unroll
-    for (int i = 0; i < L::Q; ++i) {
+    for (int i = 1; i < L::Q; ++i) {
         const int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
         const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
         const int sy = cy == 0 ? y : (cy > 0 ? ym : yp);
         const int sz = cz == 0 ? z : (cz > 0 ? zm : zp);
-        size_t a = (size_t)i * qs + ((size_t)sz * ny + sy) * nx + sx;
-        if (OPEN) {
-            const bool out = (cx > 0 && xm_o) || (cx < 0 && xp_o) || (cy > 0 && ym_o) ||
-                             (cy < 0 && yp_o) || (cz > 0 && zm_o) || (cz < 0 && zp_o);
-            if (out) a = (size_t)i * qs + idx;   // not streamed: keeps its own value
+        bool out = false;
+        if (OPEN)
+            out = (cx > 0 && xm_o) || (cx < 0 && xp_o) || (cy > 0 && ym_o) || (cy < 0 && yp_o) ||
+                  (cz > 0 && zm_o) || (cz < 0 && zp_o);
+        cn[i] = out ? kNone : cidx[((size_t)sz * ny + sy) * nx + sx];
+    }
+}
+
+// Stage 2: the populations.  A link whose source lies beyond a wall takes the
+// node's own value, P_opp(i) (half-way bounce-back) or P_i (on-node, fixed
+// afterwards by onnode_walls) (S/GpuStream.cu:8-28, :160-194;
+// S/GpuBoundaries.cu:400-490); a source outside a non-periodic face keeps the
+// node's own P_i (the reference's "stream not allowed"), to be replaced by
+// the open-boundary condition.
+template <class L, int S, bool OPEN>
+__device__ inline void gather(const typename Io<L, S>::raw* __restrict__ src, const KParams& kp,
+                              uint32_t own, uint32_t nd, const uint32_t* cn, typename Io<L, S>::T* h) {
+    const size_t qs = kp.qstride;
+    h[0] = Io<L, S>::ld(src, own, 0);
+#pragma unroll
+    for (int i = 1; i < L::Q; ++i) {
+        const int o = L::opp(i);
+        uint32_t j = cn[i];
+        int pl = i;
+        if (((nd >> o) & 1u) || j == kNone) {
+            j = own;
+            pl = ((nd >> o) & 1u) && kp.wall == 0 ? o : i;
         }
-        if (i != 0) {
-            const int o = L::opp(i);
-            if ((nd >> o) & 1u) a = (size_t)(kp.wall == 0 ? o : i) * qs + idx;
-        }
-        h[i] = Io<L, S>::ld(src, a, i);
+        h[i] = Io<L, S>::ld(src, (size_t)pl * qs + j, i);
     }
 }
 
@@ -294,32 +354,21 @@ __device__ inline void collide(const KParams& kp, const T* h, T drho, T rho, con
     const bool guo = kp.force == 2;
     if (guo) guo_terms<L, T>(kp, rho, u, S);
     if (C == 2) {
-        T m[L::Q], dm[L::Q];
+        T m[L::Q], dm[L::Q], gs[L::Q], dd[L::Q];
         mrt_dmeq<L, T>(kp, drho, rho, u, dm);
+        m_apply<L, T>(h, m, std::make_integer_sequence<int, L::Q>{});
+        if (guo) m_apply<L, T>(S, gs, std::make_integer_sequence<int, L::Q>{});
 #pragma unroll
         for (int r = 0; r < L::Q; ++r) {
-            T acc = 0, gs = 0;
-#pragma unroll
-            for (int i = 0; i < L::Q; ++i) {
-                const double c = mrow<L>(r, i);
-                if (c != 0.0) { acc += (T)c * h[i]; if (guo) gs += (T)c * S[i]; }
-            }
             const T s = (T)kp.s[r];
-            m[r] = s * (acc - dm[r]);
-            if (guo) m[r] -= ((T)1 - (T)0.5 * s) * gs;
+            m[r] = s * (m[r] - dm[r]);
+            if (guo) m[r] -= ((T)1 - (T)0.5 * s) * gs[r];
         }
-        const bool nof18 = L::Q == 19 && (kp.parity & 0x2u);
+        mi_apply<L, T>(m, dd, std::make_integer_sequence<int, L::Q>{});
 #pragma unroll
-        for (int i = 0; i < L::Q; ++i) {
-            T acc = 0;
-#pragma unroll
-            for (int r = 0; r < L::Q; ++r) {
-                const double c = mrow<L>(r, i);
-                if (c != 0.0) acc += (T)(c / mnorm2<L>(r)) * m[r];
-            }
-            out[i] = h[i] - acc;
-        }
-        if (nof18) out[L::Q - 1] = -(T)L::w(L::Q - 1);   // fColl_18 stays at its zero initial value
+        for (int i = 0; i < L::Q; ++i) out[i] = h[i] - dd[i];
+        if (L::Q == 19 && (kp.parity & 0x2u))
+            out[L::Q - 1] = -(T)L::w(L::Q - 1);   // fColl_18 stays at its zero initial value
         return;
     }
     const T uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
@@ -362,21 +411,44 @@ __device__ inline void collide(const KParams& kp, const T* h, T drho, T rho, con
 
 // ---------------------------------------------------------------------------
 // kernels
+// Quotient of a 32-bit n by an invariant divisor d: q = (((m n) >> 32) + n) >> s
+// with m = floor(2^32 (2^s - d) / d) + 1, s = ceil(log2 d) (exact for all n < 2^32).
+__device__ inline uint32_t fdiv(uint32_t n, uint64_t m, uint32_t s) {
+    return (uint32_t)((((m * n) >> 32) + n) >> s);
+}
+
+// Workgroups are dealt round-robin to the XCDs; renumber them so that each XCD
+// runs one contiguous range of the lattice and neighbouring rows (and their
+// cache lines of cidx and P) meet in the same L2.  Bijective for any count.
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb, uint32_t nxcd) {
+    if (nxcd <= 1) return b;
+    const uint32_t q = nb / nxcd, r = nb % nxcd, x = b % nxcd, k = b / nxcd;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+// One thread per non-solid node, in compact order: reads of the node's own
+// data and all writes are contiguous and aligned across a wave.
 template <class L, int S, int C, bool OPEN>
 __global__ __launch_bounds__(256) void k_step(const typename Io<L, S>::raw* __restrict__ src,
                                               typename Io<L, S>::raw* __restrict__ dst,
-                                              const uint32_t* __restrict__ node, KParams kp,
+                                              const uint32_t* __restrict__ didx,
+                                              const uint32_t* __restrict__ nodew,
+                                              const uint32_t* __restrict__ cidx, KParams kp,
                                               Fields fl, int write_macros) {
     using T = typename Io<L, S>::T;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y, z = blockIdx.z;
-    if (x >= kp.nx) return;
-    const size_t idx = ((size_t)z * kp.ny + y) * kp.nx + x;
-    const uint32_t nd = node[idx];
+    const uint32_t own = xcd_block(blockIdx.x, gridDim.x, kp.nxcd) * blockDim.x + threadIdx.x;
+    if (own >= kp.nf) return;
+    const uint32_t idx = didx[own];
+    const uint32_t nd = nodew[own];
+    const uint32_t q1 = fdiv(idx, kp.mx, kp.sx);
+    const int x = (int)(idx - q1 * (uint32_t)kp.nx);
+    const uint32_t zq = fdiv(q1, kp.my, kp.sy);
+    const int y = (int)(q1 - zq * (uint32_t)kp.ny), z = (int)zq;
+    uint32_t cn[L::Q];
+    source_index<L, OPEN>(cidx, kp, x, y, z, cn);
     const uint32_t cls = nd >> kClsShift;
-    if (cls == SOLID) return;
     T h[L::Q];
-    gather<L, S, OPEN>(src, kp, x, y, z, idx, nd, h);
+    gather<L, S, OPEN>(src, kp, own, nd, cn, h);
     if (OPEN && cls == VEL_W) zou_inlet<L, T>(kp, fl, idx, nd, h);
     if (kp.wall == 1 && (nd & kBBMask)) onnode_walls<L, T>(nd, h);
     if (OPEN && cls == PRES_E) zou_outlet<L, T>(kp, h);
@@ -389,28 +461,32 @@ __global__ __launch_bounds__(256) void k_step(const typename Io<L, S>::raw* __re
     T out[L::Q];
     collide<L, T, C>(kp, h, drho, rho, u, out);
 #pragma unroll
-    for (int i = 0; i < L::Q; ++i) Io<L, S>::st(dst, (size_t)i * kp.qstride + idx, out[i], i);
+    for (int i = 0; i < L::Q; ++i) Io<L, S>::st(dst, (size_t)i * kp.qstride + own, out[i], i);
 }
 
-// Post-boundary populations and moments of the state held in src.
+// Post-boundary populations (dense, full, FP64) and moments of the state in src.
 template <class L, int S, bool OPEN>
 __global__ __launch_bounds__(256) void k_recon(const typename Io<L, S>::raw* __restrict__ src,
-                                               const uint32_t* __restrict__ node, KParams kp, Fields fl) {
+                                               const uint32_t* __restrict__ node,
+                                               const uint32_t* __restrict__ cidx, KParams kp, Fields fl) {
     using T = typename Io<L, S>::T;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y, z = blockIdx.z;
     if (x >= kp.nx) return;
     const size_t n = (size_t)kp.nx * kp.ny * kp.nz;
     const size_t idx = ((size_t)z * kp.ny + y) * kp.nx + x;
-    const uint32_t nd = node[idx];
-    const uint32_t cls = nd >> kClsShift;
-    if (cls == SOLID) {
+    const uint32_t own = cidx[idx];
+    if (own == kNone) {
 #pragma unroll
         for (int i = 0; i < L::Q; ++i) fl.fout[i * n + idx] = 0.0;
         return;
     }
+    const uint32_t nd = node[idx];
+    const uint32_t cls = nd >> kClsShift;
+    uint32_t cn[L::Q];
+    source_index<L, OPEN>(cidx, kp, x, y, z, cn);
     T h[L::Q];
-    gather<L, S, OPEN>(src, kp, x, y, z, idx, nd, h);
+    gather<L, S, OPEN>(src, kp, own, nd, cn, h);
     if (OPEN && cls == VEL_W) zou_inlet<L, T>(kp, fl, idx, nd, h);
     if (kp.wall == 1 && (nd & kBBMask)) onnode_walls<L, T>(nd, h);
     if (OPEN && cls == PRES_E) zou_outlet<L, T>(kp, h);
@@ -425,7 +501,8 @@ __global__ __launch_bounds__(256) void k_recon(const typename Io<L, S>::raw* __r
 // First post-collision state from an initial f and macroscopic fields.
 template <class L, int S, int C>
 __global__ __launch_bounds__(256) void k_init(typename Io<L, S>::raw* __restrict__ dst,
-                                              const uint32_t* __restrict__ node, KParams kp,
+                                              const uint32_t* __restrict__ node,
+                                              const uint32_t* __restrict__ cidx, KParams kp,
                                               Fields fl, int mode) {
     using T = typename Io<L, S>::T;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -433,13 +510,10 @@ __global__ __launch_bounds__(256) void k_init(typename Io<L, S>::raw* __restrict
     if (x >= kp.nx) return;
     const size_t n = (size_t)kp.nx * kp.ny * kp.nz;
     const size_t idx = ((size_t)z * kp.ny + y) * kp.nx + x;
+    const uint32_t own = cidx[idx];
+    if (own == kNone) return;
     const uint32_t nd = node[idx];
     const uint32_t cls = nd >> kClsShift;
-    if (cls == SOLID) {
-#pragma unroll
-        for (int i = 0; i < L::Q; ++i) Io<L, S>::st(dst, (size_t)i * kp.qstride + idx, (T)0, i);
-        return;
-    }
     T h[L::Q], drho, rho, u[3];
     if (mode == INIT_FROM_F) {
 #pragma unroll
@@ -466,7 +540,7 @@ __global__ __launch_bounds__(256) void k_init(typename Io<L, S>::raw* __restrict
     T out[L::Q];
     collide<L, T, C>(kp, h, drho, rho, u, out);
 #pragma unroll
-    for (int i = 0; i < L::Q; ++i) Io<L, S>::st(dst, (size_t)i * kp.qstride + idx, out[i], i);
+    for (int i = 0; i < L::Q; ++i) Io<L, S>::st(dst, (size_t)i * kp.qstride + own, out[i], i);
 }
 
 // FP64 reduction: per-block partials (wave shuffles, then LDS), then one block
@@ -535,99 +609,101 @@ static dim3 node_grid(const KParams& kp) {
 }
 
 template <class L, int S, int C, bool OPEN>
-static hipError_t step_t(const KParams& kp, const void* src, void* dst, const uint32_t* node, Fields fl,
-                         bool wm, hipStream_t st) {
+static hipError_t step_t(const KParams& kp, const void* src, void* dst, const uint32_t* node, const uint32_t* cidx,
+                         Fields fl, bool wm, hipStream_t st) {
     using raw = typename Io<L, S>::raw;
-    hipLaunchKernelGGL((k_step<L, S, C, OPEN>), node_grid(kp), node_block(kp), 0, st,
-                       (const raw*)src, (raw*)dst, node, kp, fl, wm ? 1 : 0);
+    const unsigned blocks = (unsigned)((kp.nf + 255) / 256);
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_step<L, S, C, OPEN>), dim3(blocks), dim3(256), 0, st, (const raw*)src, (raw*)dst,
+                       kp.didx, kp.nodew, cidx, kp, fl, wm ? 1 : 0);
     return hipGetLastError();
 }
 template <class L, int S, int C>
-static hipError_t step_o(bool open, const KParams& kp, const void* src, void* dst, const uint32_t* node,
+static hipError_t step_o(bool open, const KParams& kp, const void* src, void* dst, const uint32_t* node, const uint32_t* cidx,
                          Fields fl, bool wm, hipStream_t st) {
-    return open ? step_t<L, S, C, true>(kp, src, dst, node, fl, wm, st)
-                : step_t<L, S, C, false>(kp, src, dst, node, fl, wm, st);
+    return open ? step_t<L, S, C, true>(kp, src, dst, node, cidx, fl, wm, st)
+                : step_t<L, S, C, false>(kp, src, dst, node, cidx, fl, wm, st);
 }
 template <class L, int S>
 static hipError_t step_c(int c, bool open, const KParams& kp, const void* src, void* dst,
-                         const uint32_t* node, Fields fl, bool wm, hipStream_t st) {
+                         const uint32_t* node, const uint32_t* cidx, Fields fl, bool wm, hipStream_t st) {
     switch (c) {
-    case 0: return step_o<L, S, 0>(open, kp, src, dst, node, fl, wm, st);
-    case 1: return step_o<L, S, 1>(open, kp, src, dst, node, fl, wm, st);
-    default: return step_o<L, S, 2>(open, kp, src, dst, node, fl, wm, st);
+    case 0: return step_o<L, S, 0>(open, kp, src, dst, node, cidx, fl, wm, st);
+    case 1: return step_o<L, S, 1>(open, kp, src, dst, node, cidx, fl, wm, st);
+    default: return step_o<L, S, 2>(open, kp, src, dst, node, cidx, fl, wm, st);
     }
 }
 template <class L>
 static hipError_t step_s(int s, int c, bool open, const KParams& kp, const void* src, void* dst,
-                         const uint32_t* node, Fields fl, bool wm, hipStream_t st) {
+                         const uint32_t* node, const uint32_t* cidx, Fields fl, bool wm, hipStream_t st) {
     switch (s) {
-    case 0: return step_c<L, 0>(c, open, kp, src, dst, node, fl, wm, st);
-    case 1: return step_c<L, 1>(c, open, kp, src, dst, node, fl, wm, st);
-    case 2: return step_c<L, 2>(c, open, kp, src, dst, node, fl, wm, st);
-    default: return step_c<L, 3>(c, open, kp, src, dst, node, fl, wm, st);
+    case 0: return step_c<L, 0>(c, open, kp, src, dst, node, cidx, fl, wm, st);
+    case 1: return step_c<L, 1>(c, open, kp, src, dst, node, cidx, fl, wm, st);
+    case 2: return step_c<L, 2>(c, open, kp, src, dst, node, cidx, fl, wm, st);
+    default: return step_c<L, 3>(c, open, kp, src, dst, node, cidx, fl, wm, st);
     }
 }
 hipError_t launch_step(int q, int storage, int collision, bool open, const KParams& kp, const void* src,
-                       void* dst, const uint32_t* node, Fields fl, bool wm, hipStream_t st) {
-    return q == 9 ? step_s<D2Q9>(storage, collision, open, kp, src, dst, node, fl, wm, st)
-                  : step_s<D3Q19>(storage, collision, open, kp, src, dst, node, fl, wm, st);
+                       void* dst, const uint32_t* node, const uint32_t* cidx, Fields fl, bool wm, hipStream_t st) {
+    return q == 9 ? step_s<D2Q9>(storage, collision, open, kp, src, dst, node, cidx, fl, wm, st)
+                  : step_s<D3Q19>(storage, collision, open, kp, src, dst, node, cidx, fl, wm, st);
 }
 
 template <class L, int S>
-static hipError_t recon_t(bool open, const KParams& kp, const void* src, const uint32_t* node, Fields fl,
+static hipError_t recon_t(bool open, const KParams& kp, const void* src, const uint32_t* node, const uint32_t* cidx, Fields fl,
                           hipStream_t st) {
     using raw = typename Io<L, S>::raw;
     if (open)
-        hipLaunchKernelGGL((k_recon<L, S, true>), node_grid(kp), node_block(kp), 0, st, (const raw*)src, node, kp, fl);
+        hipLaunchKernelGGL((k_recon<L, S, true>), node_grid(kp), node_block(kp), 0, st, (const raw*)src, node, cidx, kp, fl);
     else
-        hipLaunchKernelGGL((k_recon<L, S, false>), node_grid(kp), node_block(kp), 0, st, (const raw*)src, node, kp, fl);
+        hipLaunchKernelGGL((k_recon<L, S, false>), node_grid(kp), node_block(kp), 0, st, (const raw*)src, node, cidx, kp, fl);
     return hipGetLastError();
 }
 template <class L>
 static hipError_t recon_s(int s, bool open, const KParams& kp, const void* src, const uint32_t* node,
-                          Fields fl, hipStream_t st) {
+                          const uint32_t* cidx, Fields fl, hipStream_t st) {
     switch (s) {
-    case 0: return recon_t<L, 0>(open, kp, src, node, fl, st);
-    case 1: return recon_t<L, 1>(open, kp, src, node, fl, st);
-    case 2: return recon_t<L, 2>(open, kp, src, node, fl, st);
-    default: return recon_t<L, 3>(open, kp, src, node, fl, st);
+    case 0: return recon_t<L, 0>(open, kp, src, node, cidx, fl, st);
+    case 1: return recon_t<L, 1>(open, kp, src, node, cidx, fl, st);
+    case 2: return recon_t<L, 2>(open, kp, src, node, cidx, fl, st);
+    default: return recon_t<L, 3>(open, kp, src, node, cidx, fl, st);
     }
 }
 hipError_t launch_recon(int q, int storage, bool open, const KParams& kp, const void* src, const uint32_t* node,
-                        Fields fl, hipStream_t st) {
-    return q == 9 ? recon_s<D2Q9>(storage, open, kp, src, node, fl, st)
-                  : recon_s<D3Q19>(storage, open, kp, src, node, fl, st);
+                        const uint32_t* cidx, Fields fl, hipStream_t st) {
+    return q == 9 ? recon_s<D2Q9>(storage, open, kp, src, node, cidx, fl, st)
+                  : recon_s<D3Q19>(storage, open, kp, src, node, cidx, fl, st);
 }
 
 template <class L, int S, int C>
-static hipError_t init_t(int mode, const KParams& kp, void* dst, const uint32_t* node, Fields fl, hipStream_t st) {
+static hipError_t init_t(int mode, const KParams& kp, void* dst, const uint32_t* node, const uint32_t* cidx, Fields fl, hipStream_t st) {
     using raw = typename Io<L, S>::raw;
-    hipLaunchKernelGGL((k_init<L, S, C>), node_grid(kp), node_block(kp), 0, st, (raw*)dst, node, kp, fl, mode);
+    hipLaunchKernelGGL((k_init<L, S, C>), node_grid(kp), node_block(kp), 0, st, (raw*)dst, node, cidx, kp, fl, mode);
     return hipGetLastError();
 }
 template <class L, int S>
-static hipError_t init_c(int c, int mode, const KParams& kp, void* dst, const uint32_t* node, Fields fl,
+static hipError_t init_c(int c, int mode, const KParams& kp, void* dst, const uint32_t* node, const uint32_t* cidx, Fields fl,
                          hipStream_t st) {
     switch (c) {
-    case 0: return init_t<L, S, 0>(mode, kp, dst, node, fl, st);
-    case 1: return init_t<L, S, 1>(mode, kp, dst, node, fl, st);
-    default: return init_t<L, S, 2>(mode, kp, dst, node, fl, st);
+    case 0: return init_t<L, S, 0>(mode, kp, dst, node, cidx, fl, st);
+    case 1: return init_t<L, S, 1>(mode, kp, dst, node, cidx, fl, st);
+    default: return init_t<L, S, 2>(mode, kp, dst, node, cidx, fl, st);
     }
 }
 template <class L>
-static hipError_t init_s(int s, int c, int mode, const KParams& kp, void* dst, const uint32_t* node, Fields fl,
+static hipError_t init_s(int s, int c, int mode, const KParams& kp, void* dst, const uint32_t* node, const uint32_t* cidx, Fields fl,
                          hipStream_t st) {
     switch (s) {
-    case 0: return init_c<L, 0>(c, mode, kp, dst, node, fl, st);
-    case 1: return init_c<L, 1>(c, mode, kp, dst, node, fl, st);
-    case 2: return init_c<L, 2>(c, mode, kp, dst, node, fl, st);
-    default: return init_c<L, 3>(c, mode, kp, dst, node, fl, st);
+    case 0: return init_c<L, 0>(c, mode, kp, dst, node, cidx, fl, st);
+    case 1: return init_c<L, 1>(c, mode, kp, dst, node, cidx, fl, st);
+    case 2: return init_c<L, 2>(c, mode, kp, dst, node, cidx, fl, st);
+    default: return init_c<L, 3>(c, mode, kp, dst, node, cidx, fl, st);
     }
 }
 hipError_t launch_init(int q, int storage, int collision, int mode, const KParams& kp, void* dst,
-                       const uint32_t* node, Fields fl, hipStream_t st) {
-    return q == 9 ? init_s<D2Q9>(storage, collision, mode, kp, dst, node, fl, st)
-                  : init_s<D3Q19>(storage, collision, mode, kp, dst, node, fl, st);
+                       const uint32_t* node, const uint32_t* cidx, Fields fl, hipStream_t st) {
+    return q == 9 ? init_s<D2Q9>(storage, collision, mode, kp, dst, node, cidx, fl, st)
+                  : init_s<D3Q19>(storage, collision, mode, kp, dst, node, cidx, fl, st);
 }
 
 hipError_t launch_reduce(const KParams& kp, const uint32_t* node, Fields fl, double* uprev, double* vprev,
