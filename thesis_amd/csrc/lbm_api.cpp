@@ -86,12 +86,15 @@ Fields fields(lbm_ctx* c) {
 
 size_t storage_bytes(int s) { return s == LBM_STORE_F64 ? 8 : s == LBM_STORE_F32 ? 4 : 2; }
 
-// Population plane stride: rounded to 256 B, plus an offset of one 4 KiB page
-// and 256 B so that the Q planes of one node start on different HBM channels.
+// Population plane stride: whole 64-node tiles, rounded to 256 B, plus an offset
+// of one 4 KiB page and 256 B so that the Q planes of one node start on
+// different HBM channels.
 size_t plane_stride(size_t n, size_t eb) {
     size_t pad_bytes = 4096 + 256;
     if (const char* e = std::getenv("LBM_QPAD_BYTES")) pad_bytes = (size_t)std::strtoull(e, nullptr, 10);
-    size_t per = 256 / eb;
+    // whole 64-node tiles (the tiled layout addresses ceil(n/64)*64*Q elements)
+    // and at least 256 B alignment of every plane
+    size_t per = 256 / eb > 64 ? 256 / eb : 64;
     size_t s = (n + per - 1) / per * per;
     return s + (pad_bytes + eb - 1) / eb;
 }
@@ -242,7 +245,9 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     };
     magic((uint32_t)p->nx, k.mx, k.sx);
     magic((uint32_t)p->ny, k.my, k.sy);
-    k.nxcd = 8;   // MI355X: 8 XCDs
+    k.nxcd = 1;   // renumbering for the 8 XCDs measured slower with the tiled layout
+    k.tile = 1;
+    if (const char* e = std::getenv("LBM_LAYOUT")) k.tile = std::strcmp(e, "soa") == 0 ? 0 : 1;
     if (const char* e = std::getenv("LBM_XCD")) k.nxcd = (uint32_t)std::strtoul(e, nullptr, 10);
     k.omega = p->omega;
     const double lam = p->trt_magic > 0 ? p->trt_magic : 3.0 / 16.0;

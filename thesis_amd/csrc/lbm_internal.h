@@ -30,6 +30,7 @@ struct KParams {
     uint64_t mx, my;         // invariant-divisor multipliers for nx, ny
     uint32_t sx, sy;
     uint32_t nxcd;           // XCDs the workgroups are dealt over (1: no renumbering)
+    int tile;                // population layout: 0 SoA planes, 1 tiles of 64 nodes x Q
     const uint32_t* didx;    // compact -> dense node index
     const uint32_t* nodew;   // compact node words
 };

@@ -4,7 +4,7 @@
 // fixed (y, z), so the (y, z) neighbour rows of the pull are wave-uniform.
 //
 // Populations of the non-solid nodes only are stored, in lattice order, as SoA
-// planes P[q*qstride + j] (j = compact index of the node), in two buffers
+// planes P[q*qstride + j] or 64-node tiles (j = compact index of the node), in two buffers
 // (A/B).  A dense map cidx[node] -> j (all-ones for solid nodes, 4 B per node,
 // small enough to stay in the last-level cache) turns the pull into a gather.
 // A step reads the post-collision populations of the previous step from A,
@@ -216,11 +216,20 @@ __device__ inline void source_index(const uint32_t* __restrict__ cidx, const KPa
 // S/GpuBoundaries.cu:400-490); a source outside a non-periodic face keeps the
 // node's own P_i (the reference's "stream not allowed"), to be replaced by
 // the open-boundary condition.
+// Element offset of population i of compact node j: SoA planes
+// P[i*qstride + j], or tiles of 64 nodes holding all Q planes
+// P[(j/64)*64*Q + i*64 + j%64], so one wave's populations sit in one
+// contiguous 64*Q-element block.
+template <class L>
+__device__ inline size_t paddr(const KParams& kp, int i, uint32_t j) {
+    if (kp.tile) return (size_t)(j >> 6) * (64 * L::Q) + (size_t)i * 64 + (j & 63u);
+    return (size_t)i * kp.qstride + j;
+}
+
 template <class L, int S, bool OPEN>
 __device__ inline void gather(const typename Io<L, S>::raw* __restrict__ src, const KParams& kp,
                               uint32_t own, uint32_t nd, const uint32_t* cn, typename Io<L, S>::T* h) {
-    const size_t qs = kp.qstride;
-    h[0] = Io<L, S>::ld(src, own, 0);
+    h[0] = Io<L, S>::ld(src, paddr<L>(kp, 0, own), 0);
 #pragma unroll
     for (int i = 1; i < L::Q; ++i) {
         const int o = L::opp(i);
@@ -230,7 +239,7 @@ __device__ inline void gather(const typename Io<L, S>::raw* __restrict__ src, co
             j = own;
             pl = ((nd >> o) & 1u) && kp.wall == 0 ? o : i;
         }
-        h[i] = Io<L, S>::ld(src, (size_t)pl * qs + j, i);
+        h[i] = Io<L, S>::ld(src, paddr<L>(kp, pl, j), i);
     }
 }
 
@@ -461,7 +470,7 @@ __global__ __launch_bounds__(256) void k_step(const typename Io<L, S>::raw* __re
     T out[L::Q];
     collide<L, T, C>(kp, h, drho, rho, u, out);
 #pragma unroll
-    for (int i = 0; i < L::Q; ++i) Io<L, S>::st(dst, (size_t)i * kp.qstride + own, out[i], i);
+    for (int i = 0; i < L::Q; ++i) Io<L, S>::st(dst, paddr<L>(kp, i, own), out[i], i);
 }
 
 // Post-boundary populations (dense, full, FP64) and moments of the state in src.
@@ -540,7 +549,7 @@ __global__ __launch_bounds__(256) void k_init(typename Io<L, S>::raw* __restrict
     T out[L::Q];
     collide<L, T, C>(kp, h, drho, rho, u, out);
 #pragma unroll
-    for (int i = 0; i < L::Q; ++i) Io<L, S>::st(dst, (size_t)i * kp.qstride + own, out[i], i);
+    for (int i = 0; i < L::Q; ++i) Io<L, S>::st(dst, paddr<L>(kp, i, own), out[i], i);
 }
 
 // FP64 reduction: per-block partials (wave shuffles, then LDS), then one block
