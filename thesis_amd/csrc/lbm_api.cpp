@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -36,6 +37,8 @@ struct lbm_ctx {
     uint32_t* cidx = nullptr;     // dense node -> compact population index (all ones: solid)
     uint32_t* didx = nullptr;     // compact -> dense
     uint32_t* nodew = nullptr;    // compact node words
+    uint32_t* rowmeta = nullptr;  // per row and word: fluid mask, first compact index
+    uint32_t* blk = nullptr;      // per workgroup: first row, rows spanned
     size_t nf = 0;                // non-solid nodes
     double *rho = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
     double *up = nullptr, *vp = nullptr, *wp = nullptr;
@@ -171,6 +174,7 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
         const uint32_t bb = lat->bb_links ? (lat->bb_links[i] & mask) : 0u;
         node[i] = (cls << kClsShift) | bb;
     }
+    bool simple = p->wall_model == LBM_WALL_HALFWAY;
     // A link into a solid node is a wall: complete the bounce-back bits from the
     // node classes (periodic axes wrap; links leaving a non-periodic box are
     // left to the lattice's own WALL links or the open boundaries).
@@ -185,6 +189,8 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
         const int per[3] = {p->periodic[0] != 0, p->periodic[1] != 0, p->q == 9 ? 1 : p->periodic[2] != 0};
         for (size_t i = 0; i < n; ++i) {
             if (lat->node_class[i] == LBM_SOLID) continue;
+            const uint32_t user = node[i] & kBBMask;
+            uint32_t derived = 0;
             const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((size_t)nx * ny));
             for (int d = 1; d < p->q; ++d) {
                 const int* c = p->q == 9 ? C9[d] : C19[d];
@@ -199,8 +205,10 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
                 }
                 if (out) continue;
                 const size_t j = ((size_t)xyz[2] * ny + xyz[1]) * nx + xyz[0];
-                if (lat->node_class[j] == LBM_SOLID) node[i] |= 1u << d;
+                if (lat->node_class[j] == LBM_SOLID) derived |= 1u << d;
             }
+            node[i] |= derived;
+            if ((user & ~derived) != 0 || lat->node_class[i] != LBM_FLUID) simple = false;
         }
     }
     if (n >= 0xFFFFFFFFu) return reject("lattice too large for 32-bit node indices");
@@ -215,6 +223,35 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
             nodew.push_back(node[i]);
         }
     }
+    // Row metadata of the rank step: per row and 32-node word, the fluid mask
+    // and the compact index of the word's first fluid node; per 256-node
+    // workgroup, its first row and the number of rows it spans.
+    const int words = (p->nx + 31) / 32;
+    const size_t nrows = (size_t)p->ny * nz;
+    std::vector<uint32_t> rowmeta(nrows * words * 2);
+    {
+        uint32_t cnt = 0;
+        for (size_t r = 0; r < nrows; ++r)
+            for (int w = 0; w < words; ++w) {
+                uint32_t m = 0;
+                for (int b = 0; b < 32 && 32 * w + b < p->nx; ++b)
+                    if (lat->node_class[r * p->nx + 32 * w + b] != LBM_SOLID) m |= 1u << b;
+                rowmeta[(r * words + w) * 2] = m;
+                rowmeta[(r * words + w) * 2 + 1] = cnt;
+                cnt += (uint32_t)__builtin_popcount(m);
+            }
+    }
+    const size_t nblk = (nf + 255) / 256;
+    std::vector<uint32_t> blk(2 * (nblk > 0 ? nblk : 1));
+    int rmax = 1;
+    for (size_t b = 0; b < nblk; ++b) {
+        const size_t first = didx[b * 256], last = didx[std::min(nf, (b + 1) * 256) - 1];
+        const uint32_t r0 = (uint32_t)(first / p->nx), r1 = (uint32_t)(last / p->nx);
+        blk[2 * b] = r0;
+        blk[2 * b + 1] = r1 - r0 + 1;
+        rmax = std::max(rmax, (int)(r1 - r0 + 1));
+    }
+    const bool rank_ok = (size_t)rmax * 9 * words * 8 <= 48 * 1024;
     if (any_open_cls && !lat->u_bc) {
         bool need = false;
         for (size_t i = 0; i < n; ++i) need |= lat->node_class[i] == LBM_VEL_W;
@@ -237,6 +274,11 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     for (int a = 0; a < 3; ++a) k.per[a] = p->periodic[a] ? 1 : 0;
     if (p->q == 9) k.per[2] = 1;
     k.qstride = plane_stride(nf > 0 ? nf : 1, c->raw_bytes);
+    k.qstride32 = (uint32_t)k.qstride;
+    if (k.qstride * c->q * c->raw_bytes >= (1ull << 32) || n * 4 >= (1ull << 32)) {
+        delete c;
+        return reject("population buffer or node map above 4 GiB (32-bit offsets)");
+    }
     k.nf = (uint32_t)nf;
     auto magic = [](uint32_t d, uint64_t& m, uint32_t& s) {
         s = 0;
@@ -290,6 +332,8 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     if (st == LBM_OK) st = dalloc(c, (void**)&c->cidx, sizeof(uint32_t) * n);
     if (st == LBM_OK) st = dalloc(c, (void**)&c->didx, sizeof(uint32_t) * (nf > 0 ? nf : 1));
     if (st == LBM_OK) st = dalloc(c, (void**)&c->nodew, sizeof(uint32_t) * (nf > 0 ? nf : 1));
+    if (st == LBM_OK) st = dalloc(c, (void**)&c->rowmeta, sizeof(uint32_t) * rowmeta.size());
+    if (st == LBM_OK) st = dalloc(c, (void**)&c->blk, sizeof(uint32_t) * blk.size());
     double** fl[] = {&c->rho, &c->u, &c->v, &c->w, &c->up, &c->vp, &c->wp};
     for (double** a : fl)
         if (st == LBM_OK) st = dalloc(c, (void**)a, sizeof(double) * n);
@@ -306,11 +350,20 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     }
     k.didx = c->didx;
     k.nodew = c->nodew;
+    k.rowmeta = (const uint2*)c->rowmeta;
+    k.blk = (const uint2*)c->blk;
+    k.words = words;
+    k.rmax = rmax;
+    k.simple = simple ? 1 : 0;
+    k.rank = rank_ok ? 1 : 0;
+    if (const char* e = std::getenv("LBM_RANK")) k.rank = rank_ok && e[0] == '1';
     auto h2d = [&](void* d, const void* h, size_t b) { return hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, c->stream); };
     if (h2d(c->node, node.data(), sizeof(uint32_t) * n) != hipSuccess ||
         h2d(c->cidx, cidx.data(), sizeof(uint32_t) * n) != hipSuccess ||
         (nf > 0 && h2d(c->didx, didx.data(), sizeof(uint32_t) * nf) != hipSuccess) ||
-        (nf > 0 && h2d(c->nodew, nodew.data(), sizeof(uint32_t) * nf) != hipSuccess))
+        (nf > 0 && h2d(c->nodew, nodew.data(), sizeof(uint32_t) * nf) != hipSuccess) ||
+        h2d(c->rowmeta, rowmeta.data(), sizeof(uint32_t) * rowmeta.size()) != hipSuccess ||
+        h2d(c->blk, blk.data(), sizeof(uint32_t) * blk.size()) != hipSuccess)
         return cleanup(LBM_EDEVICE);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(LBM_EDEVICE);
     if (c->ubc) {
@@ -332,7 +385,7 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
 void lbm_destroy(lbm_ctx* c) {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
-    void* bufs[] = {c->Palloc[0], c->Palloc[1], c->node, c->cidx, c->didx, c->nodew, c->rho, c->u, c->v, c->w, c->up, c->vp, c->wp,
+    void* bufs[] = {c->Palloc[0], c->Palloc[1], c->node, c->cidx, c->didx, c->nodew, c->rowmeta, c->blk, c->rho, c->u, c->v, c->w, c->up, c->vp, c->wp,
                     c->ubc, c->vbc, c->wbc, c->fout, c->partials, c->out4};
     for (void* b : bufs)
         if (b) hipFree(b);

@@ -18,6 +18,7 @@ struct KParams {
     int nx, ny, nz;
     int per[3];
     size_t qstride;          // elements between population planes
+    uint32_t qstride32;
     double omega, omega_m;   // BGK/TRT rates
     double s[19];            // MRT diag(S)
     double g[3];
@@ -33,6 +34,12 @@ struct KParams {
     int tile;                // population layout: 0 SoA planes, 1 tiles of 64 nodes x Q
     const uint32_t* didx;    // compact -> dense node index
     const uint32_t* nodew;   // compact node words
+    int rank;                // use the row-bitmask step (no per-node index loads)
+    int simple;              // walls are exactly the links into solid nodes; all fluid plain
+    int words;               // 32-node words per row
+    int rmax;                // rows staged per workgroup (LDS sizing)
+    const uint2* rowmeta;    // per row and word: (fluid mask, compact index of first fluid node)
+    const uint2* blk;        // per 256-node workgroup: (first row, rows spanned)
 };
 
 struct Fields {              // FP64 side arrays (N each)

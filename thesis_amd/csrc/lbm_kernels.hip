@@ -137,42 +137,44 @@ __device__ inline uint16_t f32_to_bf16(float f) {   // round to nearest even
     return (uint16_t)(u >> 16);
 }
 
+// Loads and stores take 32-bit element offsets; the address is formed as
+// base + 32-bit byte offset so it maps onto global_load/store with a scalar
+// base register and one vector offset (population buffers stay below 4 GiB).
+template <class R> __device__ inline R ldo(const R* base, uint32_t k) {
+    return *(const R*)((const char*)base + k * (uint32_t)sizeof(R));
+}
+template <class R> __device__ inline void sto(R* base, uint32_t k, R v) {
+    __builtin_nontemporal_store(v, (R*)((char*)base + k * (uint32_t)sizeof(R)));
+}
+
 template <class L, int S> struct Io;
 template <class L> struct Io<L, 0> {
     using raw = double; using T = double;
     __device__ static T cv(raw r, int) { return r; }
     __device__ static raw pk(T v, int) { return v; }
-    __device__ static T ld(const raw* p, size_t k, int) { return p[k]; }
-    __device__ static void st(raw* p, size_t k, T v, int) { __builtin_nontemporal_store(v, p + k); }
+    __device__ static T ld(const raw* p, uint32_t k, int) { return ldo(p, k); }
+    __device__ static void st(raw* p, uint32_t k, T v, int) { sto(p, k, v); }
 };
 template <class L> struct Io<L, 1> {
     using raw = float; using T = float;
     __device__ static T cv(raw r, int) { return r; }
     __device__ static raw pk(T v, int) { return v; }
-    __device__ static T ld(const raw* p, size_t k, int) { return p[k]; }
-    __device__ static void st(raw* p, size_t k, T v, int) { __builtin_nontemporal_store(v, p + k); }
+    __device__ static T ld(const raw* p, uint32_t k, int) { return ldo(p, k); }
+    __device__ static void st(raw* p, uint32_t k, T v, int) { sto(p, k, v); }
 };
 template <class L> struct Io<L, 2> {
     using raw = uint16_t; using T = float;
     __device__ static T cv(raw r, int i) { return (float)__builtin_bit_cast(_Float16, r) * f16_a<L>(i); }
     __device__ static raw pk(T v, int i) { return __builtin_bit_cast(uint16_t, (_Float16)(v * f16_ainv<L>(i))); }
-    __device__ static T ld(const raw* p, size_t k, int i) {
-        _Float16 h = __builtin_bit_cast(_Float16, p[k]);
-        return (float)h * f16_a<L>(i);
-    }
-    __device__ static void st(raw* p, size_t k, T v, int i) {
-        _Float16 h = (_Float16)(v * f16_ainv<L>(i));
-        __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, h), p + k);
-    }
+    __device__ static T ld(const raw* p, uint32_t k, int i) { return cv(ldo(p, k), i); }
+    __device__ static void st(raw* p, uint32_t k, T v, int i) { sto(p, k, pk(v, i)); }
 };
 template <class L> struct Io<L, 3> {
     using raw = uint16_t; using T = float;
     __device__ static T cv(raw r, int i) { return bf16_to_f32(r) * bf16_a<L>(i); }
     __device__ static raw pk(T v, int i) { return f32_to_bf16(v * bf16_ainv<L>(i)); }
-    __device__ static T ld(const raw* p, size_t k, int i) { return bf16_to_f32(p[k]) * bf16_a<L>(i); }
-    __device__ static void st(raw* p, size_t k, T v, int i) {
-        __builtin_nontemporal_store(f32_to_bf16(v * bf16_ainv<L>(i)), p + k);
-    }
+    __device__ static T ld(const raw* p, uint32_t k, int i) { return cv(ldo(p, k), i); }
+    __device__ static void st(raw* p, uint32_t k, T v, int i) { sto(p, k, pk(v, i)); }
 };
 
 // ---------------------------------------------------------------------------
@@ -196,17 +198,19 @@ __device__ inline void source_index(const uint32_t* __restrict__ cidx, const KPa
         ym_o = !kp.per[1] && y == 0;  yp_o = !kp.per[1] && y == ny - 1;
         zm_o = !kp.per[2] && z == 0;  zp_o = !kp.per[2] && z == nz - 1;
     }
+    // row bases of the three source rows in y and planes in z
+    const uint32_t ux = (uint32_t)nx, plane = (uint32_t)nx * (uint32_t)ny;
+    const uint32_t yb[3] = {(uint32_t)ym * ux, (uint32_t)y * ux, (uint32_t)yp * ux};     // index 1 - cy
+    const uint32_t zb[3] = {(uint32_t)zm * plane, (uint32_t)z * plane, (uint32_t)zp * plane};
 #pragma unroll
     for (int i = 1; i < L::Q; ++i) {
         const int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
         const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
-        const int sy = cy == 0 ? y : (cy > 0 ? ym : yp);
-        const int sz = cz == 0 ? z : (cz > 0 ? zm : zp);
         bool out = false;
         if (OPEN)
             out = (cx > 0 && xm_o) || (cx < 0 && xp_o) || (cy > 0 && ym_o) || (cy < 0 && yp_o) ||
                   (cz > 0 && zm_o) || (cz < 0 && zp_o);
-        cn[i] = out ? kNone : cidx[((size_t)sz * ny + sy) * nx + sx];
+        cn[i] = out ? kNone : ldo(cidx, zb[1 - cz] + yb[1 - cy] + (uint32_t)sx);
     }
 }
 
@@ -221,9 +225,9 @@ __device__ inline void source_index(const uint32_t* __restrict__ cidx, const KPa
 // P[(j/64)*64*Q + i*64 + j%64], so one wave's populations sit in one
 // contiguous 64*Q-element block.
 template <class L>
-__device__ inline size_t paddr(const KParams& kp, int i, uint32_t j) {
-    if (kp.tile) return (size_t)(j >> 6) * (64 * L::Q) + (size_t)i * 64 + (j & 63u);
-    return (size_t)i * kp.qstride + j;
+__device__ inline uint32_t paddr(const KParams& kp, int i, uint32_t j) {
+    if (kp.tile) return (j >> 6) * (64u * L::Q) + (uint32_t)i * 64u + (j & 63u);
+    return (uint32_t)i * kp.qstride32 + j;
 }
 
 template <class L, int S, bool OPEN>
@@ -285,6 +289,9 @@ __device__ inline void onnode_walls(uint32_t nd, T* h) {
         if ((nd >> d) & 1u) h[L::opp(d)] = h[d];
 }
 
+__device__ inline double recip(double x) { return 1.0 / x; }
+__device__ inline float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // Moments of h and the stored-velocity semantics of the force model
 // (S/GpuUpdateMacro.cu:6-41, :137-213).
 template <class L, class T>
@@ -299,7 +306,7 @@ __device__ inline void moments(const KParams& kp, uint32_t cls, const T* h, T& d
     }
     drho = d;
     rho = (T)1 + d;
-    const T ir = (T)1 / rho;
+    const T ir = recip(rho);
     u[0] = jx * ir; u[1] = jy * ir; u[2] = jz * ir;
     if (kp.force == 1) {
         const T io = (T)(1.0 / kp.omega);
@@ -359,10 +366,10 @@ __device__ inline void mrt_dmeq(const KParams& kp, T drho, T rho, const T* u, T*
 // TRT: :917-969 (general Lambda); MRT: :971-1037, :1039-1170.
 template <class L, class T, int C>
 __device__ inline void collide(const KParams& kp, const T* h, T drho, T rho, const T* u, T* out) {
-    T S[L::Q];
     const bool guo = kp.force == 2;
-    if (guo) guo_terms<L, T>(kp, rho, u, S);
     if (C == 2) {
+        T S[L::Q];
+        if (guo) guo_terms<L, T>(kp, rho, u, S);
         T m[L::Q], dm[L::Q], gs[L::Q], dd[L::Q];
         mrt_dmeq<L, T>(kp, drho, rho, u, dm);
         m_apply<L, T>(h, m, std::make_integer_sequence<int, L::Q>{});
@@ -380,40 +387,42 @@ __device__ inline void collide(const KParams& kp, const T* h, T drho, T rho, con
             out[L::Q - 1] = -(T)L::w(L::Q - 1);   // fColl_18 stays at its zero initial value
         return;
     }
-    const T uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-    T e[L::Q];
-#pragma unroll
-    for (int i = 0; i < L::Q; ++i) {
-        const T cu = (T)L::cx(i) * u[0] + (T)L::cy(i) * u[1] + (T)L::cz(i) * u[2];
-        e[i] = (T)L::w(i) * (drho + rho * ((T)3 * cu + (T)4.5 * cu * cu - (T)1.5 * uu));
-    }
+    // BGK and TRT in symmetric/antisymmetric pair form: for a link pair
+    // (i, opp i) the parts of f_eq and of the Guo source are
+    //   e+ = w (drho + rho (4.5 cu^2 - 1.5 u.u)),  e- = 3 w rho cu,
+    //   S+ = w (9 cu cF - 3 u.F),                  S- = 3 w cF,
+    // relaxed with omega+ and omega- (equal for BGK) and prefactors 1 - omega/2.
     const T op = (T)kp.omega;
-    if (C == 0) {
-        const T gf = (T)1 - (T)0.5 * op;
-#pragma unroll
-        for (int i = 0; i < L::Q; ++i) {
-            out[i] = h[i] - op * (h[i] - e[i]);
-            if (guo) out[i] += gf * S[i];
-        }
-        return;
-    }
-    const T om = (T)kp.omega_m;
+    const T om = C == 0 ? op : (T)kp.omega_m;
     const T gp = (T)1 - (T)0.5 * op, gm = (T)1 - (T)0.5 * om;
-    out[0] = h[0] - op * (h[0] - e[0]);
-    if (guo) out[0] += gp * S[0];
+    const T uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    T Fx = 0, Fy = 0, Fz = 0, uF = 0;
+    if (guo) {
+        Fx = rho * (T)kp.g[0]; Fy = rho * (T)kp.g[1]; Fz = rho * (T)kp.g[2];
+        uF = u[0] * Fx + u[1] * Fy + u[2] * Fz;
+    }
+    {
+        const T w0 = (T)L::w(0);
+        out[0] = h[0] - op * (h[0] - w0 * (drho - (T)1.5 * rho * uu));
+        if (guo) out[0] -= gp * (T)3 * w0 * uF;
+    }
 #pragma unroll
     for (int i = 1; i < L::Q; ++i) {
         const int o = L::opp(i);
         if (i < o) {
-            const T dp = op * (T)0.5 * ((h[i] + h[o]) - (e[i] + e[o]));
-            const T dn = om * (T)0.5 * ((h[i] - h[o]) - (e[i] - e[o]));
-            out[i] = h[i] - dp - dn;
-            out[o] = h[o] - dp + dn;
+            const T w = (T)L::w(i);
+            const T cu = (T)L::cx(i) * u[0] + (T)L::cy(i) * u[1] + (T)L::cz(i) * u[2];
+            const T ep = w * (drho + rho * ((T)4.5 * cu * cu - (T)1.5 * uu));
+            const T em = (T)3 * w * rho * cu;
+            T dp = op * ((T)0.5 * (h[i] + h[o]) - ep);
+            T dm = om * ((T)0.5 * (h[i] - h[o]) - em);
             if (guo) {
-                const T sp = gp * (T)0.5 * (S[i] + S[o]), sn = gm * (T)0.5 * (S[i] - S[o]);
-                out[i] += sp + sn;
-                out[o] += sp - sn;
+                const T cF = (T)L::cx(i) * Fx + (T)L::cy(i) * Fy + (T)L::cz(i) * Fz;
+                dp -= gp * w * ((T)9 * cu * cF - (T)3 * uF);
+                dm -= gm * (T)3 * w * cF;
             }
+            out[i] = h[i] - dp - dm;
+            out[o] = h[o] - dp + dm;
         }
     }
 }
@@ -458,6 +467,121 @@ __global__ __launch_bounds__(256) void k_step(const typename Io<L, S>::raw* __re
     const uint32_t cls = nd >> kClsShift;
     T h[L::Q];
     gather<L, S, OPEN>(src, kp, own, nd, cn, h);
+    if (OPEN && cls == VEL_W) zou_inlet<L, T>(kp, fl, idx, nd, h);
+    if (kp.wall == 1 && (nd & kBBMask)) onnode_walls<L, T>(nd, h);
+    if (OPEN && cls == PRES_E) zou_outlet<L, T>(kp, h);
+    T drho, rho, u[3];
+    moments<L, T>(kp, cls, h, drho, rho, u);
+    if (write_macros) {
+        fl.rho[idx] = (double)rho; fl.u[idx] = (double)u[0];
+        fl.v[idx] = (double)u[1]; fl.w[idx] = (double)u[2];
+    }
+    T out[L::Q];
+    collide<L, T, C>(kp, h, drho, rho, u, out);
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) Io<L, S>::st(dst, paddr<L>(kp, i, own), out[i], i);
+}
+
+// Rank variant of the step: no per-node index loads.  For every lattice row
+// (fixed y, z) rowmeta holds, per 32-node word, the fluid bitmask and the
+// compact index of the word's first fluid node.  A workgroup (256 compact
+// nodes, spanning R rows) stages the words of its rows and of their eight
+// neighbour rows in LDS; a node's own position and the compact index of every
+// source node are then a popcount away.  With SIMPLE (every wall is a link into
+// a solid node, all non-solid nodes plain fluid) the node words are not read
+// either, and a step moves only the populations.
+__device__ inline int select_bit(uint32_t m, uint32_t t) {   // position of the t-th set bit
+    int pos = 0;
+    uint32_t c = __popc(m & 0xFFFFu);
+    if (t >= c) { t -= c; m >>= 16; pos += 16; }
+    c = __popc(m & 0xFFu);
+    if (t >= c) { t -= c; m >>= 8; pos += 8; }
+    c = __popc(m & 0xFu);
+    if (t >= c) { t -= c; m >>= 4; pos += 4; }
+    c = __popc(m & 0x3u);
+    if (t >= c) { t -= c; m >>= 2; pos += 2; }
+    if (t >= (m & 1u)) pos += 1;
+    return pos;
+}
+
+constexpr uint32_t kSolidSrc = 0xFFFFFFFEu;
+
+template <class L, int S, int C, bool OPEN, bool SIMPLE>
+__global__ __launch_bounds__(256) void k_stepr(const typename Io<L, S>::raw* __restrict__ src,
+                                               typename Io<L, S>::raw* __restrict__ dst,
+                                               const uint2* __restrict__ rowmeta,
+                                               const uint2* __restrict__ blk,
+                                               const uint32_t* __restrict__ nodew, KParams kp,
+                                               Fields fl, int write_macros) {
+    using T = typename Io<L, S>::T;
+    extern __shared__ uint2 smeta[];
+    const int W = kp.words, nx = kp.nx, ny = kp.ny, nz = kp.nz;
+    const uint2 bi = blk[blockIdx.x];
+    const uint32_t r0 = bi.x;
+    const int R = (int)bi.y;
+    // stage the words of rows r0..r0+R-1 and of their (dy, dz) neighbours;
+    // slot (k, nb) with nb = (dz+1)*3 + (dy+1); rows outside a non-periodic
+    // face read as all-solid (outside sources are resolved by coordinates)
+    const int tot = R * 9 * W;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int k = e / (9 * W), rem = e - k * 9 * W, nb = rem / W, w = rem - nb * W;
+        const uint32_t r = r0 + (uint32_t)k;
+        const uint32_t zq = fdiv(r, kp.my, kp.sy);
+        int y = (int)(r - zq * (uint32_t)ny), z = (int)zq;
+        y += nb % 3 - 1;
+        z += nb / 3 - 1;
+        bool out = false;
+        if (y < 0 || y >= ny) { if (kp.per[1]) y = (y + ny) % ny; else out = true; }
+        if (z < 0 || z >= nz) { if (kp.per[2]) z = (z + nz) % nz; else out = true; }
+        smeta[e] = out ? make_uint2(0u, 0u) : rowmeta[((size_t)z * ny + y) * W + w];
+    }
+    __syncthreads();
+    const uint32_t own = blockIdx.x * blockDim.x + threadIdx.x;
+    if (own >= kp.nf) return;
+    // own row: the last staged own row whose first compact index is <= own
+    int k = 0;
+    while (k + 1 < R && smeta[((k + 1) * 9 + 4) * W].y <= own) ++k;
+    const uint2* me = smeta + (k * 9 + 4) * W;
+    int w = 0;
+    while (w + 1 < W && me[w + 1].y <= own) ++w;
+    const int x = 32 * w + select_bit(me[w].x, own - me[w].y);
+    const uint32_t r = r0 + (uint32_t)k;
+    const uint32_t zq = fdiv(r, kp.my, kp.sy);
+    const int y = (int)(r - zq * (uint32_t)ny), z = (int)zq;
+    const uint32_t idx = r * (uint32_t)nx + (uint32_t)x;
+    const uint32_t nd = SIMPLE ? ((uint32_t)FLUID << kClsShift) : nodew[own];
+    const uint32_t cls = nd >> kClsShift;
+    bool xm_o = false, xp_o = false, ym_o = false, yp_o = false, zm_o = false, zp_o = false;
+    if (OPEN) {
+        xm_o = !kp.per[0] && x == 0;  xp_o = !kp.per[0] && x == nx - 1;
+        ym_o = !kp.per[1] && y == 0;  yp_o = !kp.per[1] && y == ny - 1;
+        zm_o = !kp.per[2] && z == 0;  zp_o = !kp.per[2] && z == nz - 1;
+    }
+    const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+    T h[L::Q];
+    h[0] = Io<L, S>::ld(src, paddr<L>(kp, 0, own), 0);
+#pragma unroll
+    for (int i = 1; i < L::Q; ++i) {
+        const int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i), o = L::opp(i);
+        const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
+        bool out = false;
+        if (OPEN)
+            out = (cx > 0 && xm_o) || (cx < 0 && xp_o) || (cy > 0 && ym_o) || (cy < 0 && yp_o) ||
+                  (cz > 0 && zm_o) || (cz < 0 && zp_o);
+        uint32_t j = kNone;
+        if (!out) {
+            const uint2 e = smeta[(k * 9 + (1 - cz) * 3 + (1 - cy)) * W + (sx >> 5)];
+            const uint32_t bit = (uint32_t)sx & 31u;
+            j = ((e.x >> bit) & 1u) ? e.y + __popc(e.x & ((1u << bit) - 1u)) : kSolidSrc;
+        }
+        const bool wall = ((nd >> o) & 1u) || j == kSolidSrc;
+        int pl = i;
+        if (wall || j == kNone) {
+            j = own;
+            pl = wall && kp.wall == 0 ? o : i;
+        }
+        h[i] = Io<L, S>::ld(src, paddr<L>(kp, pl, j), i);
+    }
     if (OPEN && cls == VEL_W) zou_inlet<L, T>(kp, fl, idx, nd, h);
     if (kp.wall == 1 && (nd & kBBMask)) onnode_walls<L, T>(nd, h);
     if (OPEN && cls == PRES_E) zou_outlet<L, T>(kp, h);
@@ -621,10 +745,20 @@ template <class L, int S, int C, bool OPEN>
 static hipError_t step_t(const KParams& kp, const void* src, void* dst, const uint32_t* node, const uint32_t* cidx,
                          Fields fl, bool wm, hipStream_t st) {
     using raw = typename Io<L, S>::raw;
+    if (kp.nf == 0) return hipSuccess;
     const unsigned blocks = (unsigned)((kp.nf + 255) / 256);
-    if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_step<L, S, C, OPEN>), dim3(blocks), dim3(256), 0, st, (const raw*)src, (raw*)dst,
-                       kp.didx, kp.nodew, cidx, kp, fl, wm ? 1 : 0);
+    if (kp.rank) {
+        const size_t lds = (size_t)kp.rmax * 9 * kp.words * sizeof(uint2);
+        if (kp.simple)
+            hipLaunchKernelGGL((k_stepr<L, S, C, OPEN, true>), dim3(blocks), dim3(256), lds, st, (const raw*)src,
+                               (raw*)dst, kp.rowmeta, kp.blk, kp.nodew, kp, fl, wm ? 1 : 0);
+        else
+            hipLaunchKernelGGL((k_stepr<L, S, C, OPEN, false>), dim3(blocks), dim3(256), lds, st, (const raw*)src,
+                               (raw*)dst, kp.rowmeta, kp.blk, kp.nodew, kp, fl, wm ? 1 : 0);
+    } else {
+        hipLaunchKernelGGL((k_step<L, S, C, OPEN>), dim3(blocks), dim3(256), 0, st, (const raw*)src, (raw*)dst,
+                           kp.didx, kp.nodew, cidx, kp, fl, wm ? 1 : 0);
+    }
     return hipGetLastError();
 }
 template <class L, int S, int C>
