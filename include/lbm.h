@@ -140,15 +140,20 @@ int64_t lbm_device_bytes(const lbm_ctx* ctx);
 
 /* ---- lattice files (host only; no device needed) ------------------------ */
 /* latticegen2 reader: <name>_Node.dat ("i j [k] x y [z] fluid") and
- * <name>_BC.dat ("i j [k] dir type x y [z] id"), S/FilesReading.cu:270-347,
- * L/main.cpp:59-66, L/FluidTracking.cpp:229-236.  First call with
+ * <name>_BC.dat ("i j [k] dir type x y [z] id"), replacing readNodeFile /
+ * readConnFile / getGridSpacing (S/FilesReading.cu:270-347, :473-490); format
+ * of L/main.cpp:59-66 and L/FluidTracking.cpp:229-236.  First call with
  * node_class == NULL returns dims (nx, ny, nz) and the link count; the second
- * fills node_class (0/1 from the fluid column), bb_links (WALL links) and
- * type_counts[6] (links per type 0..5).  Returns LBM_EIO on a missing or
- * malformed file. */
+ * fills node_class (0 solid / 1 fluid from the fluid column; in 2D, nodes with
+ * INLE links on x = 0 become LBM_VEL_W and nodes with OUTL links on
+ * x = nx-1 become LBM_PRES_E), bb_links (WALL links), type_counts[6] (links
+ * per type 0..5), dir_counts[19] (links per direction), *delta (grid
+ * spacing) and cyclic[3] (an axis whose faces carry CYCL links).  Any output
+ * pointer may be NULL.  Returns LBM_EIO on a missing or malformed file. */
 lbm_status lbm_read_lattice(const char* node_path, const char* bc_path, int32_t dims[3],
                             int64_t* nlinks, uint8_t* node_class, uint32_t* bb_links,
-                            int64_t type_counts[6], int64_t dir_counts[19]);
+                            int64_t type_counts[6], int64_t dir_counts[19], double* delta,
+                            int32_t cyclic[3]);
 
 #ifdef __cplusplus
 }

@@ -58,6 +58,8 @@ def test_read_lattice_2d_and_crlf(tmp_path):
     bc.write_text("0 0 4 1 0.0 -0.25 3\n0 0 3 2 -0.25 0.0 1\n3 2 2 1 1.5 1.25 3\n")
     r = L.read_lattice(str(node), str(bc))
     assert r["dims"] == (4, 3, 1)
-    assert r["node_class"].sum() == 11 and r["node_class"][0, 1, 2] == 0
+    assert (r["node_class"] > 0).sum() == 11 and r["node_class"][0, 1, 2] == 0
+    assert r["node_class"][0, 0, 0] == L.VEL_W            # INLE link on the west face
+    assert abs(r["delta"] - 0.5) < 1e-12 and r["cyclic"] == (0, 0, 0)
     assert r["nlinks"] == 3 and list(r["type_counts"][:3]) == [0, 2, 1]
     assert r["bb_links"][0, 0, 0] == (1 << 4) and r["bb_links"][0, 2, 3] == (1 << 2)

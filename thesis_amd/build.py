@@ -40,8 +40,7 @@ def build_lib(jobs=None):
     for src in sources():
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
         objs.append(obj)
-        lang = ["-x", "hip"] if src.endswith(".hip") else ["-x", "hip"]
-        cmd = [HIPCC] + COMMON + lang + ["-c", src, "-o", obj]
+        cmd = [HIPCC] + COMMON + ["-x", "hip", "-c", src, "-o", obj]
         print(" ".join(cmd), flush=True)
         procs.append(subprocess.Popen(cmd))
     bad = [p.wait() for p in procs]
@@ -60,9 +59,18 @@ def build_cli():
           "-L", HERE, "-llbm", "-Wl,-rpath,$ORIGIN"])
 
 
+def build_calib():
+    """Copy-bandwidth calibration program (counter scale factors)."""
+    src = os.path.join(HERE, "calib", "copy_bandwidth.hip")
+    if os.path.exists(src):
+        _run([HIPCC, "-O3", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+              src, "-o", os.path.join(HERE, "calib", "copy_bandwidth")])
+
+
 def build():
     build_lib()
     build_cli()
+    build_calib()
 
 
 if __name__ == "__main__":

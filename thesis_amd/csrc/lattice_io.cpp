@@ -11,6 +11,7 @@
 // '\n' only, so CRLF files count correctly (unlike S/FilesReading.cu:27-31).
 #include "lbm.h"
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -55,7 +56,8 @@ int parse_line(const char*& p, const char* end, double* vals, int maxc) {
 
 extern "C" lbm_status lbm_read_lattice(const char* node_path, const char* bc_path, int32_t dims[3],
                                        int64_t* nlinks, uint8_t* node_class, uint32_t* bb_links,
-                                       int64_t type_counts[6], int64_t dir_counts[19]) {
+                                       int64_t type_counts[6], int64_t dir_counts[19], double* delta,
+                                       int32_t cyclic[3]) {
     if (!node_path || !dims) return LBM_EINVAL;
     std::string txt;
     if (!slurp(node_path, txt)) return LBM_EIO;
@@ -64,7 +66,7 @@ extern "C" lbm_status lbm_read_lattice(const char* node_path, const char* bc_pat
     double v[16];
     int cols = 0;
     int64_t mx[3] = {-1, -1, -1};
-    struct Rec { int64_t i, j, k; int fluid; };
+    struct Rec { int64_t i, j, k; int fluid; double x; };
     std::vector<Rec> recs;
     while (p < end) {
         int c = parse_line(p, end, v, 16);
@@ -74,6 +76,7 @@ extern "C" lbm_status lbm_read_lattice(const char* node_path, const char* bc_pat
         Rec r;
         r.i = (int64_t)v[0]; r.j = (int64_t)v[1]; r.k = cols == 7 ? (int64_t)v[2] : 0;
         r.fluid = (int)v[cols - 1];
+        r.x = v[cols == 7 ? 3 : 2];
         if (r.i < 0 || r.j < 0 || r.k < 0) return LBM_EIO;
         if (r.i > mx[0]) mx[0] = r.i;
         if (r.j > mx[1]) mx[1] = r.j;
@@ -100,6 +103,16 @@ extern "C" lbm_status lbm_read_lattice(const char* node_path, const char* bc_pat
         }
         return LBM_OK;
     }
+    if (delta) {   // |x(1,0,0) - x(0,0,0)| (S/FilesReading.cu:473-490)
+        double x0 = 0, x1 = 0;
+        bool h0 = false, h1 = false;
+        for (const Rec& r : recs) {
+            if (r.j == 0 && r.k == 0 && r.i == 0) { x0 = r.x; h0 = true; }
+            if (r.j == 0 && r.k == 0 && r.i == 1) { x1 = r.x; h1 = true; }
+        }
+        *delta = (h0 && h1) ? std::fabs(x1 - x0) : 1.0;
+    }
+    if (cyclic) cyclic[0] = cyclic[1] = cyclic[2] = 0;
     std::memset(node_class, 0, (size_t)n);
     if (bb_links) std::memset(bb_links, 0, sizeof(uint32_t) * (size_t)n);
     if (type_counts) std::memset(type_counts, 0, sizeof(int64_t) * 6);
@@ -121,7 +134,22 @@ extern "C" lbm_status lbm_read_lattice(const char* node_path, const char* bc_pat
             if (dir < 1 || dir >= qmax || type < 0 || type > 5) return LBM_EIO;
             if (type_counts) type_counts[type]++;
             if (dir_counts) dir_counts[dir]++;
-            if (type == 1 && bb_links) bb_links[i + j * nx + k * nx * ny] |= 1u << dir;
+            const int64_t at = i + j * nx + k * nx * ny;
+            if (type == 1 && bb_links) bb_links[at] |= 1u << dir;
+            if (!three_d && type == 2 && i == 0 && node_class[at]) node_class[at] = LBM_VEL_W;
+            if (!three_d && type == 3 && i == nx - 1 && node_class[at]) node_class[at] = LBM_PRES_E;
+            if (type == 4 && cyclic) {
+                static const int c9[9][2] = {{0, 0}, {1, 0}, {0, 1}, {-1, 0}, {0, -1}, {1, 1}, {-1, 1}, {-1, -1}, {1, -1}};
+                static const int c19[19][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1},
+                                               {0, 0, -1}, {1, 1, 0}, {-1, 1, 0}, {1, -1, 0}, {-1, -1, 0}, {1, 0, 1},
+                                               {-1, 0, 1}, {1, 0, -1}, {-1, 0, -1}, {0, 1, 1}, {0, -1, 1}, {0, 1, -1},
+                                               {0, -1, -1}};
+                const int cx = three_d ? c19[dir][0] : c9[dir][0], cy = three_d ? c19[dir][1] : c9[dir][1];
+                const int cz = three_d ? c19[dir][2] : 0;
+                if ((cx < 0 && i == 0) || (cx > 0 && i == nx - 1)) cyclic[0] = 1;
+                if ((cy < 0 && j == 0) || (cy > 0 && j == ny - 1)) cyclic[1] = 1;
+                if ((cz < 0 && k == 0) || (cz > 0 && k == dims[2] - 1)) cyclic[2] = 1;
+            }
             ++cnt;
         }
     }

@@ -129,3 +129,52 @@ def droplet2d(nx, ny, radius, cx=None, cy=None):
     inside = (xx - cx) ** 2 + (yy - cy) ** 2 <= radius ** 2
     r = inside.astype(np.float64)
     return r, 1.0 - r
+
+
+def write_latticegen2(path_prefix, node_class, bb=None, delta=1.0, inlet_dirs=None, outlet_dirs=None):
+    """Write <prefix>_Node.dat and <prefix>_BC.dat in the latticegen2 layout
+    (L/main.cpp:59-66, L/FluidTracking.cpp:229-236): 2D lines "i j x y fluid"
+    and "i j dir type x y id", 3D with the k column.  WALL links come from
+    `bb`; VEL_W / PRES_E nodes of a 2D lattice get INLE / OUTL links in the
+    directions leaving the west / east face."""
+    cls = np.asarray(node_class)
+    two_d = cls.ndim == 2
+    c3 = cls[None] if two_d else cls
+    nz, ny, nx = c3.shape
+    b3 = np.zeros(c3.shape, np.uint32) if bb is None else (np.asarray(bb)[None] if two_d else np.asarray(bb))
+    q = 9 if two_d else 19
+    c = velocities(q)
+    inlet_dirs = [d for d in range(1, q) if c[d, 0] < 0] if inlet_dirs is None else inlet_dirs
+    outlet_dirs = [d for d in range(1, q) if c[d, 0] > 0] if outlet_dirs is None else outlet_dirs
+    with open(path_prefix + "_Node.dat", "w") as fh:
+        for k in range(nz):
+            for j in range(ny):
+                for i in range(nx):
+                    fl = 1 if c3[k, j, i] != SOLID else 0
+                    if two_d:
+                        fh.write(f"{i}\t{j}\t{i * delta:.6f}\t{j * delta:.6f}\t{fl}\n")
+                    else:
+                        fh.write(f"{i}\t{j}\t{k}\t{i * delta:.6f}\t{j * delta:.6f}\t{k * delta:.6f}\t{fl}\n")
+    with open(path_prefix + "_BC.dat", "w") as fh:
+        def link(i, j, k, d, t):
+            x, y, z = (i + 0.5 * c[d, 0]) * delta, (j + 0.5 * c[d, 1]) * delta, (k + 0.5 * c[d, 2]) * delta
+            if two_d:
+                fh.write(f"{i}\t{j}\t{d}\t{t}\t{x:.6f}\t{y:.6f}\t1\n")
+            else:
+                fh.write(f"{i}\t{j}\t{k}\t{d}\t{t}\t{x:.6f}\t{y:.6f}\t{z:.6f}\t1\n")
+        for k in range(nz):
+            for j in range(ny):
+                for i in range(nx):
+                    if c3[k, j, i] == SOLID:
+                        continue
+                    for d in range(1, q):
+                        if (int(b3[k, j, i]) >> d) & 1:
+                            link(i, j, k, d, 1)
+                    if c3[k, j, i] == VEL_W:
+                        for d in inlet_dirs:
+                            if not (int(b3[k, j, i]) >> d) & 1:
+                                link(i, j, k, d, 2)
+                    if c3[k, j, i] == PRES_E:
+                        for d in outlet_dirs:
+                            if not (int(b3[k, j, i]) >> d) & 1:
+                                link(i, j, k, d, 3)

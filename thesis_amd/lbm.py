@@ -89,7 +89,7 @@ def load():
     L.lbm_device_bytes.argtypes = [vp]
     L.lbm_device_bytes.restype = C.c_int64
     L.lbm_read_lattice.argtypes = [C.c_char_p, C.c_char_p, P(C.c_int32), P(C.c_int64), P(C.c_uint8),
-                                   P(C.c_uint32), P(C.c_int64), P(C.c_int64)]
+                                   P(C.c_uint32), P(C.c_int64), P(C.c_int64), P(C.c_double), P(C.c_int32)]
     for name in EXPORTED:
         fn = getattr(L, name)
         if fn.restype is C.c_int and name not in ("lbm_abi_version", "lbm_device_count"):
@@ -126,7 +126,7 @@ def read_lattice(node_path, bc_path=None):
     dims = (C.c_int32 * 3)()
     nl = C.c_int64()
     nb = bc_path.encode() if bc_path else None
-    st = L.lbm_read_lattice(node_path.encode(), nb, dims, C.byref(nl), None, None, None, None)
+    st = L.lbm_read_lattice(node_path.encode(), nb, dims, C.byref(nl), None, None, None, None, None, None)
     if st:
         raise LbmError(st, f"cannot read {node_path} / {bc_path}")
     nx, ny, nz = dims[:]
@@ -135,13 +135,16 @@ def read_lattice(node_path, bc_path=None):
     bb = np.zeros(n, np.uint32)
     tc = np.zeros(6, np.int64)
     dc = np.zeros(19, np.int64)
+    delta = C.c_double()
+    cyc = (C.c_int32 * 3)()
     st = L.lbm_read_lattice(node_path.encode(), nb, dims, C.byref(nl),
                             cls.ctypes.data_as(C.POINTER(C.c_uint8)), bb.ctypes.data_as(C.POINTER(C.c_uint32)),
-                            tc.ctypes.data_as(C.POINTER(C.c_int64)), dc.ctypes.data_as(C.POINTER(C.c_int64)))
+                            tc.ctypes.data_as(C.POINTER(C.c_int64)), dc.ctypes.data_as(C.POINTER(C.c_int64)),
+                            C.byref(delta), cyc)
     if st:
         raise LbmError(st, f"cannot read {node_path} / {bc_path}")
     return dict(dims=(nx, ny, nz), node_class=cls.reshape(nz, ny, nx), bb_links=bb.reshape(nz, ny, nx),
-                nlinks=nl.value, type_counts=tc, dir_counts=dc)
+                nlinks=nl.value, type_counts=tc, dir_counts=dc, delta=delta.value, cyclic=tuple(cyc[:]))
 
 
 class Solver:
