@@ -30,6 +30,8 @@ EXPORTED = [
     "lbm_abi_version", "lbm_device_count", "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_init",
     "lbm_step", "lbm_run_to_residual", "lbm_reduce", "lbm_get_macros", "lbm_get_f", "lbm_set_f",
     "lbm_time_steps", "lbm_device_bytes", "lbm_read_lattice",
+    "lbm_cg_create", "lbm_cg_destroy", "lbm_cg_last_error", "lbm_cg_init", "lbm_cg_step", "lbm_cg_get_macros",
+    "lbm_cg_get_f", "lbm_cg_reduce", "lbm_cg_time_steps",
 ]
 
 
@@ -47,6 +49,15 @@ class Params(C.Structure):
         ("force_model", C.c_int32), ("force", C.c_double * 3), ("storage", C.c_int32),
         ("wall_model", C.c_int32), ("inlet_model", C.c_int32), ("parity_flags", C.c_uint32),
         ("rho_out", C.c_double),
+    ]
+
+
+class CgParams(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_uint32), ("nx", C.c_int32), ("ny", C.c_int32), ("r_alpha", C.c_double),
+        ("b_alpha", C.c_double), ("r_nu", C.c_double), ("b_nu", C.c_double), ("beta", C.c_double),
+        ("A", C.c_double), ("g_limit", C.c_double), ("high_order", C.c_int32), ("kappa", C.c_double),
+        ("storage", C.c_int32),
     ]
 
 
@@ -90,6 +101,17 @@ def load():
     L.lbm_device_bytes.restype = C.c_int64
     L.lbm_read_lattice.argtypes = [C.c_char_p, C.c_char_p, P(C.c_int32), P(C.c_int64), P(C.c_uint8),
                                    P(C.c_uint32), P(C.c_int64), P(C.c_int64), P(C.c_double), P(C.c_int32)]
+    L.lbm_cg_create.argtypes = [P(CgParams), P(vp)]
+    L.lbm_cg_destroy.argtypes = [vp]
+    L.lbm_cg_destroy.restype = None
+    L.lbm_cg_last_error.argtypes = [vp]
+    L.lbm_cg_last_error.restype = C.c_char_p
+    L.lbm_cg_init.argtypes = [vp, dp, dp]
+    L.lbm_cg_step.argtypes = [vp, C.c_int64]
+    L.lbm_cg_get_macros.argtypes = [vp, dp, dp, dp, dp, dp]
+    L.lbm_cg_get_f.argtypes = [vp, dp, dp]
+    L.lbm_cg_reduce.argtypes = [vp, dp]
+    L.lbm_cg_time_steps.argtypes = [vp, C.c_int64, dp]
     for name in EXPORTED:
         fn = getattr(L, name)
         if fn.restype is C.c_int and name not in ("lbm_abi_version", "lbm_device_count"):
@@ -243,3 +265,65 @@ class Solver:
 
     def device_bytes(self):
         return self._L.lbm_device_bytes(self.h)
+
+
+class CgSolver:
+    """Colour-gradient D2Q9 two-phase step on a periodic nx x ny box (GPU).
+    Same keyword arguments as the oracle's CgSim; arrays are (ny, nx)."""
+
+    def __init__(self, nx, ny, *, r_alpha=4 / 9, b_alpha=4 / 9, r_nu=1 / 6, b_nu=1 / 6, beta=0.7, A=0.01125,
+                 g_limit=0.0, high_order=0, kappa=0.9, storage=STORE_F64):
+        L = load()
+        if isinstance(storage, str):
+            storage = STORAGE_NAMES[storage]
+        p = CgParams()
+        p.abi_version, p.nx, p.ny = ABI_VERSION, nx, ny
+        p.r_alpha, p.b_alpha, p.r_nu, p.b_nu, p.beta, p.A = r_alpha, b_alpha, r_nu, b_nu, beta, A
+        p.g_limit, p.high_order, p.kappa, p.storage = g_limit, high_order, kappa, storage
+        self.nx, self.ny, self.n = nx, ny, nx * ny
+        h = C.c_void_p()
+        st = L.lbm_cg_create(C.byref(p), C.byref(h))
+        if st:
+            raise LbmError(st, "lbm_cg_create failed")
+        self.h, self._L = h, L
+
+    def _check(self, st):
+        if st:
+            raise LbmError(st, self._L.lbm_cg_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.lbm_cg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def init(self, r_rho, b_rho):
+        r, b = _f64(r_rho, self.n), _f64(b_rho, self.n)
+        self._check(self._L.lbm_cg_init(self.h, _dp(r), _dp(b)))
+
+    def step(self, n=1):
+        self._check(self._L.lbm_cg_step(self.h, int(n)))
+
+    def macros(self):
+        out = [np.zeros(self.n) for _ in range(5)]
+        self._check(self._L.lbm_cg_get_macros(self.h, *[_dp(a) for a in out]))
+        return dict(zip(["r_rho", "b_rho", "rho", "u", "v"], [a.reshape(self.ny, self.nx) for a in out]))
+
+    def get_f(self):
+        r, b = np.zeros(9 * self.n), np.zeros(9 * self.n)
+        self._check(self._L.lbm_cg_get_f(self.h, _dp(r), _dp(b)))
+        return r.reshape(9, self.ny, self.nx), b.reshape(9, self.ny, self.nx)
+
+    def reduce(self):
+        out = np.zeros(12)
+        self._check(self._L.lbm_cg_reduce(self.h, _dp(out)))
+        keys = ["mass_r", "mass_b", "pin_sum", "pin_n", "pout_sum", "pout_n", "rho_in_sum", "rho_in_n",
+                "rho_out_sum", "rho_out_n", "umax", "nonfinite"]
+        return dict(zip(keys, out))
+
+    def time_steps(self, n):
+        ms = C.c_double()
+        self._check(self._L.lbm_cg_time_steps(self.h, int(n), C.byref(ms)))
+        return ms.value
