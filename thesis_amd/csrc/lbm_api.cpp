@@ -110,6 +110,10 @@ lbm_status ensure_fout(lbm_ctx* c) {
 lbm_status run_steps(lbm_ctx* c, int64_t nsteps) {
     for (int64_t k = 0; k < nsteps; ++k) {
         const bool wm = (k == nsteps - 1);
+        // per-launch seed of the stochastic rounding: a hash of the step count
+        uint64_t z = (uint64_t)(c->steps + k + 1) * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        c->kp.seed = (uint32_t)(z ^ (z >> 31));
         hipError_t e = launch_step(c->q, c->p.storage, c->p.collision, c->open, c->kp, c->P[c->cur],
                                    c->P[c->cur ^ 1], c->node, c->cidx, fields(c), wm, c->stream);
         if (e != hipSuccess) return fail(c, LBM_EDEVICE, "step launch: %s", hipGetErrorString(e));
@@ -357,6 +361,8 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     k.simple = simple ? 1 : 0;
     k.rank = rank_ok ? 1 : 0;
     if (const char* e = std::getenv("LBM_RANK")) k.rank = rank_ok && e[0] == '1';
+    k.sr = 1;
+    if (const char* e = std::getenv("LBM_SR")) k.sr = e[0] == '1';
     auto h2d = [&](void* d, const void* h, size_t b) { return hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, c->stream); };
     if (h2d(c->node, node.data(), sizeof(uint32_t) * n) != hipSuccess ||
         h2d(c->cidx, cidx.data(), sizeof(uint32_t) * n) != hipSuccess ||

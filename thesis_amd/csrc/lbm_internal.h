@@ -35,6 +35,8 @@ struct KParams {
     const uint32_t* didx;    // compact -> dense node index
     const uint32_t* nodew;   // compact node words
     int rank;                // use the row-bitmask step (no per-node index loads)
+    int sr;                  // stochastic rounding of 16-bit stores
+    uint32_t seed;           // per-launch random seed for stochastic rounding
     int simple;              // walls are exactly the links into solid nodes; all fluid plain
     int words;               // 32-node words per row
     int rmax;                // rows staged per workgroup (LDS sizing)

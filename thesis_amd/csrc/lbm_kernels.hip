@@ -154,6 +154,7 @@ template <class L> struct Io<L, 0> {
     __device__ static raw pk(T v, int) { return v; }
     __device__ static T ld(const raw* p, uint32_t k, int) { return ldo(p, k); }
     __device__ static void st(raw* p, uint32_t k, T v, int) { sto(p, k, v); }
+    __device__ static void st_sr(raw* p, uint32_t k, T v, int, uint32_t) { sto(p, k, v); }
 };
 template <class L> struct Io<L, 1> {
     using raw = float; using T = float;
@@ -161,6 +162,7 @@ template <class L> struct Io<L, 1> {
     __device__ static raw pk(T v, int) { return v; }
     __device__ static T ld(const raw* p, uint32_t k, int) { return ldo(p, k); }
     __device__ static void st(raw* p, uint32_t k, T v, int) { sto(p, k, v); }
+    __device__ static void st_sr(raw* p, uint32_t k, T v, int, uint32_t) { sto(p, k, v); }
 };
 template <class L> struct Io<L, 2> {
     using raw = uint16_t; using T = float;
@@ -168,6 +170,12 @@ template <class L> struct Io<L, 2> {
     __device__ static raw pk(T v, int i) { return __builtin_bit_cast(uint16_t, (_Float16)(v * f16_ainv<L>(i))); }
     __device__ static T ld(const raw* p, uint32_t k, int i) { return cv(ldo(p, k), i); }
     __device__ static void st(raw* p, uint32_t k, T v, int i) { sto(p, k, pk(v, i)); }
+    // unbiased (stochastic) rounding with the random bits rnd
+    __device__ static void st_sr(raw* p, uint32_t k, T v, int i, uint32_t rnd) {
+        typedef short s2 __attribute__((ext_vector_type(2)));
+        const s2 r = __builtin_amdgcn_cvt_sr_f16_f32(s2{0, 0}, v * f16_ainv<L>(i), rnd, false);
+        sto(p, k, (uint16_t)r[0]);
+    }
 };
 template <class L> struct Io<L, 3> {
     using raw = uint16_t; using T = float;
@@ -175,6 +183,11 @@ template <class L> struct Io<L, 3> {
     __device__ static raw pk(T v, int i) { return f32_to_bf16(v * bf16_ainv<L>(i)); }
     __device__ static T ld(const raw* p, uint32_t k, int i) { return cv(ldo(p, k), i); }
     __device__ static void st(raw* p, uint32_t k, T v, int i) { sto(p, k, pk(v, i)); }
+    __device__ static void st_sr(raw* p, uint32_t k, T v, int i, uint32_t rnd) {
+        typedef short s2 __attribute__((ext_vector_type(2)));
+        const s2 r = __builtin_amdgcn_cvt_sr_bf16_f32(s2{0, 0}, v * bf16_ainv<L>(i), rnd, false);
+        sto(p, k, (uint16_t)r[0]);
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -427,6 +440,28 @@ __device__ inline void collide(const KParams& kp, const T* h, T drho, T rho, con
     }
 }
 
+// Store of one node's post-collision populations.  The 16-bit storages use
+// stochastic rounding (unbiased: a body force smaller than half an ulp of the
+// stored deviation still accumulates), with random bits from a hash of the
+// compact node index and a per-launch seed, rotated per direction; results are
+// deterministic for a given seed sequence.
+template <class L, int S>
+__device__ inline void store_node(typename Io<L, S>::raw* __restrict__ dst, const KParams& kp, uint32_t own,
+                                  const typename Io<L, S>::T* out) {
+    if (S >= 2 && kp.sr) {
+        uint32_t h = own * 0x9E3779B1u + kp.seed;
+        h ^= h >> 15;
+        h *= 0x2C1B3C6Du;
+        h ^= h >> 12;
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i)
+            Io<L, S>::st_sr(dst, paddr<L>(kp, i, own), out[i], i, (h << (i & 31)) | (h >> ((32 - i) & 31)));
+    } else {
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i) Io<L, S>::st(dst, paddr<L>(kp, i, own), out[i], i);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // kernels
 // Quotient of a 32-bit n by an invariant divisor d: q = (((m n) >> 32) + n) >> s
@@ -478,8 +513,7 @@ __global__ __launch_bounds__(256) void k_step(const typename Io<L, S>::raw* __re
     }
     T out[L::Q];
     collide<L, T, C>(kp, h, drho, rho, u, out);
-#pragma unroll
-    for (int i = 0; i < L::Q; ++i) Io<L, S>::st(dst, paddr<L>(kp, i, own), out[i], i);
+    store_node<L, S>(dst, kp, own, out);
 }
 
 // Rank variant of the step: no per-node index loads.  For every lattice row
@@ -593,8 +627,7 @@ __global__ __launch_bounds__(256) void k_stepr(const typename Io<L, S>::raw* __r
     }
     T out[L::Q];
     collide<L, T, C>(kp, h, drho, rho, u, out);
-#pragma unroll
-    for (int i = 0; i < L::Q; ++i) Io<L, S>::st(dst, paddr<L>(kp, i, own), out[i], i);
+    store_node<L, S>(dst, kp, own, out);
 }
 
 // Post-boundary populations (dense, full, FP64) and moments of the state in src.
