@@ -191,6 +191,51 @@ lbm_status lbm_cg_get_f(lbm_cg_ctx* ctx, double* r_f, double* b_f);
 lbm_status lbm_cg_reduce(lbm_cg_ctx* ctx, double out[12]);
 lbm_status lbm_cg_time_steps(lbm_cg_ctx* ctx, int64_t nsteps, double* ms);
 
+/* ---- batched channels: many D2Q9 lattices in one persistent launch ------- */
+/* The reference runs one lattice per process: Iterate2D's loop of
+ * collide/stream/boundary/macro launches with a residual check every
+ * ShowMacroDiff steps (S/Iterate.cu:385-625, S/ComputeResiduals.cu:13-63).
+ * A batch advances N independent D2Q9 channels (open x faces, walls along
+ * y given by bb_links and by solid nodes, Zou-He velocity inlet on LBM_VEL_W
+ * and pressure outlet on LBM_PRES_E nodes, half-way bounce-back) in one
+ * cooperative kernel with a grid barrier per step; each instance stops at
+ * its own residual. */
+typedef struct lbm_batch lbm_batch;
+typedef struct {
+    int32_t nx, ny;
+    const uint8_t* node_class;  /* nx*ny, x fastest; LBM_SOLID .. LBM_PRES_E */
+    const uint32_t* bb_links;   /* nx*ny wall links as in lbm_lattice; NULL: only links into solid nodes */
+    const double* u_bc;         /* nx*ny inlet velocity on LBM_VEL_W nodes (NULL: 0) */
+    const double* v_bc;
+    double omega;               /* 1/tau */
+    double trt_magic;           /* Lambda; 0 selects 3/16 */
+    double rho_out;             /* outlet density; 0 selects 1 */
+} lbm_batch_instance;
+
+/* collision LBM_BGK or LBM_TRT; storage LBM_STORE_F64 or LBM_STORE_F32.
+ * Instances start at rest (rho = 1, u = 0). */
+lbm_status lbm_batch_create(const lbm_batch_instance* inst, int32_t n, int32_t collision, int32_t storage,
+                            lbm_batch** out);
+void lbm_batch_destroy(lbm_batch* b);
+const char* lbm_batch_last_error(const lbm_batch* b);
+/* Equilibrium start of instance i (i = -1: every instance) at rho, u, v
+ * (nx*ny doubles each; NULL: 1, 0, 0).  Also the residual's reference field. */
+lbm_status lbm_batch_init(lbm_batch* b, int32_t i, const double* rho, const double* u, const double* v);
+/* One persistent launch: every instance steps until its residual
+ * sqrt(sum |u - u_prev|^2 / sum |u|^2) over check_every steps (FP64, over
+ * non-solid nodes, as lbm_run_to_residual) falls below tol, or max_steps.
+ * steps[i] = steps instance i took, residual[i] = its last residual (NaN:
+ * non-finite field), *ms = device time of the launch; any may be NULL.
+ * Returns LBM_DIVERGED when an instance went non-finite (outputs filled). */
+lbm_status lbm_batch_run(lbm_batch* b, double tol, int32_t check_every, int64_t max_steps, int64_t* steps,
+                         double* residual, double* ms);
+/* Fields of instance i at its last check step (nx*ny doubles; NULL skips). */
+lbm_status lbm_batch_get_macros(lbm_batch* b, int32_t i, double* rho, double* u, double* v);
+/* Post-boundary populations f[q*nx*ny + node] of instance i's newest state (FP64). */
+lbm_status lbm_batch_get_f(lbm_batch* b, int32_t i, double* f);
+/* Total nodes of all instances; *fluid (may be NULL) = non-solid nodes. */
+int64_t lbm_batch_nodes(const lbm_batch* b, int64_t* fluid);
+
 #ifdef __cplusplus
 }
 #endif

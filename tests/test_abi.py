@@ -41,6 +41,47 @@ def test_create_rejects_bad_parameters():
         L.Solver(9, (4, 4), bad, ubc=np.zeros(16))
 
 
+def test_batch_create_rejects_bad_parameters():
+    ch = dict(node_class=np.ones((4, 8), np.uint8), bb=None, ubc=None, vbc=None, omega=1.0)
+    cases = [
+        (dict(ch, omega=2.5), {}),                                   # tau < 1/2
+        (dict(ch, node_class=np.zeros((4, 8), np.uint8)), {}),       # no fluid node
+        (ch, dict(collision=L.MRT)),                                 # batch runs BGK/TRT
+        (ch, dict(storage="f16")),                                   # batch stores F64/F32
+    ]
+    bad = ch["node_class"].copy()
+    bad[:, 0] = L.VEL_W                                              # inlet without u_bc
+    cases.append((dict(ch, node_class=bad), {}))
+    bad = ch["node_class"].copy()
+    bad[2, 3] = L.PRES_E                                             # outlet off the east face
+    cases.append((dict(ch, node_class=bad), {}))
+    for inst, kw in cases:
+        with pytest.raises(L.LbmError) as ei:
+            L.Batch([ch, inst], **kw)
+        assert ei.value.status == 3 and "device" not in str(ei.value), str(ei.value)
+
+
+def test_channel_sweep_is_seeded_and_in_range():
+    from thesis_amd import lattices as LT
+    a = LT.channel_sweep(count=64, seed=7)
+    b = LT.channel_sweep(count=64, seed=7)
+    assert len(a) == 64
+    for d, e in zip(a, b):
+        assert (d["node_class"] == e["node_class"]).all() and d["omega"] == e["omega"]
+        assert 128 <= d["nx"] <= 1024 and 32 <= d["ny"] <= 256 and d["ny"] <= d["nx"] // 4
+        assert 1 <= len(d["obstacles"]) <= 5 and 10 <= d["Re"] <= 100
+        nu = (1 / d["omega"] - 0.5) / 3
+        assert abs(d["u_max"] * d["ny"] / nu - d["Re"]) < 1e-9 * d["Re"]
+        for (cx, cy, r) in d["obstacles"]:
+            assert 4 <= r <= max(4, min(20, d["ny"] / 8))
+            assert cx - r >= d["ny"] / 2 and cx + r <= d["nx"] - d["ny"]
+            assert cy - r >= 2 and cy + r <= d["ny"] - 3
+        # inlet and outlet columns stay fully open apart from the wall rows
+        assert (d["node_class"][:, 0] == L.VEL_W).all() and (d["node_class"][:, -1] == L.PRES_E).all()
+    sizes = [d["nx"] * d["ny"] for d in a]
+    assert max(sizes) / min(sizes) > 16
+
+
 def test_read_lattice_missing_file_is_eio(tmp_path):
     with pytest.raises(L.LbmError) as ei:
         L.read_lattice(str(tmp_path / "nope_Node.dat"), None)

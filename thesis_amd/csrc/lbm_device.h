@@ -1,0 +1,434 @@
+// Device building blocks of the lattice-Boltzmann step shared by the step
+// kernels (lbm_kernels.hip) and the batched persistent kernel (lbm_batch.hip):
+// lattices, storage formats, pull gather, boundaries, moments, collision.
+#pragma once
+#include "lbm_internal.h"
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <utility>
+
+namespace lbm {
+
+// ---------------------------------------------------------------------------
+// lattices: reference numbering (S/GpuInit.cu:94-99, :822-842)
+struct D2Q9 {
+    static constexpr int Q = 9;
+    __host__ __device__ static constexpr int cx(int i) { return i == 1 || i == 5 || i == 8 ? 1 : (i == 3 || i == 6 || i == 7 ? -1 : 0); }
+    __host__ __device__ static constexpr int cy(int i) { return i == 2 || i == 5 || i == 6 ? 1 : (i == 4 || i == 7 || i == 8 ? -1 : 0); }
+    __host__ __device__ static constexpr int cz(int) { return 0; }
+    __host__ __device__ static constexpr int opp(int i) { return i == 0 ? 0 : (i <= 4 ? (i <= 2 ? i + 2 : i - 2) : (i <= 6 ? i + 2 : i - 2)); }
+    __host__ __device__ static constexpr double w(int i) { return i == 0 ? 4.0 / 9.0 : (i <= 4 ? 1.0 / 9.0 : 1.0 / 36.0); }
+};
+
+struct D3Q19 {
+    static constexpr int Q = 19;
+    __host__ __device__ static constexpr int cx(int i) {
+        return (i == 1 || i == 7 || i == 9 || i == 11 || i == 13) ? 1 : ((i == 2 || i == 8 || i == 10 || i == 12 || i == 14) ? -1 : 0);
+    }
+    __host__ __device__ static constexpr int cy(int i) {
+        return (i == 3 || i == 7 || i == 8 || i == 15 || i == 17) ? 1 : ((i == 4 || i == 9 || i == 10 || i == 16 || i == 18) ? -1 : 0);
+    }
+    __host__ __device__ static constexpr int cz(int i) {
+        return (i == 5 || i == 11 || i == 12 || i == 15 || i == 16) ? 1 : ((i == 6 || i == 13 || i == 14 || i == 17 || i == 18) ? -1 : 0);
+    }
+    // pairs (1,2)(3,4)(5,6)(7,10)(8,9)(11,14)(12,13)(15,18)(16,17)
+    __host__ __device__ static constexpr int opp(int i) {
+        return i == 0 ? 0 : i <= 6 ? ((i & 1) ? i + 1 : i - 1)
+             : i == 7 ? 10 : i == 10 ? 7 : i == 8 ? 9 : i == 9 ? 8
+             : i == 11 ? 14 : i == 14 ? 11 : i == 12 ? 13 : i == 13 ? 12
+             : i == 15 ? 18 : i == 18 ? 15 : i == 16 ? 17 : 16;
+    }
+    __host__ __device__ static constexpr double w(int i) { return i == 0 ? 1.0 / 3.0 : (i <= 6 ? 1.0 / 18.0 : 1.0 / 36.0); }
+};
+
+static_assert(D2Q9::opp(1) == 3 && D2Q9::opp(5) == 7 && D2Q9::opp(8) == 6, "D2Q9 opp");
+static_assert(D3Q19::opp(7) == 10 && D3Q19::opp(16) == 17 && D3Q19::opp(2) == 1, "D3Q19 opp");
+
+// MRT moment polynomials (the rows of M tabulated in S/CellFunctions.cu:15-36
+// for D2Q9 and :73-114 for D3Q19).  Rows are orthogonal: Minv = M^T / |row|^2.
+template <class L>
+__host__ __device__ constexpr double mrow(int r, int i) {
+    const double x = L::cx(i), y = L::cy(i), z = L::cz(i), c2 = x * x + y * y + z * z;
+    if (L::Q == 9) {
+        return r == 0 ? 1 : r == 1 ? 3 * c2 - 4 : r == 2 ? (9 * c2 * c2 - 21 * c2 + 8) / 2
+             : r == 3 ? x : r == 4 ? (3 * c2 - 5) * x : r == 5 ? y : r == 6 ? (3 * c2 - 5) * y
+             : r == 7 ? x * x - y * y : x * y;
+    }
+    return r == 0 ? 1 : r == 1 ? 19 * c2 - 30 : r == 2 ? (21 * c2 * c2 - 53 * c2 + 24) / 2
+         : r == 3 ? x : r == 4 ? (5 * c2 - 9) * x : r == 5 ? y : r == 6 ? (5 * c2 - 9) * y
+         : r == 7 ? z : r == 8 ? (5 * c2 - 9) * z : r == 9 ? 3 * x * x - c2
+         : r == 10 ? (3 * c2 - 5) * (3 * x * x - c2) : r == 11 ? y * y - z * z
+         : r == 12 ? (3 * c2 - 5) * (y * y - z * z) : r == 13 ? x * y : r == 14 ? y * z
+         : r == 15 ? x * z : r == 16 ? (y * y - z * z) * x : r == 17 ? (z * z - x * x) * y
+         : (x * x - y * y) * z;
+}
+template <class L>
+__host__ __device__ constexpr double mnorm2(int r) {
+    double s = 0;
+    for (int i = 0; i < L::Q; ++i) s += mrow<L>(r, i) * mrow<L>(r, i);
+    return s;
+}
+template <class L>
+__host__ __device__ constexpr double mrest(int r) {   // (M w)_r: moments of the rest state
+    double s = 0;
+    for (int i = 0; i < L::Q; ++i) s += mrow<L>(r, i) * L::w(i);
+    return s;
+}
+static_assert(mrest<D2Q9>(1) > -2.0000001 && mrest<D2Q9>(1) < -1.9999999, "D2Q9 rest e");
+static_assert(mrest<D2Q9>(2) > 0.9999999 && mrest<D2Q9>(2) < 1.0000001, "D2Q9 rest eps");
+static_assert(mrest<D3Q19>(1) > -11.0000001 && mrest<D3Q19>(1) < -10.9999999, "D3Q19 rest e");
+static_assert(mrest<D3Q19>(2) > 2.9999999 && mrest<D3Q19>(2) < 3.0000001, "D3Q19 rest eps");
+
+// Compile-time coefficients of M and Minv; the products below are expanded by
+// fold expressions so every zero drops out and every +-1 becomes an add.
+template <class L, int R, int I> struct Mc {
+    static constexpr double m = mrow<L>(R, I);
+    static constexpr double mi = mrow<L>(R, I) / mnorm2<L>(R);
+};
+template <class L, class T, int R, int... I>
+__device__ inline T m_row_dot(const T* h, std::integer_sequence<int, I...>) {
+    T acc = 0;
+    ((Mc<L, R, I>::m != 0.0 ? (void)(acc += (T)Mc<L, R, I>::m * h[I]) : (void)0), ...);
+    return acc;
+}
+template <class L, class T, int... R>
+__device__ inline void m_apply(const T* h, T* m, std::integer_sequence<int, R...>) {
+    ((m[R] = m_row_dot<L, T, R>(h, std::make_integer_sequence<int, L::Q>{})), ...);
+}
+template <class L, class T, int I, int... R>
+__device__ inline T mi_col_dot(const T* m, std::integer_sequence<int, R...>) {
+    T acc = 0;
+    ((Mc<L, R, I>::mi != 0.0 ? (void)(acc += (T)Mc<L, R, I>::mi * m[R]) : (void)0), ...);
+    return acc;
+}
+template <class L, class T, int... I>
+__device__ inline void mi_apply(const T* m, T* out, std::integer_sequence<int, I...>) {
+    ((out[I] = mi_col_dot<L, T, I>(m, std::make_integer_sequence<int, L::Q>{})), ...);
+}
+
+// ---------------------------------------------------------------------------
+// storage: deviations h_i, raw in F64/F32; scaled by 1/(w_i a) in 16 bits
+// FP16: stored value = h / w_i * 2^10 keeps typical deviations in the normal range.
+template <class L> __device__ inline float f16_a(int i) { return (float)(L::w(i) / 1024.0); }
+template <class L> __device__ inline float f16_ainv(int i) { return (float)(1024.0 / L::w(i)); }
+template <class L> __device__ inline float bf16_a(int i) { return (float)L::w(i); }
+template <class L> __device__ inline float bf16_ainv(int i) { return (float)(1.0 / L::w(i)); }
+
+__device__ inline float bf16_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ inline uint16_t f32_to_bf16(float f) {   // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// Loads and stores take 32-bit element offsets; the address is formed as
+// base + 32-bit byte offset so it maps onto global_load/store with a scalar
+// base register and one vector offset (population buffers stay below 4 GiB).
+template <class R> __device__ inline R ldo(const R* base, uint32_t k) {
+    return *(const R*)((const char*)base + k * (uint32_t)sizeof(R));
+}
+template <class R> __device__ inline void sto(R* base, uint32_t k, R v) {
+    __builtin_nontemporal_store(v, (R*)((char*)base + k * (uint32_t)sizeof(R)));
+}
+
+template <class L, int S> struct Io;
+template <class L> struct Io<L, 0> {
+    using raw = double; using T = double;
+    __device__ static T cv(raw r, int) { return r; }
+    __device__ static raw pk(T v, int) { return v; }
+    __device__ static T ld(const raw* p, uint32_t k, int) { return ldo(p, k); }
+    __device__ static void st(raw* p, uint32_t k, T v, int) { sto(p, k, v); }
+    __device__ static void st_sr(raw* p, uint32_t k, T v, int, uint32_t) { sto(p, k, v); }
+};
+template <class L> struct Io<L, 1> {
+    using raw = float; using T = float;
+    __device__ static T cv(raw r, int) { return r; }
+    __device__ static raw pk(T v, int) { return v; }
+    __device__ static T ld(const raw* p, uint32_t k, int) { return ldo(p, k); }
+    __device__ static void st(raw* p, uint32_t k, T v, int) { sto(p, k, v); }
+    __device__ static void st_sr(raw* p, uint32_t k, T v, int, uint32_t) { sto(p, k, v); }
+};
+template <class L> struct Io<L, 2> {
+    using raw = uint16_t; using T = float;
+    __device__ static T cv(raw r, int i) { return (float)__builtin_bit_cast(_Float16, r) * f16_a<L>(i); }
+    __device__ static raw pk(T v, int i) { return __builtin_bit_cast(uint16_t, (_Float16)(v * f16_ainv<L>(i))); }
+    __device__ static T ld(const raw* p, uint32_t k, int i) { return cv(ldo(p, k), i); }
+    __device__ static void st(raw* p, uint32_t k, T v, int i) { sto(p, k, pk(v, i)); }
+    // unbiased (stochastic) rounding with the random bits rnd
+    __device__ static void st_sr(raw* p, uint32_t k, T v, int i, uint32_t rnd) {
+        typedef short s2 __attribute__((ext_vector_type(2)));
+        const s2 r = __builtin_amdgcn_cvt_sr_f16_f32(s2{0, 0}, v * f16_ainv<L>(i), rnd, false);
+        sto(p, k, (uint16_t)r[0]);
+    }
+};
+template <class L> struct Io<L, 3> {
+    using raw = uint16_t; using T = float;
+    __device__ static T cv(raw r, int i) { return bf16_to_f32(r) * bf16_a<L>(i); }
+    __device__ static raw pk(T v, int i) { return f32_to_bf16(v * bf16_ainv<L>(i)); }
+    __device__ static T ld(const raw* p, uint32_t k, int i) { return cv(ldo(p, k), i); }
+    __device__ static void st(raw* p, uint32_t k, T v, int i) { sto(p, k, pk(v, i)); }
+    __device__ static void st_sr(raw* p, uint32_t k, T v, int i, uint32_t rnd) {
+        typedef short s2 __attribute__((ext_vector_type(2)));
+        const s2 r = __builtin_amdgcn_cvt_sr_bf16_f32(s2{0, 0}, v * bf16_ainv<L>(i), rnd, false);
+        sto(p, k, (uint16_t)r[0]);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// node pipeline pieces (T = double for F64 storage, float otherwise)
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// Quotient of a 32-bit n by an invariant divisor d: q = (((m n) >> 32) + n) >> s
+// with m = floor(2^32 (2^s - d) / d) + 1, s = ceil(log2 d) (exact for all n < 2^32).
+__device__ inline uint32_t fdiv(uint32_t n, uint64_t m, uint32_t s) {
+    return (uint32_t)((((m * n) >> 32) + n) >> s);
+}
+
+// Stage 1 of the pull: compact indices of the source node of every direction
+// (periodic wrap; kNone marks a source outside a non-periodic face).  These
+// loads do not depend on the node's own word, so they issue together.
+template <class L, bool OPEN>
+__device__ inline void source_index(const uint32_t* __restrict__ cidx, const KParams& kp, int x, int y, int z,
+                                    uint32_t* cn) {
+    const int nx = kp.nx, ny = kp.ny, nz = kp.nz;
+    const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+    const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
+    const int zm = z == 0 ? nz - 1 : z - 1, zp = z == nz - 1 ? 0 : z + 1;
+    bool xm_o = false, xp_o = false, ym_o = false, yp_o = false, zm_o = false, zp_o = false;
+    if (OPEN) {
+        xm_o = !kp.per[0] && x == 0;  xp_o = !kp.per[0] && x == nx - 1;
+        ym_o = !kp.per[1] && y == 0;  yp_o = !kp.per[1] && y == ny - 1;
+        zm_o = !kp.per[2] && z == 0;  zp_o = !kp.per[2] && z == nz - 1;
+    }
+    // row bases of the three source rows in y and planes in z
+    const uint32_t ux = (uint32_t)nx, plane = (uint32_t)nx * (uint32_t)ny;
+    const uint32_t yb[3] = {(uint32_t)ym * ux, (uint32_t)y * ux, (uint32_t)yp * ux};     // index 1 - cy
+    const uint32_t zb[3] = {(uint32_t)zm * plane, (uint32_t)z * plane, (uint32_t)zp * plane};
+#pragma unroll
+    for (int i = 1; i < L::Q; ++i) {
+        const int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
+        const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
+        bool out = false;
+        if (OPEN)
+            out = (cx > 0 && xm_o) || (cx < 0 && xp_o) || (cy > 0 && ym_o) || (cy < 0 && yp_o) ||
+                  (cz > 0 && zm_o) || (cz < 0 && zp_o);
+        cn[i] = out ? kNone : ldo(cidx, zb[1 - cz] + yb[1 - cy] + (uint32_t)sx);
+    }
+}
+
+// Stage 2: the populations.  A link whose source lies beyond a wall takes the
+// node's own value, P_opp(i) (half-way bounce-back) or P_i (on-node, fixed
+// afterwards by onnode_walls) (S/GpuStream.cu:8-28, :160-194;
+// S/GpuBoundaries.cu:400-490); a source outside a non-periodic face keeps the
+// node's own P_i (the reference's "stream not allowed"), to be replaced by
+// the open-boundary condition.
+// Element offset of population i of compact node j: SoA planes
+// P[i*qstride + j], or tiles of 64 nodes holding all Q planes
+// P[(j/64)*64*Q + i*64 + j%64], so one wave's populations sit in one
+// contiguous 64*Q-element block.
+template <class L>
+__device__ inline uint32_t paddr(const KParams& kp, int i, uint32_t j) {
+    if (kp.tile) return (j >> 6) * (64u * L::Q) + (uint32_t)i * 64u + (j & 63u);
+    return (uint32_t)i * kp.qstride32 + j;
+}
+
+template <class L, int S, bool OPEN>
+__device__ inline void gather(const typename Io<L, S>::raw* __restrict__ src, const KParams& kp,
+                              uint32_t own, uint32_t nd, const uint32_t* cn, typename Io<L, S>::T* h) {
+    h[0] = Io<L, S>::ld(src, paddr<L>(kp, 0, own), 0);
+#pragma unroll
+    for (int i = 1; i < L::Q; ++i) {
+        const int o = L::opp(i);
+        uint32_t j = cn[i];
+        int pl = i;
+        if (((nd >> o) & 1u) || j == kNone) {
+            j = own;
+            pl = ((nd >> o) & 1u) && kp.wall == 0 ? o : i;
+        }
+        h[i] = Io<L, S>::ld(src, paddr<L>(kp, pl, j), i);
+    }
+}
+
+// Open boundaries on D2Q9 faces, in full populations.
+// Velocity inlet (west): Zou-He, or the reference's form (S/GpuBoundaries.cu:45-64).
+// Pressure outlet (east): T/LBM/Tom/Lecture Script/LBMchannel.m:254-270.
+template <class L, class T>
+__device__ inline void zou_inlet(const KParams& kp, const Fields& fl, size_t idx, uint32_t nd, T* h) {
+    if (L::Q != 9) return;
+    if (kp.inlet == 1 && (nd & kBBMask)) return;       // reference skips corner nodes
+    T f[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f[i] = h[i] + (T)L::w(i);
+    const T u0 = (T)fl.ubc[idx], v0 = (T)fl.vbc[idx];
+    const T r = (f[0] + f[2] + f[4] + (T)2 * (f[3] + f[6] + f[7])) / ((T)1 - u0);
+    f[1] = f[3] + (T)(2.0 / 3.0) * r * u0;
+    if (kp.inlet == 1) {
+        f[5] = f[7] + r * u0 / (T)6;
+        f[8] = f[6] + r * u0 / (T)6;
+    } else {
+        f[5] = f[7] - (T)0.5 * (f[2] - f[4]) + r * u0 / (T)6 + (T)0.5 * r * v0;
+        f[8] = f[6] + (T)0.5 * (f[2] - f[4]) + r * u0 / (T)6 - (T)0.5 * r * v0;
+    }
+    h[1] = f[1] - (T)L::w(1); h[5] = f[5] - (T)L::w(5); h[8] = f[8] - (T)L::w(8);
+}
+template <class L, class T>
+__device__ inline void zou_outlet(const KParams& kp, T* h) {
+    if (L::Q != 9) return;
+    T f[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f[i] = h[i] + (T)L::w(i);
+    const T ro = (T)kp.rho_out;
+    const T ux = (T)-1 + (f[0] + f[2] + f[4] + (T)2 * (f[1] + f[5] + f[8])) / ro;
+    f[3] = f[1] - (T)(2.0 / 3.0) * ro * ux;
+    f[7] = f[5] + (T)0.5 * (f[2] - f[4]) - ro * ux / (T)6;
+    f[6] = f[8] - (T)0.5 * (f[2] - f[4]) - ro * ux / (T)6;
+    h[3] = f[3] - (T)L::w(3); h[6] = f[6] - (T)L::w(6); h[7] = f[7] - (T)L::w(7);
+}
+template <class L, class T>
+__device__ inline void onnode_walls(uint32_t nd, T* h) {
+#pragma unroll
+    for (int d = 1; d < L::Q; ++d)
+        if ((nd >> d) & 1u) h[L::opp(d)] = h[d];
+}
+
+__device__ inline double recip(double x) { return 1.0 / x; }
+__device__ inline float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Moments of h and the stored-velocity semantics of the force model
+// (S/GpuUpdateMacro.cu:6-41, :137-213).
+template <class L, class T>
+__device__ inline void moments(const KParams& kp, uint32_t cls, const T* h, T& drho, T& rho, T* u) {
+    T d = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) {
+        d += h[i];
+        if (L::cx(i)) jx += L::cx(i) > 0 ? h[i] : -h[i];
+        if (L::cy(i)) jy += L::cy(i) > 0 ? h[i] : -h[i];
+        if (L::cz(i)) jz += L::cz(i) > 0 ? h[i] : -h[i];
+    }
+    drho = d;
+    rho = (T)1 + d;
+    const T ir = recip(rho);
+    u[0] = jx * ir; u[1] = jy * ir; u[2] = jz * ir;
+    if (kp.force == 1) {
+        const T io = (T)(1.0 / kp.omega);
+        u[0] += (T)kp.g[0] * io; u[1] += (T)kp.g[1] * io; u[2] += (T)kp.g[2] * io;
+    } else if (kp.force == 2) {
+        u[0] += (T)(0.5 * kp.g[0]); u[1] += (T)(0.5 * kp.g[1]); u[2] += (T)(0.5 * kp.g[2]);
+    }
+    if ((kp.parity & 0x8u) && cls == PRES_E) u[1] = 0;
+}
+
+template <class L, class T>
+__device__ inline void guo_terms(const KParams& kp, T rho, const T* u, T* S) {
+    const T Fx = rho * (T)kp.g[0], Fy = rho * (T)kp.g[1], Fz = rho * (T)kp.g[2];
+    const T uF = u[0] * Fx + u[1] * Fy + u[2] * Fz;
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) {
+        const T cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
+        const T cF = cx * Fx + cy * Fy + cz * Fz;
+        const T cu = cx * u[0] + cy * u[1] + cz * u[2];
+        S[i] = (T)L::w(i) * ((T)3 * (cF - uF) + (T)9 * cu * cF);
+    }
+}
+
+// Deviations of the MRT equilibrium moments from the rest moments (M w).
+// D2Q9: S/GpuCollision.cu:986-994; D3Q19: :1065-1083.
+template <class L, class T>
+__device__ inline void mrt_dmeq(const KParams& kp, T drho, T rho, const T* u, T* dm) {
+    if (L::Q == 9) {
+        const T uu = u[0] * u[0] + u[1] * u[1];
+        const T k = (kp.parity & 0x1u) ? rho : (T)1;
+        dm[0] = drho;
+        dm[1] = (T)-2 * drho + (T)3 * k * rho * uu;
+        dm[2] = drho - (T)3 * k * rho * uu;
+        dm[3] = rho * u[0]; dm[4] = -rho * u[0];
+        dm[5] = rho * u[1]; dm[6] = -rho * u[1];
+        dm[7] = rho * (u[0] * u[0] - u[1] * u[1]);
+        dm[8] = rho * u[0] * u[1];
+    } else {
+        const T jx = rho * u[0], jy = rho * u[1], jz = rho * u[2];
+        const T j2 = jx * jx + jy * jy + jz * jz;
+        dm[0] = drho;
+        dm[1] = (T)-11 * drho + (T)19 * j2;
+        dm[2] = (T)(-475.0 / 63.0) * j2 - (T)mrest<D3Q19>(2);
+        dm[3] = jx; dm[4] = (T)(-2.0 / 3.0) * jx;
+        dm[5] = jy; dm[6] = (T)(-2.0 / 3.0) * jy;
+        dm[7] = jz; dm[8] = (T)(-2.0 / 3.0) * jz;
+        dm[9] = (T)2 * jx * jx - (jy * jy + jz * jz);
+        dm[10] = 0;
+        dm[11] = jy * jy - jz * jz;
+        dm[12] = 0;
+        dm[13] = jx * jy; dm[14] = jy * jz; dm[15] = jx * jz;
+        dm[16] = dm[17] = dm[18] = 0;
+    }
+}
+
+// Collision in deviation form.  BGK: S/GpuCollision.cu:528-550, :882-916;
+// TRT: :917-969 (general Lambda); MRT: :971-1037, :1039-1170.
+template <class L, class T, int C>
+__device__ inline void collide(const KParams& kp, const T* h, T drho, T rho, const T* u, T* out) {
+    const bool guo = kp.force == 2;
+    if (C == 2) {
+        T S[L::Q];
+        if (guo) guo_terms<L, T>(kp, rho, u, S);
+        T m[L::Q], dm[L::Q], gs[L::Q], dd[L::Q];
+        mrt_dmeq<L, T>(kp, drho, rho, u, dm);
+        m_apply<L, T>(h, m, std::make_integer_sequence<int, L::Q>{});
+        if (guo) m_apply<L, T>(S, gs, std::make_integer_sequence<int, L::Q>{});
+#pragma unroll
+        for (int r = 0; r < L::Q; ++r) {
+            const T s = (T)kp.s[r];
+            m[r] = s * (m[r] - dm[r]);
+            if (guo) m[r] -= ((T)1 - (T)0.5 * s) * gs[r];
+        }
+        mi_apply<L, T>(m, dd, std::make_integer_sequence<int, L::Q>{});
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i) out[i] = h[i] - dd[i];
+        if (L::Q == 19 && (kp.parity & 0x2u))
+            out[L::Q - 1] = -(T)L::w(L::Q - 1);   // fColl_18 stays at its zero initial value
+        return;
+    }
+    // BGK and TRT in symmetric/antisymmetric pair form: for a link pair
+    // (i, opp i) the parts of f_eq and of the Guo source are
+    //   e+ = w (drho + rho (4.5 cu^2 - 1.5 u.u)),  e- = 3 w rho cu,
+    //   S+ = w (9 cu cF - 3 u.F),                  S- = 3 w cF,
+    // relaxed with omega+ and omega- (equal for BGK) and prefactors 1 - omega/2.
+    const T op = (T)kp.omega;
+    const T om = C == 0 ? op : (T)kp.omega_m;
+    const T gp = (T)1 - (T)0.5 * op, gm = (T)1 - (T)0.5 * om;
+    const T uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    T Fx = 0, Fy = 0, Fz = 0, uF = 0;
+    if (guo) {
+        Fx = rho * (T)kp.g[0]; Fy = rho * (T)kp.g[1]; Fz = rho * (T)kp.g[2];
+        uF = u[0] * Fx + u[1] * Fy + u[2] * Fz;
+    }
+    {
+        const T w0 = (T)L::w(0);
+        out[0] = h[0] - op * (h[0] - w0 * (drho - (T)1.5 * rho * uu));
+        if (guo) out[0] -= gp * (T)3 * w0 * uF;
+    }
+#pragma unroll
+    for (int i = 1; i < L::Q; ++i) {
+        const int o = L::opp(i);
+        if (i < o) {
+            const T w = (T)L::w(i);
+            const T cu = (T)L::cx(i) * u[0] + (T)L::cy(i) * u[1] + (T)L::cz(i) * u[2];
+            const T ep = w * (drho + rho * ((T)4.5 * cu * cu - (T)1.5 * uu));
+            const T em = (T)3 * w * rho * cu;
+            T dp = op * ((T)0.5 * (h[i] + h[o]) - ep);
+            T dm = om * ((T)0.5 * (h[i] - h[o]) - em);
+            if (guo) {
+                const T cF = (T)L::cx(i) * Fx + (T)L::cy(i) * Fy + (T)L::cz(i) * Fz;
+                dp -= gp * w * ((T)9 * cu * cF - (T)3 * uF);
+                dm -= gm * (T)3 * w * cF;
+            }
+            out[i] = h[i] - dp - dm;
+            out[o] = h[o] - dp + dm;
+        }
+    }
+}
+
+}  // namespace lbm

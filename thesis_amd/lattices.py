@@ -82,6 +82,47 @@ def channel2d(nx, ny, u_max, obstacles=(), rho_out=1.0):
                 periodic=(0, 0, 0), rho_out=rho_out)
 
 
+def channel_sweep(count=64, seed=7, u_max=0.1, nx_range=(128, 1024), ny_range=(32, 256), re_range=(10.0, 100.0),
+                  obstacles=(1, 5), r_range=(4.0, 20.0), trt_magic=3.0 / 16.0):
+    """Seeded family of D2Q9 channels for a batched parametric sweep.
+
+    Per channel, from numpy.random.Generator(PCG64(seed)): nx, ny uniform
+    integers in the given ranges with ny <= nx/4 (aspect ratio at least 4); Re ~ U[re_range] with Re = u_max ny / nu, so
+    nu = u_max ny / Re and omega = 1 / (3 nu + 1/2); 1-5 non-overlapping
+    circular obstacles with radius ~ U[4, min(20, ny/8)] (so the cylinder
+    Reynolds number 2 r u_max / nu <= Re/4 stays below vortex shedding), kept
+    out of the inlet strip (ny/2 nodes) and the outlet strip (ny nodes) and
+    two nodes clear of the walls.  Each entry is channel2d's dict plus omega,
+    trt_magic, Re, nu, obstacles and the shape."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(count):
+        nx = int(rng.integers(nx_range[0], nx_range[1] + 1))
+        ny = int(rng.integers(ny_range[0], max(ny_range[0], min(ny_range[1], nx // 4)) + 1))
+        re = float(rng.uniform(*re_range))
+        nu = u_max * ny / re
+        rmax = max(r_range[0], min(r_range[1], ny / 8.0))
+        nobs = int(rng.integers(obstacles[0], obstacles[1] + 1))
+        obs = []
+        x0, x1 = ny / 2.0, nx - ny
+        for _attempt in range(200):
+            if len(obs) == nobs:
+                break
+            r = float(rng.uniform(r_range[0], rmax))
+            lo_x, hi_x = x0 + r, x1 - r
+            lo_y, hi_y = r + 2.0, ny - 3.0 - r
+            if hi_x <= lo_x or hi_y <= lo_y:
+                continue
+            cx, cy = float(rng.uniform(lo_x, hi_x)), float(rng.uniform(lo_y, hi_y))
+            if all((cx - ox) ** 2 + (cy - oy) ** 2 > (r + orr + 2.0) ** 2 for ox, oy, orr in obs):
+                obs.append((cx, cy, r))
+        d = channel2d(nx, ny, u_max, obstacles=obs)
+        d.update(omega=1.0 / (3.0 * nu + 0.5), trt_magic=trt_magic, Re=re, nu=nu, obstacles=obs, nx=nx, ny=ny,
+                 u_max=u_max)
+        out.append(d)
+    return out
+
+
 def porous3d(n, seed=2024, rmin=6.0, rmax=12.0, solid_fraction=0.30, ny=None, nz=None):
     """Periodic overlapping-sphere porous medium on an n x ny x nz box.
 

@@ -32,6 +32,8 @@ EXPORTED = [
     "lbm_time_steps", "lbm_device_bytes", "lbm_read_lattice",
     "lbm_cg_create", "lbm_cg_destroy", "lbm_cg_last_error", "lbm_cg_init", "lbm_cg_step", "lbm_cg_get_macros",
     "lbm_cg_get_f", "lbm_cg_reduce", "lbm_cg_time_steps",
+    "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_last_error", "lbm_batch_init", "lbm_batch_run",
+    "lbm_batch_get_macros", "lbm_batch_get_f", "lbm_batch_nodes",
 ]
 
 
@@ -65,6 +67,14 @@ class Lattice(C.Structure):
     _fields_ = [
         ("node_class", C.POINTER(C.c_uint8)), ("bb_links", C.POINTER(C.c_uint32)),
         ("u_bc", C.POINTER(C.c_double)), ("v_bc", C.POINTER(C.c_double)), ("w_bc", C.POINTER(C.c_double)),
+    ]
+
+
+class BatchInstance(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("node_class", C.POINTER(C.c_uint8)),
+        ("bb_links", C.POINTER(C.c_uint32)), ("u_bc", C.POINTER(C.c_double)), ("v_bc", C.POINTER(C.c_double)),
+        ("omega", C.c_double), ("trt_magic", C.c_double), ("rho_out", C.c_double),
     ]
 
 
@@ -112,9 +122,20 @@ def load():
     L.lbm_cg_get_f.argtypes = [vp, dp, dp]
     L.lbm_cg_reduce.argtypes = [vp, dp]
     L.lbm_cg_time_steps.argtypes = [vp, C.c_int64, dp]
+    L.lbm_batch_create.argtypes = [P(BatchInstance), C.c_int32, C.c_int32, C.c_int32, P(vp)]
+    L.lbm_batch_destroy.argtypes = [vp]
+    L.lbm_batch_destroy.restype = None
+    L.lbm_batch_last_error.argtypes = [vp]
+    L.lbm_batch_last_error.restype = C.c_char_p
+    L.lbm_batch_init.argtypes = [vp, C.c_int32, dp, dp, dp]
+    L.lbm_batch_run.argtypes = [vp, C.c_double, C.c_int32, C.c_int64, P(C.c_int64), dp, dp]
+    L.lbm_batch_get_macros.argtypes = [vp, C.c_int32, dp, dp, dp]
+    L.lbm_batch_get_f.argtypes = [vp, C.c_int32, dp]
+    L.lbm_batch_nodes.argtypes = [vp, P(C.c_int64)]
+    L.lbm_batch_nodes.restype = C.c_int64
     for name in EXPORTED:
         fn = getattr(L, name)
-        if fn.restype is C.c_int and name not in ("lbm_abi_version", "lbm_device_count"):
+        if fn.restype is C.c_int and name not in ("lbm_abi_version", "lbm_device_count", "lbm_batch_nodes"):
             fn.restype = C.c_int
     if L.lbm_abi_version() != ABI_VERSION:
         raise LbmError(3, "liblbm.so ABI version mismatch")
@@ -327,3 +348,87 @@ class CgSolver:
         ms = C.c_double()
         self._check(self._L.lbm_cg_time_steps(self.h, int(n), C.byref(ms)))
         return ms.value
+
+
+class Batch:
+    """Many D2Q9 channels advanced together in one persistent launch (GPU).
+
+    `instances` is a list of dicts with node_class (ny, nx), bb (ny, nx)
+    wall-link masks, ubc / vbc inlet velocities, omega, and optionally
+    trt_magic and rho_out (the keys lattices.channel2d returns, plus omega)."""
+
+    def __init__(self, instances, collision=TRT, storage=STORE_F64):
+        L = load()
+        if isinstance(storage, str):
+            storage = STORAGE_NAMES[storage]
+        n = len(instances)
+        arr = (BatchInstance * n)()
+        self._keep = []
+        self.shapes = []
+        for k, d in enumerate(instances):
+            cls = np.ascontiguousarray(d["node_class"], dtype=np.uint8)
+            ny, nx = cls.shape
+            bb = np.ascontiguousarray(d["bb"], dtype=np.uint32) if d.get("bb") is not None else None
+            ubc = _f64(d.get("ubc"), nx * ny)
+            vbc = _f64(d.get("vbc"), nx * ny)
+            self._keep += [cls, bb, ubc, vbc]
+            a = arr[k]
+            a.nx, a.ny = nx, ny
+            a.node_class = cls.ctypes.data_as(C.POINTER(C.c_uint8))
+            a.bb_links = bb.ctypes.data_as(C.POINTER(C.c_uint32)) if bb is not None else None
+            a.u_bc, a.v_bc = _dp(ubc), _dp(vbc)
+            a.omega = float(d["omega"])
+            a.trt_magic = float(d.get("trt_magic", 0.0))
+            a.rho_out = float(d.get("rho_out", 1.0))
+            self.shapes.append((ny, nx))
+        h = C.c_void_p()
+        st = L.lbm_batch_create(arr, n, collision, storage, C.byref(h))
+        if st:
+            raise LbmError(st, L.lbm_batch_last_error(None).decode())
+        self.h, self._L, self.n = h, L, n
+        self._keep = []
+
+    def _check(self, st):
+        if st:
+            raise LbmError(st, self._L.lbm_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.lbm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def init(self, i=-1, rho=None, u=None, v=None):
+        """Equilibrium start of instance i (-1: all, at rest)."""
+        m = None if i < 0 else self.shapes[i][0] * self.shapes[i][1]
+        r, uu, vv = _f64(rho, m), _f64(u, m), _f64(v, m)
+        self._check(self._L.lbm_batch_init(self.h, int(i), _dp(r), _dp(uu), _dp(vv)))
+
+    def run(self, tol, every=100, max_steps=100000):
+        """One persistent launch; returns (steps, residuals, device ms)."""
+        steps = np.zeros(self.n, np.int64)
+        res = np.zeros(self.n)
+        ms = C.c_double()
+        st = self._L.lbm_batch_run(self.h, float(tol), int(every), int(max_steps),
+                                   steps.ctypes.data_as(C.POINTER(C.c_int64)), _dp(res), C.byref(ms))
+        self._check(st)
+        return steps, res, ms.value
+
+    def macros(self, i):
+        ny, nx = self.shapes[i]
+        out = [np.zeros(nx * ny) for _ in range(3)]
+        self._check(self._L.lbm_batch_get_macros(self.h, int(i), *[_dp(a) for a in out]))
+        return [a.reshape(ny, nx) for a in out]
+
+    def get_f(self, i):
+        ny, nx = self.shapes[i]
+        f = np.zeros(9 * nx * ny)
+        self._check(self._L.lbm_batch_get_f(self.h, int(i), _dp(f)))
+        return f.reshape(9, ny, nx)
+
+    def nodes(self):
+        fl = C.c_int64()
+        tot = self._L.lbm_batch_nodes(self.h, C.byref(fl))
+        return tot, fl.value
