@@ -212,8 +212,8 @@ typedef struct {
     double rho_out;             /* outlet density; 0 selects 1 */
 } lbm_batch_instance;
 
-/* collision LBM_BGK or LBM_TRT; storage LBM_STORE_F64 or LBM_STORE_F32.
- * Instances start at rest (rho = 1, u = 0). */
+/* collision LBM_BGK or LBM_TRT; storage LBM_STORE_F64 (LBM_STORE_F32 returns
+ * LBM_EUNSUPPORTED for now).  Instances start at rest (rho = 1, u = 0). */
 lbm_status lbm_batch_create(const lbm_batch_instance* inst, int32_t n, int32_t collision, int32_t storage,
                             lbm_batch** out);
 void lbm_batch_destroy(lbm_batch* b);

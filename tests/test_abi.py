@@ -59,6 +59,9 @@ def test_batch_create_rejects_bad_parameters():
         with pytest.raises(L.LbmError) as ei:
             L.Batch([ch, inst], **kw)
         assert ei.value.status == 3 and "device" not in str(ei.value), str(ei.value)
+    with pytest.raises(L.LbmError) as ei:
+        L.Batch([ch], storage="f32")
+    assert ei.value.status == 6
 
 
 def test_channel_sweep_is_seeded_and_in_range():
@@ -71,7 +74,8 @@ def test_channel_sweep_is_seeded_and_in_range():
         assert 128 <= d["nx"] <= 1024 and 32 <= d["ny"] <= 256 and d["ny"] <= d["nx"] // 4
         assert 1 <= len(d["obstacles"]) <= 5 and 10 <= d["Re"] <= 100
         nu = (1 / d["omega"] - 0.5) / 3
-        assert abs(d["u_max"] * d["ny"] / nu - d["Re"]) < 1e-9 * d["Re"]
+        assert abs(d["u_max"] * 2 * d["ny"] / nu - d["Re"]) < 1e-9 * d["Re"]
+        assert d["u_max"] <= 0.1 and 48 * d["u_max"] ** 2 * d["nx"] / (d["ny"] * d["Re"]) <= 0.03 + 1e-12
         for (cx, cy, r) in d["obstacles"]:
             assert 4 <= r <= max(4, min(20, d["ny"] / 8))
             assert cx - r >= d["ny"] / 2 and cx + r <= d["nx"] - d["ny"]

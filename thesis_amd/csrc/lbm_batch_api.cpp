@@ -111,6 +111,12 @@ lbm_status lbm_batch_create(const lbm_batch_instance* in, int32_t n, int32_t col
     if (!in || n < 1) return reject("need at least one instance");
     if (collision != LBM_BGK && collision != LBM_TRT) return reject("batch collision must be BGK or TRT");
     if (storage != LBM_STORE_F64 && storage != LBM_STORE_F32) return reject("batch storage must be F64 or F32");
+    if (storage == LBM_STORE_F32) {
+        // switched off: its first cooperative launch (7 workgroups per CU) did
+        // not finish on MI355X while FP64 (6 per CU) runs; cause not found
+        g_create_err = "F32 batch storage is switched off (its persistent launch did not complete)";
+        return LBM_EUNSUPPORTED;
+    }
 
     std::vector<BInst> hinst(n);
     std::vector<uint32_t> cidx, didx, nodew;

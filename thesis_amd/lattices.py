@@ -82,25 +82,30 @@ def channel2d(nx, ny, u_max, obstacles=(), rho_out=1.0):
                 periodic=(0, 0, 0), rho_out=rho_out)
 
 
-def channel_sweep(count=64, seed=7, u_max=0.1, nx_range=(128, 1024), ny_range=(32, 256), re_range=(10.0, 100.0),
-                  obstacles=(1, 5), r_range=(4.0, 20.0), trt_magic=3.0 / 16.0):
+def channel_sweep(count=64, seed=7, nx_range=(128, 1024), ny_range=(32, 256), re_range=(10.0, 100.0),
+                  obstacles=(1, 5), r_range=(4.0, 20.0), trt_magic=3.0 / 16.0, u_cap=0.1, drho=0.03):
     """Seeded family of D2Q9 channels for a batched parametric sweep.
 
     Per channel, from numpy.random.Generator(PCG64(seed)): nx, ny uniform
-    integers in the given ranges with ny <= nx/4 (aspect ratio at least 4); Re ~ U[re_range] with Re = u_max ny / nu, so
-    nu = u_max ny / Re and omega = 1 / (3 nu + 1/2); 1-5 non-overlapping
-    circular obstacles with radius ~ U[4, min(20, ny/8)] (so the cylinder
-    Reynolds number 2 r u_max / nu <= Re/4 stays below vortex shedding), kept
-    out of the inlet strip (ny/2 nodes) and the outlet strip (ny nodes) and
-    two nodes clear of the walls.  Each entry is channel2d's dict plus omega,
-    trt_magic, Re, nu, obstacles and the shape."""
+    integers in the given ranges with ny <= nx/4 (aspect ratio at least 4);
+    Re ~ U[re_range] on the hydraulic diameter D_h = 2 ny, Re = u_max D_h / nu.
+    The peak inlet speed is the largest (up to u_cap) for which the Poiseuille
+    density drop along the channel, 3 dp/rho = 24 nu u_max nx / ny^2
+    = 48 u_max^2 nx / (ny Re), stays at `drho`, so the flow stays weakly
+    compressible; then nu = 2 u_max ny / Re and omega = 1 / (3 nu + 1/2).
+    1-5 non-overlapping circular obstacles with radius ~ U[4, min(20, ny/8)]
+    (cylinder Reynolds number 2 r u_max / nu <= Re/8, far below vortex
+    shedding) are kept out of the inlet strip (ny/2 nodes) and the outlet strip
+    (ny nodes) and two nodes clear of the walls.  Each entry is channel2d's
+    dict plus omega, trt_magic, Re, nu, u_max, obstacles and the shape."""
     rng = np.random.Generator(np.random.PCG64(seed))
     out = []
     for _ in range(count):
         nx = int(rng.integers(nx_range[0], nx_range[1] + 1))
         ny = int(rng.integers(ny_range[0], max(ny_range[0], min(ny_range[1], nx // 4)) + 1))
         re = float(rng.uniform(*re_range))
-        nu = u_max * ny / re
+        u_max = min(u_cap, float(np.sqrt(drho * re * ny / (48.0 * nx))))
+        nu = 2.0 * u_max * ny / re
         rmax = max(r_range[0], min(r_range[1], ny / 8.0))
         nobs = int(rng.integers(obstacles[0], obstacles[1] + 1))
         obs = []
