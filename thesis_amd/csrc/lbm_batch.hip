@@ -1,11 +1,17 @@
 // Batched D2Q9 channels in one persistent launch.
 //
 // Many independent lattices (instances) of different sizes advance together in
-// one cooperative kernel.  Every step the workgroups stride over a table of
-// 256-node tiles of all instances, skip the tiles of retired instances and run
-// the single-lattice node update (pull with half-way walls, Zou-He inlet and
+// one cooperative kernel.  Every step the workgroups stride over the 1024-node
+// tiles of all instances, skip the tiles of retired instances and run the
+// single-lattice node update (pull with half-way walls, Zou-He inlet and
 // outlet, moments, BGK/TRT) with the instance's own parameters; a grid barrier
-// separates steps.  Every check_every steps each tile adds its sum |u - u_prev|^2
+// separates steps.  Nodes are stored densely per instance, so the source of
+// every pulled population is the node index minus (cx + cy nx): no index maps
+// are loaded, and the instance table, activity flags and buffer parities are
+// staged in LDS, so a tile's only global loads are its node words and
+// populations.  The grid barrier costs about 0.1 us per workgroup on MI355X
+// (each one writes back and invalidates its caches), so the grid is one
+// 1024-thread workgroup per CU rather than many small ones.  Every check_every steps each tile adds its sum |u - u_prev|^2
 // and sum |u|^2 to its instance (FP64 atomics); after the barrier workgroup 0
 // turns them into the instance's residual sqrt(sum du^2 / sum u^2), retires the
 // instances below the tolerance (and non-finite ones), and publishes how many
@@ -34,69 +40,106 @@ __device__ inline KParams inst_params(const BInst& I) {
     return kp;
 }
 
-// Post-boundary deviations h of compact node `own` of instance I; idx, cls out.
+// Population slot of plane q of dense node j: tiles of 64 nodes x 9 planes.
+__device__ inline uint32_t bslot(int q, uint32_t j) { return (j >> 6) * (64u * 9u) + (uint32_t)q * 64u + (j & 63u); }
+
+// Post-boundary deviations h of node `own` (local dense index, not solid) of
+// instance I, pulled from the instance's population block src.  Walls (bit
+// opp(i) of the node word) return the node's own opposite population
+// (half-way bounce-back); sources beyond the open x faces keep the node's own
+// P_i, replaced by the Zou-He conditions on inlet and outlet nodes.
 template <int S>
-__device__ inline void batch_gather(const BInst& I, const KParams& kp, const typename Io<D2Q9, S>::raw* isrc,
-                                    const uint32_t* __restrict__ didx, const uint32_t* __restrict__ nodew,
-                                    const uint32_t* __restrict__ cidx, const Fields& fl, uint32_t own,
-                                    uint32_t& idx, uint32_t& cls, typename Io<D2Q9, S>::T* h) {
+__device__ inline void dense_gather(const BInst& I, const KParams& kp, const typename Io<D2Q9, S>::raw* src,
+                                    const Fields& fl, uint32_t own, uint32_t nd, typename Io<D2Q9, S>::T* h) {
     using L = D2Q9;
     using T = typename Io<L, S>::T;
-    idx = didx[I.cbase + own];
-    const uint32_t nd = nodew[I.cbase + own];
-    const uint32_t yq = fdiv(idx, I.mx, I.sx);
-    const int x = (int)(idx - yq * (uint32_t)I.nx), y = (int)yq;
-    uint32_t cn[L::Q];
-    source_index<L, true>(cidx + I.dbase, kp, x, y, 0, cn);
-    cls = nd >> kClsShift;
-    gather<L, S, true>(isrc, kp, own, nd, cn, h);
+    const uint32_t yq = fdiv(own, I.mx, I.sx);
+    const int x = (int)(own - yq * (uint32_t)I.nx);
+    h[0] = Io<L, S>::ld(src, bslot(0, own), 0);
+#pragma unroll
+    for (int i = 1; i < L::Q; ++i) {
+        const int o = L::opp(i), cx = L::cx(i), cy = L::cy(i);
+        uint32_t j = own - (uint32_t)(cx + cy * I.nx);
+        int pl = i;
+        const bool wall = (nd >> o) & 1u;
+        const bool open = (cx > 0 && x == 0) || (cx < 0 && x == I.nx - 1);
+        if (wall || open) {
+            j = own;
+            pl = wall ? o : i;
+        }
+        h[i] = Io<L, S>::ld(src, bslot(pl, j), i);
+    }
+    const uint32_t cls = nd >> kClsShift;
     Fields f = fl;
     f.ubc = fl.ubc + I.dbase;
     f.vbc = fl.vbc + I.dbase;
-    if (cls == VEL_W) zou_inlet<L, T>(kp, f, idx, nd, h);
+    if (cls == VEL_W) zou_inlet<L, T>(kp, f, own, nd, h);
     if (cls == PRES_E) zou_outlet<L, T>(kp, h);
+}
+
+__device__ inline int find_inst(const BInst* I, int ninst, uint32_t t) {
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (I[mid].tbase <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 }  // namespace
 
 template <int S, int C>
-__global__ __launch_bounds__(256) void k_batch(const BInst* __restrict__ inst, int ninst,
-                                               const uint2* __restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(kBatchTile) void k_batch(const BInst* __restrict__ inst_g, int ninst, int ntiles,
                                                typename Io<D2Q9, S>::raw* P0, typename Io<D2Q9, S>::raw* P1,
-                                               const uint32_t* __restrict__ didx, const uint32_t* __restrict__ nodew,
-                                               const uint32_t* __restrict__ cidx, Fields fl, double* uprev,
+                                               const uint32_t* __restrict__ nodew, Fields fl, double* uprev,
                                                double* vprev, BState bs, double tol, int check_every,
                                                int max_steps) {
     using L = D2Q9;
     using T = typename Io<L, S>::T;
+    using raw = typename Io<L, S>::raw;
     namespace cgr = cooperative_groups;
     cgr::grid_group grid = cgr::this_grid();
-    __shared__ double red[2][4];
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    BInst* inst = (BInst*)lds_raw;
+    int* act = (int*)(inst + ninst);
+    int* par = act + ninst;
+    constexpr int W = kBatchTile / 64;
+    __shared__ double red[2][W];
     __shared__ int left;
+    for (int k = threadIdx.x; k < ninst; k += blockDim.x) inst[k] = inst_g[k];
     for (int step = 0; step < max_steps; ++step) {
-        // written by workgroup 0 before the last barrier: the same in every workgroup
-        if (*(volatile int*)bs.remaining == 0) break;
+        // activity and parity were last written by workgroup 0 before the
+        // previous barrier: every workgroup sees the same values
+        for (int k = threadIdx.x; k < ninst; k += blockDim.x) {
+            act[k] = *(volatile int*)(bs.active + k);
+            par[k] = *(volatile int*)(bs.par + k);
+        }
+        const int remaining = *(volatile int*)bs.remaining;
+        __syncthreads();
+        if (remaining == 0) break;
         const bool check = (step + 1) % check_every == 0 || step + 1 == max_steps;
+        int ci = 0;
         for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-            const uint2 tl = tiles[t];
-            const int i = (int)tl.x;
-            if (!*(volatile int*)(bs.active + i)) continue;   // uniform over the workgroup
+            while (ci + 1 < ninst && inst[ci + 1].tbase <= (uint32_t)t) ++ci;   // tiles ascend with t
+            const int i = ci;
+            if (!act[i]) continue;                          // uniform over the workgroup
             const BInst I = inst[i];
             const KParams kp = inst_params(I);
-            const int sb = (bs.par[i] + step) & 1;             // buffer holding the instance's newest state
-            const size_t off = (size_t)I.cbase * L::Q;
-            const typename Io<L, S>::raw* isrc = (sb ? P1 : P0) + off;
-            typename Io<L, S>::raw* idst = (sb ? P0 : P1) + off;
-            const uint32_t own = tl.y + threadIdx.x;
+            const int sb = (par[i] + step) & 1;             // buffer holding the instance's newest state
+            const size_t off = (size_t)I.pbase * L::Q;
+            const raw* isrc = (sb ? P1 : P0) + off;
+            raw* idst = (sb ? P0 : P1) + off;
+            const uint32_t own = ((uint32_t)t - I.tbase) * (uint32_t)kBatchTile + threadIdx.x;
             double du2 = 0, uu2 = 0;
-            if (own < I.nf) {
-                uint32_t idx, cls;
+            const uint32_t nd = own < I.n ? nodew[I.dbase + own] : 0u;
+            if ((nd >> kClsShift) != SOLID) {
                 T h[L::Q];
-                batch_gather<S>(I, kp, isrc, didx, nodew, cidx, fl, own, idx, cls, h);
+                dense_gather<S>(I, kp, isrc, fl, own, nd, h);
                 T drho, rho, u[3];
-                moments<L, T>(kp, cls, h, drho, rho, u);
+                moments<L, T>(kp, nd >> kClsShift, h, drho, rho, u);
                 if (check) {
-                    const size_t d = (size_t)I.dbase + idx;
+                    const size_t d = (size_t)I.dbase + own;
                     const double ud = (double)u[0], vd = (double)u[1];
                     const double dx = ud - uprev[d], dy = vd - vprev[d];
                     du2 = dx * dx + dy * dy;
@@ -108,7 +151,7 @@ __global__ __launch_bounds__(256) void k_batch(const BInst* __restrict__ inst, i
                 T out[L::Q];
                 collide<L, T, C>(kp, h, drho, rho, u, out);
 #pragma unroll
-                for (int q = 0; q < L::Q; ++q) Io<L, S>::st(idst, paddr<L>(kp, q, own), out[q], q);
+                for (int q = 0; q < L::Q; ++q) Io<L, S>::st(idst, bslot(q, own), out[q], q);
             }
             if (check) {
 #pragma unroll
@@ -120,8 +163,10 @@ __global__ __launch_bounds__(256) void k_batch(const BInst* __restrict__ inst, i
                 if (lane == 0) { red[0][wv] = du2; red[1][wv] = uu2; }
                 __syncthreads();
                 if (threadIdx.x == 0) {
-                    atomicAdd(bs.acc + 2 * i, (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]));
-                    atomicAdd(bs.acc + 2 * i + 1, (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]));
+                    double a = 0, b = 0;
+                    for (int k = 0; k < W; ++k) { a += red[0][k]; b += red[1][k]; }
+                    atomicAdd(bs.acc + 2 * i, a);
+                    atomicAdd(bs.acc + 2 * i + 1, b);
                 }
                 __syncthreads();
             }
@@ -132,7 +177,7 @@ __global__ __launch_bounds__(256) void k_batch(const BInst* __restrict__ inst, i
                 if (threadIdx.x == 0) left = 0;
                 __syncthreads();
                 for (int i = threadIdx.x; i < ninst; i += blockDim.x) {
-                    if (!bs.active[i]) continue;
+                    if (!act[i]) continue;
                     const double a = bs.acc[2 * i], b = bs.acc[2 * i + 1];
                     const double r = b > 0 ? sqrt(a / b) : sqrt(a);
                     bs.res[i] = r;
@@ -142,7 +187,7 @@ __global__ __launch_bounds__(256) void k_batch(const BInst* __restrict__ inst, i
                     if (r < tol || bad || step + 1 == max_steps) {
                         bs.active[i] = 0;
                         bs.done[i] = step + 1;
-                        bs.par[i] = ((bs.par[i] + step) & 1) ^ 1;   // the buffer just written
+                        bs.par[i] = ((par[i] + step) & 1) ^ 1;   // the buffer just written
                     } else {
                         atomicAdd(&left, 1);
                     }
@@ -158,18 +203,19 @@ __global__ __launch_bounds__(256) void k_batch(const BInst* __restrict__ inst, i
 // First post-collision state: equilibrium at the instance's rho, u (dense
 // fields), then one collision, as the single-lattice initialisation does.
 template <int S, int C>
-__global__ __launch_bounds__(256) void k_batch_init(const BInst* __restrict__ inst, const uint2* __restrict__ tiles,
-                                                    typename Io<D2Q9, S>::raw* P, const uint32_t* __restrict__ didx,
+__global__ __launch_bounds__(kBatchTile) void k_batch_init(const BInst* __restrict__ inst, int ninst,
+                                                    typename Io<D2Q9, S>::raw* P, const uint32_t* __restrict__ nodew,
                                                     Fields fl, int only) {
     using L = D2Q9;
     using T = typename Io<L, S>::T;
-    const uint2 tl = tiles[blockIdx.x];
-    if (only >= 0 && (int)tl.x != only) return;
-    const BInst I = inst[tl.x];
-    const uint32_t own = tl.y + threadIdx.x;
-    if (own >= I.nf) return;
+    const int i = find_inst(inst, ninst, blockIdx.x);
+    if (only >= 0 && i != only) return;
+    const BInst I = inst[i];
+    const uint32_t own = (blockIdx.x - I.tbase) * (uint32_t)kBatchTile + threadIdx.x;
+    if (own >= I.n) return;
+    if ((nodew[I.dbase + own] >> kClsShift) == SOLID) return;
     const KParams kp = inst_params(I);
-    const size_t d = (size_t)I.dbase + didx[I.cbase + own];
+    const size_t d = (size_t)I.dbase + own;
     const double r = fl.rho[d];
     const T drho = (T)(r - 1.0), rho = (T)r;
     const T u[3] = {(T)fl.u[d], (T)fl.v[d], (T)0};
@@ -182,34 +228,34 @@ __global__ __launch_bounds__(256) void k_batch_init(const BInst* __restrict__ in
     }
     T out[L::Q];
     collide<L, T, C>(kp, h, drho, rho, u, out);
-    typename Io<L, S>::raw* base = P + (size_t)I.cbase * L::Q;
+    typename Io<L, S>::raw* base = P + (size_t)I.pbase * L::Q;
 #pragma unroll
-    for (int q = 0; q < L::Q; ++q) Io<L, S>::st(base, paddr<L>(kp, q, own), out[q], q);
+    for (int q = 0; q < L::Q; ++q) Io<L, S>::st(base, bslot(q, own), out[q], q);
 }
 
 // Full post-boundary populations of instance i (dense, FP64) from buffer P.
 template <int S>
 __global__ __launch_bounds__(256) void k_batch_recon(const BInst* __restrict__ inst, int i,
                                                      const typename Io<D2Q9, S>::raw* __restrict__ P,
-                                                     const uint32_t* __restrict__ didx,
-                                                     const uint32_t* __restrict__ nodew,
-                                                     const uint32_t* __restrict__ cidx, Fields fl, double* fout) {
+                                                     const uint32_t* __restrict__ nodew, Fields fl, double* fout) {
     using L = D2Q9;
     using T = typename Io<L, S>::T;
     const BInst I = inst[i];
     const uint32_t own = blockIdx.x * blockDim.x + threadIdx.x;
-    if (own >= I.nf) return;
+    if (own >= I.n) return;
+    const uint32_t nd = nodew[I.dbase + own];
+    if ((nd >> kClsShift) == SOLID) return;
     const KParams kp = inst_params(I);
-    uint32_t idx, cls;
     T h[L::Q];
-    batch_gather<S>(I, kp, P + (size_t)I.cbase * L::Q, didx, nodew, cidx, fl, own, idx, cls, h);
-    const size_t n = (size_t)I.nx * I.ny;
+    dense_gather<S>(I, kp, P + (size_t)I.pbase * L::Q, fl, own, nd, h);
 #pragma unroll
-    for (int q = 0; q < L::Q; ++q) fout[q * n + idx] = (double)h[q] + L::w(q);
+    for (int q = 0; q < L::Q; ++q) fout[(size_t)q * I.n + own] = (double)h[q] + L::w(q);
 }
 
 // ---------------------------------------------------------------------------
 // launchers
+
+size_t batch_lds_bytes(int ninst) { return (size_t)ninst * (sizeof(BInst) + 2 * sizeof(int)); }
 
 template <int S, int C> static const void* batch_kernel() { return (const void*)k_batch<S, C>; }
 
@@ -218,58 +264,57 @@ static const void* batch_kernel_for(int storage, int collision) {
     return collision == 0 ? batch_kernel<1, 0>() : batch_kernel<1, 1>();
 }
 
-int batch_max_blocks(int storage, int collision) {
+int batch_max_blocks(int storage, int collision, int ninst) {
     int dev = 0, ncu = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_kernel_for(storage, collision), 256, 0) !=
-        hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_kernel_for(storage, collision), kBatchTile,
+                                                     batch_lds_bytes(ninst)) != hipSuccess)
         return 0;
     return ncu * per;
 }
 
-hipError_t batch_run(int storage, int collision, const BInst* inst, int ninst, const uint2* tiles, int ntiles,
-                     void* P0, void* P1, const uint32_t* didx, const uint32_t* nodew, const uint32_t* cidx,
-                     Fields fl, double* uprev, double* vprev, BState bs, double tol, int check_every, int max_steps,
-                     int grid_blocks, hipStream_t st) {
-    void* args[] = {(void*)&inst, (void*)&ninst, (void*)&tiles, (void*)&ntiles, (void*)&P0, (void*)&P1,
-                    (void*)&didx, (void*)&nodew, (void*)&cidx, (void*)&fl, (void*)&uprev, (void*)&vprev,
-                    (void*)&bs, (void*)&tol, (void*)&check_every, (void*)&max_steps};
-    return hipLaunchCooperativeKernel(batch_kernel_for(storage, collision), dim3(grid_blocks), dim3(256), args, 0,
-                                      st);
+hipError_t batch_run(int storage, int collision, const BInst* inst, int ninst, int ntiles, void* P0, void* P1,
+                     const uint32_t* nodew, Fields fl, double* uprev, double* vprev, BState bs, double tol,
+                     int check_every, int max_steps, int grid_blocks, hipStream_t st) {
+    void* args[] = {(void*)&inst, (void*)&ninst, (void*)&ntiles, (void*)&P0, (void*)&P1, (void*)&nodew,
+                    (void*)&fl, (void*)&uprev, (void*)&vprev, (void*)&bs, (void*)&tol, (void*)&check_every,
+                    (void*)&max_steps};
+    return hipLaunchCooperativeKernel(batch_kernel_for(storage, collision), dim3(grid_blocks), dim3(kBatchTile), args,
+                                      (unsigned)batch_lds_bytes(ninst), st);
 }
 
-hipError_t batch_init(int storage, int collision, const BInst* inst, const uint2* tiles, int ntiles, void* P,
-                      const uint32_t* didx, Fields fl, int only, hipStream_t st) {
+hipError_t batch_init(int storage, int collision, const BInst* inst, int ninst, int ntiles, void* P,
+                      const uint32_t* nodew, Fields fl, int only, hipStream_t st) {
     if (ntiles == 0) return hipSuccess;
     if (storage == 0) {
         if (collision == 0)
-            hipLaunchKernelGGL((k_batch_init<0, 0>), dim3(ntiles), dim3(256), 0, st, inst, tiles, (double*)P, didx,
+            hipLaunchKernelGGL((k_batch_init<0, 0>), dim3(ntiles), dim3(kBatchTile), 0, st, inst, ninst, (double*)P, nodew,
                                fl, only);
         else
-            hipLaunchKernelGGL((k_batch_init<0, 1>), dim3(ntiles), dim3(256), 0, st, inst, tiles, (double*)P, didx,
+            hipLaunchKernelGGL((k_batch_init<0, 1>), dim3(ntiles), dim3(kBatchTile), 0, st, inst, ninst, (double*)P, nodew,
                                fl, only);
     } else {
         if (collision == 0)
-            hipLaunchKernelGGL((k_batch_init<1, 0>), dim3(ntiles), dim3(256), 0, st, inst, tiles, (float*)P, didx,
+            hipLaunchKernelGGL((k_batch_init<1, 0>), dim3(ntiles), dim3(kBatchTile), 0, st, inst, ninst, (float*)P, nodew,
                                fl, only);
         else
-            hipLaunchKernelGGL((k_batch_init<1, 1>), dim3(ntiles), dim3(256), 0, st, inst, tiles, (float*)P, didx,
+            hipLaunchKernelGGL((k_batch_init<1, 1>), dim3(ntiles), dim3(kBatchTile), 0, st, inst, ninst, (float*)P, nodew,
                                fl, only);
     }
     return hipGetLastError();
 }
 
-hipError_t batch_recon(int storage, const BInst* inst, int i, uint32_t nf, const void* P, const uint32_t* didx,
-                       const uint32_t* nodew, const uint32_t* cidx, Fields fl, double* fout, hipStream_t st) {
-    const unsigned blocks = (nf + 255) / 256;
+hipError_t batch_recon(int storage, const BInst* inst, int i, uint32_t n, const void* P, const uint32_t* nodew,
+                       Fields fl, double* fout, hipStream_t st) {
+    const unsigned blocks = (n + 255) / 256;
     if (blocks == 0) return hipSuccess;
     if (storage == 0)
-        hipLaunchKernelGGL(k_batch_recon<0>, dim3(blocks), dim3(256), 0, st, inst, i, (const double*)P, didx, nodew,
-                           cidx, fl, fout);
+        hipLaunchKernelGGL(k_batch_recon<0>, dim3(blocks), dim3(256), 0, st, inst, i, (const double*)P, nodew, fl,
+                           fout);
     else
-        hipLaunchKernelGGL(k_batch_recon<1>, dim3(blocks), dim3(256), 0, st, inst, i, (const float*)P, didx, nodew,
-                           cidx, fl, fout);
+        hipLaunchKernelGGL(k_batch_recon<1>, dim3(blocks), dim3(256), 0, st, inst, i, (const float*)P, nodew, fl,
+                           fout);
     return hipGetLastError();
 }
 

@@ -1,10 +1,11 @@
 // C ABI of the batched channels (declared in include/lbm.h).
 //
-// All instances share one compact population index space (each instance's
-// block starts on a 64-node tile), one dense per-node index space for the
-// FP64 fields, and one tile table (instance, first compact node) that the
-// persistent kernel strides over.  The reference runs one lattice per
-// process through Iterate2D (S/Iterate.cu:23-625).
+// Each instance keeps its nodes densely in row-major order: its populations
+// in one block of 64-node tiles (the block starts on a tile), its node words
+// and FP64 fields in one concatenated per-node index space, and its
+// 1024-node workgroup tiles numbered after those of the previous instances.  The
+// reference runs one lattice per process through Iterate2D
+// (S/Iterate.cu:23-625).
 #include "lbm.h"
 #include "lbm_batch_internal.h"
 
@@ -28,15 +29,14 @@ struct lbm_batch {
     std::vector<uint8_t> ready;     // instance initialised
     std::vector<int64_t> steps;     // steps since its initialisation
     std::vector<int> par;           // host copy of the buffer holding each instance's newest state
-    size_t ndense = 0, ncompact = 0, nfluid = 0, maxn = 0;
+    size_t ndense = 0, nslots = 0, nfluid = 0, maxn = 0;
     int ntiles = 0, grid = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<void*> allocs;
     BInst* inst = nullptr;
-    uint2* tiles = nullptr;
     void* P[2] = {nullptr, nullptr};
-    uint32_t *cidx = nullptr, *didx = nullptr, *nodew = nullptr;
+    uint32_t* nodew = nullptr;
     double *rho = nullptr, *u = nullptr, *v = nullptr, *up = nullptr, *vp = nullptr;
     double *ubc = nullptr, *vbc = nullptr, *fout = nullptr;
     BState bs{};
@@ -118,17 +118,18 @@ lbm_status lbm_batch_create(const lbm_batch_instance* in, int32_t n, int32_t col
         return LBM_EUNSUPPORTED;
     }
 
+    if (n > kBatchMaxInstances) return reject("at most %d instances per batch", kBatchMaxInstances);
+
     std::vector<BInst> hinst(n);
-    std::vector<uint32_t> cidx, didx, nodew;
-    std::vector<uint2> tiles;
+    std::vector<uint32_t> nodew;
     std::vector<double> ubc, vbc;
-    size_t dbase = 0, cbase = 0, maxn = 0, nfluid = 0;
+    size_t dbase = 0, pbase = 0, tbase = 0, maxn = 0, nfluid = 0;
     for (int k = 0; k < n; ++k) {
         const lbm_batch_instance& I = in[k];
         if (I.nx < 3 || I.ny < 1 || !I.node_class) return reject("instance %d: need nx >= 3, ny >= 1, node_class", k);
         if (!(I.omega > 0.0 && I.omega < 2.0)) return reject("instance %d: omega must lie in (0, 2)", k);
         const size_t nn = (size_t)I.nx * I.ny;
-        if (nn >= 0xFFFFFFFFull) return reject("instance %d too large", k);
+        if (nn * 9 * 8 >= (1ull << 32)) return reject("instance %d too large for 32-bit offsets", k);
         std::vector<uint32_t> word(nn);
         size_t nf = 0;
         bool need_u = false;
@@ -144,52 +145,49 @@ lbm_status lbm_batch_create(const lbm_batch_instance* in, int32_t n, int32_t col
         }
         if (nf == 0) return reject("instance %d has no fluid nodes", k);
         if (need_u && !I.u_bc) return reject("instance %d: velocity inlet nodes need u_bc", k);
-        // a link into a solid node is a wall
+        // a link into a solid node, or out through the y faces, is a wall
         for (size_t j = 0; j < nn; ++j) {
-            if (I.node_class[j] == LBM_SOLID) continue;
+            if (I.node_class[j] == LBM_SOLID) {
+                word[j] = 0;
+                continue;
+            }
             const int x = (int)(j % I.nx), y = (int)(j / I.nx);
             for (int d = 1; d < 9; ++d) {
                 const int xs = x + kC9[d][0], ys = y + kC9[d][1];
-                if (xs < 0 || xs >= I.nx || ys < 0 || ys >= I.ny) continue;
+                if (ys < 0 || ys >= I.ny) {
+                    word[j] |= 1u << d;
+                    continue;
+                }
+                if (xs < 0 || xs >= I.nx) continue;
                 if (I.node_class[(size_t)ys * I.nx + xs] == LBM_SOLID) word[j] |= 1u << d;
             }
         }
         BInst& B = hinst[k];
         B.nx = I.nx; B.ny = I.ny;
-        B.nf = (uint32_t)nf;
-        B.cbase = (uint32_t)cbase;
+        B.n = (uint32_t)nn;
+        B.pbase = (uint32_t)pbase;
         B.dbase = (uint32_t)dbase;
+        B.tbase = (uint32_t)tbase;
+        B.pad = 0;
         magic((uint32_t)I.nx, B.mx, B.sx);
         B.omega = I.omega;
         const double lam = I.trt_magic > 0 ? I.trt_magic : 3.0 / 16.0;
         B.omega_m = collision == LBM_TRT ? 1.0 / (0.5 + lam / (1.0 / I.omega - 0.5)) : I.omega;
         B.rho_out = I.rho_out > 0 ? I.rho_out : 1.0;
-        uint32_t loc = 0;
+        nodew.insert(nodew.end(), word.begin(), word.end());
         for (size_t j = 0; j < nn; ++j) {
-            if (I.node_class[j] == LBM_SOLID) {
-                cidx.push_back(0xFFFFFFFFu);
-            } else {
-                cidx.push_back(loc++);
-                didx.push_back((uint32_t)j);
-                nodew.push_back(word[j]);
-            }
             ubc.push_back(I.u_bc ? I.u_bc[j] : 0.0);
             vbc.push_back(I.v_bc ? I.v_bc[j] : 0.0);
         }
-        // compact arrays padded to the instance's 64-node tile boundary
-        const size_t padded = (nf + 63) / 64 * 64;
-        didx.resize(cbase + padded, 0u);
-        nodew.resize(cbase + padded, 0u);
-        for (size_t t = 0; t < nf; t += 256) tiles.push_back(make_uint2((uint32_t)k, (uint32_t)t));
         dbase += nn;
-        cbase += padded;
+        pbase += (nn + 63) / 64 * 64;
+        tbase += (nn + kBatchTile - 1) / kBatchTile;
         nfluid += nf;
         maxn = nn > maxn ? nn : maxn;
-        if (dbase >= 0xFFFFFFFFull || cbase * 9 * 8 >= (1ull << 36)) return reject("batch too large");
+        if (dbase >= 0xFFFFFFFFull || pbase >= 0xFFFFFFFFull || tbase >= (size_t)INT_MAX)
+            return reject("batch too large");
     }
     const size_t raw = storage == LBM_STORE_F64 ? 8 : 4;
-    if (maxn * 9 * 8 >= (1ull << 32)) return reject("instance too large for 32-bit offsets");
-    if (tiles.size() > (size_t)INT_MAX) return reject("too many tiles");
 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return reject("no HIP device");
@@ -208,16 +206,16 @@ lbm_status lbm_batch_create(const lbm_batch_instance* in, int32_t n, int32_t col
     b->steps.assign(n, 0);
     b->par.assign(n, 0);
     b->ndense = dbase;
-    b->ncompact = cbase;
+    b->nslots = pbase;
     b->nfluid = nfluid;
     b->maxn = maxn;
-    b->ntiles = (int)tiles.size();
+    b->ntiles = (int)tbase;
     auto bad = [&](lbm_status s) {
         g_create_err = b->err;
         lbm_batch_destroy(b);
         return s;
     };
-    const int maxb = batch_max_blocks(storage == LBM_STORE_F64 ? 0 : 1, collision);
+    const int maxb = batch_max_blocks(storage == LBM_STORE_F64 ? 0 : 1, collision, n);
     if (maxb <= 0) {
         b->err = "occupancy query for the batch kernel failed";
         return bad(LBM_EDEVICE);
@@ -230,10 +228,9 @@ lbm_status lbm_batch_create(const lbm_batch_instance* in, int32_t n, int32_t col
     }
     lbm_status s;
     const size_t D = b->ndense;
-    if ((s = balloc(b, &b->inst, (size_t)n)) != LBM_OK || (s = balloc(b, &b->tiles, tiles.size())) != LBM_OK ||
-        (s = balloc(b, (char**)&b->P[0], cbase * 9 * raw)) != LBM_OK ||
-        (s = balloc(b, (char**)&b->P[1], cbase * 9 * raw)) != LBM_OK || (s = balloc(b, &b->cidx, D)) != LBM_OK ||
-        (s = balloc(b, &b->didx, cbase)) != LBM_OK || (s = balloc(b, &b->nodew, cbase)) != LBM_OK ||
+    if ((s = balloc(b, &b->inst, (size_t)n)) != LBM_OK ||
+        (s = balloc(b, (char**)&b->P[0], pbase * 9 * raw)) != LBM_OK ||
+        (s = balloc(b, (char**)&b->P[1], pbase * 9 * raw)) != LBM_OK || (s = balloc(b, &b->nodew, D)) != LBM_OK ||
         (s = balloc(b, &b->rho, D)) != LBM_OK || (s = balloc(b, &b->u, D)) != LBM_OK ||
         (s = balloc(b, &b->v, D)) != LBM_OK || (s = balloc(b, &b->up, D)) != LBM_OK ||
         (s = balloc(b, &b->vp, D)) != LBM_OK || (s = balloc(b, &b->ubc, D)) != LBM_OK ||
@@ -244,10 +241,7 @@ lbm_status lbm_batch_create(const lbm_batch_instance* in, int32_t n, int32_t col
         return bad(s);
     const hipMemcpyKind h2d = hipMemcpyHostToDevice;
     if (hipMemcpy(b->inst, hinst.data(), sizeof(BInst) * n, h2d) != hipSuccess ||
-        hipMemcpy(b->tiles, tiles.data(), sizeof(uint2) * tiles.size(), h2d) != hipSuccess ||
-        hipMemcpy(b->cidx, cidx.data(), 4 * D, h2d) != hipSuccess ||
-        hipMemcpy(b->didx, didx.data(), 4 * cbase, h2d) != hipSuccess ||
-        hipMemcpy(b->nodew, nodew.data(), 4 * cbase, h2d) != hipSuccess ||
+        hipMemcpy(b->nodew, nodew.data(), 4 * D, h2d) != hipSuccess ||
         hipMemcpy(b->ubc, ubc.data(), 8 * D, h2d) != hipSuccess ||
         hipMemcpy(b->vbc, vbc.data(), 8 * D, h2d) != hipSuccess) {
         b->err = "host-to-device copy failed";
@@ -293,8 +287,8 @@ lbm_status lbm_batch_init(lbm_batch* b, int32_t i, const double* rho, const doub
         b->ready[k] = 1;
         b->steps[k] = 0;
     }
-    BCHK(b, batch_init(b->storage == LBM_STORE_F64 ? 0 : 1, b->collision, b->inst, b->tiles, b->ntiles, b->P[0],
-                       b->didx, bfields(b), i, b->stream));
+    BCHK(b, batch_init(b->storage == LBM_STORE_F64 ? 0 : 1, b->collision, b->inst, b->n, b->ntiles, b->P[0],
+                       b->nodew, bfields(b), i, b->stream));
     BCHK(b, hipMemcpyAsync(b->bs.par, b->par.data(), sizeof(int) * b->n, hipMemcpyHostToDevice, b->stream));
     BCHK(b, hipStreamSynchronize(b->stream));
     return LBM_OK;
@@ -319,9 +313,9 @@ lbm_status lbm_batch_run(lbm_batch* b, double tol, int32_t check_every, int64_t 
     BCHK(b, hipMemcpyAsync(b->bs.remaining, &remaining, sizeof(int), h2d, b->stream));
     BCHK(b, hipEventRecord(b->ev0, b->stream));
     if (max_steps > 0)
-        BCHK(b, batch_run(b->storage == LBM_STORE_F64 ? 0 : 1, b->collision, b->inst, n, b->tiles, b->ntiles, b->P[0],
-                          b->P[1], b->didx, b->nodew, b->cidx, bfields(b), b->up, b->vp, b->bs, tol, check_every,
-                          (int)max_steps, b->grid, b->stream));
+        BCHK(b, batch_run(b->storage == LBM_STORE_F64 ? 0 : 1, b->collision, b->inst, n, b->ntiles, b->P[0],
+                          b->P[1], b->nodew, bfields(b), b->up, b->vp, b->bs, tol, check_every, (int)max_steps,
+                          b->grid, b->stream));
     BCHK(b, hipEventRecord(b->ev1, b->stream));
     std::vector<int> done(n);
     std::vector<double> res(n);
@@ -366,8 +360,8 @@ lbm_status lbm_batch_get_f(lbm_batch* b, int32_t i, double* f) {
     // the reference's f after that iteration
     const int src = b->par[i] ^ 1;
     BCHK(b, hipMemsetAsync(b->fout, 0, 8 * 9 * nn, b->stream));
-    BCHK(b, batch_recon(b->storage == LBM_STORE_F64 ? 0 : 1, b->inst, i, B.nf, b->P[src], b->didx, b->nodew, b->cidx,
-                        bfields(b), b->fout, b->stream));
+    BCHK(b, batch_recon(b->storage == LBM_STORE_F64 ? 0 : 1, b->inst, i, B.n, b->P[src], b->nodew, bfields(b),
+                        b->fout, b->stream));
     BCHK(b, hipMemcpyAsync(f, b->fout, 8 * 9 * nn, hipMemcpyDeviceToHost, b->stream));
     BCHK(b, hipStreamSynchronize(b->stream));
     return LBM_OK;
