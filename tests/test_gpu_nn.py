@@ -1,0 +1,203 @@
+"""Warm-start network kernels (include/lbm_nn.h) against the FP64 oracle.
+
+GEMM: operands that are small multiples of 1/4 make every FP32 sum exact, so
+all layout variants, tile shapes, reduction splits and the transposed store
+are compared bit for bit.  Data-movement kernels match the oracle to FP32
+rounding.  The whole network's forward and backward match the oracle with
+bf16 operand rounding emulated: relative L2 <= 5e-3 (outputs) and <= 2e-2
+(parameter gradient), the room FP32 accumulation and the bf16 rounding flips
+it causes leave over 15 layers (measured 2.2e-3 and 2.6e-3; without the
+emulation about twice that; a wrong tap, padding or skip gives O(0.1-1)).
+Each case prints its name first."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import nn_oracle as O
+from thesis_amd import nn
+from thesis_amd import surrogate as S
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def rng(seed):
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def bf16_words(a):
+    """Raw bf16 words (int16 tensor on the device) of values exact in bf16."""
+    w = (np.asarray(a, np.float32).view(np.uint32) >> 16).astype(np.uint16)
+    return torch.from_numpy(w.view(np.int16).copy()).to(DEV)
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32).ravel()).to(DEV)
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy().astype(np.float64)
+
+
+@pytest.mark.parametrize("ta", [0, 1])
+@pytest.mark.parametrize("tb", [0, 1])
+@pytest.mark.parametrize("N", [3, 32, 64, 200])
+def test_gemm_exact(ta, tb, N):
+    print(f"case gemm ta={ta} tb={tb} N={N}", flush=True)
+    r = rng(10 + ta + 2 * tb + N)
+    M, R = 301, 77
+    A = r.integers(-4, 5, (M, R)) / 4.0
+    B = r.integers(-4, 5, (N, R)) / 4.0
+    bias = r.integers(-8, 9, N) / 8.0
+    Ad = bf16_words(A.T.copy() if ta else A)
+    Bd = bf16_words(B.T.copy() if tb else B)
+    ref = A @ B.T + bias
+    for flags, splits in ((0, 1), (nn.RELU, 1), (nn.ATOMIC, 3), (nn.ATOMIC | nn.TRANSC, 2)):
+        Cd = torch.zeros(M * N, dtype=torch.float32, device=DEV)
+        nn.gemm(M, N, R, Ad, M if ta else R, ta, Bd, N if tb else R, tb, Cd, M if flags & nn.TRANSC else N,
+                dev(bias), flags, splits)
+        got = host(Cd).reshape((N, M) if flags & nn.TRANSC else (M, N))
+        if flags & nn.TRANSC:
+            got = got.T
+        want = np.maximum(ref, 0) if flags & nn.RELU else ref
+        assert np.array_equal(got, want), (flags, splits, np.abs(got - want).max())
+
+
+def test_gemm_rejects_unsummed_splits():
+    z = torch.zeros(64, dtype=torch.float32, device=DEV)
+    w = torch.zeros(64, dtype=torch.int16, device=DEV)
+    with pytest.raises(Exception):
+        nn.gemm(8, 8, 8, w, 8, 0, w, 8, 0, z, 8, None, 0, 2)
+
+
+def test_data_movement_kernels_match_oracle():
+    print("case im2col/col2im/pool/up/xint", flush=True)
+    r = rng(20)
+    B, H, W, C = 2, 8, 16, 5
+    x = r.standard_normal((B, H, W, C)).astype(np.float32).astype(np.float64)
+    kp = O.kpad(C, 3)
+    col = torch.empty(B * H * W * kp, dtype=torch.int16, device=DEV)
+    nn.im2col(dev(x), B, H, W, C, kp, col)
+    got = (host(col).astype(np.int64) & 0xFFFF).astype(np.uint32) << 16
+    got = got.astype(np.uint32).view(np.float32).astype(np.float64).reshape(-1, kp)
+    assert np.array_equal(got, O.bf16(O.im2col(x, kp)))
+    dcol = r.standard_normal((B * H * W, kp)).astype(np.float32).astype(np.float64)
+    dx = torch.empty(B * H * W * C, dtype=torch.float32, device=DEV)
+    nn.col2im(dev(dcol), B, H, W, C, kp, dx)
+    assert np.abs(host(dx).reshape(x.shape) - O.col2im(dcol, x.shape)).max() < 1e-5
+    y = torch.empty(B * H * W * C // 4, dtype=torch.float32, device=DEV)
+    nn.avgpool2(dev(x), B, H, W, C, y)
+    assert np.abs(host(y).reshape(B, H // 2, W // 2, C) - O.avgpool2(x)).max() < 1e-6
+    dy = r.standard_normal((B, H // 2, W // 2, C))
+    nn.avgpool2_bwd(dev(dy), B, H, W, C, dx)
+    assert np.abs(host(dx).reshape(x.shape) - O.avgpool2_bwd(dy)).max() < 1e-6
+    u = torch.empty(B * H * W * C * 4, dtype=torch.float32, device=DEV)
+    nn.up2(dev(x), B, H, W, C, u)
+    assert np.abs(host(u).reshape(B, 2 * H, 2 * W, C) - O.up2(x)).max() < 1e-6
+    du = r.standard_normal((B, 2 * H, 2 * W, C))
+    nn.up2_bwd(dev(du), B, H, W, C, dx)
+    assert np.abs(host(dx).reshape(x.shape) - O.up2_bwd(du)).max() < 1e-5
+    xi = torch.empty(B * H * W * C, dtype=torch.float32, device=DEV)
+    nn.xint(dev(x), B, H, W, C, 0, xi)
+    assert np.abs(host(xi).reshape(x.shape) - O.xint(x)).max() < 1e-5
+    nn.xint(dev(x), B, H, W, C, 1, xi)
+    assert np.abs(host(xi).reshape(x.shape) - O.xint_bwd(x)).max() < 1e-5
+
+
+def test_mse_and_adam_kernels():
+    print("case mse/adam", flush=True)
+    r = rng(21)
+    pred = r.standard_normal((3, 4, 5, 3)).astype(np.float32).astype(np.float64)
+    tgt = r.standard_normal((3, 4, 5, 3)).astype(np.float32).astype(np.float64)
+    m = (r.random((3, 4, 5)) > 0.3).astype(np.float64)
+    w = np.array([1.0, 2.0, 0.5])
+    lo, go = O.mse(pred, tgt, m, w)
+    grad = torch.empty(pred.size, dtype=torch.float32, device=DEV)
+    loss = torch.zeros(1, dtype=torch.float32, device=DEV)
+    nn.mse(dev(pred), dev(tgt), dev(m), dev(w), m.size, 3, 1.0 / m.sum(), grad, loss)
+    assert abs(host(loss)[0] - lo) < 1e-5 * lo
+    assert np.abs(host(grad).reshape(pred.shape) - go).max() < 1e-6
+    n = 1000
+    wv, gv = r.standard_normal(n), r.standard_normal(n)
+    mv, vv = 0.1 * r.standard_normal(n), r.random(n)
+    wd, gd, md, vd = dev(wv), dev(gv), dev(mv), dev(vv)
+    wbf = torch.empty(600, dtype=torch.int16, device=DEV)
+    nn.adam(wd, gd, md, vd, 1e-3, 0.9, 0.999, 1e-8, 7, wbf, 600)
+    m2 = 0.9 * mv + 0.1 * gv
+    v2 = 0.999 * vv + 0.001 * gv * gv
+    w2 = wv - 1e-3 * (m2 / (1 - 0.9 ** 7)) / (np.sqrt(v2 / (1 - 0.999 ** 7)) + 1e-8)
+    assert np.abs(host(wd) - w2).max() < 1e-6
+    words = (host(wbf).astype(np.int64) & 0xFFFF).astype(np.uint32) << 16
+    assert np.abs(words.view(np.float32)[:600] - O.bf16(host(wd)[:600])).max() == 0
+
+
+def _setup(seed, B=2, H=32, W=64):
+    r = rng(seed)
+    p = S.init_params(seed=seed).astype(np.float64)
+    wo, bo, nw, n = S.layout()
+    p[nw:] = 0.05 * r.standard_normal(n - nw)
+    p = p.astype(np.float32).astype(np.float64)
+    x = r.standard_normal((B, H, W, 3)).astype(np.float32).astype(np.float64)
+    t = r.standard_normal((B, H, W, 3)).astype(np.float32).astype(np.float64)
+    m = (r.random((B, H, W)) > 0.2).astype(np.float64)
+    return p, x, t, m
+
+
+def test_network_forward_and_backward_match_oracle():
+    print("case network fwd/bwd B=2 H=32 W=64", flush=True)
+    p, x, t, m = _setup(30)
+    B, H, W = x.shape[:3]
+    ref = O.Net(S.ARCH, p, emulate=True)
+    out_o, c = ref.forward(x)
+    lo, dout = O.mse(out_o, t, m)
+    g_o = ref.backward(dout, c)
+    net = S.Net(p.astype(np.float32))
+    out, cache = net.forward(dev(x), B, H, W)
+    got = host(out).reshape(out_o.shape)
+    rel = np.linalg.norm(got - out_o) / np.linalg.norm(out_o)
+    plain = O.Net(S.ARCH, p).forward(x)[0]
+    print("forward relative L2", rel, "(against the un-emulated oracle:",
+          np.linalg.norm(got - plain) / np.linalg.norm(plain), ")", flush=True)
+    assert rel < 5e-3
+    nn.fill(0.0, net.g)
+    net.backward(dev(dout), cache)
+    g = host(net.g)
+    relg = np.linalg.norm(g - g_o) / np.linalg.norm(g_o)
+    print("gradient relative L2", relg, flush=True)
+    assert relg < 2e-2
+    wo, bo, nw, n = S.layout()
+    for li, (name, cin, cout, k, _) in enumerate(S.ARCH):
+        blk = slice(wo[li], wo[li] + cout * S.kpad(cin, k))
+        e = np.linalg.norm(g[blk] - g_o[blk]) / max(np.linalg.norm(g_o[blk]), 1e-30)
+        assert e < 5e-2, (name, e)
+
+
+def test_training_reduces_the_loss():
+    """A few hundred Adam steps on four fixed samples lower the masked loss
+    at least five-fold (the whole training path runs on the GPU kernels)."""
+    print("case overfit 4 samples", flush=True)
+    r = rng(40)
+    B, H, W = 4, 32, 64
+    yy, xx = np.mgrid[0:H, 0:W]
+    x = np.zeros((B, H, W, 3), np.float32)
+    t = np.zeros((B, H, W, 3), np.float32)
+    for b in range(B):
+        cx, cy = r.uniform(16, 48), r.uniform(8, 24)
+        solid = (xx - cx) ** 2 + (yy - cy) ** 2 < 16
+        x[b, ..., 0] = solid
+        x[b, ..., 1] = 4 * (yy + 0.5) * (H - yy - 0.5) / H ** 2
+        x[b, ..., 2] = xx / (W - 1)
+        t[b, ..., 0] = np.where(solid, 0, x[b, ..., 1] * (1 - np.exp(-((xx - cx) ** 2 + (yy - cy) ** 2) / 30)))
+        t[b, ..., 1] = np.where(solid, 0, 0.1 * np.sin(xx / 7.0) * np.cos(yy / 5.0))
+        t[b, ..., 2] = np.where(solid, 0, 0.5 * (1 - xx / W))
+    m = (x[..., 0] == 0).astype(np.float32)
+    net = S.Net()
+    wch = dev([1.0, 1.0, 1.0])
+    xd, td, md = dev(x), dev(t), dev(m)
+    first = float(net.step(xd, td, md, B, H, W, 1e-3, wch).item())
+    for _ in range(300):
+        last = float(net.step(xd, td, md, B, H, W, 1e-3, wch).item())
+    print("loss", first, "->", last, flush=True)
+    assert last < first / 5

@@ -128,6 +128,46 @@ def channel_sweep(count=64, seed=7, nx_range=(128, 1024), ny_range=(32, 256), re
     return out
 
 
+def surrogate_channels(count, seed, nx=512, ny=128, obstacles=(1, 6), r_range=(4.0, 16.0), u_range=(0.04, 0.1),
+                       tau=1.0):
+    """Seeded D2Q9 channels of one size for the learned warm start.
+
+    Per channel, from numpy.random.Generator(PCG64(seed)): peak inlet speed
+    u_max ~ U[u_range] at fixed tau (nu = (tau - 1/2)/3, so Re = 2 u_max ny / nu
+    on the hydraulic diameter runs over about 60-150 for the defaults), and
+    1-6 non-overlapping circular obstacles with radius ~ U[r_range], kept two
+    nodes clear of the walls and out of the inlet strip (ny/2 nodes) and the
+    outlet strip (ny nodes).  Each entry is channel2d's dict plus omega, nu,
+    u_max, Re and the obstacles."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nu = (tau - 0.5) / 3.0
+    out = []
+    for _ in range(count):
+        u_max = float(rng.uniform(*u_range))
+        nobs = int(rng.integers(obstacles[0], obstacles[1] + 1))
+        obs = []
+        for _attempt in range(400):
+            if len(obs) == nobs:
+                break
+            r = float(rng.uniform(*r_range))
+            cx = float(rng.uniform(ny / 2.0 + r, nx - ny - r))
+            cy = float(rng.uniform(r + 2.0, ny - 3.0 - r))
+            if all((cx - ox) ** 2 + (cy - oy) ** 2 > (r + orr + 2.0) ** 2 for ox, oy, orr in obs):
+                obs.append((cx, cy, r))
+        d = channel2d(nx, ny, u_max, obstacles=obs)
+        d.update(omega=1.0 / tau, trt_magic=3.0 / 16.0, nu=nu, u_max=u_max, Re=2.0 * u_max * ny / nu,
+                 obstacles=obs, nx=nx, ny=ny)
+        out.append(d)
+    return out
+
+
+def poiseuille_start(d):
+    """Fully developed channel profile of d's inlet in every column, zero in
+    solid nodes, rho = 1: the analytic start the surrogate has to beat."""
+    u = np.where(d["node_class"] != SOLID, d["profile"][:, None], 0.0)
+    return np.ones_like(u), u, np.zeros_like(u)
+
+
 def porous3d(n, seed=2024, rmin=6.0, rmax=12.0, solid_fraction=0.30, ny=None, nz=None):
     """Periodic overlapping-sphere porous medium on an n x ny x nz box.
 

@@ -1,0 +1,413 @@
+"""Learned warm start for D2Q9 channels: a 4-level convolutional
+encoder-decoder trained on the GPU with the library's own HIP kernels.
+
+The reference starts every run from uniform fields or from a macro file
+(S/Iterate.cu:211-275, S/FilesReading.cu:224-268); here a network maps the
+obstacle mask and inlet velocity of a 512 x 128 channel to its steady
+(u, v, rho), and the solver starts from f = f_eq(rho, u) of that prediction.
+
+Network (about 1.0 M parameters, bf16 weights, FP32 accumulation and
+activations, NHWC): 3x3 convolutions with ReLU, two per level at 32, 64, 128
+and 128 channels, 2x2 average pooling down, nearest x2 upsampling up with
+additive skip connections, and at the coarsest level a running integral
+along the channel axis (x + (1/W) sum_{x' >= x} x), which carries the
+downstream obstacles' pressure drop upstream.  A 1x1 convolution gives
+(u/u_max, v/u_max, (rho-1)/u_max).  Inputs: solid mask, inlet velocity
+u_max p(y)/0.1 broadcast along x, and x/(nx-1).
+
+Device memory is torch's (plumbing); every layer runs a kernel of
+include/lbm_nn.h.  Training data comes from the batched solver
+(lbm_batch_*), run from rest to a 1e-6 residual.
+"""
+import json
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import lattices as LT
+from . import lbm as L
+from . import nn
+
+# name, input channels, output channels, kernel size, ReLU
+ARCH = [
+    ("e1a", 3, 32, 3, True), ("e1b", 32, 32, 3, True),
+    ("e2a", 32, 64, 3, True), ("e2b", 64, 64, 3, True),
+    ("e3a", 64, 128, 3, True), ("e3b", 128, 128, 3, True),
+    ("b1", 128, 128, 3, True), ("b2", 128, 128, 3, True),
+    ("d3a", 128, 128, 3, True), ("d3b", 128, 128, 3, True),
+    ("d2a", 128, 64, 3, True), ("d2b", 64, 64, 3, True),
+    ("d1a", 64, 32, 3, True), ("d1b", 32, 32, 3, True),
+    ("out", 32, 3, 1, False),
+]
+U_SCALE = 0.1
+INDEX = {a[0]: i for i, a in enumerate(ARCH)}
+
+
+def kpad(cin, k):
+    """Columns of a layer's im2col matrix: cin*k*k rounded up to 32."""
+    return (cin * k * k + 31) // 32 * 32
+
+
+def layout():
+    """Offsets of every layer's weight block [cout x Kp] and bias in the flat
+    parameter vector: all weight blocks first (the bf16 region), then biases."""
+    off, wo = 0, []
+    for (_, cin, cout, k, _) in ARCH:
+        wo.append(off)
+        off += cout * kpad(cin, k)
+    nw = off
+    bo = []
+    for (_, _, cout, _, _) in ARCH:
+        bo.append(off)
+        off += cout
+    return wo, bo, nw, off
+
+
+def init_params(seed=0):
+    """He-normal weights for ReLU layers (zero in the padded columns), a small
+    output layer, zero biases; FP32 numpy vector in layout() order."""
+    wo, bo, nw, n = layout()
+    rng = np.random.Generator(np.random.PCG64(seed))
+    p = np.zeros(n, np.float32)
+    for li, (_, cin, cout, k, relu) in enumerate(ARCH):
+        kp, fan = kpad(cin, k), cin * k * k
+        std = math.sqrt(2.0 / fan) if relu else 0.1 / math.sqrt(fan)
+        w = np.zeros((cout, kp), np.float32)
+        w[:, :fan] = rng.standard_normal((cout, fan)) * std
+        p[wo[li]:wo[li] + cout * kp] = w.ravel()
+    return p
+
+
+def features(d):
+    """Network input (ny, nx, 3) of a channel dict from lattices.surrogate_channels."""
+    cls = d["node_class"]
+    ny, nx = cls.shape
+    x = np.zeros((ny, nx, 3), np.float32)
+    x[..., 0] = cls == L.SOLID
+    x[..., 1] = (d["u_max"] * 4.0 * (np.arange(ny) + 0.5) * (ny - np.arange(ny) - 0.5) / ny ** 2 / U_SCALE)[:, None]
+    x[..., 2] = (np.arange(nx) / (nx - 1.0))[None, :]
+    return x
+
+
+def targets(d, rho, u, v):
+    """Normalised steady fields (ny, nx, 3) and the fluid mask (ny, nx)."""
+    um = d["u_max"]
+    t = np.stack([u / um, v / um, (rho - 1.0) / um], axis=-1).astype(np.float32)
+    mask = (d["node_class"] != L.SOLID).astype(np.float32)
+    return np.where(mask[..., None] > 0, t, 0.0).astype(np.float32), mask
+
+
+def fields(d, pred):
+    """Physical (rho, u, v) from a normalised prediction (ny, nx, 3): zero
+    velocity in solids, the prescribed profile on the inlet column."""
+    um = d["u_max"]
+    solid = d["node_class"] == L.SOLID
+    u = np.where(solid, 0.0, pred[..., 0] * um)
+    v = np.where(solid, 0.0, pred[..., 1] * um)
+    rho = np.where(solid, 1.0, 1.0 + pred[..., 2] * um)
+    inlet = d["node_class"] == L.VEL_W
+    u = np.where(inlet, d["ubc"], u)
+    v = np.where(inlet, 0.0, v)
+    return rho, u, v
+
+
+class Net:
+    """Parameters, Adam state and the explicit forward/backward passes."""
+
+    def __init__(self, params=None, device="cuda"):
+        nn.load()
+        self.dev = torch.device(device)
+        self.wo, self.bo, self.nw, self.n = layout()
+        p = init_params() if params is None else np.asarray(params, np.float32)
+        assert p.size == self.n
+        self.w = torch.from_numpy(p.copy()).to(self.dev)
+        self.wbf = torch.empty(self.nw, dtype=torch.int16, device=self.dev)
+        nn.to_bf16(self.w[:self.nw], self.wbf)
+        self.g = torch.zeros(self.n, dtype=torch.float32, device=self.dev)
+        self.m = torch.zeros_like(self.g)
+        self.v = torch.zeros_like(self.g)
+        self.t = 0
+
+    # -- layers -----------------------------------------------------------
+    def _conv(self, x, Bn, H, W, li, cache):
+        _, cin, cout, k, relu = ARCH[li]
+        kp = kpad(cin, k)
+        M = Bn * H * W
+        col = torch.empty(M * kp, dtype=torch.int16, device=self.dev)
+        if k == 3:
+            nn.im2col(x, Bn, H, W, cin, kp, col)
+        else:
+            assert kp == cin
+            nn.to_bf16(x, col)
+        y = torch.empty(M * cout, dtype=torch.float32, device=self.dev)
+        wo, bo = self.wo[li], self.bo[li]
+        nn.gemm(M, cout, kp, col, kp, 0, self.wbf[wo:wo + cout * kp], kp, 0, y, cout,
+                self.w[bo:bo + cout], nn.RELU if relu else 0, 1)
+        cache[li] = (col, y, Bn, H, W)
+        return y
+
+    def _conv_bwd(self, dy, li, cache, need_dx=True):
+        _, cin, cout, k, relu = ARCH[li]
+        kp = kpad(cin, k)
+        col, y, Bn, H, W = cache[li]
+        M = Bn * H * W
+        wo, bo = self.wo[li], self.bo[li]
+        dz = torch.empty(M * cout, dtype=torch.int16, device=self.dev)
+        nn.grad_bf16(dy, y, relu, dz)
+        nn.colsum(dy, y, relu, M, cout, self.g[bo:bo + cout])
+        bm = 256 if cout <= 32 else 128
+        tiles = -(-kp // bm) * -(-cout // (32 if cout <= 32 else 64 if cout <= 64 else 128))
+        splits = max(1, min(2048 // max(tiles, 1), M // 2048))
+        nn.gemm(kp, cout, M, col, kp, 1, dz, cout, 1, self.g[wo:wo + cout * kp], kp, None,
+                nn.ATOMIC | nn.TRANSC, splits)
+        if not need_dx:
+            return None
+        dcol = torch.empty(M * kp, dtype=torch.float32, device=self.dev)
+        nn.gemm(M, kp, cout, dz, cout, 0, self.wbf[wo:wo + cout * kp], kp, 1, dcol, kp, None, 0, 1)
+        if k == 1:
+            return dcol
+        dx = torch.empty(M * cin, dtype=torch.float32, device=self.dev)
+        nn.col2im(dcol, Bn, H, W, cin, kp, dx)
+        return dx
+
+    def _new(self, n):
+        return torch.empty(n, dtype=torch.float32, device=self.dev)
+
+    # -- network ----------------------------------------------------------
+    def forward(self, x, Bn, H, W):
+        """x: FP32 device tensor (Bn*H*W*3) NHWC; returns (out, cache)."""
+        c = {}
+        I = INDEX
+        a = self._conv(x, Bn, H, W, I["e1a"], c)
+        s1 = self._conv(a, Bn, H, W, I["e1b"], c)
+        H2, W2 = H // 2, W // 2
+        p = self._new(Bn * H2 * W2 * 32)
+        nn.avgpool2(s1, Bn, H, W, 32, p)
+        a = self._conv(p, Bn, H2, W2, I["e2a"], c)
+        s2 = self._conv(a, Bn, H2, W2, I["e2b"], c)
+        H3, W3 = H2 // 2, W2 // 2
+        p = self._new(Bn * H3 * W3 * 64)
+        nn.avgpool2(s2, Bn, H2, W2, 64, p)
+        a = self._conv(p, Bn, H3, W3, I["e3a"], c)
+        s3 = self._conv(a, Bn, H3, W3, I["e3b"], c)
+        H4, W4 = H3 // 2, W3 // 2
+        p = self._new(Bn * H4 * W4 * 128)
+        nn.avgpool2(s3, Bn, H3, W3, 128, p)
+        a = self._conv(p, Bn, H4, W4, I["b1"], c)
+        xi = self._new(a.numel())
+        nn.xint(a, Bn, H4, W4, 128, 0, xi)
+        a = self._conv(xi, Bn, H4, W4, I["b2"], c)
+        u = self._new(Bn * H3 * W3 * 128)
+        nn.up2(a, Bn, H4, W4, 128, u)
+        a = self._conv(u, Bn, H3, W3, I["d3a"], c)
+        t = self._new(a.numel())
+        nn.add(a, s3, t)        # the convolution output stays intact for its ReLU mask
+        a = t
+        a = self._conv(a, Bn, H3, W3, I["d3b"], c)
+        u = self._new(Bn * H2 * W2 * 128)
+        nn.up2(a, Bn, H3, W3, 128, u)
+        a = self._conv(u, Bn, H2, W2, I["d2a"], c)
+        t = self._new(a.numel())
+        nn.add(a, s2, t)        # the convolution output stays intact for its ReLU mask
+        a = t
+        a = self._conv(a, Bn, H2, W2, I["d2b"], c)
+        u = self._new(Bn * H * W * 64)
+        nn.up2(a, Bn, H2, W2, 64, u)
+        a = self._conv(u, Bn, H, W, I["d1a"], c)
+        t = self._new(a.numel())
+        nn.add(a, s1, t)        # the convolution output stays intact for its ReLU mask
+        a = t
+        a = self._conv(a, Bn, H, W, I["d1b"], c)
+        out = self._conv(a, Bn, H, W, I["out"], c)
+        c["dims"] = (Bn, H, W)
+        return out, c
+
+    def backward(self, dout, c):
+        """Accumulates parameter gradients of the loss whose output gradient
+        is dout into self.g (zero it first)."""
+        I = INDEX
+        Bn, H, W = c["dims"]
+        H2, W2, H3, W3, H4, W4 = H // 2, W // 2, H // 4, W // 4, H // 8, W // 8
+        d = self._conv_bwd(dout, I["out"], c)
+        d = self._conv_bwd(d, I["d1b"], c)            # gradient of relu(d1a) + s1
+        ds1 = d
+        d = self._conv_bwd(d, I["d1a"], c)
+        da = self._new(Bn * H2 * W2 * 64)
+        nn.up2_bwd(d, Bn, H2, W2, 64, da)
+        d = self._conv_bwd(da, I["d2b"], c)
+        ds2 = d
+        d = self._conv_bwd(d, I["d2a"], c)
+        da = self._new(Bn * H3 * W3 * 128)
+        nn.up2_bwd(d, Bn, H3, W3, 128, da)
+        d = self._conv_bwd(da, I["d3b"], c)
+        ds3 = d
+        d = self._conv_bwd(d, I["d3a"], c)
+        da = self._new(Bn * H4 * W4 * 128)
+        nn.up2_bwd(d, Bn, H4, W4, 128, da)
+        d = self._conv_bwd(da, I["b2"], c)
+        dxi = self._new(d.numel())
+        nn.xint(d, Bn, H4, W4, 128, 1, dxi)
+        d = self._conv_bwd(dxi, I["b1"], c)
+        dp = self._new(Bn * H3 * W3 * 128)
+        nn.avgpool2_bwd(d, Bn, H3, W3, 128, dp)
+        nn.axpy(1.0, ds3, dp)
+        d = self._conv_bwd(dp, I["e3b"], c)
+        d = self._conv_bwd(d, I["e3a"], c)
+        dp = self._new(Bn * H2 * W2 * 64)
+        nn.avgpool2_bwd(d, Bn, H2, W2, 64, dp)
+        nn.axpy(1.0, ds2, dp)
+        d = self._conv_bwd(dp, I["e2b"], c)
+        d = self._conv_bwd(d, I["e2a"], c)
+        dp = self._new(Bn * H * W * 32)
+        nn.avgpool2_bwd(d, Bn, H, W, 32, dp)
+        nn.axpy(1.0, ds1, dp)
+        d = self._conv_bwd(dp, I["e1b"], c)
+        self._conv_bwd(d, I["e1a"], c, need_dx=False)
+
+    def loss_grad(self, out, target, mask, wch):
+        """Masked MSE (per fluid pixel, summed over channels) and its gradient."""
+        npix = mask.numel()
+        grad = torch.empty_like(out)
+        loss = torch.zeros(1, dtype=torch.float32, device=self.dev)
+        inv = 1.0 / max(float(mask.sum().item()), 1.0)
+        nn.mse(out, target, mask, wch, npix, 3, inv, grad, loss)
+        return loss, grad
+
+    def step(self, x, target, mask, Bn, H, W, lr, wch):
+        nn.fill(0.0, self.g)
+        out, c = self.forward(x, Bn, H, W)
+        loss, grad = self.loss_grad(out, target, mask, wch)
+        self.backward(grad, c)
+        self.t += 1
+        nn.adam(self.w, self.g, self.m, self.v, lr, 0.9, 0.999, 1e-8, self.t, self.wbf, self.nw)
+        return loss
+
+    def predict(self, X, Bn, H, W):
+        out, _ = self.forward(X, Bn, H, W)
+        return out
+
+    # -- persistence --------------------------------------------------------
+    def save(self, path, meta=None):
+        w = self.w.detach().cpu().numpy()
+        np.savez(path, wbf=self.wbf.detach().cpu().numpy().view(np.uint16), bias=w[self.nw:],
+                 meta=np.frombuffer(json.dumps(meta or {}).encode(), dtype=np.uint8))
+
+    @classmethod
+    def load(cls, path, device="cuda"):
+        z = np.load(path, allow_pickle=False)
+        wbf = z["wbf"].astype(np.uint32) << 16
+        p = np.concatenate([wbf.view(np.float32), z["bias"].astype(np.float32)])
+        return cls(p, device=device)
+
+
+# ---------------------------------------------------------------------------
+# data, training and the warm-start comparison
+
+
+def solve(ds, tol=1e-6, every=100, max_steps=400000, starts=None, chunk=64):
+    """Runs channels to the residual with the batched solver (FP64), 64 per
+    persistent launch; returns per channel (rho, u, v, steps, residual) and
+    the device milliseconds of the launches."""
+    out, total_ms = [], 0.0
+    for k in range(0, len(ds), chunk):
+        part = ds[k:k + chunk]
+        b = L.Batch(part, collision=L.TRT, storage="f64")
+        if starts is not None:
+            for i, s in enumerate(starts[k:k + chunk]):
+                b.init(i, *s)
+        steps, res, ms = b.run(tol, every=every, max_steps=max_steps)
+        total_ms += ms
+        for i in range(len(part)):
+            rho, u, v = b.macros(i)
+            out.append((rho, u, v, int(steps[i]), float(res[i])))
+        b.close()
+    return out, total_ms
+
+
+def dataset(count, seed, log=print):
+    """Input features, normalised targets and masks (numpy float32) for
+    `count` channels of lattices.surrogate_channels(count, seed)."""
+    ds = LT.surrogate_channels(count, seed)
+    ny, nx = ds[0]["node_class"].shape
+    X = np.zeros((count, ny, nx, 3), np.float32)
+    Y = np.zeros((count, ny, nx, 3), np.float32)
+    Mk = np.zeros((count, ny, nx), np.float32)
+    t0 = time.time()
+    for k in range(0, count, 64):
+        res, ms = solve(ds[k:k + 64])
+        for i, (rho, u, v, _, _) in enumerate(res):
+            d = ds[k + i]
+            X[k + i] = features(d)
+            Y[k + i], Mk[k + i] = targets(d, rho, u, v)
+        log(f"data {min(k + 64, count)}/{count} ({time.time() - t0:.0f} s)")
+    return X, Y, Mk
+
+
+def train(X, Y, Mk, epochs=30, batch=16, lr=2e-3, seed=0, log=print, net=None):
+    """Adam with a cosine learning-rate decay over the epochs."""
+    dev = torch.device("cuda")
+    net = net or Net()
+    n, H, W, _ = X.shape
+    Xd = torch.from_numpy(X).to(dev)
+    Yd = torch.from_numpy(Y).to(dev)
+    Md = torch.from_numpy(Mk).to(dev)
+    wch = torch.tensor([1.0, 1.0, 1.0], dtype=torch.float32, device=dev)
+    g = torch.Generator(device="cpu")
+    g.manual_seed(seed)
+    steps_per_epoch = n // batch
+    total = epochs * steps_per_epoch
+    history = []
+    t0 = time.time()
+    it = 0
+    for ep in range(epochs):
+        perm = torch.randperm(n, generator=g).to(dev)
+        acc = torch.zeros(1, dtype=torch.float32, device=dev)
+        for s in range(steps_per_epoch):
+            idx = perm[s * batch:(s + 1) * batch]
+            xb = Xd.index_select(0, idx).contiguous().view(-1)
+            yb = Yd.index_select(0, idx).contiguous().view(-1)
+            mb = Md.index_select(0, idx).contiguous().view(-1)
+            lr_t = lr * 0.5 * (1.0 + math.cos(math.pi * it / total))
+            acc += net.step(xb, yb, mb, batch, H, W, lr_t, wch)
+            it += 1
+        mean = float(acc.item()) / steps_per_epoch
+        history.append(mean)
+        log(f"epoch {ep + 1}/{epochs} loss {mean:.3e} ({time.time() - t0:.0f} s)")
+    return net, history
+
+
+def predict_fields(net, ds, batch=16):
+    """Physical (rho, u, v) predictions for channel dicts."""
+    dev = torch.device("cuda")
+    ny, nx = ds[0]["node_class"].shape
+    out = []
+    for k in range(0, len(ds), batch):
+        part = ds[k:k + batch]
+        X = torch.from_numpy(np.stack([features(d) for d in part])).to(dev).view(-1)
+        P = net.predict(X, len(part), ny, nx).view(len(part), ny, nx, 3).cpu().numpy()
+        out += [fields(d, P[i]) for i, d in enumerate(part)]
+    return out
+
+
+def warm_start_single(net, d, tol=1e-6, every=100, max_steps=400000):
+    """Wall time of one warm-started single-lattice run, inference and
+    injection included: returns (steps, seconds, solver)."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rho, u, v = predict_fields(net, [d])[0]
+    s = L.Solver(9, d["node_class"].shape, d["node_class"], d["bb"], collision=L.TRT, omega=d["omega"],
+                 trt_magic=d["trt_magic"], ubc=d["ubc"], vbc=d["vbc"], rho_out=d["rho_out"], storage="f64")
+    s.init(0, rho=rho, u=u, v=v)
+    n, _ = s.run_to_residual(tol, every=every, max_steps=max_steps)
+    return n, time.perf_counter() - t0, s
+
+
+def rest_start_single(d, tol=1e-6, every=100, max_steps=400000):
+    t0 = time.perf_counter()
+    s = L.Solver(9, d["node_class"].shape, d["node_class"], d["bb"], collision=L.TRT, omega=d["omega"],
+                 trt_magic=d["trt_magic"], ubc=d["ubc"], vbc=d["vbc"], rho_out=d["rho_out"], storage="f64")
+    s.init(0)
+    n, _ = s.run_to_residual(tol, every=every, max_steps=max_steps)
+    return n, time.perf_counter() - t0, s
