@@ -28,6 +28,16 @@ def test_library_exports_every_declared_symbol():
     assert lib.lbm_device_count() >= 0
 
 
+def test_nn_header_matches_binding_and_library():
+    from thesis_amd import nn
+    txt = open(os.path.join(ROOT, "include", "lbm_nn.h")).read()
+    declared = sorted(set(re.findall(r"\b(nn_[a-z_0-9]+)\s*\(", txt)))
+    assert declared == sorted(nn.EXPORTED)
+    lib = nn.load()
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
 def test_create_rejects_bad_parameters():
     cls = np.ones((4, 4), np.uint8)
     with pytest.raises(L.LbmError) as ei:

@@ -139,7 +139,7 @@ def _setup(seed, B=2, H=32, W=64):
     wo, bo, nw, n = S.layout()
     p[nw:] = 0.05 * r.standard_normal(n - nw)
     p = p.astype(np.float32).astype(np.float64)
-    x = r.standard_normal((B, H, W, 3)).astype(np.float32).astype(np.float64)
+    x = r.standard_normal((B, H, W, S.ARCH[0][1])).astype(np.float32).astype(np.float64)
     t = r.standard_normal((B, H, W, 3)).astype(np.float32).astype(np.float64)
     m = (r.random((B, H, W)) > 0.2).astype(np.float64)
     return p, x, t, m
@@ -181,7 +181,7 @@ def test_training_reduces_the_loss():
     r = rng(40)
     B, H, W = 4, 32, 64
     yy, xx = np.mgrid[0:H, 0:W]
-    x = np.zeros((B, H, W, 3), np.float32)
+    x = np.zeros((B, H, W, S.ARCH[0][1]), np.float32)
     t = np.zeros((B, H, W, 3), np.float32)
     for b in range(B):
         cx, cy = r.uniform(16, 48), r.uniform(8, 24)

@@ -62,7 +62,7 @@ def test_network_gradient_by_central_differences():
     wo, bo, nw, n = S.layout()
     p[nw:] = 0.05 * r.standard_normal(n - nw)        # non-zero biases exercise every ReLU branch
     net = O.Net(S.ARCH, p)
-    x = r.standard_normal((1, 16, 32, 3))
+    x = r.standard_normal((1, 16, 32, S.ARCH[0][1]))
     t = r.standard_normal((1, 16, 32, 3))
     m = (r.random((1, 16, 32)) > 0.2).astype(float)
     out, c = net.forward(x)
@@ -88,7 +88,7 @@ def test_gradient_check_catches_a_wrong_skip():
     r = rng(4)
     p = S.init_params(seed=6).astype(np.float64)
     net = O.Net(S.ARCH, p)
-    x = r.standard_normal((1, 16, 32, 3))
+    x = r.standard_normal((1, 16, 32, S.ARCH[0][1]))
     t = r.standard_normal((1, 16, 32, 3))
     m = np.ones((1, 16, 32))
     out, c = net.forward(x)

@@ -255,7 +255,11 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
         blk[2 * b + 1] = r1 - r0 + 1;
         rmax = std::max(rmax, (int)(r1 - r0 + 1));
     }
-    const bool rank_ok = (size_t)rmax * 9 * words * 8 <= 48 * 1024;
+    int lgw = 0;
+    while ((1 << lgw) < words) ++lgw;
+    // LDS: staged row words at a power-of-two pitch plus 256 node positions;
+    // positions pack x in 16 bits
+    const bool rank_ok = (((size_t)rmax * 9) << lgw) * 8 + 1024 <= 48 * 1024 && p->nx < 65536;
     if (any_open_cls && !lat->u_bc) {
         bool need = false;
         for (size_t i = 0; i < n; ++i) need |= lat->node_class[i] == LBM_VEL_W;
@@ -357,10 +361,11 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     k.rowmeta = (const uint2*)c->rowmeta;
     k.blk = (const uint2*)c->blk;
     k.words = words;
+    k.lgw = lgw;
     k.rmax = rmax;
     k.simple = simple ? 1 : 0;
-    k.rank = rank_ok ? 1 : 0;
-    if (const char* e = std::getenv("LBM_RANK")) k.rank = rank_ok && e[0] == '1';
+    k.rank = rank_ok && k.tile ? 1 : 0;   // the rank step addresses the tile layout only
+    if (const char* e = std::getenv("LBM_RANK")) k.rank = rank_ok && k.tile && e[0] == '1';
     k.sr = 1;
     if (const char* e = std::getenv("LBM_SR")) k.sr = e[0] == '1';
     auto h2d = [&](void* d, const void* h, size_t b) { return hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, c->stream); };

@@ -182,8 +182,9 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;
 
 // Quotient of a 32-bit n by an invariant divisor d: q = (((m n) >> 32) + n) >> s
 // with m = floor(2^32 (2^s - d) / d) + 1, s = ceil(log2 d) (exact for all n < 2^32).
+// m < 2^32 for every d <= 2^31, so the high product is one v_mul_hi_u32.
 __device__ inline uint32_t fdiv(uint32_t n, uint64_t m, uint32_t s) {
-    return (uint32_t)((((m * n) >> 32) + n) >> s);
+    return (uint32_t)(((uint64_t)__umulhi((uint32_t)m, n) + n) >> s);
 }
 
 // Stage 1 of the pull: compact indices of the source node of every direction
@@ -232,6 +233,13 @@ template <class L>
 __device__ inline uint32_t paddr(const KParams& kp, int i, uint32_t j) {
     if (kp.tile) return (j >> 6) * (64u * L::Q) + (uint32_t)i * 64u + (j & 63u);
     return (uint32_t)i * kp.qstride32 + j;
+}
+
+// Tile-layout slot of plane i of compact node j.  j >> 6 < 2^24 whenever the
+// buffer is below 4 GiB, so the tile stride is a full-rate 24-bit multiply.
+template <class L>
+__device__ inline uint32_t tslot(int i, uint32_t j) {
+    return __umul24(j >> 6, 64u * L::Q) + (uint32_t)i * 64u + (j & 63u);
 }
 
 template <class L, int S, bool OPEN>

@@ -26,9 +26,11 @@ namespace lbm {
 // stored deviation still accumulates), with random bits from a hash of the
 // compact node index and a per-launch seed, rotated per direction; results are
 // deterministic for a given seed sequence.
-template <class L, int S>
+// TILE: the buffer is known to use the tile layout (no runtime layout test).
+template <class L, int S, bool TILE = false>
 __device__ inline void store_node(typename Io<L, S>::raw* __restrict__ dst, const KParams& kp, uint32_t own,
                                   const typename Io<L, S>::T* out) {
+    auto slot = [&](int i) { return TILE ? tslot<L>(i, own) : paddr<L>(kp, i, own); };
     if (S >= 2 && kp.sr) {
         uint32_t h = own * 0x9E3779B1u + kp.seed;
         h ^= h >> 15;
@@ -36,10 +38,10 @@ __device__ inline void store_node(typename Io<L, S>::raw* __restrict__ dst, cons
         h ^= h >> 12;
 #pragma unroll
         for (int i = 0; i < L::Q; ++i)
-            Io<L, S>::st_sr(dst, paddr<L>(kp, i, own), out[i], i, (h << (i & 31)) | (h >> ((32 - i) & 31)));
+            Io<L, S>::st_sr(dst, slot(i), out[i], i, (h << (i & 31)) | (h >> ((32 - i) & 31)));
     } else {
 #pragma unroll
-        for (int i = 0; i < L::Q; ++i) Io<L, S>::st(dst, paddr<L>(kp, i, own), out[i], i);
+        for (int i = 0; i < L::Q; ++i) Io<L, S>::st(dst, slot(i), out[i], i);
     }
 }
 
@@ -99,19 +101,6 @@ __global__ __launch_bounds__(256) void k_step(const typename Io<L, S>::raw* __re
 // source node are then a popcount away.  With SIMPLE (every wall is a link into
 // a solid node, all non-solid nodes plain fluid) the node words are not read
 // either, and a step moves only the populations.
-__device__ inline int select_bit(uint32_t m, uint32_t t) {   // position of the t-th set bit
-    int pos = 0;
-    uint32_t c = __popc(m & 0xFFFFu);
-    if (t >= c) { t -= c; m >>= 16; pos += 16; }
-    c = __popc(m & 0xFFu);
-    if (t >= c) { t -= c; m >>= 8; pos += 8; }
-    c = __popc(m & 0xFu);
-    if (t >= c) { t -= c; m >>= 4; pos += 4; }
-    c = __popc(m & 0x3u);
-    if (t >= c) { t -= c; m >>= 2; pos += 2; }
-    if (t >= (m & 1u)) pos += 1;
-    return pos;
-}
 
 constexpr uint32_t kSolidSrc = 0xFFFFFFFEu;
 
@@ -124,36 +113,53 @@ __global__ __launch_bounds__(256) void k_stepr(const typename Io<L, S>::raw* __r
                                                Fields fl, int write_macros) {
     using T = typename Io<L, S>::T;
     extern __shared__ uint2 smeta[];
-    const int W = kp.words, nx = kp.nx, ny = kp.ny, nz = kp.nz;
+    const int W = kp.words, lg = kp.lgw, nx = kp.nx, ny = kp.ny, nz = kp.nz;
     const uint2 bi = blk[blockIdx.x];
     const uint32_t r0 = bi.x;
     const int R = (int)bi.y;
-    // stage the words of rows r0..r0+R-1 and of their (dy, dz) neighbours;
-    // slot (k, nb) with nb = (dz+1)*3 + (dy+1); rows outside a non-periodic
-    // face read as all-solid (outside sources are resolved by coordinates)
-    const int tot = R * 9 * W;
+    // Stage the words of rows r0..r0+R-1 and of their (dy, dz) neighbours at
+    // smeta[((k*9 + nb) << lg) + w], nb = (dz+1)*3 + (dy+1), rows padded to a
+    // power of two; rows outside a non-periodic face read as all-solid
+    // (outside sources are resolved by coordinates).
+    const int tot = (R * 9) << lg;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-        const int k = e / (9 * W), rem = e - k * 9 * W, nb = rem / W, w = rem - nb * W;
-        const uint32_t r = r0 + (uint32_t)k;
-        const uint32_t zq = fdiv(r, kp.my, kp.sy);
-        int y = (int)(r - zq * (uint32_t)ny), z = (int)zq;
-        y += nb % 3 - 1;
-        z += nb / 3 - 1;
-        bool out = false;
-        if (y < 0 || y >= ny) { if (kp.per[1]) y = (y + ny) % ny; else out = true; }
-        if (z < 0 || z >= nz) { if (kp.per[2]) z = (z + nz) % nz; else out = true; }
-        smeta[e] = out ? make_uint2(0u, 0u) : rowmeta[((size_t)z * ny + y) * W + w];
+        const int w = e & ((1 << lg) - 1), slot = e >> lg;
+        const int k = slot / 9, nb = slot - 9 * k;
+        uint2 val = make_uint2(0u, 0u);
+        if (w < W) {
+            const uint32_t r = r0 + (uint32_t)k;
+            const uint32_t zq = fdiv(r, kp.my, kp.sy);
+            int y = (int)(r - zq * (uint32_t)ny) + nb % 3 - 1, z = (int)zq + nb / 3 - 1;
+            bool out = false;
+            if (y < 0) { y += ny; out = !kp.per[1]; } else if (y >= ny) { y -= ny; out = !kp.per[1]; }
+            if (z < 0) { z += nz; out |= !kp.per[2]; } else if (z >= nz) { z -= nz; out |= !kp.per[2]; }
+            if (!out) val = rowmeta[((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)W + (uint32_t)w];
+        }
+        smeta[e] = val;
     }
     __syncthreads();
-    const uint32_t own = blockIdx.x * blockDim.x + threadIdx.x;
+    // Position of every node of this workgroup: the threads owning a staged own
+    // row word walk its fluid bits and write (row, x) at the node's slot, so no
+    // thread searches for its own node.
+    const uint32_t own0 = blockIdx.x * blockDim.x;
+    uint32_t* pos = (uint32_t*)(smeta + ((kp.rmax * 9) << lg));
+    for (int t = threadIdx.x; t < (R << lg); t += blockDim.x) {
+        const int k = t >> lg, w = t & ((1 << lg) - 1);
+        if (w >= W) continue;
+        const uint2 me = smeta[((k * 9 + 4) << lg) + w];
+        uint32_t m = me.x, rank = me.y;
+        while (m) {
+            const uint32_t slot = rank - own0;
+            if (slot < blockDim.x) pos[slot] = ((uint32_t)k << 16) | (uint32_t)(32 * w + __builtin_ctz(m));
+            m &= m - 1u;
+            ++rank;
+        }
+    }
+    __syncthreads();
+    const uint32_t own = own0 + threadIdx.x;
     if (own >= kp.nf) return;
-    // own row: the last staged own row whose first compact index is <= own
-    int k = 0;
-    while (k + 1 < R && smeta[((k + 1) * 9 + 4) * W].y <= own) ++k;
-    const uint2* me = smeta + (k * 9 + 4) * W;
-    int w = 0;
-    while (w + 1 < W && me[w + 1].y <= own) ++w;
-    const int x = 32 * w + select_bit(me[w].x, own - me[w].y);
+    const uint32_t p = pos[threadIdx.x];
+    const int k = (int)(p >> 16), x = (int)(p & 0xFFFFu);
     const uint32_t r = r0 + (uint32_t)k;
     const uint32_t zq = fdiv(r, kp.my, kp.sy);
     const int y = (int)(r - zq * (uint32_t)ny), z = (int)zq;
@@ -168,7 +174,8 @@ __global__ __launch_bounds__(256) void k_stepr(const typename Io<L, S>::raw* __r
     }
     const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
     T h[L::Q];
-    h[0] = Io<L, S>::ld(src, paddr<L>(kp, 0, own), 0);
+    h[0] = Io<L, S>::ld(src, tslot<L>(0, own), 0);
+    const int kb = (k * 9) << lg;
 #pragma unroll
     for (int i = 1; i < L::Q; ++i) {
         const int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i), o = L::opp(i);
@@ -179,7 +186,7 @@ __global__ __launch_bounds__(256) void k_stepr(const typename Io<L, S>::raw* __r
                   (cz > 0 && zm_o) || (cz < 0 && zp_o);
         uint32_t j = kNone;
         if (!out) {
-            const uint2 e = smeta[(k * 9 + (1 - cz) * 3 + (1 - cy)) * W + (sx >> 5)];
+            const uint2 e = smeta[kb + (((1 - cz) * 3 + (1 - cy)) << lg) + (sx >> 5)];
             const uint32_t bit = (uint32_t)sx & 31u;
             j = ((e.x >> bit) & 1u) ? e.y + __popc(e.x & ((1u << bit) - 1u)) : kSolidSrc;
         }
@@ -189,7 +196,7 @@ __global__ __launch_bounds__(256) void k_stepr(const typename Io<L, S>::raw* __r
             j = own;
             pl = wall && kp.wall == 0 ? o : i;
         }
-        h[i] = Io<L, S>::ld(src, paddr<L>(kp, pl, j), i);
+        h[i] = Io<L, S>::ld(src, tslot<L>(pl, j), i);
     }
     if (OPEN && cls == VEL_W) zou_inlet<L, T>(kp, fl, idx, nd, h);
     if (kp.wall == 1 && (nd & kBBMask)) onnode_walls<L, T>(nd, h);
@@ -202,7 +209,7 @@ __global__ __launch_bounds__(256) void k_stepr(const typename Io<L, S>::raw* __r
     }
     T out[L::Q];
     collide<L, T, C>(kp, h, drho, rho, u, out);
-    store_node<L, S>(dst, kp, own, out);
+    store_node<L, S, true>(dst, kp, own, out);
 }
 
 // Post-boundary populations (dense, full, FP64) and moments of the state in src.
@@ -356,7 +363,7 @@ static hipError_t step_t(const KParams& kp, const void* src, void* dst, const ui
     if (kp.nf == 0) return hipSuccess;
     const unsigned blocks = (unsigned)((kp.nf + 255) / 256);
     if (kp.rank) {
-        const size_t lds = (size_t)kp.rmax * 9 * kp.words * sizeof(uint2);
+        const size_t lds = (((size_t)kp.rmax * 9) << kp.lgw) * sizeof(uint2) + 256 * sizeof(uint32_t);
         if (kp.simple)
             hipLaunchKernelGGL((k_stepr<L, S, C, OPEN, true>), dim3(blocks), dim3(256), lds, st, (const raw*)src,
                                (raw*)dst, kp.rowmeta, kp.blk, kp.nodew, kp, fl, wm ? 1 : 0);

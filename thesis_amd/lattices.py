@@ -162,10 +162,19 @@ def surrogate_channels(count, seed, nx=512, ny=128, obstacles=(1, 6), r_range=(4
 
 
 def poiseuille_start(d):
-    """Fully developed channel profile of d's inlet in every column, zero in
-    solid nodes, rho = 1: the analytic start the surrogate has to beat."""
-    u = np.where(d["node_class"] != SOLID, d["profile"][:, None], 0.0)
-    return np.ones_like(u), u, np.zeros_like(u)
+    """Fully developed channel flow of d's inlet in every column: the inlet
+    profile, zero in solid nodes, and the Poiseuille pressure drop
+    rho = 1 + 3 G (nx - 1 - x) with G = 8 nu u_max / ny^2 down to the outlet
+    density 1 (walls at y = -1/2 and ny - 1/2).  The analytic start the
+    learned one has to beat."""
+    cls = d["node_class"]
+    ny, nx = cls.shape
+    nu = (1.0 / d["omega"] - 0.5) / 3.0
+    G = 8.0 * nu * d["u_max"] / ny ** 2
+    solid = cls == SOLID
+    u = np.where(solid, 0.0, d["profile"][:, None])
+    rho = np.where(solid, 1.0, 1.0 + 3.0 * G * (nx - 1.0 - np.arange(nx))[None, :])
+    return rho, u, np.zeros_like(u)
 
 
 def porous3d(n, seed=2024, rmin=6.0, rmax=12.0, solid_fraction=0.30, ny=None, nz=None):

@@ -11,9 +11,9 @@ activations, NHWC): 3x3 convolutions with ReLU, two per level at 32, 64, 128
 and 128 channels, 2x2 average pooling down, nearest x2 upsampling up with
 additive skip connections, and at the coarsest level a running integral
 along the channel axis (x + (1/W) sum_{x' >= x} x), which carries the
-downstream obstacles' pressure drop upstream.  A 1x1 convolution gives
-(u/u_max, v/u_max, (rho-1)/u_max).  Inputs: solid mask, inlet velocity
-u_max p(y)/0.1 broadcast along x, and x/(nx-1).
+downstream obstacles' pressure drop upstream.  A 1x1 convolution gives the
+deviations ((u - u_p), v, (rho - rho_p)) / u_max.  Inputs: solid mask, the
+inlet profile p(y), x/(nx-1) and u_max/0.1.
 
 Device memory is torch's (plumbing); every layer runs a kernel of
 include/lbm_nn.h.  Training data comes from the batched solver
@@ -32,7 +32,7 @@ from . import nn
 
 # name, input channels, output channels, kernel size, ReLU
 ARCH = [
-    ("e1a", 3, 32, 3, True), ("e1b", 32, 32, 3, True),
+    ("e1a", 4, 32, 3, True), ("e1b", 32, 32, 3, True),
     ("e2a", 32, 64, 3, True), ("e2b", 64, 64, 3, True),
     ("e3a", 64, 128, 3, True), ("e3b", 128, 128, 3, True),
     ("b1", 128, 128, 3, True), ("b2", 128, 128, 3, True),
@@ -81,32 +81,41 @@ def init_params(seed=0):
 
 
 def features(d):
-    """Network input (ny, nx, 3) of a channel dict from lattices.surrogate_channels."""
+    """Network input (ny, nx, 4) of a channel dict from lattices.surrogate_channels:
+    solid mask, inlet profile p(y) = 4 (y+1/2)(ny-y-1/2)/ny^2 broadcast along x,
+    x/(nx-1), and u_max/0.1."""
     cls = d["node_class"]
     ny, nx = cls.shape
-    x = np.zeros((ny, nx, 3), np.float32)
+    x = np.zeros((ny, nx, 4), np.float32)
+    y = np.arange(ny) + 0.5
     x[..., 0] = cls == L.SOLID
-    x[..., 1] = (d["u_max"] * 4.0 * (np.arange(ny) + 0.5) * (ny - np.arange(ny) - 0.5) / ny ** 2 / U_SCALE)[:, None]
+    x[..., 1] = (4.0 * y * (ny - y) / ny ** 2)[:, None]
     x[..., 2] = (np.arange(nx) / (nx - 1.0))[None, :]
+    x[..., 3] = d["u_max"] / U_SCALE
     return x
 
 
 def targets(d, rho, u, v):
-    """Normalised steady fields (ny, nx, 3) and the fluid mask (ny, nx)."""
+    """Normalised deviation (ny, nx, 3) of the steady fields from the analytic
+    channel start (lattices.poiseuille_start), ((u - u_p), v, (rho - rho_p)) / u_max,
+    and the fluid mask (ny, nx)."""
     um = d["u_max"]
-    t = np.stack([u / um, v / um, (rho - 1.0) / um], axis=-1).astype(np.float32)
+    rp, up, _ = LT.poiseuille_start(d)
+    t = np.stack([(u - up) / um, v / um, (rho - rp) / um], axis=-1).astype(np.float32)
     mask = (d["node_class"] != L.SOLID).astype(np.float32)
     return np.where(mask[..., None] > 0, t, 0.0).astype(np.float32), mask
 
 
 def fields(d, pred):
-    """Physical (rho, u, v) from a normalised prediction (ny, nx, 3): zero
-    velocity in solids, the prescribed profile on the inlet column."""
+    """Physical (rho, u, v) from a normalised prediction (ny, nx, 3): the
+    analytic start plus the predicted deviation, zero velocity in solids, the
+    prescribed profile on the inlet column."""
     um = d["u_max"]
+    rp, up, _ = LT.poiseuille_start(d)
     solid = d["node_class"] == L.SOLID
-    u = np.where(solid, 0.0, pred[..., 0] * um)
+    u = np.where(solid, 0.0, up + pred[..., 0] * um)
     v = np.where(solid, 0.0, pred[..., 1] * um)
-    rho = np.where(solid, 1.0, 1.0 + pred[..., 2] * um)
+    rho = np.where(solid, 1.0, rp + pred[..., 2] * um)
     inlet = d["node_class"] == L.VEL_W
     u = np.where(inlet, d["ubc"], u)
     v = np.where(inlet, 0.0, v)
@@ -306,11 +315,13 @@ class Net:
 # data, training and the warm-start comparison
 
 
-def solve(ds, tol=1e-6, every=100, max_steps=400000, starts=None, chunk=64):
+def solve(ds, tol=1e-6, every=100, max_steps=400000, starts=None, chunk=64, then=None):
     """Runs channels to the residual with the batched solver (FP64), 64 per
     persistent launch; returns per channel (rho, u, v, steps, residual) and
-    the device milliseconds of the launches."""
-    out, total_ms = [], 0.0
+    the device milliseconds of the launches.  With `then`, each batch then
+    continues from its exact state to that tighter residual, and a second
+    list (the references) is returned too."""
+    out, refs, total_ms = [], [], 0.0
     for k in range(0, len(ds), chunk):
         part = ds[k:k + chunk]
         b = L.Batch(part, collision=L.TRT, storage="f64")
@@ -322,8 +333,13 @@ def solve(ds, tol=1e-6, every=100, max_steps=400000, starts=None, chunk=64):
         for i in range(len(part)):
             rho, u, v = b.macros(i)
             out.append((rho, u, v, int(steps[i]), float(res[i])))
+        if then is not None:
+            steps2, res2, _ = b.run(then, every=every, max_steps=max_steps)
+            for i in range(len(part)):
+                rho, u, v = b.macros(i)
+                refs.append((rho, u, v, int(steps[i] + steps2[i]), float(res2[i])))
         b.close()
-    return out, total_ms
+    return (out, total_ms, refs) if then is not None else (out, total_ms)
 
 
 def dataset(count, seed, log=print):
@@ -331,7 +347,7 @@ def dataset(count, seed, log=print):
     `count` channels of lattices.surrogate_channels(count, seed)."""
     ds = LT.surrogate_channels(count, seed)
     ny, nx = ds[0]["node_class"].shape
-    X = np.zeros((count, ny, nx, 3), np.float32)
+    X = np.zeros((count, ny, nx, ARCH[0][1]), np.float32)
     Y = np.zeros((count, ny, nx, 3), np.float32)
     Mk = np.zeros((count, ny, nx), np.float32)
     t0 = time.time()
@@ -411,3 +427,118 @@ def rest_start_single(d, tol=1e-6, every=100, max_steps=400000):
     s.init(0)
     n, _ = s.run_to_residual(tol, every=every, max_steps=max_steps)
     return n, time.perf_counter() - t0, s
+
+
+def rel_err(a, b, mask):
+    """(max, L2) velocity error of field a against b over mask, relative to b."""
+    ua, va = a
+    ub, vb = b
+    d = np.sqrt((ua - ub) ** 2 + (va - vb) ** 2)[mask]
+    s = np.sqrt(ub ** 2 + vb ** 2)[mask]
+    return float(d.max() / s.max()), float(np.sqrt((d ** 2).sum() / (s ** 2).sum()))
+
+
+def evaluate(net, test, log=print, single=16, tol=1e-6):
+    """Rest, analytic (Poiseuille) and learned starts of the held-out channels
+    to the residual: steps, device time of the batched launches, single-lattice
+    wall time (inference and injection included) on the first `single`
+    channels, and errors against a 1e-8 reference."""
+    rep = {}
+    t0 = time.time()
+    rest, ms_rest, ref = solve(test, tol=tol, then=1e-8)
+    log(f"rest starts and their 1e-8 continuations done ({time.time() - t0:.0f} s)")
+    pois, ms_pois = solve(test, tol=tol, starts=[LT.poiseuille_start(d) for d in test])
+    torch.cuda.synchronize()
+    ti = time.perf_counter()
+    pred = predict_fields(net, test)
+    torch.cuda.synchronize()
+    t_inf = time.perf_counter() - ti
+    warm, ms_warm = solve(test, tol=tol, starts=pred)
+    log(f"warm starts done ({time.time() - t0:.0f} s)")
+    rows = []
+    for i, d in enumerate(test):
+        fl = d["node_class"] != L.SOLID
+        rr = (ref[i][1], ref[i][2])
+        rows.append(dict(
+            steps_rest=rest[i][3], steps_pois=pois[i][3], steps_warm=warm[i][3],
+            err_rest=rel_err((rest[i][1], rest[i][2]), rr, fl), err_pois=rel_err((pois[i][1], pois[i][2]), rr, fl),
+            err_warm=rel_err((warm[i][1], warm[i][2]), rr, fl), err_pred=rel_err((pred[i][1], pred[i][2]), rr, fl),
+            err_pois_start=rel_err(LT.poiseuille_start(d)[1:], rr, fl),
+            err_rho_pred=float(np.abs(pred[i][0] - ref[i][0])[fl].max() / np.abs(ref[i][0] - 1.0)[fl].max())))
+    rep["per_channel"] = rows
+    sr = np.array([r["steps_rest"] for r in rows], float)
+    sw = np.array([r["steps_warm"] for r in rows], float)
+    sp = np.array([r["steps_pois"] for r in rows], float)
+    rep["steps"] = dict(rest_mean=sr.mean(), warm_mean=sw.mean(), pois_mean=sp.mean(),
+                        ratio_mean=float((sr / sw).mean()), ratio_total=float(sr.sum() / sw.sum()))
+    rep["batched_ms"] = dict(rest=ms_rest, pois=ms_pois, warm=ms_warm, inference_s=t_inf,
+                             speedup=ms_rest / (ms_warm + 1e3 * t_inf))
+    e = np.array([r["err_rho_pred"] for r in rows])
+    rep["err_rho_pred"] = dict(mean=float(e.mean()), max=float(e.max()))
+    for key in ("err_rest", "err_pois", "err_warm", "err_pred", "err_pois_start"):
+        e = np.array([r[key] for r in rows])
+        rep[key] = dict(max_mean=float(e[:, 0].mean()), max_max=float(e[:, 0].max()), l2_mean=float(e[:, 1].mean()),
+                        l2_max=float(e[:, 1].max()))
+    # single-lattice wall time, inference and injection included
+    wr, ww = [], []
+    for d in test[:single]:
+        n_r, t_r, s = rest_start_single(d, tol=tol)
+        s.close()
+        n_w, t_w, s = warm_start_single(net, d, tol=tol)
+        s.close()
+        wr.append(t_r)
+        ww.append(t_w)
+    rep["single_wall_s"] = dict(rest_mean=float(np.mean(wr)), warm_mean=float(np.mean(ww)),
+                                speedup=float(np.sum(wr) / np.sum(ww)), n=len(wr))
+    log(f"single-lattice runs done ({time.time() - t0:.0f} s)")
+    return rep
+
+
+def main(argv=None):
+    import argparse
+    import os
+    ap = argparse.ArgumentParser(description="train and evaluate the learned warm start on the GPU")
+    ap.add_argument("--train", type=int, default=2000)
+    ap.add_argument("--test", type=int, default=200)
+    ap.add_argument("--train-seed", type=int, default=1)
+    ap.add_argument("--test-seed", type=int, default=9001)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--out", default="gpurun_out/surrogate")
+    ap.add_argument("--weights", default=None, help="skip training, evaluate these weights")
+    a = ap.parse_args(argv)
+    os.makedirs(a.out, exist_ok=True)
+    t0 = time.time()
+
+    def log(msg):
+        print(f"[{time.time() - t0:7.1f} s] {msg}", flush=True)
+
+    report = dict(args=vars(a))
+    if a.weights:
+        net = Net.load(a.weights)
+    else:
+        X, Y, Mk = dataset(a.train, a.train_seed, log=log)
+        report["data_s"] = time.time() - t0
+        t1 = time.time()
+        net, hist = train(X, Y, Mk, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
+        report["train_s"] = time.time() - t1
+        report["loss_history"] = hist
+        del X, Y, Mk
+        net.save(os.path.join(a.out, "weights.npz"), meta=dict(train=a.train, seed=a.train_seed, epochs=a.epochs))
+    test = LT.surrogate_channels(a.test, a.test_seed)
+    report["eval"] = evaluate(net, test, log=log)
+    with open(os.path.join(a.out, "report.json"), "w") as f:
+        json.dump(report, f, indent=1, default=float)
+    ev = report["eval"]
+    log(f"steps rest {ev['steps']['rest_mean']:.0f} warm {ev['steps']['warm_mean']:.0f} "
+        f"poiseuille {ev['steps']['pois_mean']:.0f}; batched speedup {ev['batched_ms']['speedup']:.2f}; "
+        f"single-lattice wall speedup {ev['single_wall_s']['speedup']:.2f}")
+    log(f"errors vs 1e-8 reference (max-norm mean): rest {ev['err_rest']['max_mean']:.2e} "
+        f"warm {ev['err_warm']['max_mean']:.2e} poiseuille {ev['err_pois']['max_mean']:.2e} "
+        f"prediction {ev['err_pred']['max_mean']:.2e} (L2 {ev['err_pred']['l2_mean']:.2e}; analytic start "
+        f"{ev['err_pois_start']['max_mean']:.2e}, L2 {ev['err_pois_start']['l2_mean']:.2e})")
+
+
+if __name__ == "__main__":
+    main()
