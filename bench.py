@@ -121,6 +121,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variants", action="store_true", help="also time f16/bf16 storage, MRT, BGK and the batch")
     ap.add_argument("--no-warm-start", action="store_true")
+    ap.add_argument("--batch-only", action="store_true", help="only the 64-channel batch line (for profiling)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -128,6 +129,10 @@ def main():
         print("bench.py: this build runs on one GPU; --gpus above 1 is not supported "
               "(the multi-GPU slab decomposition is a design note only).", file=sys.stderr)
         sys.exit(2)
+
+    if args.batch_only:
+        print(json.dumps({"batch64": batch64()}), flush=True)
+        return
 
     from thesis_amd import lattices as LT
     from thesis_amd import lbm as L

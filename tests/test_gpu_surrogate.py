@@ -1,6 +1,6 @@
 """The committed warm-start weights on held-out channels (seed 9001, never
-trained on): the learned start is closer to the steady state than the
-untrained network (whose zero deviation is the analytic channel start), and
+trained on): the learned start is closer to the steady state than the same
+network with its random initial weights (the trained-vs-untrained ablation), and
 single-lattice runs from it reach the 1e-6 residual in fewer steps than runs
 from rest, ending within a few per cent of the rest-started state (the
 Zou-He channel's steady state depends on the start)."""

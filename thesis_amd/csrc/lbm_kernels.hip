@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void k_step(const typename Io<L, S>::raw* __re
 constexpr uint32_t kSolidSrc = 0xFFFFFFFEu;
 
 template <class L, int S, int C, bool OPEN, bool SIMPLE>
-__global__ __launch_bounds__(256) void k_stepr(const typename Io<L, S>::raw* __restrict__ src,
+__global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __restrict__ src,
                                                typename Io<L, S>::raw* __restrict__ dst,
                                                const uint2* __restrict__ rowmeta,
                                                const uint2* __restrict__ blk,
@@ -363,12 +363,13 @@ static hipError_t step_t(const KParams& kp, const void* src, void* dst, const ui
     if (kp.nf == 0) return hipSuccess;
     const unsigned blocks = (unsigned)((kp.nf + 255) / 256);
     if (kp.rank) {
-        const size_t lds = (((size_t)kp.rmax * 9) << kp.lgw) * sizeof(uint2) + 256 * sizeof(uint32_t);
+        const size_t lds = (((size_t)kp.rmax * 9) << kp.lgw) * sizeof(uint2) + (size_t)kp.rblock * sizeof(uint32_t);
+        const unsigned rb = (unsigned)((kp.nf + kp.rblock - 1) / kp.rblock);
         if (kp.simple)
-            hipLaunchKernelGGL((k_stepr<L, S, C, OPEN, true>), dim3(blocks), dim3(256), lds, st, (const raw*)src,
+            hipLaunchKernelGGL((k_stepr<L, S, C, OPEN, true>), dim3(rb), dim3(kp.rblock), lds, st, (const raw*)src,
                                (raw*)dst, kp.rowmeta, kp.blk, kp.nodew, kp, fl, wm ? 1 : 0);
         else
-            hipLaunchKernelGGL((k_stepr<L, S, C, OPEN, false>), dim3(blocks), dim3(256), lds, st, (const raw*)src,
+            hipLaunchKernelGGL((k_stepr<L, S, C, OPEN, false>), dim3(rb), dim3(kp.rblock), lds, st, (const raw*)src,
                                (raw*)dst, kp.rowmeta, kp.blk, kp.nodew, kp, fl, wm ? 1 : 0);
     } else {
         hipLaunchKernelGGL((k_step<L, S, C, OPEN>), dim3(blocks), dim3(256), 0, st, (const raw*)src, (raw*)dst,
