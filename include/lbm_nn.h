@@ -47,6 +47,8 @@ int nn_up2_bwd(const float* dy, int Bn, int H, int W, int C, float* dx, void* st
 /* y = x + (1/W) sum_{x' >= x} x along the width (bwd = 0), or its adjoint
  * dx = dy + (1/W) sum_{x' <= x} dy (bwd = 1) */
 int nn_xint(const float* x, int Bn, int H, int W, int C, int bwd, float* y, void* stream);
+/* y = x + (1/H) sum over the height of x (symmetric: also its own adjoint) */
+int nn_ymix(const float* x, int Bn, int H, int W, int C, float* y, void* stream);
 int nn_axpy(float a, const float* x, int64_t n, float* y, void* stream);
 int nn_add(const float* x, const float* y, int64_t n, float* z, void* stream);   /* z = x + y */
 int nn_fill(float v, int64_t n, float* y, void* stream);

@@ -115,6 +115,11 @@ def xint_bwd(dy):
     return dy + np.cumsum(dy, axis=2) / dy.shape[2]
 
 
+def ymix(x):
+    """x + the mean over the height axis (symmetric, its own adjoint)."""
+    return x + x.mean(axis=1, keepdims=True)
+
+
 class Net:
     """Forward/backward of the encoder-decoder for a flat parameter vector."""
 
@@ -171,7 +176,7 @@ class Net:
         a = self.conv(avgpool2(s2), "e3a", c)
         s3 = self.conv(a, "e3b", c)
         a = self.conv(avgpool2(s3), "b1", c)
-        a = self.conv(xint(a), "b2", c)
+        a = self.conv(ymix(xint(a)), "b2", c)
         a = self.conv(up2(a), "d3a", c) + s3
         a = self.conv(a, "d3b", c)
         a = self.conv(up2(a), "d2a", c) + s2
@@ -192,7 +197,7 @@ class Net:
         d = self.conv_bwd(d, "d3b", c, g)
         ds3 = d
         d = up2_bwd(self.conv_bwd(d, "d3a", c, g))
-        d = xint_bwd(self.conv_bwd(d, "b2", c, g))
+        d = xint_bwd(ymix(self.conv_bwd(d, "b2", c, g)))
         d = avgpool2_bwd(self.conv_bwd(d, "b1", c, g)) + ds3
         d = self.conv_bwd(self.conv_bwd(d, "e3b", c, g), "e3a", c, g)
         d = avgpool2_bwd(d) + ds2

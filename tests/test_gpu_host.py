@@ -55,6 +55,14 @@ def test_lbmsolver_matches_abi_run(tmp_path):
     data = np.loadtxt(tmp_path / "Results" / "FinalData.csv", delimiter=",", skiprows=1)
     assert data.shape == (nx * ny, 9)
     assert (tmp_path / "Results" / "residuals.dat").exists()
+    # runtimes.dat and lbmsolver.log in the reference's layout (S/LogWriter.cu:5-188)
+    keys = [l.rsplit(" ", 1)[0] for l in (tmp_path / "Results" / "runtimes.dat").read_text().splitlines()]
+    assert keys == ["Overall", "Iteration", "Initialisation", "Collision", "Streaming", "Boundaries", "Macroscopic",
+                    "Residuals", "File writing"]
+    log = (tmp_path / "Results" / "lbmsolver.log").read_text()
+    for line in (">>> CollisionModel    : TRT", ">>> # of nodes in x (n)      = 64", ">>> NumInletNodes            = 16",
+                 ":::: Start Iterations ::::", "Main while loop took", ":::: Iterations done! ::::"):
+        assert line in log, line
     # same lattice through the ABI: reread the written files, same profile rule
     lat = L.read_lattice(str(tmp_path / "Mesh" / "chan_Node.dat"), str(tmp_path / "Mesh" / "chan_BC.dat"))
     cls = lat["node_class"][0]
@@ -71,3 +79,58 @@ def test_lbmsolver_matches_abi_run(tmp_path):
     rho, u, v, w = s.macros()
     assert np.abs(data[:, 3] - u.ravel()).max() < 1e-9     # CSV keeps 10 decimals
     assert np.abs(data[:, 7] - rho.ravel()).max() < 1e-9
+
+
+@pytest.mark.parametrize("fmt,ext", [("CSV", "csv"), ("DAT", "dat"), ("VTI", "vti")])
+def test_lbmsolver_restart_from_results_file(tmp_path, fmt, ext):
+    """InitialConditionFromFile (S/FilesReading.cu:224-268): a run restarted
+    from its own FinalData file with 0 further iterations writes the same
+    fields back, in each output format; a restart that then continues equals
+    the uninterrupted run from the same fields through the C ABI (f = 0 start
+    with those macros, the reference's restart)."""
+    print(f"case: lbmsolver restart {fmt}", flush=True)
+    nx, ny, U, nu = 64, 16, 0.02, 0.1
+    ch = LT.channel2d(nx, ny, 1.5 * U, obstacles=((20, 8, 3),))
+    os.makedirs(tmp_path / "Mesh")
+    LT.write_latticegen2(str(tmp_path / "Mesh" / "chan"), ch["node_class"], ch["bb"])
+    ini = INI.format(U=U, nu=nu, iters=300).replace("OutputFormat = CSV", f"OutputFormat = {fmt}")
+    (tmp_path / "SetUpData.ini").write_text(ini)
+    r = subprocess.run([CLI, "-f", "SetUpData.ini", "-o", "Run1/"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    first = tmp_path / "Run1" / f"FinalData.{ext}"
+    ini2 = (ini.replace("InitialConditionFromFile = no", "InitialConditionFromFile = yes")
+               .replace('InitialFile = "none"', f'InitialFile = "{first}"')
+               .replace("Number of iterations = 300", "Number of iterations = 0")
+               .replace("init = equilibrium\n", ""))
+    (tmp_path / "restart.ini").write_text(ini2)
+    r = subprocess.run([CLI, "-f", "restart.ini", "-o", "Run2/"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout[-300:], r.stderr[-300:])
+    assert r.returncode == 0 and "Initial conditions loaded" in r.stdout
+    r = subprocess.run([CLI, "compare", str(first), str(tmp_path / "Run2" / f"FinalData.{ext}")], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 0, r.stdout
+    if fmt == "CSV":
+        (tmp_path / "restart200.ini").write_text(ini2.replace("Number of iterations = 0", "Number of iterations = 200"))
+        r = subprocess.run([CLI, "-f", "restart200.ini", "-o", "Run3/"], cwd=tmp_path, capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0
+        d0 = np.loadtxt(first, delimiter=",", skiprows=1)
+        d3 = np.loadtxt(tmp_path / "Run3" / "FinalData.csv", delimiter=",", skiprows=1)
+        lat = L.read_lattice(str(tmp_path / "Mesh" / "chan_Node.dat"), str(tmp_path / "Mesh" / "chan_BC.dat"))
+        cls = lat["node_class"][0]
+        ys = np.arange(ny)
+        ubc = np.zeros((ny, nx))
+        ubc[:, 0] = 6 * U * (ys + 0.5) * (ny - 0.5 - ys) / ny ** 2
+        s = L.Solver(9, (ny, nx), cls, lat["bb_links"][0], collision=L.TRT, omega=1 / (3 * nu + 0.5),
+                     ubc=np.where(cls == L.VEL_W, ubc, 0.0), vbc=np.zeros((ny, nx)), rho_out=1.0)
+        s.init(1, rho=d0[:, 7].reshape(ny, nx), u=d0[:, 3].reshape(ny, nx), v=d0[:, 4].reshape(ny, nx))
+        s.step(200)
+        rho, u, v, w = s.macros()
+        assert np.abs(d3[:, 3] - u.ravel()).max() < 1e-9 and np.abs(d3[:, 7] - rho.ravel()).max() < 1e-9
+    # a wrong file falls back to the uniform start with a message
+    (tmp_path / "bad.ini").write_text(ini2.replace(str(first), str(tmp_path / "missing.csv")))
+    r = subprocess.run([CLI, "-f", "bad.ini", "-o", "Run4/"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0 and "uniform start" in r.stderr

@@ -104,6 +104,8 @@ def test_data_movement_kernels_match_oracle():
     assert np.abs(host(xi).reshape(x.shape) - O.xint(x)).max() < 1e-5
     nn.xint(dev(x), B, H, W, C, 1, xi)
     assert np.abs(host(xi).reshape(x.shape) - O.xint_bwd(x)).max() < 1e-5
+    nn.ymix(dev(x), B, H, W, C, xi)
+    assert np.abs(host(xi).reshape(x.shape) - O.ymix(x)).max() < 1e-5
 
 
 def test_mse_and_adam_kernels():

@@ -24,7 +24,7 @@ def test_im2col_matches_loop_convolution():
     assert (O.im2col(x, kp)[:, 27:] == 0).all()
 
 
-@pytest.mark.parametrize("op", ["im2col", "pool", "up", "xint"])
+@pytest.mark.parametrize("op", ["im2col", "pool", "up", "xint", "ymix"])
 def test_adjoints(op):
     r = rng(2)
     x = r.standard_normal((2, 4, 8, 3))
@@ -40,10 +40,14 @@ def test_adjoints(op):
         fx = O.up2(x)
         y = r.standard_normal(fx.shape)
         ty = O.up2_bwd(y)
-    else:
+    elif op == "xint":
         fx = O.xint(x)
         y = r.standard_normal(fx.shape)
         ty = O.xint_bwd(y)
+    else:
+        fx = O.ymix(x)
+        y = r.standard_normal(fx.shape)
+        ty = O.ymix(y)
     assert abs((fx * y).sum() - (x * ty).sum()) < 1e-10 * (1 + abs((fx * y).sum()))
 
 
@@ -106,7 +110,7 @@ def test_gradient_check_catches_a_wrong_skip():
             d = self.conv_bwd(d, "d3b", c, g)
             ds3 = d
             d = O.up2_bwd(self.conv_bwd(d, "d3a", c, g))
-            d = O.xint_bwd(self.conv_bwd(d, "b2", c, g))
+            d = O.xint_bwd(O.ymix(self.conv_bwd(d, "b2", c, g)))
             d = O.avgpool2_bwd(self.conv_bwd(d, "b1", c, g)) + ds3
             d = self.conv_bwd(self.conv_bwd(d, "e3b", c, g), "e3a", c, g)
             d = O.avgpool2_bwd(d) + ds2

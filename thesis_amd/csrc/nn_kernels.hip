@@ -308,6 +308,25 @@ __global__ void k_xint(const float* __restrict__ x, int Bn, int H, int W, int C,
     }
 }
 
+// y = x + (1/H) sum_{y'} x (mean over the height broadcast back); the operator
+// is symmetric, so it is its own adjoint.  One thread per (b, x, c) column.
+__global__ void k_ymix(const float* __restrict__ x, int Bn, int H, int W, int C, float* __restrict__ y) {
+    const size_t cols = (size_t)Bn * W * C;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < cols; e += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(e % C);
+        const size_t bw = e / C;
+        const int xw = (int)(bw % W);
+        const size_t b = bw / W;
+        float s = 0.f;
+        for (int yh = 0; yh < H; ++yh) s += x[((b * H + yh) * W + xw) * C + c];
+        s /= (float)H;
+        for (int yh = 0; yh < H; ++yh) {
+            const size_t i = ((b * H + yh) * W + xw) * C + c;
+            y[i] = x[i] + s;
+        }
+    }
+}
+
 __global__ void k_axpy(float a, const float* __restrict__ x, size_t n, float* __restrict__ y) {
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
         y[e] += a * x[e];
@@ -432,6 +451,9 @@ int nn_up2_bwd(const float* dy, int Bn, int H, int W, int C, float* dx, void* st
 }
 int nn_xint(const float* x, int Bn, int H, int W, int C, int bwd, float* y, void* stream) {
     NN_LAUNCH(k_xint, (size_t)Bn * H * C, x, Bn, H, W, C, bwd, y);
+}
+int nn_ymix(const float* x, int Bn, int H, int W, int C, float* y, void* stream) {
+    NN_LAUNCH(k_ymix, (size_t)Bn * W * C, x, Bn, H, W, C, y);
 }
 int nn_axpy(float a, const float* x, int64_t n, float* y, void* stream) {
     NN_LAUNCH(k_axpy, (size_t)n, a, x, (size_t)n, y);

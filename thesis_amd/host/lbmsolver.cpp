@@ -6,7 +6,8 @@
 // (S/Arguments.cu:155-315), the latticegen2 Mesh/<name>_Node.dat and _BC.dat,
 // runs the time loop through the C ABI (include/lbm.h) and writes
 // Results/FinalData.{csv,dat,vti} (S/FilesWriting.cu:146-258) and
-// Results/residuals.dat (S/LogWriter.cu:190-205).  Exit codes follow the
+// Results/residuals.dat (S/LogWriter.cu:190-205), lbmsolver.log
+// (:5-174) and runtimes.dat (:176-188).  Exit codes follow the
 // reference driver: 0 ok, 1 diverged, 2 file error (S/include/Iterate.h:62).
 //
 // Lines after the 23 positional ones are ignored by the reference; here an
@@ -16,6 +17,9 @@
 //   force_model = shift|guo       rho_out = <density>
 #include "lbm.h"
 
+#include <hip/hip_runtime_api.h>
+
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -34,10 +38,14 @@ struct Args {
     int inlet_profile = 2;           // 1 INLET_PROF, 2 INLET_CTE, 3 INLET_PUL
     int collision = LBM_BGK;
     int curved = 0, hh_walls = 0, outlet = 1;
-    long iterations = 1000, residual_every = 100;
+    long iterations = 1000, residual_every = 100, autosave_after = 0, autosave_every = 0;
+    std::string residual_model = "L2";
+    int boundary_id = 0, update_inlet_outlet = 1;
     double stop[4] = {0, 0, 0, 0};
     int format = 3;                  // 1 CSV, 2 DAT (Tecplot), 3 VTI
     // new knobs
+    bool use_init_file = false;      // ini line 17: start from the macroscopic fields of a results file
+    std::string init_file;           // ini line 18
     int storage = LBM_STORE_F64, init_eq = 0, wall_model = LBM_WALL_ONNODE, force_model = LBM_FORCE_SHIFT;
     double trt_magic = 0, rho_out = -1;
 };
@@ -79,7 +87,14 @@ bool read_ini(const std::string& path, Args& a) {
     a.iterations = atol(v[11].c_str());
     sscanf(v[14].c_str(), "%lf %lf %lf %lf", &a.stop[0], &a.stop[1], &a.stop[2], &a.stop[3]);
     a.format = has(v[15], "CSV") ? 1 : has(v[15], "DAT") ? 2 : 3;
+    a.use_init_file = has(v[16], "yes");
+    a.init_file = unquote(v[17]);
     a.residual_every = atol(v[19].c_str());
+    a.autosave_after = atol(v[12].c_str());
+    a.autosave_every = atol(v[13].c_str());
+    a.residual_model = trim(v[18]);
+    a.boundary_id = atoi(v[21].c_str());
+    a.update_inlet_outlet = has(v[22], "yes");
     a.g = atof(v[20].c_str());
     // optional section after line 23
     std::map<std::string, std::string> kv;
@@ -156,6 +171,46 @@ int compare_files(const char* f1, const char* f2) {
     return bad;
 }
 
+// Initial macroscopic fields from a results file, the reference's
+// readInitConditionsFile/readResultFile (S/FilesReading.cu:224-268,
+// :348-407): CSV ("x," header, one line per node), Tecplot ("Ti" header and
+// two more lines) or ASCII VTI ("<?" header, 3N + 18 lines: velocity
+// vectors, then the density).  The node count must match; columns are
+// x, y, z, u, v, w, |u|, rho, p.  Returns false (uniform start) otherwise.
+bool read_initial_conditions(const std::string& path, size_t N, std::vector<double>& u, std::vector<double>& v,
+                             std::vector<double>& w, std::vector<double>& rho) {
+    std::ifstream in(path);
+    if (!in) return false;
+    std::vector<std::string> lines;
+    std::string line;
+    while (std::getline(in, line)) lines.push_back(line);
+    if (lines.empty()) return false;
+    const std::string& h = lines[0];
+    u.assign(N, 0.0); v.assign(N, 0.0); w.assign(N, 0.0); rho.assign(N, 1.0);
+    auto row9 = [&](size_t first) {
+        for (size_t i = 0; i < N; ++i) {
+            std::string t = lines[first + i];
+            for (char& ch : t) if (ch == ',') ch = ' ';
+            double c[9];
+            if (std::sscanf(t.c_str(), "%lf %lf %lf %lf %lf %lf %lf %lf %lf", c, c + 1, c + 2, c + 3, c + 4, c + 5,
+                            c + 6, c + 7, c + 8) != 9)
+                return false;
+            u[i] = c[3]; v[i] = c[4]; w[i] = c[5]; rho[i] = c[7];
+        }
+        return true;
+    };
+    if (h.rfind("x,", 0) == 0 && lines.size() - 1 == N) return row9(1);
+    if (h.rfind("Ti", 0) == 0 && lines.size() - 3 == N) return row9(3);
+    if (h.rfind("<?", 0) == 0 && lines.size() == 18 + 3 * N) {
+        for (size_t i = 0; i < N; ++i)
+            if (std::sscanf(lines[7 + i].c_str(), "%lf %lf %lf", &u[i], &v[i], &w[i]) != 3) return false;
+        for (size_t i = 0; i < N; ++i)
+            if (std::sscanf(lines[9 + N + i].c_str(), "%lf", &rho[i]) != 1) return false;
+        return true;
+    }
+    return false;
+}
+
 void write_results(const std::string& path, int fmt, int n, int m, int h, double dx, const std::vector<double>& rho,
                    const std::vector<double>& u, const std::vector<double>& v, const std::vector<double>& w) {
     FILE* fp = std::fopen(path.c_str(), "w");
@@ -192,9 +247,95 @@ void write_results(const std::string& path, int fmt, int n, int m, int h, double
     std::fclose(fp);
 }
 
+// Results/lbmsolver.log in the layout of the reference's writeInitLog,
+// writeNodeNumbers and writeEndLog (S/LogWriter.cu:5-174).
+void write_init_log(const std::string& path, const Args& a, double delta, int n, int m, int h, long inlet_nodes,
+                    double ymax, double ymin) {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) return;
+    std::fprintf(f, "This is the 3D lattice Boltzmann *.log file\n\n");
+    std::fprintf(f, "\n:::: Imported variables from the *.ini file :::: \n");
+    std::fprintf(f, ">>> Uavg              : %3.6f\n", a.u);
+    std::fprintf(f, ">>> Vavg              : %3.6f\n", a.v);
+    std::fprintf(f, ">>> Wavg              : %3.6f\n", a.w);
+    std::fprintf(f, ">>> Force             : %3.6f\n", a.g);
+    std::fprintf(f, ">>> Initial density   : %2.1f\n", a.rho);
+    std::fprintf(f, ">>> Viscosity         : %3.8f\n", a.viscosity);
+    std::fprintf(f, ">>> # of iterations   : %1.1ld\n", a.iterations);
+    std::fprintf(f, ">>> Autosave after    : %1.1ld\n", a.autosave_after);
+    std::fprintf(f, ">>> Autosave every    : %1.1ld\n", a.autosave_every);
+    std::fprintf(f, ">>> CollisionModel    : %s\n", a.collision == LBM_TRT ? "TRT" : a.collision == LBM_MRT ? "MRT" : "BGKW");
+    std::fprintf(f, ">>> InletProfile      : %s\n", a.inlet_profile == 1 ? "ON" : a.inlet_profile == 3 ? "PULSATILE" : "CONSTANT");
+    std::fprintf(f, ">>> OutletProfile     : %s\n",
+                 a.outlet == 1 ? "ON" : a.outlet == 2 ? "OPEN (2nd)" : a.outlet == 3 ? "OPEN (1st)" : "He model");
+    std::fprintf(f, ">>> CurvedBoundaries  : %s\n", a.curved ? "ON" : "OFF");
+    std::fprintf(f, ">>> Residuals type    : %s\n", a.residual_model.find("MacroDiff") != std::string::npos ? "MaxMacroDiffs"
+                                                     : a.residual_model.find("FdRelDiff") != std::string::npos ? "MaxRelativeDiff" : "L2");
+    std::fprintf(f, ">>> Boundary model    : %s\n", a.hh_walls ? "HHmodel" : "BBack");
+    std::fprintf(f, ">>> Results format    : %s\n", a.format == 1 ? "excel " : a.format == 2 ? "Tecplot (*.dat)" : "Paraview (*.vti)");
+    std::fprintf(f, ">>> Initial conditions from file : %s\n", a.use_init_file ? "YES" : "NO");
+    std::fprintf(f, ">>> Initial conditions filename: %s\n", a.init_file.c_str());
+    std::fprintf(f, ">>> Residuals and macro diffs calculated each: %ld iterations\n", a.residual_every);
+    std::fprintf(f, ">>> Stop conditions: %f %f %f %f\n", a.stop[0], a.stop[1], a.stop[2], a.stop[3]);
+    if (a.boundary_id > 0) std::fprintf(f, ">>> Drag, lift @ BC   : %d\n", a.boundary_id);
+    else std::fprintf(f, ">>> Drag, lift was not calculated\n");
+    std::fprintf(f, ">>> UpdateInltOutl\t:\t%s\n", a.update_inlet_outlet ? "yes" : "no");
+    std::fprintf(f, "\n:::: Calculated variables from mesh :::: \n");
+    std::fprintf(f, ">>> Grid spacing             = %f\n", delta);
+    std::fprintf(f, ">>> # of nodes in x (n)      = %d\n", n);
+    std::fprintf(f, ">>> # of nodes in y (m)      = %d\n", m);
+    std::fprintf(f, ">>> # of nodes in z (h)      = %d\n", h);
+    std::fprintf(f, ">>> NumInletNodes            = %ld\n", inlet_nodes);
+    std::fprintf(f, ">>> MaxInletCoordY           = %f\n", ymax);
+    std::fprintf(f, ">>> MinInletCoordY           = %f\n", ymin);
+    std::fprintf(f, ">>> MaxInletCoordZ           = %f\n", 0.0);
+    std::fprintf(f, ">>> MinInletCoordZ           = %f\n", 0.0);
+    std::fclose(f);
+}
+
+// Phase times in seconds, keys of S/include/LogWriter.h:248-259.  The step is
+// one fused kernel, so collision holds streaming, boundaries and moments too.
+struct Times { double overall = 0, iter = 0, init = 0, coll = 0, strm = 0, bndc = 0, macr = 0, resi = 0, writ = 0; };
+
+void write_end_log(const std::string& path, const Times& t) {
+    FILE* f = std::fopen(path.c_str(), "a");
+    if (!f) return;
+    size_t fr = 0, tot = 1;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) { fr = 0; tot = 1; }
+    std::fprintf(f, "^^^^ Free : %llu Mbytes \n", (unsigned long long)fr / 1024 / 1024);
+    std::fprintf(f, "^^^^ Total: %llu Mbytes \n", (unsigned long long)tot / 1024 / 1024);
+    std::fprintf(f, "^^^^ %f%% free, %f%% used\n", 100.0 * fr / (double)tot, 100.0 * (tot - fr) / (double)tot);
+    std::fprintf(f, "\nOverall calculations took %f seconds", t.overall);
+    std::fprintf(f, "\nMain while loop took %f seconds\n", t.iter);
+    std::fprintf(f, "Initialization took %f seconds\n", t.init);
+    std::fprintf(f, "Collision took %f seconds\n", t.coll);
+    std::fprintf(f, "Streaming took %f seconds\n", t.strm);
+    std::fprintf(f, "Calculating Boundaries took %f seconds\n", t.bndc);
+    std::fprintf(f, "Update Macroscopic took %f seconds\n", t.macr);
+    std::fprintf(f, "Calculating residuals took %f seconds\n", t.resi);
+    std::fprintf(f, "Writing results took %f seconds\n", t.writ);
+    std::fprintf(f, "\n:::: Iterations done! ::::\n");
+    std::fclose(f);
+}
+
+// Results/runtimes.dat (S/LogWriter.cu:176-188).
+void write_times(const std::string& path, const Times& t) {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) return;
+    std::fprintf(f, "Overall %f\nIteration %f\nInitialisation %f\nCollision %f\nStreaming %f\nBoundaries %f\n"
+                    "Macroscopic %f\nResiduals %f\nFile writing %f\n",
+                 t.overall, t.iter, t.init, t.coll, t.strm, t.bndc, t.macr, t.resi, t.writ);
+    std::fclose(f);
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
+    const double t_start = now_s();
     Args a;
     std::string ini = "SetUpData.ini";
     bool from_cli = false;
@@ -306,12 +447,55 @@ int main(int argc, char** argv) {
     if (lbm_create(&p, &lat, &ctx) != LBM_OK) { std::fprintf(stderr, "lbm_create failed\n"); return 2; }
     std::vector<double> rho(NS, a.rho), u(NS, 0.0), v(NS, 0.0), w(NS, 0.0);
     for (size_t i = 0; i < NS; ++i) { u[i] = scls[i] == LBM_VEL_W ? ubc[i] : a.u; v[i] = a.v; w[i] = a.w; }
+    if (a.use_init_file) {
+        std::vector<double> fu, fv, fw, fr;
+        if (read_initial_conditions(a.init_file, N, fu, fv, fw, fr)) {
+            for (int z = 0; z < sim[2]; ++z)
+                for (int y = 0; y < sim[1]; ++y)
+                    for (int x = 0; x < sim[0]; ++x) {
+                        const size_t s = ((size_t)z * sim[1] + y) * sim[0] + x, d = ((size_t)z * dims[1] + y) * dims[0] + x;
+                        u[s] = fu[d]; v[s] = fv[d]; w[s] = fw[d]; rho[s] = fr[d];
+                    }
+            std::printf("Initial conditions loaded from file %s\n", a.init_file.c_str());
+        } else {
+            std::fprintf(stderr, "wrong or missing initial condition file %s: uniform start\n", a.init_file.c_str());
+        }
+    }
     if (lbm_init(ctx, a.init_eq ? 0 : 1, rho.data(), u.data(), v.data(), w.data()) != LBM_OK) {
         std::fprintf(stderr, "lbm_init: %s\n", lbm_last_error(ctx));
         lbm_destroy(ctx);
         return 2;
     }
     mkdir(a.out_dir.c_str(), 0755);
+    Times tm;
+    {
+        long inlet_nodes = 0;
+        double ylo = 1e300, yhi = -1e300;
+        for (size_t i = 0; i < NS; ++i)
+            if (scls[i] == LBM_VEL_W) {
+                ++inlet_nodes;
+                const double y = (double)((i / sim[0]) % sim[1]) * dx;
+                ylo = std::min(ylo, y);
+                yhi = std::max(yhi, y);
+            }
+        if (!inlet_nodes) ylo = yhi = 0;
+        const std::string logp = a.out_dir + "lbmsolver.log";
+        write_init_log(logp, a, dx, dims[0], dims[1], dims[2], inlet_nodes, yhi, ylo);
+        if (FILE* lf = std::fopen(logp.c_str(), "a")) {
+            long fluid = 0;
+            for (size_t i = 0; i < NS; ++i) fluid += scls[i] != LBM_SOLID;
+            std::fprintf(lf, "\n:::: Initializing ::::\n");
+            std::fprintf(lf, ">>> Number of nodes                      = %zu\n", N);
+            std::fprintf(lf, ">>> Number of fluid nodes                = %ld\n", fluid);
+            std::fprintf(lf, ">>> Device memory of the solver          = %lld Mbytes\n",
+                         (long long)(lbm_device_bytes(ctx) / 1024 / 1024));
+            std::fprintf(lf, "\n:::: Initialization done! ::::\n");
+            std::fprintf(lf, "\n:::: Start Iterations ::::\n");
+            std::fclose(lf);
+        }
+    }
+    tm.init = now_s() - t_start;
+    const double t_loop = now_s();
     FILE* rf = std::fopen((a.out_dir + "residuals.dat").c_str(), "w");
     if (rf) std::fprintf(rf, "Iter L2_norm L2_norm_weighted Drag Lift\n");
     double r4[4];
@@ -321,10 +505,14 @@ int main(int argc, char** argv) {
     int code = 0;
     while (done < a.iterations) {
         const long k = std::min(every, a.iterations - done);
+        double t0 = now_s();
         lbm_status st = lbm_step(ctx, k);
+        tm.coll += now_s() - t0;
         if (st != LBM_OK) { std::fprintf(stderr, "step: %s\n", lbm_last_error(ctx)); code = 2; break; }
         done += k;
+        t0 = now_s();
         lbm_reduce(ctx, r4);
+        tm.resi += now_s() - t0;
         const double res = r4[1] > 0 ? std::sqrt(r4[0] / r4[1]) : std::sqrt(r4[0]);
         if (rf) std::fprintf(rf, "%ld %5.4e %5.4e %f %f\n", done, std::sqrt(r4[0]), res, 0.0, 0.0);
         std::printf("iteration %ld residual %.4e mass %.10f\n", done, res, r4[2]);
@@ -332,6 +520,8 @@ int main(int argc, char** argv) {
         if (a.stop[0] > 0 && res < a.stop[0]) { std::printf("simulation converged!\n"); break; }
     }
     if (rf) std::fclose(rf);
+    tm.iter = now_s() - t_loop;
+    const double t_write = now_s();
     lbm_get_macros(ctx, rho.data(), u.data(), v.data(), w.data());
     lbm_destroy(ctx);
     // back to the file's node set (cyclic planes copied from plane 0)
@@ -346,5 +536,9 @@ int main(int argc, char** argv) {
     const char* ext = a.format == 1 ? "csv" : a.format == 2 ? "dat" : "vti";
     write_results(a.out_dir + "FinalData." + ext, a.format, dims[0], dims[1], dims[2], dx, R, U, V, W);
     std::printf("results written to %sFinalData.%s\n", a.out_dir.c_str(), ext);
+    tm.writ = now_s() - t_write;
+    tm.overall = now_s() - t_start;
+    write_end_log(a.out_dir + "lbmsolver.log", tm);
+    write_times(a.out_dir + "runtimes.dat", tm);
     return code;
 }

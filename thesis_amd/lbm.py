@@ -88,6 +88,14 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise LbmError(4, f"{LIB_PATH} is missing: run `python -m thesis_amd.build` first")
+    # PyTorch ships its own HIP and HSA runtimes.  A process must hold only
+    # one: when this library's runtime (ROCm's) is loaded first, a later torch
+    # finds no GPU, while the reverse order makes this library bind to the
+    # runtime torch already loaded.  So torch, when installed, goes first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     P = C.POINTER
     dp = P(C.c_double)

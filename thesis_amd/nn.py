@@ -14,7 +14,7 @@ RELU, ATOMIC, TRANSC = 0x1, 0x2, 0x4
 
 EXPORTED = [
     "nn_gemm_bf16", "nn_im2col3x3", "nn_col2im3x3", "nn_grad_bf16", "nn_colsum", "nn_to_bf16", "nn_avgpool2",
-    "nn_avgpool2_bwd", "nn_up2", "nn_up2_bwd", "nn_xint", "nn_axpy", "nn_add", "nn_fill", "nn_mse", "nn_adam",
+    "nn_avgpool2_bwd", "nn_up2", "nn_up2_bwd", "nn_xint", "nn_ymix", "nn_axpy", "nn_add", "nn_fill", "nn_mse", "nn_adam",
 ]
 
 _lib = None
@@ -38,6 +38,7 @@ def load():
         "nn_up2": [vp, i32, i32, i32, i32, vp, vp],
         "nn_up2_bwd": [vp, i32, i32, i32, i32, vp, vp],
         "nn_xint": [vp, i32, i32, i32, i32, i32, vp, vp],
+        "nn_ymix": [vp, i32, i32, i32, i32, vp, vp],
         "nn_axpy": [f32, vp, i64, vp, vp],
         "nn_add": [vp, vp, i64, vp, vp],
         "nn_fill": [f32, i64, vp, vp],
@@ -108,6 +109,10 @@ def up2_bwd(dy, Bn, H, W, Cc, dx):
 
 def xint(x, Bn, H, W, Cc, bwd, y):
     _check(load().nn_xint(_p(x), Bn, H, W, Cc, int(bwd), _p(y), _stream()), "nn_xint")
+
+
+def ymix(x, Bn, H, W, Cc, y):
+    _check(load().nn_ymix(_p(x), Bn, H, W, Cc, _p(y), _stream()), "nn_ymix")
 
 
 def axpy(a, x, y):

@@ -11,7 +11,8 @@ activations, NHWC): 3x3 convolutions with ReLU, two per level at 32, 64, 128
 and 128 channels, 2x2 average pooling down, nearest x2 upsampling up with
 additive skip connections, and at the coarsest level a running integral
 along the channel axis (x + (1/W) sum_{x' >= x} x), which carries the
-downstream obstacles' pressure drop upstream.  A 1x1 convolution gives the
+downstream obstacles' pressure drop upstream, followed by a mean over the
+channel width added back (x + mean_y x), which couples the two walls.  A 1x1 convolution gives the
 deviations ((u - u_p), v, (rho - rho_p)) / u_max.  Inputs: solid mask, the
 inlet profile p(y), x/(nx-1) and u_max/0.1.
 
@@ -207,7 +208,9 @@ class Net:
         a = self._conv(p, Bn, H4, W4, I["b1"], c)
         xi = self._new(a.numel())
         nn.xint(a, Bn, H4, W4, 128, 0, xi)
-        a = self._conv(xi, Bn, H4, W4, I["b2"], c)
+        ym = self._new(a.numel())
+        nn.ymix(xi, Bn, H4, W4, 128, ym)
+        a = self._conv(ym, Bn, H4, W4, I["b2"], c)
         u = self._new(Bn * H3 * W3 * 128)
         nn.up2(a, Bn, H4, W4, 128, u)
         a = self._conv(u, Bn, H3, W3, I["d3a"], c)
@@ -256,8 +259,10 @@ class Net:
         da = self._new(Bn * H4 * W4 * 128)
         nn.up2_bwd(d, Bn, H4, W4, 128, da)
         d = self._conv_bwd(da, I["b2"], c)
+        dym = self._new(d.numel())
+        nn.ymix(d, Bn, H4, W4, 128, dym)
         dxi = self._new(d.numel())
-        nn.xint(d, Bn, H4, W4, 128, 1, dxi)
+        nn.xint(dym, Bn, H4, W4, 128, 1, dxi)
         d = self._conv_bwd(dxi, I["b1"], c)
         dp = self._new(Bn * H3 * W3 * 128)
         nn.avgpool2_bwd(d, Bn, H3, W3, 128, dp)
@@ -502,7 +507,7 @@ def main(argv=None):
     ap.add_argument("--test", type=int, default=200)
     ap.add_argument("--train-seed", type=int, default=1)
     ap.add_argument("--test-seed", type=int, default=9001)
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=80)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", default="gpurun_out/surrogate")
