@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 1
+#define LBM_ABI_VERSION 2
 
 typedef struct lbm_ctx lbm_ctx;
 
@@ -39,8 +39,11 @@ typedef enum {
     LBM_EUNSUPPORTED = 6
 } lbm_status;
 
-/* node classes: one byte per node (latticegen2 fluid flag + boundary kind) */
-enum { LBM_SOLID = 0, LBM_FLUID = 1, LBM_VEL_W = 2, LBM_PRES_E = 3 };
+/* node classes: one byte per node (latticegen2 fluid flag + boundary kind).
+ * LBM_VEL_W: velocity inlet on the west face; LBM_PRES_E: outlet on the east
+ * face (lbm_params.outlet_model); LBM_VEL_N: velocity (moving lid) on the
+ * north face, the reference's 2D north "inlet" (S/GpuBoundaries.cu:21-41). */
+enum { LBM_SOLID = 0, LBM_FLUID = 1, LBM_VEL_W = 2, LBM_PRES_E = 3, LBM_VEL_N = 4 };
 /* collision models (S/include/Arguments.h: BGKW, TRT, MRT) */
 enum { LBM_BGK = 0, LBM_TRT = 1, LBM_MRT = 2 };
 /* population storage; compute is FP64 for F64 storage and FP32 otherwise */
@@ -51,9 +54,16 @@ enum { LBM_FORCE_NONE = 0, LBM_FORCE_SHIFT = 1, LBM_FORCE_GUO = 2 };
 /* walls: half-way bounce-back, or the reference's on-node
  * f[opp] = f[dir] (S/GpuBoundaries.cu:435-437, :482-484) */
 enum { LBM_WALL_HALFWAY = 0, LBM_WALL_ONNODE = 1 };
-/* velocity inlet: textbook Zou-He, or the reference's 2D west form
- * without transverse terms (S/GpuBoundaries.cu:45-64) */
+/* velocity faces: textbook Zou-He, or the reference's 2D forms without
+ * transverse terms, corners skipped (west S/GpuBoundaries.cu:45-64; north
+ * lid :21-41); the reference forms impose u0 exactly on equilibrium states */
 enum { LBM_INLET_ZOUHE = 0, LBM_INLET_REF2D = 1 };
+/* east outlet (2D): Zou-He pressure (rho_out; T/LBM/Tom/Lecture Script/LBMchannel.m:254-270),
+ * the reference's velocity outlet with its (1 - u0) density denominator and
+ * u0 = u_bc at the node (S/GpuBoundaries.cu:1571-1605), or its second- and
+ * first-order extrapolations, which overwrite the streamed-in f1, f5, f8 from
+ * the two upstream nodes (:1609-1640) */
+enum { LBM_OUTLET_PRESSURE = 0, LBM_OUTLET_REF_VELOCITY = 1, LBM_OUTLET_EXTRAP2 = 2, LBM_OUTLET_EXTRAP1 = 3 };
 /* reference quirks reproduced on request */
 #define LBM_PARITY_MRT_RHO2   0x1u /* 2D MRT m_eq with rho^2 (S/GpuCollision.cu:987-988) */
 #define LBM_PARITY_MRT_NO_F18 0x2u /* 3D MRT leaves f18 unwritten (S/GpuCollision.cu:1161) */
@@ -77,12 +87,14 @@ typedef struct {
     int32_t inlet_model;
     uint32_t parity_flags;
     double rho_out;         /* Zou-He pressure outlet density */
+    int32_t outlet_model;   /* LBM_OUTLET_*; applies to LBM_PRES_E nodes */
 } lbm_params;
 
 /* Lattice description; node_class is required, the rest may be NULL.
  * bb_links[i] bit d (1 <= d < q): the link from node i in direction d
  * crosses a wall (the _BC.dat line "i j k d WALL").
- * u_bc/v_bc/w_bc: prescribed velocity on LBM_VEL_W nodes. */
+ * u_bc/v_bc/w_bc: prescribed velocity on LBM_VEL_W and LBM_VEL_N nodes, and
+ * u0 of the reference velocity outlet on LBM_PRES_E nodes. */
 typedef struct {
     const uint8_t* node_class;
     const uint32_t* bb_links;

@@ -386,13 +386,35 @@ static inline long wrap_src(const orc_sim* s, int x, int y, int z, int i, int* o
 /* Zou-He boundaries on D2Q9 faces.
  * Velocity inlet, west face: textbook (Zou & He 1997) or the reference's
  * simplified form without the transverse terms (S/GpuBoundaries.cu:45-64).
- * Pressure outlet, east face: closed form of
- * T/LBM/Tom/Lecture Script/LBMchannel.m:254-270 (absent from the solver). */
+ * Moving lid, north face: textbook, solved from the density, the two
+ * momentum components and non-equilibrium bounce-back of f4, or the
+ * reference's form (S/GpuBoundaries.cu:21-41): f4 = f2 and
+ * f8, f7 = f6, f5 -+ B u0/6 with B = f0 + f1 + f3 + 2(f2 + f5 + f6), corner
+ * nodes skipped.
+ * East outlet: Zou-He pressure, the closed form of
+ * T/LBM/Tom/Lecture Script/LBMchannel.m:254-270 (absent from the solver), or
+ * the reference's velocity outlet (S/GpuBoundaries.cu:1571-1605) whose
+ * density R = (f0 + f2 + f4 + 2(f1 + f5 + f8)) / (1 - u0) keeps its sign.
+ * The extrapolating outlets are applied afterwards (outlet_extrap). */
 static void zouhe_node(const orc_sim* s, long ind, double* f)
 {
     uint8_t c = s->cls[ind];
     if (s->Q != 9) return;
-    if (c == ORC_VEL_W) {
+    if (c == ORC_VEL_N) {
+        double u0 = s->ubc[ind], v0 = s->vbc[ind];
+        if (s->p.inlet_model == ORC_INLET_REF2D) {
+            if (s->bb[ind]) return;
+            double B = f[0] + f[1] + f[3] + 2.0 * (f[2] + f[6] + f[5]);
+            f[4] = f[2];
+            f[8] = f[6] + B * u0 / 6.0;
+            f[7] = f[5] - B * u0 / 6.0;
+        } else {
+            double r = (f[0] + f[1] + f[3] + 2.0 * (f[2] + f[5] + f[6])) / (1.0 + v0);
+            f[4] = f[2] - 2.0 / 3.0 * r * v0;
+            f[7] = f[5] + 0.5 * (f[1] - f[3]) - 0.5 * r * u0 - r * v0 / 6.0;
+            f[8] = f[6] - 0.5 * (f[1] - f[3]) + 0.5 * r * u0 - r * v0 / 6.0;
+        }
+    } else if (c == ORC_VEL_W) {
         double u0 = s->ubc[ind], v0 = s->vbc[ind];
         if (s->p.inlet_model == ORC_INLET_REF2D) {
             if (s->bb[ind]) return;   /* corner nodes are skipped (BC_CORNER) */
@@ -406,12 +428,39 @@ static void zouhe_node(const orc_sim* s, long ind, double* f)
             f[5] = f[7] - 0.5 * (f[2] - f[4]) + r * u0 / 6.0 + 0.5 * r * v0;
             f[8] = f[6] + 0.5 * (f[2] - f[4]) + r * u0 / 6.0 - 0.5 * r * v0;
         }
-    } else if (c == ORC_PRES_E) {
+    } else if (c == ORC_PRES_E && s->p.outlet_model == ORC_OUTLET_REF_VELOCITY) {
+        double u0 = s->ubc[ind];
+        double R = (f[0] + f[2] + f[4] + 2.0 * (f[1] + f[5] + f[8])) / (1.0 - u0);
+        f[3] = f[1] - 2.0 * R * u0 / 3.0;
+        f[7] = f[5] - R * u0 / 6.0;
+        f[6] = f[8] - R * u0 / 6.0;
+    } else if (c == ORC_PRES_E && s->p.outlet_model == ORC_OUTLET_PRESSURE) {
         double ro = s->p.rho_out;
         double ux = -1.0 + (f[0] + f[2] + f[4] + 2.0 * (f[1] + f[5] + f[8])) / ro;
         f[3] = f[1] - 2.0 / 3.0 * ro * ux;
         f[7] = f[5] + 0.5 * (f[2] - f[4]) - ro * ux / 6.0;
         f[6] = f[8] - 0.5 * (f[2] - f[4]) - ro * ux / 6.0;
+    }
+}
+
+void orc_open_boundary(const orc_sim* s, long ind, double* f) { zouhe_node(s, ind, f); }
+
+/* Extrapolating east outlet (S/GpuBoundaries.cu:1609-1640): the populations
+ * that streamed in from the east face's upstream side, f1, f5, f8, become
+ * 2 f(x-1) - f(x-2) (second order) or f(x-1) (first order), from the two
+ * upstream nodes' populations after their inlet and wall conditions.  The
+ * node keeps its streamed values when an upstream node is solid or missing. */
+static void outlet_extrap(orc_sim* s, long ind)
+{
+    const long n = s->n;
+    const int nx = s->p.nx;
+    int x = (int)(ind % nx);
+    if (x < 2 || s->cls[ind - 1] == ORC_SOLID || s->cls[ind - 2] == ORC_SOLID) return;
+    static const int Q3[3] = {1, 5, 8};
+    for (int k = 0; k < 3; ++k) {
+        size_t q = (size_t)Q3[k] * n;
+        double a = s->f[q + ind - 1], b = s->f[q + ind - 2];
+        s->f[q + ind] = s->p.outlet_model == ORC_OUTLET_EXTRAP2 ? 2.0 * a - b : a;
     }
 }
 
@@ -452,7 +501,8 @@ void orc_step(orc_sim* s, int nsteps)
                 long src = wrap_src(s, x, y, z, i, &outside);
                 f[i] = outside ? s->fcoll[(size_t)i * n + ind] : s->fcoll[(size_t)i * n + src];
             }
-            if (s->cls[ind] == ORC_VEL_W) zouhe_node(s, ind, f);
+            /* inlet -> wall -> outlet order of S/Iterate.cu:468-473 */
+            if (s->cls[ind] == ORC_VEL_W || s->cls[ind] == ORC_VEL_N) zouhe_node(s, ind, f);
             if (s->p.wall_model == ORC_WALL_ONNODE && bb) {
                 /* on-node bounce-back, S/GpuBoundaries.cu:400-443, :445-490 */
                 for (int d = 1; d < Q; ++d)
@@ -460,6 +510,12 @@ void orc_step(orc_sim* s, int nsteps)
             }
             if (s->cls[ind] == ORC_PRES_E) zouhe_node(s, ind, f);
             for (int i = 0; i < Q; ++i) s->f[(size_t)i * n + ind] = f[i];
+        }
+        if (Q == 9 && s->p.outlet_model >= ORC_OUTLET_EXTRAP2) {
+            /* east-face nodes only read nodes off the face: no ordering issue */
+#pragma omp parallel for
+            for (long ind = 0; ind < n; ++ind)
+                if (s->cls[ind] == ORC_PRES_E) outlet_extrap(s, ind);
         }
         /* macroscopic update */
 #pragma omp parallel for

@@ -29,11 +29,14 @@ extern "C" {
 #endif
 
 /* node classes (shared with the product's lattice description) */
-enum { ORC_SOLID = 0, ORC_FLUID = 1, ORC_VEL_W = 2, ORC_PRES_E = 3 };
+enum { ORC_SOLID = 0, ORC_FLUID = 1, ORC_VEL_W = 2, ORC_PRES_E = 3, ORC_VEL_N = 4 };
 enum { ORC_BGK = 0, ORC_TRT = 1, ORC_MRT = 2 };
 enum { ORC_FORCE_NONE = 0, ORC_FORCE_SHIFT = 1, ORC_FORCE_GUO = 2 };
 enum { ORC_WALL_HALFWAY = 0, ORC_WALL_ONNODE = 1 };
 enum { ORC_INLET_ZOUHE = 0, ORC_INLET_REF2D = 1 };
+/* east outlet: Zou-He pressure, the reference's velocity outlet, its second-
+ * and first-order extrapolations (S/GpuBoundaries.cu:1562-1653) */
+enum { ORC_OUTLET_PRESSURE = 0, ORC_OUTLET_REF_VELOCITY = 1, ORC_OUTLET_EXTRAP2 = 2, ORC_OUTLET_EXTRAP1 = 3 };
 /* parity flags: reference quirks reproduced on request */
 #define ORC_PARITY_MRT_RHO2      0x1u /* 2D MRT m_eq uses rho^2 (S/GpuCollision.cu:987-988) */
 #define ORC_PARITY_MRT_NO_F18    0x2u /* 3D MRT never writes f18 (S/GpuCollision.cu:1161) */
@@ -55,6 +58,7 @@ typedef struct {
     unsigned parity;
     double rho_out;     /* Zou-He pressure outlet density */
     int nthreads;       /* OpenMP threads; <= 0 uses the runtime default */
+    int outlet_model;   /* ORC_OUTLET_*, on ORC_PRES_E nodes */
 } orc_params;
 
 typedef struct orc_sim orc_sim;
@@ -69,7 +73,8 @@ int orc_mrt_matrices(int q, double omega, double* M, double* Minv, double* s);
 
 /* Create a simulation.  node_class[N], bb[N] (bit d: the link from the node
  * in direction d crosses a wall), ubc/vbc/wbc: prescribed velocity on
- * velocity-inlet nodes (may be NULL). */
+ * velocity nodes (west inlet, north lid) and u0 of the reference's velocity
+ * outlet (may be NULL). */
 orc_sim* orc_create(const orc_params* p, const uint8_t* node_class, const uint32_t* bb,
                     const double* ubc, const double* vbc, const double* wbc);
 void orc_destroy(orc_sim* s);
@@ -87,6 +92,9 @@ double orc_mass(const orc_sim* s);                   /* sum of f over non-solid 
 /* Kernels exposed for closed-form tests. */
 void orc_collide_node(const orc_sim* s, const double* f, double rho, const double* uvw, double* out);
 void orc_equilibrium(int q, double rho, const double* uvw, double* feq);
+/* The node-local open-boundary condition of node ind (west inlet, north lid,
+ * east outlet models 0 and 1) applied to its streamed populations f[Q]. */
+void orc_open_boundary(const orc_sim* s, long ind, double* f);
 
 /* ---- colour-gradient two-phase model (D2Q9) ----------------------------- */
 typedef struct {

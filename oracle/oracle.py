@@ -14,11 +14,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblbm_oracle.so")
 REF_MRT_PATH = os.path.join(HERE, "_ref", "libref_cellfunctions.so")
 
-SOLID, FLUID, VEL_W, PRES_E = 0, 1, 2, 3
+SOLID, FLUID, VEL_W, PRES_E, VEL_N = 0, 1, 2, 3, 4
 BGK, TRT, MRT = 0, 1, 2
 FORCE_NONE, FORCE_SHIFT, FORCE_GUO = 0, 1, 2
 WALL_HALFWAY, WALL_ONNODE = 0, 1
 INLET_ZOUHE, INLET_REF2D = 0, 1
+OUTLET_PRESSURE, OUTLET_REF_VELOCITY, OUTLET_EXTRAP2, OUTLET_EXTRAP1 = 0, 1, 2, 3
 PARITY_MRT_RHO2, PARITY_MRT_NO_F18, PARITY_INLET_FROZEN, PARITY_OUTLET_V0 = 1, 2, 4, 8
 
 
@@ -29,6 +30,7 @@ class OrcParams(C.Structure):
         ("trt_magic", C.c_double), ("mrt_s", C.c_double * 19), ("force_model", C.c_int),
         ("g", C.c_double * 3), ("wall_model", C.c_int), ("inlet_model", C.c_int),
         ("parity", C.c_uint), ("rho_out", C.c_double), ("nthreads", C.c_int),
+        ("outlet_model", C.c_int),
     ]
 
 
@@ -67,6 +69,7 @@ def lib():
         L.orc_mass.argtypes = [C.c_void_p]
         L.orc_mass.restype = C.c_double
         L.orc_collide_node.argtypes = [C.c_void_p, dp, C.c_double, dp, dp]
+        L.orc_open_boundary.argtypes = [C.c_void_p, C.c_long, dp]
         L.orc_equilibrium.argtypes = [C.c_int, C.c_double, dp, dp]
         L.orc_lattice.argtypes = [C.c_int, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), dp]
         L.orc_mrt_matrices.argtypes = [C.c_int, C.c_double, dp, dp, dp]
@@ -118,7 +121,7 @@ class Sim:
     def __init__(self, q, shape, node_class, bb=None, *, collision=BGK, omega=1.0, trt_magic=0.0,
                  mrt_s=None, force_model=FORCE_NONE, g=(0.0, 0.0, 0.0), wall_model=WALL_HALFWAY,
                  inlet_model=INLET_ZOUHE, parity=0, rho_out=1.0, periodic=(0, 0, 0), ubc=None,
-                 vbc=None, wbc=None, nthreads=0):
+                 vbc=None, wbc=None, nthreads=0, outlet_model=OUTLET_PRESSURE):
         nz, ny, nx = (1,) + tuple(shape) if len(shape) == 2 else tuple(shape)
         self.q, self.shape3 = q, (nz, ny, nx)
         self.n = nx * ny * nz
@@ -133,6 +136,7 @@ class Sim:
                 p.mrt_s[i] = v
         p.force_model, p.wall_model, p.inlet_model = force_model, wall_model, inlet_model
         p.parity, p.rho_out, p.nthreads = parity, rho_out, nthreads
+        p.outlet_model = outlet_model
         self.params = p
         cls = np.ascontiguousarray(node_class, dtype=np.uint8).ravel()
         assert cls.size == self.n
@@ -178,6 +182,12 @@ class Sim:
         out = np.zeros(self.q)
         lib().orc_collide_node(self.h, _dp(f), rho, _dp(u), _dp(out))
         return out
+
+    def open_boundary(self, ind, f):
+        """Node-local open-boundary condition of dense node ind on streamed f[Q]."""
+        g = np.array(f, dtype=np.float64)
+        lib().orc_open_boundary(self.h, int(ind), _dp(g))
+        return g
 
 
 class CgSim:

@@ -47,3 +47,20 @@ def mass_balance_error(f1, rho0, rho1, cls):
     gained = np.cumsum(dm[2:cls.shape[1] - 1])                  # columns 2..p for p = 2 .. nx-2
     lhs = fx[0] - fx[1:]                                          # planes p = 2 .. nx-3
     return float(np.abs(lhs - gained[:lhs.size]).max() / abs(fx[0]))
+
+
+# Ghia, Ghia & Shin (1982), J. Comput. Phys. 48:387-411, Table I, Re = 100:
+# u / U_lid along the vertical centre line at height y / L.
+GHIA_RE100_U = ((0.9766, 0.84123), (0.9688, 0.78871), (0.9609, 0.73722), (0.9531, 0.68717),
+                (0.8516, 0.23151), (0.7344, 0.00332), (0.6172, -0.13641), (0.5000, -0.20581),
+                (0.4531, -0.21090), (0.2813, -0.15662), (0.1719, -0.10150), (0.1016, -0.06434),
+                (0.0703, -0.04775), (0.0625, -0.04192), (0.0547, -0.03717))
+
+
+def ghia_centreline_error(u, u_lid):
+    """Largest |u/U - Ghia| on the vertical centre line of an n x n cavity
+    with walls on nodes 0 and n-1 (side n - 1), u indexed [y, x]."""
+    n = u.shape[-1]
+    col = 0.5 * (u[:, (n - 1) // 2] + u[:, n // 2]) / u_lid
+    ys = np.arange(n) / (n - 1)
+    return max(abs(float(np.interp(y, ys, col)) - g) for y, g in GHIA_RE100_U)

@@ -117,4 +117,6 @@ def test_read_lattice_2d_and_crlf(tmp_path):
     assert r["node_class"][0, 0, 0] == L.VEL_W            # INLE link on the west face
     assert abs(r["delta"] - 0.5) < 1e-12 and r["cyclic"] == (0, 0, 0)
     assert r["nlinks"] == 3 and list(r["type_counts"][:3]) == [0, 2, 1]
-    assert r["bb_links"][0, 0, 0] == (1 << 4) and r["bb_links"][0, 2, 3] == (1 << 2)
+    # node (0, 0) has a wall (4) and an inlet (3) link: a corner, whose
+    # south-west diagonal becomes a wall (S/GpuInit.cu:1073-1116)
+    assert r["bb_links"][0, 0, 0] == (1 << 4) | (1 << 7) and r["bb_links"][0, 2, 3] == (1 << 2)

@@ -39,6 +39,7 @@ UpdateInltOutl = yes
 [mi355x]
 walls = halfway
 init = equilibrium
+outlet = pressure
 """
 
 

@@ -9,6 +9,13 @@
 // the largest index + 1 (S/FilesReading.cu:469-471).  BC types: 1 WALL,
 // 2 INLE, 3 OUTL, 4 CYCL, 5 SYMP (L/GeomRead.cpp:85-99).  Lines are split on
 // '\n' only, so CRLF files count correctly (unlike S/FilesReading.cu:27-31).
+//
+// 2D node classes follow the reference's per-direction link masks
+// (S/GpuInit.cu:1066-1116, S/include/BcMacros.h): an INLE link in direction 3
+// on x = 0 makes a west velocity inlet, one in direction 2 on y = m-1 a north
+// lid, an OUTL link in direction 1 on x = n-1 an east outlet.  A node whose
+// links have different types is a corner: each diagonal whose two axis
+// directions both carry links becomes a wall link.
 #include "lbm.h"
 
 #include <cmath>
@@ -119,6 +126,10 @@ extern "C" lbm_status lbm_read_lattice(const char* node_path, const char* bc_pat
     if (dir_counts) std::memset(dir_counts, 0, sizeof(int64_t) * 19);
     for (const Rec& r : recs) node_class[r.i + r.j * nx + r.k * nx * ny] = r.fluid ? LBM_FLUID : LBM_SOLID;
     int64_t cnt = 0;
+    // 2D: first link type per node, corner flag, 2-bit type per direction
+    std::vector<uint8_t> main_t, corner;
+    std::vector<uint32_t> dmask;
+    if (!three_d && have_bc) { main_t.assign((size_t)n, 0); corner.assign((size_t)n, 0); dmask.assign((size_t)n, 0); }
     if (have_bc) {
         const char* b = bct.data();
         const char* be = b + bct.size();
@@ -136,8 +147,11 @@ extern "C" lbm_status lbm_read_lattice(const char* node_path, const char* bc_pat
             if (dir_counts) dir_counts[dir]++;
             const int64_t at = i + j * nx + k * nx * ny;
             if (type == 1 && bb_links) bb_links[at] |= 1u << dir;
-            if (!three_d && type == 2 && i == 0 && node_class[at]) node_class[at] = LBM_VEL_W;
-            if (!three_d && type == 3 && i == nx - 1 && node_class[at]) node_class[at] = LBM_PRES_E;
+            if (!three_d && type >= 1 && type <= 3) {
+                dmask[at] |= (uint32_t)type << (2 * (dir - 1));
+                if (!main_t[at]) main_t[at] = (uint8_t)type;
+                else if (main_t[at] != type) corner[at] = 1;
+            }
             if (type == 4 && cyclic) {
                 static const int c9[9][2] = {{0, 0}, {1, 0}, {0, 1}, {-1, 0}, {0, -1}, {1, 1}, {-1, 1}, {-1, -1}, {1, -1}};
                 static const int c19[19][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1},
@@ -151,6 +165,24 @@ extern "C" lbm_status lbm_read_lattice(const char* node_path, const char* bc_pat
                 if ((cz < 0 && k == 0) || (cz > 0 && k == dims[2] - 1)) cyclic[2] = 1;
             }
             ++cnt;
+        }
+    }
+    if (!three_d && have_bc) {
+        // direction d's 2-bit field: bits 2(d-1), 2(d-1)+1
+        auto any = [&](int64_t at, int d) { return ((dmask[at] >> (2 * (d - 1))) & 3u) != 0; };
+        auto is = [&](int64_t at, int d, uint32_t t) { return ((dmask[at] >> (2 * (d - 1))) & 3u) == t; };
+        for (int64_t at = 0; at < n; ++at) {
+            if (!main_t[at] || node_class[at] == LBM_SOLID) continue;
+            const int64_t i = at % nx, j = at / nx;
+            if (corner[at] && bb_links) {
+                // diagonal d between axis directions a and b: NE(5) E,N; NW(6) W,N; SW(7) W,S; SE(8) E,S
+                static const int dg[4][3] = {{5, 1, 2}, {6, 3, 2}, {7, 3, 4}, {8, 1, 4}};
+                for (const auto& e : dg)
+                    if (any(at, e[1]) && any(at, e[2])) bb_links[at] |= 1u << e[0];
+            }
+            if (i == 0 && is(at, 3, 2)) node_class[at] = LBM_VEL_W;
+            else if (j == ny - 1 && is(at, 2, 2)) node_class[at] = LBM_VEL_N;
+            else if (i == nx - 1 && is(at, 1, 3)) node_class[at] = LBM_PRES_E;
         }
     }
     if (nlinks) *nlinks = cnt;

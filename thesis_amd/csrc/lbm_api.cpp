@@ -157,23 +157,27 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     if (p->force_model < 0 || p->force_model > 2) return reject("bad force model");
     if (p->wall_model < 0 || p->wall_model > 1) return reject("bad wall model");
     if (p->inlet_model < 0 || p->inlet_model > 1) return reject("bad inlet model");
+    if (p->outlet_model < 0 || p->outlet_model > 3) return reject("bad outlet model");
     const size_t n = (size_t)p->nx * p->ny * nz;
     if (n > (size_t)1 << 33) return reject("lattice too large");
 
     // node words and open-boundary checks
     std::vector<uint32_t> node(n);
-    bool any_open_cls = false;
+    bool any_open_cls = false, any_outlet = false;
     const uint32_t mask = p->q == 9 ? 0x1FEu : 0x7FFFEu;
     for (size_t i = 0; i < n; ++i) {
         const uint32_t cls = lat->node_class[i];
-        if (cls > LBM_PRES_E) return reject("unknown node class");
-        if (cls == LBM_VEL_W || cls == LBM_PRES_E) {
+        if (cls > LBM_VEL_N) return reject("unknown node class");
+        if (cls == LBM_VEL_W || cls == LBM_PRES_E || cls == LBM_VEL_N) {
             if (p->q != 9) return reject("velocity/pressure faces are D2Q9 only");
-            const int x = (int)(i % p->nx);
+            const int x = (int)(i % p->nx), y = (int)(i / p->nx);
             if (cls == LBM_VEL_W && x != 0) return reject("velocity inlet nodes must lie on x = 0");
-            if (cls == LBM_PRES_E && x != p->nx - 1) return reject("pressure outlet nodes must lie on x = nx-1");
-            if (p->periodic[0]) return reject("open x faces need a non-periodic x axis");
+            if (cls == LBM_PRES_E && x != p->nx - 1) return reject("outlet nodes must lie on x = nx-1");
+            if (cls == LBM_VEL_N && y != p->ny - 1) return reject("lid nodes must lie on y = ny-1");
+            if (cls != LBM_VEL_N && p->periodic[0]) return reject("open x faces need a non-periodic x axis");
+            if (cls == LBM_VEL_N && p->periodic[1]) return reject("a lid needs a non-periodic y axis");
             any_open_cls = true;
+            any_outlet |= cls == LBM_PRES_E;
         }
         const uint32_t bb = lat->bb_links ? (lat->bb_links[i] & mask) : 0u;
         node[i] = (cls << kClsShift) | bb;
@@ -270,8 +274,11 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     const bool rank_ok = (((size_t)rmax * 9) << lgw) * 8 + 4 * rblock <= 48 * 1024 && p->nx < 65536;
     if (any_open_cls && !lat->u_bc) {
         bool need = false;
-        for (size_t i = 0; i < n; ++i) need |= lat->node_class[i] == LBM_VEL_W;
-        if (need) return reject("velocity inlet nodes need u_bc");
+        for (size_t i = 0; i < n; ++i) {
+            const uint8_t cl = lat->node_class[i];
+            need |= cl == LBM_VEL_W || cl == LBM_VEL_N || (cl == LBM_PRES_E && p->outlet_model == LBM_OUTLET_REF_VELOCITY);
+        }
+        if (need) return reject("velocity inlet, lid and velocity outlet nodes need u_bc");
     }
 
     int ndev = 0;
@@ -327,6 +334,7 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     k.force = p->force_model;
     k.wall = p->wall_model;
     k.inlet = p->inlet_model;
+    k.outlet = p->outlet_model;
     k.parity = p->parity_flags;
     k.rho_out = p->rho_out > 0 ? p->rho_out : 1.0;
 
@@ -373,8 +381,10 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     k.rblock = (int)rblock;
     k.rmax = rmax;
     k.simple = simple ? 1 : 0;
-    k.rank = rank_ok && k.tile ? 1 : 0;   // the rank step addresses the tile layout only
-    if (const char* e = std::getenv("LBM_RANK")) k.rank = rank_ok && k.tile && e[0] == '1';
+    // the rank step addresses the tile layout only, and has no extrapolating outlet
+    const bool rank_fits = rank_ok && k.tile && !(any_outlet && p->outlet_model >= LBM_OUTLET_EXTRAP2);
+    k.rank = rank_fits ? 1 : 0;
+    if (const char* e = std::getenv("LBM_RANK")) k.rank = rank_fits && e[0] == '1';
     k.sr = 1;
     if (const char* e = std::getenv("LBM_SR")) k.sr = e[0] == '1';
     auto h2d = [&](void* d, const void* h, size_t b) { return hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, c->stream); };

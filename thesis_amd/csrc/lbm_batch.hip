@@ -74,7 +74,7 @@ __device__ inline void dense_gather(const BInst& I, const KParams& kp, const typ
     f.ubc = fl.ubc + I.dbase;
     f.vbc = fl.vbc + I.dbase;
     if (cls == VEL_W) zou_inlet<L, T>(kp, f, own, nd, h);
-    if (cls == PRES_E) zou_outlet<L, T>(kp, h);
+    if (cls == PRES_E) zou_outlet<L, T>(kp, f, own, h);   // kp.outlet = 0: Zou-He pressure
 }
 
 __device__ inline int find_inst(const BInst* I, int ninst, uint32_t t) {

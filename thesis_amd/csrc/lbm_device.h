@@ -259,7 +259,8 @@ __device__ inline void gather(const typename Io<L, S>::raw* __restrict__ src, co
     }
 }
 
-// Open boundaries on D2Q9 faces, in full populations.
+// Open boundaries on D2Q9 faces, in full populations (deviations h = f - w
+// in and out).
 // Velocity inlet (west): Zou-He, or the reference's form (S/GpuBoundaries.cu:45-64).
 // Pressure outlet (east): T/LBM/Tom/Lecture Script/LBMchannel.m:254-270.
 template <class L, class T>
@@ -281,17 +282,55 @@ __device__ inline void zou_inlet(const KParams& kp, const Fields& fl, size_t idx
     }
     h[1] = f[1] - (T)L::w(1); h[5] = f[5] - (T)L::w(5); h[8] = f[8] - (T)L::w(8);
 }
+// Moving lid (north): Zou-He from the density, both momentum components and
+// non-equilibrium bounce-back of f4; or the reference's form
+// (S/GpuBoundaries.cu:21-41), f4 = f2, f8/f7 = f6/f5 +- B u0/6 with
+// B = f0 + f1 + f3 + 2(f2 + f5 + f6), corner nodes skipped.
 template <class L, class T>
-__device__ inline void zou_outlet(const KParams& kp, T* h) {
+__device__ inline void zou_lid(const KParams& kp, const Fields& fl, size_t idx, uint32_t nd, T* h) {
     if (L::Q != 9) return;
+    if (kp.inlet == 1 && (nd & kBBMask)) return;
     T f[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) f[i] = h[i] + (T)L::w(i);
-    const T ro = (T)kp.rho_out;
-    const T ux = (T)-1 + (f[0] + f[2] + f[4] + (T)2 * (f[1] + f[5] + f[8])) / ro;
-    f[3] = f[1] - (T)(2.0 / 3.0) * ro * ux;
-    f[7] = f[5] + (T)0.5 * (f[2] - f[4]) - ro * ux / (T)6;
-    f[6] = f[8] - (T)0.5 * (f[2] - f[4]) - ro * ux / (T)6;
+    const T u0 = (T)fl.ubc[idx], v0 = (T)fl.vbc[idx];
+    const T k = f[0] + f[1] + f[3] + (T)2 * (f[2] + f[5] + f[6]);
+    if (kp.inlet == 1) {
+        f[4] = f[2];
+        f[8] = f[6] + k * u0 / (T)6;
+        f[7] = f[5] - k * u0 / (T)6;
+    } else {
+        const T r = k / ((T)1 + v0);
+        const T t = (T)0.5 * (f[1] - f[3]);
+        f[4] = f[2] - (T)(2.0 / 3.0) * r * v0;
+        f[7] = f[5] + t - (T)0.5 * r * u0 - r * v0 / (T)6;
+        f[8] = f[6] - t + (T)0.5 * r * u0 - r * v0 / (T)6;
+    }
+    h[4] = f[4] - (T)L::w(4); h[7] = f[7] - (T)L::w(7); h[8] = f[8] - (T)L::w(8);
+}
+// East outlet, node-local models: Zou-He pressure, or the reference's velocity
+// outlet (S/GpuBoundaries.cu:1571-1605) with R = (f0 + f2 + f4 + 2(f1 + f5 + f8)) / (1 - u0),
+// u0 = ubc at the node.  The extrapolating models are applied by outlet_extrap.
+template <class L, class T>
+__device__ inline void zou_outlet(const KParams& kp, const Fields& fl, size_t idx, T* h) {
+    if (L::Q != 9 || kp.outlet >= 2) return;
+    T f[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f[i] = h[i] + (T)L::w(i);
+    const T a = f[0] + f[2] + f[4] + (T)2 * (f[1] + f[5] + f[8]);
+    if (kp.outlet == 1) {
+        const T u0 = (T)fl.ubc[idx];
+        const T r = a / ((T)1 - u0);
+        f[3] = f[1] - (T)(2.0 / 3.0) * r * u0;
+        f[7] = f[5] - r * u0 / (T)6;
+        f[6] = f[8] - r * u0 / (T)6;
+    } else {
+        const T ro = (T)kp.rho_out;
+        const T ux = (T)-1 + a / ro;
+        f[3] = f[1] - (T)(2.0 / 3.0) * ro * ux;
+        f[7] = f[5] + (T)0.5 * (f[2] - f[4]) - ro * ux / (T)6;
+        f[6] = f[8] - (T)0.5 * (f[2] - f[4]) - ro * ux / (T)6;
+    }
     h[3] = f[3] - (T)L::w(3); h[6] = f[6] - (T)L::w(6); h[7] = f[7] - (T)L::w(7);
 }
 template <class L, class T>
@@ -299,6 +338,50 @@ __device__ inline void onnode_walls(uint32_t nd, T* h) {
 #pragma unroll
     for (int d = 1; d < L::Q; ++d)
         if ((nd >> d) & 1u) h[L::opp(d)] = h[d];
+}
+
+// Populations of a node after streaming and its inlet and wall conditions,
+// i.e. everything before the outlet (the reference orders inlet, wall,
+// outlet: S/Iterate.cu:468-473).
+template <class L, int S, bool OPEN>
+__device__ inline void inflow_state(const typename Io<L, S>::raw* __restrict__ src, const KParams& kp,
+                                    const Fields& fl, const uint32_t* __restrict__ cidx, uint32_t own,
+                                    uint32_t nd, int x, int y, int z, size_t idx, typename Io<L, S>::T* h) {
+    using T = typename Io<L, S>::T;
+    uint32_t cn[L::Q];
+    source_index<L, OPEN>(cidx, kp, x, y, z, cn);
+    gather<L, S, OPEN>(src, kp, own, nd, cn, h);
+    const uint32_t cls = nd >> kClsShift;
+    if (OPEN && cls == VEL_W) zou_inlet<L, T>(kp, fl, idx, nd, h);
+    if (OPEN && cls == VEL_N) zou_lid<L, T>(kp, fl, idx, nd, h);
+    if (kp.wall == 1 && (nd & kBBMask)) onnode_walls<L, T>(nd, h);
+}
+
+// Extrapolating east outlet (S/GpuBoundaries.cu:1609-1640): f1, f5, f8 of an
+// outlet node become 2 f(x-1) - f(x-2) (kp.outlet 2) or f(x-1) (kp.outlet 3),
+// from the upstream nodes' populations after their own inlet and wall
+// conditions, recomputed here (they lie off the east face, so no outlet
+// condition applies to them).  The node keeps its streamed values when an
+// upstream node is solid.  Node words are read from nodew at the compact
+// index (compact) or at the dense index.
+template <class L, int S, bool OPEN>
+__device__ inline void outlet_extrap(const typename Io<L, S>::raw* __restrict__ src, const KParams& kp,
+                                     const Fields& fl, const uint32_t* __restrict__ cidx,
+                                     const uint32_t* __restrict__ nodew, bool compact, int x, int y, int z,
+                                     size_t idx, typename Io<L, S>::T* h) {
+    using T = typename Io<L, S>::T;
+    if (L::Q != 9 || kp.outlet < 2 || x < 2) return;
+    const uint32_t a = cidx[idx - 1], b = cidx[idx - 2];
+    if (a == kNone || b == kNone) return;
+    T ha[L::Q];
+    inflow_state<L, S, OPEN>(src, kp, fl, cidx, a, compact ? nodew[a] : nodew[idx - 1], x - 1, y, z, idx - 1, ha);
+    if (kp.outlet == 2) {
+        T hb[L::Q];
+        inflow_state<L, S, OPEN>(src, kp, fl, cidx, b, compact ? nodew[b] : nodew[idx - 2], x - 2, y, z, idx - 2, hb);
+        h[1] = (T)2 * ha[1] - hb[1]; h[5] = (T)2 * ha[5] - hb[5]; h[8] = (T)2 * ha[8] - hb[8];
+    } else {
+        h[1] = ha[1]; h[5] = ha[5]; h[8] = ha[8];
+    }
 }
 
 __device__ inline double recip(double x) { return 1.0 / x; }

@@ -6,7 +6,7 @@
 
 namespace lbm {
 
-enum Cls : uint32_t { SOLID = 0, FLUID = 1, VEL_W = 2, PRES_E = 3 };
+enum Cls : uint32_t { SOLID = 0, FLUID = 1, VEL_W = 2, PRES_E = 3, VEL_N = 4 };
 enum Mode : int { INIT_EQ = 0, INIT_ZERO = 1, INIT_FROM_F = 2 };
 
 // Per-node word: bits 0..18 bounce-back links, bits 24..31 node class.
@@ -24,7 +24,8 @@ struct KParams {
     double g[3];
     int force;               // 0 none, 1 shift, 2 guo
     int wall;                // 0 halfway, 1 on-node
-    int inlet;               // 0 zou-he, 1 reference 2D
+    int inlet;               // 0 zou-he, 1 reference 2D (west inlet and north lid)
+    int outlet;              // 0 zou-he pressure, 1 reference velocity, 2/3 extrapolation
     uint32_t parity;
     double rho_out;
     uint32_t nf;             // non-solid nodes (compact population count)

@@ -74,14 +74,13 @@ __global__ __launch_bounds__(256) void k_step(const typename Io<L, S>::raw* __re
     const int x = (int)(idx - q1 * (uint32_t)kp.nx);
     const uint32_t zq = fdiv(q1, kp.my, kp.sy);
     const int y = (int)(q1 - zq * (uint32_t)kp.ny), z = (int)zq;
-    uint32_t cn[L::Q];
-    source_index<L, OPEN>(cidx, kp, x, y, z, cn);
     const uint32_t cls = nd >> kClsShift;
     T h[L::Q];
-    gather<L, S, OPEN>(src, kp, own, nd, cn, h);
-    if (OPEN && cls == VEL_W) zou_inlet<L, T>(kp, fl, idx, nd, h);
-    if (kp.wall == 1 && (nd & kBBMask)) onnode_walls<L, T>(nd, h);
-    if (OPEN && cls == PRES_E) zou_outlet<L, T>(kp, h);
+    inflow_state<L, S, OPEN>(src, kp, fl, cidx, own, nd, x, y, z, idx, h);
+    if (OPEN && cls == PRES_E) {
+        zou_outlet<L, T>(kp, fl, idx, h);
+        outlet_extrap<L, S, OPEN>(src, kp, fl, cidx, nodew, true, x, y, z, idx, h);
+    }
     T drho, rho, u[3];
     moments<L, T>(kp, cls, h, drho, rho, u);
     if (write_macros) {
@@ -198,9 +197,11 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
         }
         h[i] = Io<L, S>::ld(src, tslot<L>(pl, j), i);
     }
+    // the extrapolating outlets need upstream nodes: the host keeps them off this kernel
     if (OPEN && cls == VEL_W) zou_inlet<L, T>(kp, fl, idx, nd, h);
+    if (OPEN && cls == VEL_N) zou_lid<L, T>(kp, fl, idx, nd, h);
     if (kp.wall == 1 && (nd & kBBMask)) onnode_walls<L, T>(nd, h);
-    if (OPEN && cls == PRES_E) zou_outlet<L, T>(kp, h);
+    if (OPEN && cls == PRES_E) zou_outlet<L, T>(kp, fl, idx, h);
     T drho, rho, u[3];
     moments<L, T>(kp, cls, h, drho, rho, u);
     if (write_macros) {
@@ -231,13 +232,12 @@ __global__ __launch_bounds__(256) void k_recon(const typename Io<L, S>::raw* __r
     }
     const uint32_t nd = node[idx];
     const uint32_t cls = nd >> kClsShift;
-    uint32_t cn[L::Q];
-    source_index<L, OPEN>(cidx, kp, x, y, z, cn);
     T h[L::Q];
-    gather<L, S, OPEN>(src, kp, own, nd, cn, h);
-    if (OPEN && cls == VEL_W) zou_inlet<L, T>(kp, fl, idx, nd, h);
-    if (kp.wall == 1 && (nd & kBBMask)) onnode_walls<L, T>(nd, h);
-    if (OPEN && cls == PRES_E) zou_outlet<L, T>(kp, h);
+    inflow_state<L, S, OPEN>(src, kp, fl, cidx, own, nd, x, y, z, idx, h);
+    if (OPEN && cls == PRES_E) {
+        zou_outlet<L, T>(kp, fl, idx, h);
+        outlet_extrap<L, S, OPEN>(src, kp, fl, cidx, node, false, x, y, z, idx, h);
+    }
     T drho, rho, u[3];
     moments<L, T>(kp, cls, h, drho, rho, u);
     fl.rho[idx] = (double)rho; fl.u[idx] = (double)u[0];

@@ -15,6 +15,11 @@
 //   storage = f64|f32|f16|bf16    init = zero|equilibrium
 //   walls = onnode|halfway        trt_magic = <Lambda>
 //   force_model = shift|guo       rho_out = <density>
+//   inlet = zouhe|reference       outlet = pressure|velocity|extrap2|extrap1
+// BCoutlet maps as the reference reads it (S/FilesReading.cu:180-190 with
+// S/include/Arguments.h:22-28): Vout -> velocity outlet, 1stE -> second-order
+// extrapolation, 2ndE -> first-order extrapolation, HEmodel -> the Zou-He
+// pressure outlet (the reference's 2D solver applies no outlet for it).
 #include "lbm.h"
 
 #include <hip/hip_runtime_api.h>
@@ -37,10 +42,12 @@ struct Args {
     double u = 0, v = 0, w = 0, rho = 1.0, viscosity = 1.0 / 6.0, g = 0;
     int inlet_profile = 2;           // 1 INLET_PROF, 2 INLET_CTE, 3 INLET_PUL
     int collision = LBM_BGK;
-    int curved = 0, hh_walls = 0, outlet = 1;
+    int curved = 0, hh_walls = 0;
+    int outlet = 2;                  // S/include/Arguments.h:22-28 numbering; CLI default "second" (S/main.cu:34)
     long iterations = 1000, residual_every = 100, autosave_after = 0, autosave_every = 0;
     std::string residual_model = "L2";
     int boundary_id = 0, update_inlet_outlet = 1;
+    int inlet_model = LBM_INLET_ZOUHE;
     double stop[4] = {0, 0, 0, 0};
     int format = 3;                  // 1 CSV, 2 DAT (Tecplot), 3 VTI
     // new knobs
@@ -114,6 +121,11 @@ bool read_ini(const std::string& path, Args& a) {
     if (kv.count("force_model")) a.force_model = kv["force_model"] == "guo" ? LBM_FORCE_GUO : LBM_FORCE_SHIFT;
     if (kv.count("trt_magic")) a.trt_magic = atof(kv["trt_magic"].c_str());
     if (kv.count("rho_out")) a.rho_out = atof(kv["rho_out"].c_str());
+    if (kv.count("inlet")) a.inlet_model = kv["inlet"] == "reference" ? LBM_INLET_REF2D : LBM_INLET_ZOUHE;
+    if (kv.count("outlet")) {
+        const std::string& o = kv["outlet"];
+        a.outlet = o == "velocity" ? 1 : o == "extrap2" ? 2 : o == "extrap1" ? 3 : 4;
+    }
     return true;
 }
 
@@ -122,6 +134,7 @@ void usage() {
         "usage: lbmsolver [-f SetUpData.ini] [-n node.dat -b bc.dat] [-o outdir] [-u U] [-v V] [-r rho]\n"
         "                 [-s viscosity] [-c BGKW|TRT|MRT] [-i iterations] [--every N]\n"
         "                 [--format paraview|tecplot|csv] [--storage f64|f32|f16|bf16]\n"
+        "                 [-l|--outlet yes|second|first]\n"
         "       lbmsolver compare <file1> <file2>\n");
 }
 
@@ -357,6 +370,13 @@ int main(int argc, char** argv) {
         else if (s == "-s") { a.viscosity = atof(next()); from_cli = true; }
         else if (s == "-i") { a.iterations = atol(next()); from_cli = true; }
         else if (s == "--every") a.residual_every = atol(next());
+        else if (s == "-l" || s == "--outlet") {   // S/Arguments.cu:55-75
+            std::string o = next();
+            if (o == "yes") a.outlet = 1;
+            else if (o == "second") a.outlet = 2;
+            else if (o == "first") a.outlet = 3;
+            else std::fprintf(stderr, "Invalid outlet profile (%s), going with default\n", o.c_str());
+        }
         else if (s == "-c") { std::string c = next(); a.collision = c == "TRT" ? LBM_TRT : c == "MRT" ? LBM_MRT : LBM_BGK; from_cli = true; }
         else if (s == "--format") { std::string f = next(); a.format = f == "tecplot" ? 2 : f == "csv" ? 1 : 3; }
         else if (s == "--storage") { std::string f = next(); a.storage = f == "f32" ? LBM_STORE_F32 : f == "f16" ? LBM_STORE_F16 : f == "bf16" ? LBM_STORE_BF16 : LBM_STORE_F64; }
@@ -409,8 +429,11 @@ int main(int argc, char** argv) {
                 scls[s] = cls[d];
                 sbb[s] = bb[d];
             }
-    // inlet profile on velocity-inlet nodes (S/GpuInit.cu:994-1006): parabolic
-    // between half a spacing beyond the extreme inlet nodes, or constant
+    // inlet profile u0 (S/GpuInit.cu:994-1006): parabolic between half a
+    // spacing beyond the extreme west-inlet nodes, or constant; the reference
+    // defines it at every node, and its velocity outlet reads it on the east
+    // face.  A north lid moves at the constant U (the parabola spans the west
+    // inlet's y range).
     std::vector<double> ubc(NS, 0.0), vbc(NS, 0.0);
     {
         int ymin = 1 << 30, ymax = -1;
@@ -420,11 +443,15 @@ int main(int argc, char** argv) {
                 ymin = std::min(ymin, y);
                 ymax = std::max(ymax, y);
             }
+        const bool prof = a.inlet_profile == 1 && ymax >= 0;
         for (size_t i = 0; i < NS; ++i)
-            if (scls[i] == LBM_VEL_W) {
+            if (scls[i] == LBM_VEL_W || scls[i] == LBM_PRES_E) {
                 const double y = (double)((i / sim[0]) % sim[1]), lo = ymin - 0.5, hi = ymax + 0.5;
-                ubc[i] = a.inlet_profile == 1 ? 6.0 * a.u * (y - lo) * (hi - y) / ((hi - lo) * (hi - lo)) : a.u;
+                ubc[i] = prof ? 6.0 * a.u * (y - lo) * (hi - y) / ((hi - lo) * (hi - lo)) : a.u;
                 vbc[i] = a.v;
+            } else if (scls[i] == LBM_VEL_N) {
+                ubc[i] = a.u;
+                vbc[i] = 0.0;
             }
     }
     lbm_params p;
@@ -440,7 +467,9 @@ int main(int argc, char** argv) {
     p.force[0] = a.g;
     p.storage = a.storage;
     p.wall_model = a.wall_model;
-    p.inlet_model = LBM_INLET_ZOUHE;
+    p.inlet_model = a.inlet_model;
+    p.outlet_model = a.outlet == 1 ? LBM_OUTLET_REF_VELOCITY : a.outlet == 2 ? LBM_OUTLET_EXTRAP2
+                   : a.outlet == 3 ? LBM_OUTLET_EXTRAP1 : LBM_OUTLET_PRESSURE;
     p.rho_out = a.rho_out > 0 ? a.rho_out : a.rho;
     lbm_lattice lat = {scls.data(), sbb.data(), ubc.data(), vbc.data(), nullptr};
     lbm_ctx* ctx = nullptr;

@@ -1,6 +1,6 @@
 """Lattice builders in the solver's node description.
 
-A lattice is `node_class` (uint8 per node: SOLID, FLUID, VEL_W, PRES_E) plus
+A lattice is `node_class` (uint8 per node: SOLID, FLUID, VEL_W, PRES_E, VEL_N) plus
 `bb` (uint32 per node, bit d set when the link from the node in direction d
 crosses a wall -- the content of a latticegen2 _BC.dat WALL line,
 L/FluidTracking.cpp:129-156), in the reference's node order
@@ -8,7 +8,7 @@ ind = i + j*n + k*n*m (S/include/Iterate.h:10-35).
 """
 import numpy as np
 
-from .lbm import SOLID, FLUID, VEL_W, PRES_E
+from .lbm import SOLID, FLUID, VEL_W, PRES_E, VEL_N
 
 # reference direction numbering (S/GpuInit.cu:94-99, :822-842)
 D2Q9_C = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, -1, 0],
@@ -80,6 +80,25 @@ def channel2d(nx, ny, u_max, obstacles=(), rho_out=1.0):
     ubc[:, 0] = prof
     return dict(node_class=cls, bb=bb, ubc=ubc, vbc=np.zeros((ny, nx)), solid=solid, profile=prof,
                 periodic=(0, 0, 0), rho_out=rho_out)
+
+
+def cavity2d(n, u_lid):
+    """D2Q9 lid-driven cavity of n x n nodes in the reference's layout for a
+    latticegen2 box (S/GpuBoundaries.cu:21-41 for the lid): every node fluid,
+    wall links on every link leaving the west, east and south faces (for
+    on-node bounce-back, walls at x = 0, n-1 and y = 0), the top row a moving
+    lid (VEL_N) at u_lid.  The two top corner nodes carry the wall links of
+    their side wall; the cavity side is n - 1."""
+    c = velocities(9)
+    yy, xx = np.mgrid[0:n, 0:n]
+    bb = np.zeros((n, n), np.uint32)
+    for d in range(1, 9):
+        sx, sy = xx + c[d, 0], yy + c[d, 1]
+        bb |= np.where((sx < 0) | (sx >= n) | (sy < 0), np.uint32(1 << d), np.uint32(0))
+    cls = np.full((n, n), FLUID, np.uint8)
+    cls[-1, :] = VEL_N
+    ubc = np.where(cls == VEL_N, float(u_lid), 0.0)
+    return dict(node_class=cls, bb=bb, ubc=ubc, vbc=np.zeros((n, n)), periodic=(0, 0, 0))
 
 
 def channel_sweep(count=64, seed=7, nx_range=(128, 1024), ny_range=(32, 256), re_range=(10.0, 100.0),
@@ -230,8 +249,9 @@ def write_latticegen2(path_prefix, node_class, bb=None, delta=1.0, inlet_dirs=No
     """Write <prefix>_Node.dat and <prefix>_BC.dat in the latticegen2 layout
     (L/main.cpp:59-66, L/FluidTracking.cpp:229-236): 2D lines "i j x y fluid"
     and "i j dir type x y id", 3D with the k column.  WALL links come from
-    `bb`; VEL_W / PRES_E nodes of a 2D lattice get INLE / OUTL links in the
-    directions leaving the west / east face."""
+    `bb`; VEL_W / PRES_E / VEL_N nodes of a 2D lattice get INLE / OUTL / INLE
+    links in the directions leaving the west / east / north face (wall links
+    excepted)."""
     cls = np.asarray(node_class)
     two_d = cls.ndim == 2
     c3 = cls[None] if two_d else cls
@@ -268,6 +288,10 @@ def write_latticegen2(path_prefix, node_class, bb=None, delta=1.0, inlet_dirs=No
                     if c3[k, j, i] == VEL_W:
                         for d in inlet_dirs:
                             if not (int(b3[k, j, i]) >> d) & 1:
+                                link(i, j, k, d, 2)
+                    if c3[k, j, i] == VEL_N:
+                        for d in range(1, q):
+                            if c[d, 1] > 0 and not (int(b3[k, j, i]) >> d) & 1:
                                 link(i, j, k, d, 2)
                     if c3[k, j, i] == PRES_E:
                         for d in outlet_dirs:

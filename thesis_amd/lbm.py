@@ -13,14 +13,15 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblbm.so")
-ABI_VERSION = 1
+ABI_VERSION = 2
 
-SOLID, FLUID, VEL_W, PRES_E = 0, 1, 2, 3
+SOLID, FLUID, VEL_W, PRES_E, VEL_N = 0, 1, 2, 3, 4
 BGK, TRT, MRT = 0, 1, 2
 STORE_F64, STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2, 3
 FORCE_NONE, FORCE_SHIFT, FORCE_GUO = 0, 1, 2
 WALL_HALFWAY, WALL_ONNODE = 0, 1
 INLET_ZOUHE, INLET_REF2D = 0, 1
+OUTLET_PRESSURE, OUTLET_REF_VELOCITY, OUTLET_EXTRAP2, OUTLET_EXTRAP1 = 0, 1, 2, 3
 PARITY_MRT_RHO2, PARITY_MRT_NO_F18, PARITY_OUTLET_V0 = 0x1, 0x2, 0x8
 
 STATUS = {0: "OK", 1: "DIVERGED", 2: "EIO", 3: "EINVAL", 4: "EDEVICE", 5: "ENOMEM", 6: "EUNSUPPORTED"}
@@ -50,7 +51,7 @@ class Params(C.Structure):
         ("omega", C.c_double), ("trt_magic", C.c_double), ("mrt_s", C.c_double * 19),
         ("force_model", C.c_int32), ("force", C.c_double * 3), ("storage", C.c_int32),
         ("wall_model", C.c_int32), ("inlet_model", C.c_int32), ("parity_flags", C.c_uint32),
-        ("rho_out", C.c_double),
+        ("rho_out", C.c_double), ("outlet_model", C.c_int32),
     ]
 
 
@@ -206,7 +207,7 @@ class Solver:
     def __init__(self, q, shape, node_class, bb=None, *, collision=BGK, omega=1.0, trt_magic=0.0,
                  mrt_s=None, force_model=FORCE_NONE, g=(0.0, 0.0, 0.0), wall_model=WALL_HALFWAY,
                  inlet_model=INLET_ZOUHE, parity=0, rho_out=1.0, periodic=(0, 0, 0), ubc=None,
-                 vbc=None, wbc=None, storage=STORE_F64):
+                 vbc=None, wbc=None, storage=STORE_F64, outlet_model=OUTLET_PRESSURE):
         L = load()
         if isinstance(storage, str):
             storage = STORAGE_NAMES[storage]
@@ -224,7 +225,7 @@ class Solver:
             for i, s in enumerate(mrt_s):
                 p.mrt_s[i] = s
         p.force_model, p.storage, p.wall_model, p.inlet_model = force_model, storage, wall_model, inlet_model
-        p.parity_flags, p.rho_out = parity, float(rho_out)
+        p.parity_flags, p.rho_out, p.outlet_model = parity, float(rho_out), outlet_model
         self.params = p
         cls = np.ascontiguousarray(node_class, dtype=np.uint8).ravel()
         if cls.size != self.n:
