@@ -40,10 +40,18 @@ typedef enum {
 } lbm_status;
 
 /* node classes: one byte per node (latticegen2 fluid flag + boundary kind).
- * LBM_VEL_W: velocity inlet on the west face; LBM_PRES_E: outlet on the east
- * face (lbm_params.outlet_model); LBM_VEL_N: velocity (moving lid) on the
- * north face, the reference's 2D north "inlet" (S/GpuBoundaries.cu:21-41). */
-enum { LBM_SOLID = 0, LBM_FLUID = 1, LBM_VEL_W = 2, LBM_PRES_E = 3, LBM_VEL_N = 4 };
+ * Faces of the box: W x = 0, E x = nx-1, S y = 0, N y = ny-1, B z = 0,
+ * T z = nz-1.  LBM_VEL_*: velocity face (u_bc, v_bc, w_bc); in 2D only W (the
+ * inlet) and N (the moving lid, the reference's 2D north "inlet",
+ * S/GpuBoundaries.cu:21-41).  LBM_PRES_E and LBM_OUT_*: outlets
+ * (lbm_params.outlet_model); 2D has the east one only.  LBM_SYM_*: symmetry
+ * planes (3D, S/GpuBoundaries.cu:2086-2167). */
+enum {
+    LBM_SOLID = 0, LBM_FLUID = 1, LBM_VEL_W = 2, LBM_PRES_E = 3, LBM_VEL_N = 4,
+    LBM_VEL_E = 5, LBM_VEL_S = 6, LBM_VEL_B = 7, LBM_VEL_T = 8,
+    LBM_OUT_W = 9, LBM_OUT_S = 10, LBM_OUT_N = 11, LBM_OUT_B = 12, LBM_OUT_T = 13,
+    LBM_SYM_W = 14, LBM_SYM_E = 15, LBM_SYM_S = 16, LBM_SYM_N = 17, LBM_SYM_B = 18, LBM_SYM_T = 19
+};
 /* collision models (S/include/Arguments.h: BGKW, TRT, MRT) */
 enum { LBM_BGK = 0, LBM_TRT = 1, LBM_MRT = 2 };
 /* population storage; compute is FP64 for F64 storage and FP32 otherwise */
@@ -53,7 +61,11 @@ enum { LBM_STORE_F64 = 0, LBM_STORE_F32 = 1, LBM_STORE_F16 = 2, LBM_STORE_BF16 =
 enum { LBM_FORCE_NONE = 0, LBM_FORCE_SHIFT = 1, LBM_FORCE_GUO = 2 };
 /* walls: half-way bounce-back, or the reference's on-node
  * f[opp] = f[dir] (S/GpuBoundaries.cu:435-437, :482-484) */
-enum { LBM_WALL_HALFWAY = 0, LBM_WALL_ONNODE = 1 };
+/* LBM_WALL_HH: the reference's wet-node walls ("HHmodel",
+ * S/GpuBoundaries.cu:492-1560), D3Q19: nodes with 5, 9 or 12 wall links are
+ * closed as a face, an edge or a corner of a box; other wall links keep the
+ * node's own population, as in the reference */
+enum { LBM_WALL_HALFWAY = 0, LBM_WALL_ONNODE = 1, LBM_WALL_HH = 2 };
 /* velocity faces: textbook Zou-He, or the reference's 2D forms without
  * transverse terms, corners skipped (west S/GpuBoundaries.cu:45-64; north
  * lid :21-41); the reference forms impose u0 exactly on equilibrium states */
@@ -63,11 +75,19 @@ enum { LBM_INLET_ZOUHE = 0, LBM_INLET_REF2D = 1 };
  * u0 = u_bc at the node (S/GpuBoundaries.cu:1571-1605), or its second- and
  * first-order extrapolations, which overwrite the streamed-in f1, f5, f8 from
  * the two upstream nodes (:1609-1640) */
-enum { LBM_OUTLET_PRESSURE = 0, LBM_OUTLET_REF_VELOCITY = 1, LBM_OUTLET_EXTRAP2 = 2, LBM_OUTLET_EXTRAP1 = 3 };
+enum { LBM_OUTLET_PRESSURE = 0, LBM_OUTLET_REF_VELOCITY = 1, LBM_OUTLET_EXTRAP2 = 2, LBM_OUTLET_EXTRAP1 = 3,
+       LBM_OUTLET_FEQ = 4 };
+/* In 3D (S/GpuBoundaries.cu:1655-1874) the outlets act on any LBM_OUT_* face
+ * and LBM_PRES_E: LBM_OUTLET_PRESSURE is a textbook pressure face (rho_out,
+ * zero tangential velocity; the reference has none), the extrapolations set
+ * the face's unknown populations from the two upstream nodes, and
+ * LBM_OUTLET_FEQ sets them to f_eq of the node's macros of the previous step
+ * (the reference's HEmodel outlet); LBM_OUTLET_REF_VELOCITY is 2D only. */
 /* reference quirks reproduced on request */
 #define LBM_PARITY_MRT_RHO2   0x1u /* 2D MRT m_eq with rho^2 (S/GpuCollision.cu:987-988) */
 #define LBM_PARITY_MRT_NO_F18 0x2u /* 3D MRT leaves f18 unwritten (S/GpuCollision.cu:1161) */
 #define LBM_PARITY_OUTLET_V0  0x8u /* 2D v := 0 on outlet nodes (S/GpuUpdateMacro.cu:30) */
+#define LBM_PARITY_INLET3D_FLIP 0x10u /* 3D W/E velocity faces divide by 1 + G u (S/GpuBoundaries.cu:102,148) */
 
 /* Solver parameters; mirrors the numerical part of the reference's
  * Arguments struct (S/include/Arguments.h:86-135) plus the new knobs. */

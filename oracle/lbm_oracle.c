@@ -324,7 +324,8 @@ static void macro_node(orc_sim* s, long ind)
     const long n = s->n;
     uint8_t c = s->cls[ind];
     if (c == ORC_SOLID) return;
-    if ((s->p.parity & ORC_PARITY_INLET_FROZEN) && c == ORC_VEL_W) return;
+    if ((s->p.parity & ORC_PARITY_INLET_FROZEN) && (c == ORC_VEL_W || c == ORC_VEL_N || (c >= ORC_VEL_E && c <= ORC_VEL_T)))
+        return;
     double r = 0, jx = 0, jy = 0, jz = 0;
     for (int i = 0; i < Q; ++i) {
         double fi = s->f[(size_t)i * n + ind];
@@ -443,7 +444,219 @@ static void zouhe_node(const orc_sim* s, long ind, double* f)
     }
 }
 
-void orc_open_boundary(const orc_sim* s, long ind, double* f) { zouhe_node(s, ind, f); }
+/* ---- D3Q19 faces, walls, outlets, symmetry ---------------------------------
+ * A face is (axis a, inward sign g): W (0,+1), E (0,-1), S (1,+1), N (1,-1),
+ * B (2,+1), T (2,-1).  Unknowns of a face node are the populations with
+ * g c_a > 0 (they would stream in from outside the lattice). */
+static int dir3(int x, int y, int z)
+{
+    for (int i = 0; i < 19; ++i)
+        if (CX19[i] == x && CY19[i] == y && CZ19[i] == z) return i;
+    return -1;
+}
+static int comp(int i, int a) { return a == 0 ? CX19[i] : a == 1 ? CY19[i] : CZ19[i]; }
+static int dir_ax(int a, int ca, int b, int cb)   /* direction with c_a = ca, c_b = cb, others 0 */
+{
+    int c[3] = {0, 0, 0};
+    c[a] += ca;
+    c[b] += cb;
+    return dir3(c[0], c[1], c[2]);
+}
+
+/* face of a 3D class: returns kind (1 velocity, 2 outlet, 3 symmetry, 0 none) */
+static int face_of(uint8_t c, int* a, int* g)
+{
+    static const int VA[6] = {0, 0, 1, 1, 2, 2}, VG[6] = {1, -1, 1, -1, 1, -1};
+    int k = 0, f = -1;
+    switch (c) {
+    case ORC_VEL_W: k = 1; f = 0; break;  case ORC_VEL_E: k = 1; f = 1; break;
+    case ORC_VEL_S: k = 1; f = 2; break;  case ORC_VEL_N: k = 1; f = 3; break;
+    case ORC_VEL_B: k = 1; f = 4; break;  case ORC_VEL_T: k = 1; f = 5; break;
+    case ORC_OUT_W: k = 2; f = 0; break;  case ORC_PRES_E: k = 2; f = 1; break;
+    case ORC_OUT_S: k = 2; f = 2; break;  case ORC_OUT_N: k = 2; f = 3; break;
+    case ORC_OUT_B: k = 2; f = 4; break;  case ORC_OUT_T: k = 2; f = 5; break;
+    default:
+        if (c >= ORC_SYM_W && c <= ORC_SYM_T) { k = 3; f = c - ORC_SYM_W; }
+    }
+    if (f >= 0) { *a = VA[f]; *g = VG[f]; }
+    return k;
+}
+
+/* Velocity face after Hecht & Harting (2010), the formulation of
+ * gpuBcInlet3D (S/GpuBoundaries.cu:73-399): rho from the tangential and
+ * outgoing populations, the normal unknown by bounce-back of the
+ * non-equilibrium part, the four diagonal unknowns with the transverse
+ * momentum corrections N_ab.  flip: the reference's W/E denominator. */
+static void velocity_face(double* f, int a, int g, const double* U, int flip)
+{
+    double K = 0;
+    for (int i = 0; i < 19; ++i) {
+        int ca = comp(i, a);
+        if (ca == 0) K += f[i];
+        else if (ca == -g) K += 2.0 * f[i];
+    }
+    double rho = K / (flip ? 1.0 + g * U[a] : 1.0 - g * U[a]);
+    f[dir_ax(a, g, a, 0)] = f[dir_ax(a, -g, a, 0)] + g * rho * U[a] / 3.0;
+    for (int b = 0; b < 3; ++b) {
+        if (b == a) continue;
+        double N = 0;
+        for (int i = 0; i < 19; ++i)
+            if (comp(i, a) == 0) N += 0.5 * comp(i, b) * f[i];
+        N -= rho * U[b] / 3.0;
+        for (int t = -1; t <= 1; t += 2)
+            f[dir_ax(a, g, b, t)] = f[dir_ax(a, -g, b, -t)] + rho * (g * U[a] + t * U[b]) / 6.0 - t * N;
+    }
+}
+
+/* Wet-node walls of gpuBcComplexWall3D: a node with 5 wall links is a face
+ * (the velocity face with u = 0), 9 an edge, 12 a corner; the inward normal
+ * along an axis is -1 when the +axis link is a wall, +1 for the -axis link.
+ * Edge with normals (a, ga), (b, gb) and tangent t: bounce-back of the normal
+ * and in-plane diagonal unknowns, N_t = -(f_+t - f_-t)/4 on the diagonals
+ * with a t component, the two buried diagonals and the rest population share
+ * the sum S of the other 16: S/22 each, 12 S/22.  Corner: bounce-back of the
+ * three normal and three diagonal unknowns, the six buried diagonals C =
+ * (sum of the 12 others)/18, rest 12 C.  Other nodes keep their values. */
+static void hh_node(uint32_t bb, double* f)
+{
+    int cnt = 0, nrm[3] = {0, 0, 0}, nn = 0;
+    for (int d = 1; d < 19; ++d) cnt += (bb >> d) & 1u;
+    for (int a = 0; a < 3; ++a) {
+        if ((bb >> dir_ax(a, 1, a, 0)) & 1u) nrm[a] = -1;
+        else if ((bb >> dir_ax(a, -1, a, 0)) & 1u) nrm[a] = 1;
+        nn += nrm[a] != 0;
+    }
+    if (cnt == 5 && nn == 1) {
+        const double U[3] = {0, 0, 0};
+        for (int a = 0; a < 3; ++a)
+            if (nrm[a]) velocity_face(f, a, nrm[a], U, 0);
+    } else if (cnt == 9 && nn == 2) {
+        int a = -1, b = -1, t = -1;
+        for (int k = 0; k < 3; ++k) {
+            if (!nrm[k]) t = k;
+            else if (a < 0) a = k;
+            else b = k;
+        }
+        int ga = nrm[a], gb = nrm[b];
+        f[dir_ax(a, ga, a, 0)] = f[dir_ax(a, -ga, a, 0)];
+        f[dir_ax(b, gb, b, 0)] = f[dir_ax(b, -gb, b, 0)];
+        f[dir_ax(a, ga, b, gb)] = f[dir_ax(a, -ga, b, -gb)];
+        double N = -0.25 * (f[dir_ax(t, 1, t, 0)] - f[dir_ax(t, -1, t, 0)]);
+        f[dir_ax(a, ga, t, 1)] = f[dir_ax(a, -ga, t, -1)] + N;
+        f[dir_ax(a, ga, t, -1)] = f[dir_ax(a, -ga, t, 1)] - N;
+        f[dir_ax(b, gb, t, 1)] = f[dir_ax(b, -gb, t, -1)] + N;
+        f[dir_ax(b, gb, t, -1)] = f[dir_ax(b, -gb, t, 1)] - N;
+        int q1 = dir_ax(a, ga, b, -gb), q2 = dir_ax(a, -ga, b, gb);
+        double S = 0;
+        for (int i = 1; i < 19; ++i)
+            if (i != q1 && i != q2) S += f[i];
+        f[q1] = f[q2] = S / 22.0;
+        f[0] = 12.0 * S / 22.0;
+    } else if (cnt == 12 && nn == 3) {
+        int bur[6], nb = 0;
+        for (int a = 0; a < 3; ++a) f[dir_ax(a, nrm[a], a, 0)] = f[dir_ax(a, -nrm[a], a, 0)];
+        for (int a = 0; a < 3; ++a)
+            for (int b = a + 1; b < 3; ++b) {
+                f[dir_ax(a, nrm[a], b, nrm[b])] = f[dir_ax(a, -nrm[a], b, -nrm[b])];
+                bur[nb++] = dir_ax(a, nrm[a], b, -nrm[b]);
+                bur[nb++] = dir_ax(a, -nrm[a], b, nrm[b]);
+            }
+        double S = 0;
+        for (int i = 1; i < 19; ++i) {
+            int buried = 0;
+            for (int k = 0; k < 6; ++k) buried |= bur[k] == i;
+            if (!buried) S += f[i];
+        }
+        for (int k = 0; k < 6; ++k) f[bur[k]] = S / 18.0;
+        f[0] = 12.0 * S / 18.0;
+    }
+}
+
+/* 3D outlet of the face (a, g).  ORC_OUTLET_PRESSURE is a textbook pressure
+ * face (the reference's 3D OUTLET prints "not implemented",
+ * S/GpuBoundaries.cu:1671-1706).  The others follow the intended semantics of gpuBcOutlet3D
+ * (S/GpuBoundaries.cu:1655-1874): the unknowns become f(x+n) (first order) or
+ * 2 f(x+n) - f(x+2n) (second order) of the upstream nodes' populations after
+ * their inlet and wall conditions, or f_eq of the node's stored macros.  The
+ * reference's population stride there is n*m instead of n*m*h and its
+ * second-order north face reads downstream; neither is reproduced.  An
+ * upstream node that is solid or outside keeps the node's streamed values. */
+static void outlet3d(const orc_sim* s, const double* src, double* dst, long ind, int a, int g)
+{
+    const long n = s->n;
+    const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
+    int x = (int)(ind % nx), y = (int)((ind / nx) % ny), z = (int)(ind / ((long)nx * ny));
+    if (s->p.outlet_model == ORC_OUTLET_PRESSURE) {
+        /* textbook pressure face (Hecht & Harting 2010): rho = rho_out, zero
+         * tangential velocity, the normal velocity from the known populations */
+        double f[19], K = 0, U[3] = {0, 0, 0};
+        for (int i = 0; i < 19; ++i) {
+            f[i] = src[(size_t)i * n + ind];
+            int ca = comp(i, a);
+            if (ca == 0) K += f[i];
+            else if (ca == -g) K += 2.0 * f[i];
+        }
+        U[a] = g * (1.0 - K / s->p.rho_out);
+        velocity_face(f, a, g, U, 0);
+        for (int i = 0; i < 19; ++i) dst[(size_t)i * n + ind] = f[i];
+        return;
+    }
+    if (s->p.outlet_model == ORC_OUTLET_FEQ) {
+        double uvw[3] = {s->u[ind], s->v[ind], s->w3[ind]}, feq[19];
+        feq_all(s, s->rho[ind], uvw, feq);
+        for (int i = 0; i < 19; ++i)
+            if (g * comp(i, a) > 0) dst[(size_t)i * n + ind] = feq[i];
+        return;
+    }
+    int p[3] = {x, y, z}, dims[3] = {nx, ny, nz};
+    long up[2];
+    for (int k = 1; k <= 2; ++k) {
+        int q[3] = {p[0], p[1], p[2]};
+        q[a] += g * k;
+        if (q[a] < 0 || q[a] >= dims[a]) return;
+        up[k - 1] = ((long)q[2] * ny + q[1]) * nx + q[0];
+        if (s->cls[up[k - 1]] == ORC_SOLID) return;
+    }
+    for (int i = 0; i < 19; ++i) {
+        if (g * comp(i, a) <= 0) continue;
+        double f1 = src[(size_t)i * n + up[0]], f2 = src[(size_t)i * n + up[1]];
+        dst[(size_t)i * n + ind] = s->p.outlet_model == ORC_OUTLET_EXTRAP2 ? 2.0 * f1 - f2 : f1;
+    }
+}
+
+/* symmetry plane (gpuBcSymm3D, S/GpuBoundaries.cu:2086-2167): each unknown
+ * takes its mirror image in the face */
+static void symmetry3d(double* f, int a, int g)
+{
+    for (int i = 0; i < 19; ++i) {
+        if (g * comp(i, a) <= 0) continue;
+        int c[3] = {CX19[i], CY19[i], CZ19[i]};
+        c[a] = -c[a];
+        f[i] = f[dir3(c[0], c[1], c[2])];
+    }
+}
+
+void orc_open_boundary(const orc_sim* s, long ind, double* f)
+{
+    if (s->Q == 9) { zouhe_node(s, ind, f); return; }
+    int a, g;
+    int k = face_of(s->cls[ind], &a, &g);
+    if (k == 1) {
+        double U[3] = {s->ubc[ind], s->vbc[ind], s->wbc[ind]};
+        velocity_face(f, a, g, U, (s->p.parity & ORC_PARITY_INLET3D_FLIP) && a == 0);
+    } else if (k == 3) {
+        symmetry3d(f, a, g);
+    }
+}
+
+void orc_wall_node(const orc_sim* s, long ind, double* f)
+{
+    uint32_t bb = s->bb[ind];
+    if (s->p.wall_model == ORC_WALL_HH) hh_node(bb, f);
+    else if (s->p.wall_model == ORC_WALL_ONNODE)
+        for (int d = 1; d < s->Q; ++d)
+            if ((bb >> d) & 1u) f[s->opp[d]] = f[d];
+}
 
 /* Extrapolating east outlet (S/GpuBoundaries.cu:1609-1640): the populations
  * that streamed in from the east face's upstream side, f1, f5, f8, become
@@ -501,14 +714,15 @@ void orc_step(orc_sim* s, int nsteps)
                 long src = wrap_src(s, x, y, z, i, &outside);
                 f[i] = outside ? s->fcoll[(size_t)i * n + ind] : s->fcoll[(size_t)i * n + src];
             }
-            /* inlet -> wall -> outlet order of S/Iterate.cu:468-473 */
-            if (s->cls[ind] == ORC_VEL_W || s->cls[ind] == ORC_VEL_N) zouhe_node(s, ind, f);
-            if (s->p.wall_model == ORC_WALL_ONNODE && bb) {
-                /* on-node bounce-back, S/GpuBoundaries.cu:400-443, :445-490 */
-                for (int d = 1; d < Q; ++d)
-                    if ((bb >> d) & 1u) f[s->opp[d]] = f[d];
-            }
-            if (s->cls[ind] == ORC_PRES_E) zouhe_node(s, ind, f);
+            /* inlet -> wall -> outlet order of S/Iterate.cu:468-473 and
+             * S/Iterate3D.cu:602-621 (then periodic, here by wrap, and symmetry) */
+            uint8_t c = s->cls[ind];
+            int a, g, kind = Q == 19 ? face_of(c, &a, &g) : 0;
+            if (Q == 9 && (c == ORC_VEL_W || c == ORC_VEL_N)) zouhe_node(s, ind, f);
+            if (kind == 1) orc_open_boundary(s, ind, f);
+            /* on-node bounce-back, S/GpuBoundaries.cu:400-443, :445-490; wet-node walls */
+            if (bb) orc_wall_node(s, ind, f);
+            if (Q == 9 && c == ORC_PRES_E) zouhe_node(s, ind, f);
             for (int i = 0; i < Q; ++i) s->f[(size_t)i * n + ind] = f[i];
         }
         if (Q == 9 && s->p.outlet_model >= ORC_OUTLET_EXTRAP2) {
@@ -516,6 +730,30 @@ void orc_step(orc_sim* s, int nsteps)
 #pragma omp parallel for
             for (long ind = 0; ind < n; ++ind)
                 if (s->cls[ind] == ORC_PRES_E) outlet_extrap(s, ind);
+        }
+        if (Q == 19) {
+            /* outlets read their upstream nodes' pre-outlet state from a snapshot */
+            long nout = 0;
+            for (long ind = 0; ind < n; ++ind) { int a, g; nout += face_of(s->cls[ind], &a, &g) == 2; }
+            if (nout) {
+                double* snap = (double*)malloc(sizeof(double) * 19 * (size_t)n);
+                memcpy(snap, s->f, sizeof(double) * 19 * (size_t)n);
+#pragma omp parallel for
+                for (long ind = 0; ind < n; ++ind) {
+                    int a, g;
+                    if (face_of(s->cls[ind], &a, &g) == 2) outlet3d(s, snap, s->f, ind, a, g);
+                }
+                free(snap);
+            }
+#pragma omp parallel for
+            for (long ind = 0; ind < n; ++ind) {
+                int a, g;
+                if (face_of(s->cls[ind], &a, &g) != 3) continue;
+                double f[19];
+                for (int i = 0; i < 19; ++i) f[i] = s->f[(size_t)i * n + ind];
+                symmetry3d(f, a, g);
+                for (int i = 0; i < 19; ++i) s->f[(size_t)i * n + ind] = f[i];
+            }
         }
         /* macroscopic update */
 #pragma omp parallel for

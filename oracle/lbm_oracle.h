@@ -29,19 +29,27 @@ extern "C" {
 #endif
 
 /* node classes (shared with the product's lattice description) */
-enum { ORC_SOLID = 0, ORC_FLUID = 1, ORC_VEL_W = 2, ORC_PRES_E = 3, ORC_VEL_N = 4 };
+enum { ORC_SOLID = 0, ORC_FLUID = 1, ORC_VEL_W = 2, ORC_PRES_E = 3, ORC_VEL_N = 4,
+       /* 3D faces: velocity (W and N above), outlets (east = ORC_PRES_E), symmetry planes */
+       ORC_VEL_E = 5, ORC_VEL_S = 6, ORC_VEL_B = 7, ORC_VEL_T = 8,
+       ORC_OUT_W = 9, ORC_OUT_S = 10, ORC_OUT_N = 11, ORC_OUT_B = 12, ORC_OUT_T = 13,
+       ORC_SYM_W = 14, ORC_SYM_E = 15, ORC_SYM_S = 16, ORC_SYM_N = 17, ORC_SYM_B = 18, ORC_SYM_T = 19 };
 enum { ORC_BGK = 0, ORC_TRT = 1, ORC_MRT = 2 };
 enum { ORC_FORCE_NONE = 0, ORC_FORCE_SHIFT = 1, ORC_FORCE_GUO = 2 };
-enum { ORC_WALL_HALFWAY = 0, ORC_WALL_ONNODE = 1 };
+/* ORC_WALL_HH: the reference's wet-node walls (HHmodel, S/GpuBoundaries.cu:492-1560),
+ * 3D, on nodes with 5 (face), 9 (edge) or 12 (corner) wall links */
+enum { ORC_WALL_HALFWAY = 0, ORC_WALL_ONNODE = 1, ORC_WALL_HH = 2 };
 enum { ORC_INLET_ZOUHE = 0, ORC_INLET_REF2D = 1 };
 /* east outlet: Zou-He pressure, the reference's velocity outlet, its second-
  * and first-order extrapolations (S/GpuBoundaries.cu:1562-1653) */
-enum { ORC_OUTLET_PRESSURE = 0, ORC_OUTLET_REF_VELOCITY = 1, ORC_OUTLET_EXTRAP2 = 2, ORC_OUTLET_EXTRAP1 = 3 };
+enum { ORC_OUTLET_PRESSURE = 0, ORC_OUTLET_REF_VELOCITY = 1, ORC_OUTLET_EXTRAP2 = 2, ORC_OUTLET_EXTRAP1 = 3,
+       ORC_OUTLET_FEQ = 4 /* 3D: f_eq of the stored macros (S/GpuBoundaries.cu:1840-1852) */ };
 /* parity flags: reference quirks reproduced on request */
 #define ORC_PARITY_MRT_RHO2      0x1u /* 2D MRT m_eq uses rho^2 (S/GpuCollision.cu:987-988) */
 #define ORC_PARITY_MRT_NO_F18    0x2u /* 3D MRT never writes f18 (S/GpuCollision.cu:1161) */
 #define ORC_PARITY_INLET_FROZEN  0x4u /* 3D macros skip inlet nodes (S/GpuUpdateMacro.cu:150-151) */
 #define ORC_PARITY_OUTLET_V0     0x8u /* 2D v := 0 on east-outlet nodes (S/GpuUpdateMacro.cu:30) */
+#define ORC_PARITY_INLET3D_FLIP 0x10u /* 3D W/E velocity faces use 1/(1 +- u) (S/GpuBoundaries.cu:102,148) */
 
 typedef struct {
     int q;              /* 9 (D2Q9) or 19 (D3Q19) */
@@ -92,9 +100,12 @@ double orc_mass(const orc_sim* s);                   /* sum of f over non-solid 
 /* Kernels exposed for closed-form tests. */
 void orc_collide_node(const orc_sim* s, const double* f, double rho, const double* uvw, double* out);
 void orc_equilibrium(int q, double rho, const double* uvw, double* feq);
-/* The node-local open-boundary condition of node ind (west inlet, north lid,
- * east outlet models 0 and 1) applied to its streamed populations f[Q]. */
+/* The node-local open-boundary condition of node ind applied to its streamed
+ * populations f[Q]: 2D west inlet, north lid, east outlet models 0 and 1;
+ * 3D velocity faces and symmetry planes. */
 void orc_open_boundary(const orc_sim* s, long ind, double* f);
+/* The wall condition of node ind (on-node bounce-back or wet-node walls). */
+void orc_wall_node(const orc_sim* s, long ind, double* f);
 
 /* ---- colour-gradient two-phase model (D2Q9) ----------------------------- */
 typedef struct {

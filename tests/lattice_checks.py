@@ -64,3 +64,31 @@ def ghia_centreline_error(u, u_lid):
     col = 0.5 * (u[:, (n - 1) // 2] + u[:, n // 2]) / u_lid
     ys = np.arange(n) / (n - 1)
     return max(abs(float(np.interp(y, ys, col)) - g) for y, g in GHIA_RE100_U)
+
+
+def square_duct_series(n, terms=99, points=None):
+    """Series of fully developed laminar flow in a square duct of half-width
+    a = (n-1)/2 (walls on nodes 0 and n-1; White, Viscous Fluid Flow,
+    eq. 3-48): u = 16 a^2 g / (nu pi^3) * S with
+    S(y, z) = sum over odd k of (-1)^((k-1)/2) / k^3
+              [1 - cosh(k pi z / 2a) / cosh(k pi / 2)] cos(k pi y / 2a),
+    y, z measured from the axis; returns S on the n x n nodes (or on `points`
+    per axis)."""
+    a = (n - 1) / 2.0
+    y = np.arange(n) - a if points is None else points
+    Y, Z = np.meshgrid(y, y, indexing="ij")
+    S = np.zeros_like(Y, dtype=float)
+    for k in range(1, 2 * terms, 2):
+        S += (-1) ** ((k - 1) // 2) / k ** 3 * (1 - np.cosh(k * np.pi * Z / (2 * a)) / np.cosh(k * np.pi / 2)) \
+            * np.cos(k * np.pi * Y / (2 * a))
+    return S
+
+
+def square_duct_profile(n, terms=99):
+    """The series profile on the nodes normalised to mean 1 over the continuous
+    cross-section (u_max / u_mean = 2.0962 for a square)."""
+    a = (n - 1) / 2.0
+    fine = np.linspace(-a, a, 801)
+    Sf = square_duct_series(n, terms, fine)
+    mean = np.trapezoid(np.trapezoid(Sf, fine, axis=1), fine) / (2 * a) ** 2
+    return square_duct_series(n, terms) / mean

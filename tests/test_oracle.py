@@ -269,7 +269,14 @@ def test_lattice_reader_matches_reference_fixture(tmp_path):
         paths.append(str(p))
     r = L.read_lattice(*paths)
     assert r["dims"] == (30, 30, 30)
-    assert r["node_class"].sum() == 27000
+    cls = r["node_class"]
+    assert (cls != L.SOLID).sum() == 27000
+    # the lid: INLE links leave the top face; its 28 x 28 inner nodes are a
+    # velocity face, its rim (wall and inlet links: corners) is all wall
+    assert (cls == L.VEL_T).sum() == 28 * 28 and (cls[-1, 1:-1, 1:-1] == L.VEL_T).all()
+    assert (cls[:-1] == L.FLUID).all() and (cls[-1, 0] == L.FLUID).all()
+    rim = r["bb_links"][-1, 0, 5]
+    assert bin(int(rim)).count("1") == 9           # an edge of the box: 9 wall links
     assert r["nlinks"] == 26640
     assert list(r["type_counts"][:3]) == [0, 22290, 4350]
     assert list(r["dir_counts"][1:7]) == [900] * 6 and list(r["dir_counts"][7:]) == [1770] * 12

@@ -16,6 +16,13 @@
 // lid, an OUTL link in direction 1 on x = n-1 an east outlet.  A node whose
 // links have different types is a corner: each diagonal whose two axis
 // directions both carry links becomes a wall link.
+//
+// 3D (S/GpuInit.cu:1157-1238, S/include/BcMacros3D.h): per direction a
+// 3-bit type; a node whose links have different types is a corner and all
+// its links become wall links.  Other nodes take the class of the first face
+// (W, E, S, N, B, T order, as S/GpuBoundaries.cu checks them) whose
+// axis-direction link has exactly the INLE (velocity face), OUTL (outlet) or
+// SYMP (symmetry) type, when the node lies on that face of the box.
 #include "lbm.h"
 
 #include <cmath>
@@ -129,7 +136,10 @@ extern "C" lbm_status lbm_read_lattice(const char* node_path, const char* bc_pat
     // 2D: first link type per node, corner flag, 2-bit type per direction
     std::vector<uint8_t> main_t, corner;
     std::vector<uint32_t> dmask;
-    if (!three_d && have_bc) { main_t.assign((size_t)n, 0); corner.assign((size_t)n, 0); dmask.assign((size_t)n, 0); }
+    std::vector<uint64_t> dmask3;
+    if (have_bc) { main_t.assign((size_t)n, 0); corner.assign((size_t)n, 0); }
+    if (!three_d && have_bc) dmask.assign((size_t)n, 0);
+    if (three_d && have_bc) dmask3.assign((size_t)n, 0);
     if (have_bc) {
         const char* b = bct.data();
         const char* be = b + bct.size();
@@ -149,6 +159,11 @@ extern "C" lbm_status lbm_read_lattice(const char* node_path, const char* bc_pat
             if (type == 1 && bb_links) bb_links[at] |= 1u << dir;
             if (!three_d && type >= 1 && type <= 3) {
                 dmask[at] |= (uint32_t)type << (2 * (dir - 1));
+                if (!main_t[at]) main_t[at] = (uint8_t)type;
+                else if (main_t[at] != type) corner[at] = 1;
+            }
+            if (three_d && type >= 1) {
+                dmask3[at] |= (uint64_t)type << (3 * dir);
                 if (!main_t[at]) main_t[at] = (uint8_t)type;
                 else if (main_t[at] != type) corner[at] = 1;
             }
@@ -183,6 +198,32 @@ extern "C" lbm_status lbm_read_lattice(const char* node_path, const char* bc_pat
             if (i == 0 && is(at, 3, 2)) node_class[at] = LBM_VEL_W;
             else if (j == ny - 1 && is(at, 2, 2)) node_class[at] = LBM_VEL_N;
             else if (i == nx - 1 && is(at, 1, 3)) node_class[at] = LBM_PRES_E;
+        }
+    }
+    if (three_d && have_bc) {
+        const int64_t ny = dims[1], nz = dims[2];
+        // faces in the reference's order: axis direction, class of a velocity
+        // face, outlet, symmetry plane, and the face test
+        struct Face { int dir, vel, out, sym, axis, high; };
+        static const Face F[6] = {{2, LBM_VEL_W, LBM_OUT_W, LBM_SYM_W, 0, 0}, {1, LBM_VEL_E, LBM_PRES_E, LBM_SYM_E, 0, 1},
+                                  {4, LBM_VEL_S, LBM_OUT_S, LBM_SYM_S, 1, 0}, {3, LBM_VEL_N, LBM_OUT_N, LBM_SYM_N, 1, 1},
+                                  {6, LBM_VEL_B, LBM_OUT_B, LBM_SYM_B, 2, 0}, {5, LBM_VEL_T, LBM_OUT_T, LBM_SYM_T, 2, 1}};
+        for (int64_t at = 0; at < n; ++at) {
+            if (!main_t[at] || node_class[at] == LBM_SOLID) continue;
+            if (corner[at]) {
+                if (bb_links)
+                    for (int d = 1; d < 19; ++d)
+                        if ((dmask3[at] >> (3 * d)) & 7u) bb_links[at] |= 1u << d;
+                continue;
+            }
+            const int64_t xyz[3] = {at % nx, (at / nx) % ny, at / (nx * ny)};
+            const int64_t top[3] = {nx - 1, ny - 1, nz - 1};
+            for (const Face& f : F) {
+                const uint32_t t = (uint32_t)(dmask3[at] >> (3 * f.dir)) & 7u;
+                if (xyz[f.axis] != (f.high ? top[f.axis] : 0)) continue;
+                const int cls = t == 2 ? f.vel : t == 3 ? f.out : t == 5 ? f.sym : 0;
+                if (cls) { node_class[at] = (uint8_t)cls; break; }
+            }
         }
     }
     if (nlinks) *nlinks = cnt;

@@ -44,6 +44,7 @@ struct lbm_ctx {
     double *up = nullptr, *vp = nullptr, *wp = nullptr;
     double *ubc = nullptr, *vbc = nullptr, *wbc = nullptr;
     double* fout = nullptr;
+    double* om[2] = {nullptr, nullptr};   // f_eq outlet: macros of even / odd steps (4 N each)
     double* partials = nullptr;
     double* out4 = nullptr;
     int64_t bytes = 0;
@@ -84,7 +85,19 @@ Fields fields(lbm_ctx* c) {
     f.rho = c->rho; f.u = c->u; f.v = c->v; f.w = c->w;
     f.ubc = c->ubc; f.vbc = c->vbc; f.wbc = c->wbc;
     f.fout = c->fout;
+    f.om_in = nullptr;
+    f.om_out = nullptr;
     return f;
+}
+
+// The f_eq outlet of step s reads the macros of step s - 1 (slot (s-1) & 1)
+// and writes those of step s (slot s & 1); step 0 is the initial state.
+lbm_status seed_outlet_macros(lbm_ctx* c) {
+    if (!c->om[0]) return LBM_OK;
+    const double* src[4] = {c->rho, c->u, c->v, c->w};
+    for (int a = 0; a < 4; ++a)
+        HIPCHK(c, hipMemcpyAsync(c->om[0] + a * c->n, src[a], 8 * c->n, hipMemcpyDeviceToDevice, c->stream));
+    return LBM_OK;
 }
 
 size_t storage_bytes(int s) { return s == LBM_STORE_F64 ? 8 : s == LBM_STORE_F32 ? 4 : 2; }
@@ -114,8 +127,14 @@ lbm_status run_steps(lbm_ctx* c, int64_t nsteps) {
         uint64_t z = (uint64_t)(c->steps + k + 1) * 0x9E3779B97F4A7C15ull;
         z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
         c->kp.seed = (uint32_t)(z ^ (z >> 31));
+        Fields fl = fields(c);
+        if (c->om[0]) {
+            const int64_t s = c->steps + k + 1;
+            fl.om_in = c->om[(s - 1) & 1];
+            fl.om_out = c->om[s & 1];
+        }
         hipError_t e = launch_step(c->q, c->p.storage, c->p.collision, c->open, c->kp, c->P[c->cur],
-                                   c->P[c->cur ^ 1], c->node, c->cidx, fields(c), wm, c->stream);
+                                   c->P[c->cur ^ 1], c->node, c->cidx, fl, wm, c->stream);
         if (e != hipSuccess) return fail(c, LBM_EDEVICE, "step launch: %s", hipGetErrorString(e));
         c->cur ^= 1;
     }
@@ -155,29 +174,33 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     if (p->collision < 0 || p->collision > 2) return reject("bad collision model");
     if (p->storage < 0 || p->storage > 3) return reject("bad storage");
     if (p->force_model < 0 || p->force_model > 2) return reject("bad force model");
-    if (p->wall_model < 0 || p->wall_model > 1) return reject("bad wall model");
+    if (p->wall_model < 0 || p->wall_model > 2) return reject("bad wall model");
+    if (p->wall_model == LBM_WALL_HH && p->q != 19) return reject("wet-node walls are D3Q19 only");
     if (p->inlet_model < 0 || p->inlet_model > 1) return reject("bad inlet model");
-    if (p->outlet_model < 0 || p->outlet_model > 3) return reject("bad outlet model");
+    if (p->outlet_model < 0 || p->outlet_model > 4) return reject("bad outlet model");
     const size_t n = (size_t)p->nx * p->ny * nz;
     if (n > (size_t)1 << 33) return reject("lattice too large");
 
     // node words and open-boundary checks
     std::vector<uint32_t> node(n);
-    bool any_open_cls = false, any_outlet = false;
+    bool any_open_cls = false, any_outlet = false, any_face3 = false;
     const uint32_t mask = p->q == 9 ? 0x1FEu : 0x7FFFEu;
     for (size_t i = 0; i < n; ++i) {
         const uint32_t cls = lat->node_class[i];
-        if (cls > LBM_VEL_N) return reject("unknown node class");
-        if (cls == LBM_VEL_W || cls == LBM_PRES_E || cls == LBM_VEL_N) {
-            if (p->q != 9) return reject("velocity/pressure faces are D2Q9 only");
-            const int x = (int)(i % p->nx), y = (int)(i / p->nx);
-            if (cls == LBM_VEL_W && x != 0) return reject("velocity inlet nodes must lie on x = 0");
-            if (cls == LBM_PRES_E && x != p->nx - 1) return reject("outlet nodes must lie on x = nx-1");
-            if (cls == LBM_VEL_N && y != p->ny - 1) return reject("lid nodes must lie on y = ny-1");
-            if (cls != LBM_VEL_N && p->periodic[0]) return reject("open x faces need a non-periodic x axis");
-            if (cls == LBM_VEL_N && p->periodic[1]) return reject("a lid needs a non-periodic y axis");
+        if (cls > LBM_SYM_T) return reject("unknown node class");
+        if (cls >= LBM_VEL_W) {
+            // face of the class: axis and side (0 low, 1 high)
+            static const int8_t AX[20] = {0, 0, 0, 0, 1, 0, 1, 2, 2, 0, 1, 1, 2, 2, 0, 0, 1, 1, 2, 2};
+            static const int8_t HI[20] = {0, 0, 0, 1, 1, 1, 0, 0, 1, 0, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1};
+            if (p->q == 9 && cls > LBM_VEL_N) return reject("3D face classes need D3Q19");
+            const int xyz[3] = {(int)(i % p->nx), (int)((i / p->nx) % p->ny), (int)(i / ((size_t)p->nx * p->ny))};
+            const int dims[3] = {p->nx, p->ny, nz};
+            const int a = AX[cls];
+            if (xyz[a] != (HI[cls] ? dims[a] - 1 : 0)) return reject("boundary nodes must lie on their face of the box");
+            if (p->periodic[a]) return reject("a face with boundary nodes needs a non-periodic axis");
             any_open_cls = true;
-            any_outlet |= cls == LBM_PRES_E;
+            any_outlet |= cls == LBM_PRES_E || (cls >= LBM_OUT_W && cls <= LBM_OUT_T);
+            any_face3 |= p->q == 19;
         }
         const uint32_t bb = lat->bb_links ? (lat->bb_links[i] & mask) : 0u;
         node[i] = (cls << kClsShift) | bb;
@@ -217,6 +240,38 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
             }
             node[i] |= derived;
             if ((user & ~derived) != 0 || lat->node_class[i] != LBM_FLUID) simple = false;
+        }
+    }
+    if (any_outlet && p->q == 19 && p->outlet_model == LBM_OUTLET_REF_VELOCITY)
+        return reject("the reference velocity outlet is 2D; 3D outlets are pressure, extrapolation or f_eq");
+    if (any_outlet && p->q == 9 && p->outlet_model == LBM_OUTLET_FEQ) return reject("the f_eq outlet is 3D");
+    if (p->wall_model == LBM_WALL_HH) {
+        // Wet-node closure id from the wall links: 5 links with one axis
+        // normal a face, 9 with two an edge, 12 with three a corner (the
+        // inward normal along an axis is -1 when the +axis link is a wall).
+        static const int AXD[3][2] = {{2, 1}, {4, 3}, {6, 5}};   // (-axis, +axis) directions
+        for (size_t i = 0; i < n; ++i) {
+            const uint32_t bb = node[i] & kBBMask;
+            if (!bb || lat->node_class[i] == LBM_SOLID) continue;
+            int g[3] = {0, 0, 0}, nn = 0;
+            for (int a = 0; a < 3; ++a) {
+                if (bb >> AXD[a][1] & 1u) g[a] = -1;
+                else if (bb >> AXD[a][0] & 1u) g[a] = 1;
+                nn += g[a] != 0;
+            }
+            const int cnt = __builtin_popcount(bb);
+            uint32_t id = 0;
+            if (cnt == 5 && nn == 1) {
+                for (int a = 0; a < 3; ++a)
+                    if (g[a]) id = 1 + 2 * a + (g[a] < 0);
+            } else if (cnt == 9 && nn == 2) {
+                const int A = g[0] ? 0 : 1, B = g[2] ? 2 : 1;
+                const int pair = (A == 0 && B == 1) ? 0 : (A == 0 ? 1 : 2);
+                id = 7 + 4 * pair + 2 * (g[A] < 0) + (g[B] < 0);
+            } else if (cnt == 12 && nn == 3) {
+                id = 19 + 4 * (g[0] < 0) + 2 * (g[1] < 0) + (g[2] < 0);
+            }
+            node[i] |= id << kHHShift;
         }
     }
     if (n >= 0xFFFFFFFFu) return reject("lattice too large for 32-bit node indices");
@@ -276,7 +331,8 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
         bool need = false;
         for (size_t i = 0; i < n; ++i) {
             const uint8_t cl = lat->node_class[i];
-            need |= cl == LBM_VEL_W || cl == LBM_VEL_N || (cl == LBM_PRES_E && p->outlet_model == LBM_OUTLET_REF_VELOCITY);
+            need |= cl == LBM_VEL_W || cl == LBM_VEL_N || (cl >= LBM_VEL_E && cl <= LBM_VEL_T) ||
+                    (cl == LBM_PRES_E && p->outlet_model == LBM_OUTLET_REF_VELOCITY);
         }
         if (need) return reject("velocity inlet, lid and velocity outlet nodes need u_bc");
     }
@@ -366,6 +422,8 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
         if (st == LBM_OK) st = dalloc(c, (void**)&c->vbc, sizeof(double) * n);
         if (st == LBM_OK) st = dalloc(c, (void**)&c->wbc, sizeof(double) * n);
     }
+    if (st == LBM_OK && any_outlet && p->q == 19 && p->outlet_model == LBM_OUTLET_FEQ)
+        for (int b = 0; b < 2 && st == LBM_OK; ++b) st = dalloc(c, (void**)&c->om[b], sizeof(double) * 4 * n);
     if (st == LBM_OK) st = dalloc(c, (void**)&c->partials, sizeof(double) * 4 * reduce_blocks());
     if (st == LBM_OK) st = dalloc(c, (void**)&c->out4, sizeof(double) * 4);
     if (st != LBM_OK) {
@@ -381,8 +439,10 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     k.rblock = (int)rblock;
     k.rmax = rmax;
     k.simple = simple ? 1 : 0;
-    // the rank step addresses the tile layout only, and has no extrapolating outlet
-    const bool rank_fits = rank_ok && k.tile && !(any_outlet && p->outlet_model >= LBM_OUTLET_EXTRAP2);
+    // the rank step addresses the tile layout only, and has no extrapolating
+    // outlet, 3D face or wet-node wall
+    const bool rank_fits = rank_ok && k.tile && !(any_outlet && p->outlet_model >= LBM_OUTLET_EXTRAP2) &&
+                           !any_face3 && p->wall_model != LBM_WALL_HH;
     k.rank = rank_fits ? 1 : 0;
     if (const char* e = std::getenv("LBM_RANK")) k.rank = rank_fits && e[0] == '1';
     k.sr = 1;
@@ -416,7 +476,7 @@ void lbm_destroy(lbm_ctx* c) {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
     void* bufs[] = {c->Palloc[0], c->Palloc[1], c->node, c->cidx, c->didx, c->nodew, c->rowmeta, c->blk, c->rho, c->u, c->v, c->w, c->up, c->vp, c->wp,
-                    c->ubc, c->vbc, c->wbc, c->fout, c->partials, c->out4};
+                    c->ubc, c->vbc, c->wbc, c->fout, c->om[0], c->om[1], c->partials, c->out4};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -442,6 +502,8 @@ lbm_status lbm_init(lbm_ctx* c, int mode, const double* rho, const double* u, co
                           c->P[0], c->node, c->cidx, fields(c), c->stream));
     c->cur = 0;
     c->steps = 0;
+    lbm_status s = seed_outlet_macros(c);
+    if (s != LBM_OK) return s;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return LBM_OK;
 }
@@ -510,7 +572,9 @@ lbm_status lbm_get_f(lbm_ctx* c, double* f) {
     if (c->steps < 1) return fail(c, LBM_EINVAL, "post-boundary populations exist after the first step");
     lbm_status s = ensure_fout(c);
     if (s != LBM_OK) return s;
-    HIPCHK(c, launch_recon(c->q, c->p.storage, c->open, c->kp, c->P[c->cur ^ 1], c->node, c->cidx, fields(c), c->stream));
+    Fields fl = fields(c);
+    if (c->om[0]) fl.om_in = c->om[(c->steps - 1) & 1];
+    HIPCHK(c, launch_recon(c->q, c->p.storage, c->open, c->kp, c->P[c->cur ^ 1], c->node, c->cidx, fl, c->stream));
     HIPCHK(c, hipMemcpyAsync(f, c->fout, 8 * c->n * c->q, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return LBM_OK;
@@ -525,6 +589,8 @@ lbm_status lbm_set_f(lbm_ctx* c, const double* f) {
                           c->stream));
     c->cur = 0;
     c->steps = 0;
+    s = seed_outlet_macros(c);
+    if (s != LBM_OK) return s;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return LBM_OK;
 }

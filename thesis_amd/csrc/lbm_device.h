@@ -340,9 +340,223 @@ __device__ inline void onnode_walls(uint32_t nd, T* h) {
         if ((nd >> d) & 1u) h[L::opp(d)] = h[d];
 }
 
+// ---------------------------------------------------------------------------
+// D3Q19 faces.  A face is (axis A, inward sign G): W (0,+1), E (0,-1),
+// S (1,+1), N (1,-1), B (2,+1), T (2,-1); its unknowns are the populations
+// with G c_A > 0.  Every closure below is instantiated per face, edge or
+// corner, so all population indices are compile-time constants and the
+// node's populations stay in registers.
+template <class L> __host__ __device__ constexpr int cmp(int i, int a) {
+    return a == 0 ? L::cx(i) : a == 1 ? L::cy(i) : L::cz(i);
+}
+template <class L> __host__ __device__ constexpr int dir_of(int x, int y, int z) {
+    for (int i = 0; i < L::Q; ++i)
+        if (L::cx(i) == x && L::cy(i) == y && L::cz(i) == z) return i;
+    return -1;
+}
+// direction with c_a = ca and c_b = cb (a == b: the axis direction ca)
+template <class L> __host__ __device__ constexpr int dir_ax(int a, int ca, int b, int cb) {
+    return dir_of<L>((a == 0 ? ca : 0) + (b == 0 ? cb : 0), (a == 1 ? ca : 0) + (b == 1 ? cb : 0),
+                     (a == 2 ? ca : 0) + (b == 2 ? cb : 0));
+}
+static_assert(dir_ax<D3Q19>(0, 1, 1, 1) == 7 && dir_ax<D3Q19>(1, -1, 2, -1) == 18 && dir_ax<D3Q19>(2, 1, 2, 0) == 5,
+              "D3Q19 direction lookup");
+
+// Transverse part of a velocity face for one tangential axis B (Hecht &
+// Harting 2010): N = sum over c_A = 0 of c_B f / 2 - rho U_B / 3 and the two
+// diagonal unknowns f(G e_A + t e_B) = f(-G e_A - t e_B) + rho (G U_A + t U_B)/6 - t N.
+template <class L, class T, int A, int G, int B>
+__device__ inline void vface_b(T* f, const T* U, T rho) {
+    T N = 0;
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i)
+        if (cmp<L>(i, A) == 0 && cmp<L>(i, B) != 0) N += cmp<L>(i, B) > 0 ? f[i] : -f[i];
+    N = (T)0.5 * N - rho * U[B] * (T)(1.0 / 3.0);
+    constexpr int p = dir_ax<L>(A, G, B, 1), pm = dir_ax<L>(A, -G, B, -1);
+    constexpr int m = dir_ax<L>(A, G, B, -1), mm = dir_ax<L>(A, -G, B, 1);
+    f[p] = f[pm] + rho * ((T)G * U[A] + U[B]) * (T)(1.0 / 6.0) - N;
+    f[m] = f[mm] + rho * ((T)G * U[A] - U[B]) * (T)(1.0 / 6.0) + N;
+}
+// Velocity face (S/GpuBoundaries.cu:73-399) on full populations f: rho from
+// the tangential and outgoing populations over 1 - G U_A (1 + G U_A with the
+// reference's west/east flip), the normal unknown by non-equilibrium
+// bounce-back, the diagonal unknowns by vface_b.
+template <class L, class T, int A, int G>
+__device__ inline void vface(T* f, const T* U, bool flip) {
+    T K = 0;
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) {
+        if (cmp<L>(i, A) == 0) K += f[i];
+        else if (cmp<L>(i, A) == -G) K += (T)2 * f[i];
+    }
+    const T rho = K / (flip ? (T)1 + (T)G * U[A] : (T)1 - (T)G * U[A]);
+    constexpr int nin = dir_ax<L>(A, G, A, 0), nout = dir_ax<L>(A, -G, A, 0);
+    f[nin] = f[nout] + (T)G * rho * U[A] * (T)(1.0 / 3.0);
+    vface_b<L, T, A, G, (A + 1) % 3>(f, U, rho);
+    vface_b<L, T, A, G, (A + 2) % 3>(f, U, rho);
+}
+// Wet-node wall edge with inward normals (A, GA), (B, GB) (gpuBcComplexWall3D,
+// S/GpuBoundaries.cu:863-1365): bounce-back of the normal and in-plane
+// unknowns, N = -(f(+t) - f(-t))/4 on the four unknowns with a tangential
+// component, the two buried diagonals S/22 each and the rest population
+// 12 S/22 with S the sum of the other 16.
+template <class L, class T, int A, int GA, int B, int GB>
+__device__ inline void hh_edge(T* f) {
+    constexpr int C = 3 - A - B;
+    constexpr int na = dir_ax<L>(A, GA, A, 0), na_ = dir_ax<L>(A, -GA, A, 0);
+    constexpr int nb = dir_ax<L>(B, GB, B, 0), nb_ = dir_ax<L>(B, -GB, B, 0);
+    constexpr int ab = dir_ax<L>(A, GA, B, GB), ab_ = dir_ax<L>(A, -GA, B, -GB);
+    constexpr int tp = dir_ax<L>(C, 1, C, 0), tm = dir_ax<L>(C, -1, C, 0);
+    constexpr int ap = dir_ax<L>(A, GA, C, 1), ap_ = dir_ax<L>(A, -GA, C, -1);
+    constexpr int am = dir_ax<L>(A, GA, C, -1), am_ = dir_ax<L>(A, -GA, C, 1);
+    constexpr int bp = dir_ax<L>(B, GB, C, 1), bp_ = dir_ax<L>(B, -GB, C, -1);
+    constexpr int bm = dir_ax<L>(B, GB, C, -1), bm_ = dir_ax<L>(B, -GB, C, 1);
+    constexpr int q1 = dir_ax<L>(A, GA, B, -GB), q2 = dir_ax<L>(A, -GA, B, GB);
+    f[na] = f[na_];
+    f[nb] = f[nb_];
+    f[ab] = f[ab_];
+    const T N = (T)-0.25 * (f[tp] - f[tm]);
+    f[ap] = f[ap_] + N;
+    f[am] = f[am_] - N;
+    f[bp] = f[bp_] + N;
+    f[bm] = f[bm_] - N;
+    T S = 0;
+#pragma unroll
+    for (int i = 1; i < L::Q; ++i)
+        if (i != q1 && i != q2) S += f[i];
+    f[q1] = S * (T)(1.0 / 22.0);
+    f[q2] = f[q1];
+    f[0] = S * (T)(12.0 / 22.0);
+}
+// Wet-node wall corner (S/GpuBoundaries.cu:1366-1560): bounce-back of the
+// three normal and three in-plane unknowns, the six buried diagonals
+// C = (sum of the 12 others)/18, rest 12 C.
+template <class L, class T, int GX, int GY, int GZ>
+__device__ inline void hh_corner(T* f) {
+    constexpr int x = dir_ax<L>(0, GX, 0, 0), x_ = dir_ax<L>(0, -GX, 0, 0);
+    constexpr int y = dir_ax<L>(1, GY, 1, 0), y_ = dir_ax<L>(1, -GY, 1, 0);
+    constexpr int z = dir_ax<L>(2, GZ, 2, 0), z_ = dir_ax<L>(2, -GZ, 2, 0);
+    constexpr int xy = dir_ax<L>(0, GX, 1, GY), xy_ = dir_ax<L>(0, -GX, 1, -GY);
+    constexpr int xz = dir_ax<L>(0, GX, 2, GZ), xz_ = dir_ax<L>(0, -GX, 2, -GZ);
+    constexpr int yz = dir_ax<L>(1, GY, 2, GZ), yz_ = dir_ax<L>(1, -GY, 2, -GZ);
+    constexpr int b0 = dir_ax<L>(0, GX, 1, -GY), b1 = dir_ax<L>(0, -GX, 1, GY);
+    constexpr int b2 = dir_ax<L>(0, GX, 2, -GZ), b3 = dir_ax<L>(0, -GX, 2, GZ);
+    constexpr int b4 = dir_ax<L>(1, GY, 2, -GZ), b5 = dir_ax<L>(1, -GY, 2, GZ);
+    f[x] = f[x_]; f[y] = f[y_]; f[z] = f[z_];
+    f[xy] = f[xy_]; f[xz] = f[xz_]; f[yz] = f[yz_];
+    T S = 0;
+#pragma unroll
+    for (int i = 1; i < L::Q; ++i)
+        if (i != b0 && i != b1 && i != b2 && i != b3 && i != b4 && i != b5) S += f[i];
+    const T c = S * (T)(1.0 / 18.0);
+    f[b0] = c; f[b1] = c; f[b2] = c; f[b3] = c; f[b4] = c; f[b5] = c;
+    f[0] = S * (T)(12.0 / 18.0);
+}
+// Wet-node wall closure of a node word (bits kHHShift.. hold the id the host
+// derived from the wall links: 1-6 face W,E,S,N,B,T; 7-18 edge, pair
+// (x,y),(x,z),(y,z) times (GA<0)*2 + (GB<0); 19-26 corner, (GX<0)*4 +
+// (GY<0)*2 + (GZ<0)).  Deviations in and out.
+template <class L, class T>
+__device__ inline void hh_walls(uint32_t nd, T* h) {
+    if (L::Q != 19) return;
+    const uint32_t id = (nd >> kHHShift) & 31u;
+    if (!id) return;
+    T f[L::Q];
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) f[i] = h[i] + (T)L::w(i);
+    const T Z[3] = {0, 0, 0};
+    switch (id) {
+    case 1: vface<L, T, 0, 1>(f, Z, false); break;
+    case 2: vface<L, T, 0, -1>(f, Z, false); break;
+    case 3: vface<L, T, 1, 1>(f, Z, false); break;
+    case 4: vface<L, T, 1, -1>(f, Z, false); break;
+    case 5: vface<L, T, 2, 1>(f, Z, false); break;
+    case 6: vface<L, T, 2, -1>(f, Z, false); break;
+    case 7: hh_edge<L, T, 0, 1, 1, 1>(f); break;
+    case 8: hh_edge<L, T, 0, 1, 1, -1>(f); break;
+    case 9: hh_edge<L, T, 0, -1, 1, 1>(f); break;
+    case 10: hh_edge<L, T, 0, -1, 1, -1>(f); break;
+    case 11: hh_edge<L, T, 0, 1, 2, 1>(f); break;
+    case 12: hh_edge<L, T, 0, 1, 2, -1>(f); break;
+    case 13: hh_edge<L, T, 0, -1, 2, 1>(f); break;
+    case 14: hh_edge<L, T, 0, -1, 2, -1>(f); break;
+    case 15: hh_edge<L, T, 1, 1, 2, 1>(f); break;
+    case 16: hh_edge<L, T, 1, 1, 2, -1>(f); break;
+    case 17: hh_edge<L, T, 1, -1, 2, 1>(f); break;
+    case 18: hh_edge<L, T, 1, -1, 2, -1>(f); break;
+    case 19: hh_corner<L, T, 1, 1, 1>(f); break;
+    case 20: hh_corner<L, T, 1, 1, -1>(f); break;
+    case 21: hh_corner<L, T, 1, -1, 1>(f); break;
+    case 22: hh_corner<L, T, 1, -1, -1>(f); break;
+    case 23: hh_corner<L, T, -1, 1, 1>(f); break;
+    case 24: hh_corner<L, T, -1, 1, -1>(f); break;
+    case 25: hh_corner<L, T, -1, -1, 1>(f); break;
+    case 26: hh_corner<L, T, -1, -1, -1>(f); break;
+    default: break;
+    }
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) h[i] = f[i] - (T)L::w(i);
+}
+
+// 3D velocity faces by node class; ubc/vbc/wbc hold the face velocity.
+template <class L, class T>
+__device__ inline void vel3d(const KParams& kp, const Fields& fl, size_t idx, uint32_t cls, T* h) {
+    if (L::Q != 19) return;
+    if (!(cls == VEL_W || cls == VEL_N || (cls >= VEL_E && cls <= VEL_T))) return;
+    T f[L::Q];
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) f[i] = h[i] + (T)L::w(i);
+    const T U[3] = {(T)fl.ubc[idx], (T)fl.vbc[idx], (T)fl.wbc[idx]};
+    const bool flip = (kp.parity & 0x10u) != 0;
+    switch (cls) {
+    case VEL_W: vface<L, T, 0, 1>(f, U, flip); break;
+    case VEL_E: vface<L, T, 0, -1>(f, U, flip); break;
+    case VEL_S: vface<L, T, 1, 1>(f, U, false); break;
+    case VEL_N: vface<L, T, 1, -1>(f, U, false); break;
+    case VEL_B: vface<L, T, 2, 1>(f, U, false); break;
+    default: vface<L, T, 2, -1>(f, U, false); break;
+    }
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) h[i] = f[i] - (T)L::w(i);
+}
+
+// Symmetry planes (S/GpuBoundaries.cu:2086-2167): unknowns take their mirror
+// images in the face (equal weights, so deviations copy directly).
+template <class L, int A, int I> struct Mirror {
+    static constexpr int v = dir_of<L>(A == 0 ? -L::cx(I) : L::cx(I), A == 1 ? -L::cy(I) : L::cy(I),
+                                       A == 2 ? -L::cz(I) : L::cz(I));
+};
+template <class L, class T, int A, int G, int... I>
+__device__ inline void sym_face_seq(T* h, std::integer_sequence<int, I...>) {
+    ((G * cmp<L>(I, A) > 0 ? (void)(h[I] = h[Mirror<L, A, I>::v]) : (void)0), ...);
+}
+template <class L, class T, int A, int G>
+__device__ inline void sym_face(T* h) {
+    sym_face_seq<L, T, A, G>(h, std::make_integer_sequence<int, L::Q>{});
+}
+template <class L, class T>
+__device__ inline void sym3d(uint32_t cls, T* h) {
+    if (L::Q != 19 || cls < SYM_W || cls > SYM_T) return;
+    T f[L::Q];
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) f[i] = h[i];
+    switch (cls) {
+    case SYM_W: sym_face<L, T, 0, 1>(f); break;
+    case SYM_E: sym_face<L, T, 0, -1>(f); break;
+    case SYM_S: sym_face<L, T, 1, 1>(f); break;
+    case SYM_N: sym_face<L, T, 1, -1>(f); break;
+    case SYM_B: sym_face<L, T, 2, 1>(f); break;
+    default: sym_face<L, T, 2, -1>(f); break;
+    }
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) h[i] = f[i];
+}
+
+__device__ inline bool is_outlet3(uint32_t cls) { return cls == PRES_E || (cls >= OUT_W && cls <= OUT_T); }
+
 // Populations of a node after streaming and its inlet and wall conditions,
 // i.e. everything before the outlet (the reference orders inlet, wall,
-// outlet: S/Iterate.cu:468-473).
+// outlet: S/Iterate.cu:468-473, S/Iterate3D.cu:602-621).
 template <class L, int S, bool OPEN>
 __device__ inline void inflow_state(const typename Io<L, S>::raw* __restrict__ src, const KParams& kp,
                                     const Fields& fl, const uint32_t* __restrict__ cidx, uint32_t own,
@@ -352,9 +566,11 @@ __device__ inline void inflow_state(const typename Io<L, S>::raw* __restrict__ s
     source_index<L, OPEN>(cidx, kp, x, y, z, cn);
     gather<L, S, OPEN>(src, kp, own, nd, cn, h);
     const uint32_t cls = nd >> kClsShift;
-    if (OPEN && cls == VEL_W) zou_inlet<L, T>(kp, fl, idx, nd, h);
-    if (OPEN && cls == VEL_N) zou_lid<L, T>(kp, fl, idx, nd, h);
+    if (OPEN && L::Q == 9 && cls == VEL_W) zou_inlet<L, T>(kp, fl, idx, nd, h);
+    if (OPEN && L::Q == 9 && cls == VEL_N) zou_lid<L, T>(kp, fl, idx, nd, h);
+    if (OPEN && L::Q == 19) vel3d<L, T>(kp, fl, idx, cls, h);
     if (kp.wall == 1 && (nd & kBBMask)) onnode_walls<L, T>(nd, h);
+    if (L::Q == 19 && kp.wall == 2) hh_walls<L, T>(nd, h);
 }
 
 // Extrapolating east outlet (S/GpuBoundaries.cu:1609-1640): f1, f5, f8 of an
@@ -382,6 +598,109 @@ __device__ inline void outlet_extrap(const typename Io<L, S>::raw* __restrict__ 
     } else {
         h[1] = ha[1]; h[5] = ha[5]; h[8] = ha[8];
     }
+}
+
+// 3D outlet of face (A, G), model kp.outlet: 0 a textbook pressure face
+// (rho_out, zero tangential velocity, the normal velocity from the known
+// populations; the reference's 3D OUTLET is "not implemented",
+// S/GpuBoundaries.cu:1671-1706); 2/3 the unknowns become 2 f(x+n) - f(x+2n)
+// or f(x+n) of the upstream nodes' pre-outlet populations; 4 f_eq of the
+// node's macros of the previous step (S/GpuBoundaries.cu:1840-1852).  The
+// intended semantics of S/GpuBoundaries.cu:1655-1874: its population stride
+// n*m (for n*m*h) and the downstream read of its second-order north face are
+// not reproduced.  An extrapolating node keeps its streamed values when an
+// upstream node is solid or outside.
+template <class L, int S, bool OPEN, int A, int G>
+__device__ inline void outlet3_face(const KParams& kp, const Fields& fl, size_t idx, typename Io<L, S>::T* h) {
+    using T = typename Io<L, S>::T;
+    if (kp.outlet == 0) {
+        T f[L::Q];
+        T K = 0;
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i) {
+            f[i] = h[i] + (T)L::w(i);
+            if (cmp<L>(i, A) == 0) K += f[i];
+            else if (cmp<L>(i, A) == -G) K += (T)2 * f[i];
+        }
+        T U[3] = {0, 0, 0};
+        U[A] = (T)G * ((T)1 - K / (T)kp.rho_out);
+        vface<L, T, A, G>(f, U, false);
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i) h[i] = f[i] - (T)L::w(i);
+        return;
+    }
+    if (kp.outlet == 4) {
+        if (!fl.om_in) return;
+        const size_t n = (size_t)kp.nx * kp.ny * kp.nz;
+        const T r = (T)fl.om_in[idx];
+        const T u[3] = {(T)fl.om_in[n + idx], (T)fl.om_in[2 * n + idx], (T)fl.om_in[3 * n + idx]};
+        const T uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i)
+            if (G * cmp<L>(i, A) > 0) {
+                const T cu = (T)L::cx(i) * u[0] + (T)L::cy(i) * u[1] + (T)L::cz(i) * u[2];
+                h[i] = (T)L::w(i) * (r * ((T)1 + (T)3 * cu + (T)4.5 * cu * cu - (T)1.5 * uu) - (T)1);
+            }
+        return;
+    }
+}
+template <class L, class T, int A, int G>
+__device__ inline void extrap_mix(T* f, const T* ha, const T* hb, bool second) {
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i)
+        if (G * cmp<L>(i, A) > 0) f[i] = second ? (T)2 * ha[i] - hb[i] : ha[i];
+}
+template <class L, int S, bool OPEN>
+__device__ inline void outlet3d(const typename Io<L, S>::raw* __restrict__ src, const KParams& kp, const Fields& fl,
+                                const uint32_t* __restrict__ cidx, const uint32_t* __restrict__ nodew, bool compact,
+                                int x, int y, int z, size_t idx, uint32_t cls, typename Io<L, S>::T* h) {
+    using T = typename Io<L, S>::T;
+    if (L::Q != 19 || !is_outlet3(cls)) return;
+    T f[L::Q];   // a local copy keeps the caller's populations in registers across the switch
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) f[i] = h[i];
+    if (kp.outlet == 2 || kp.outlet == 3) {
+        // Extrapolation: the two upstream nodes along the inward normal are
+        // recomputed once, outside the per-face code.
+        const int A = cls == PRES_E || cls == OUT_W ? 0 : (cls == OUT_S || cls == OUT_N ? 1 : 2);
+        const int G = cls == OUT_W || cls == OUT_S || cls == OUT_B ? 1 : -1;
+        const bool second = kp.outlet == 2;
+        const int dims[3] = {kp.nx, kp.ny, kp.nz};
+        const int pa = A == 0 ? x : A == 1 ? y : z;
+        const int reach = pa + G * (second ? 2 : 1);
+        if (reach < 0 || reach >= dims[A < 1 ? 0 : (A < 2 ? 1 : 2)]) return;
+        const size_t stride = A == 0 ? 1 : A == 1 ? (size_t)kp.nx : (size_t)kp.nx * kp.ny;
+        const size_t i1 = G > 0 ? idx + stride : idx - stride;
+        const size_t i2 = G > 0 ? idx + 2 * stride : idx - 2 * stride;
+        const uint32_t a = cidx[i1];
+        const uint32_t b = second ? cidx[i2] : 0u;
+        if (a == kNone || b == kNone) return;
+        const int dx = A == 0 ? G : 0, dy = A == 1 ? G : 0, dz = A == 2 ? G : 0;
+        T ha[L::Q], hb[L::Q];
+        inflow_state<L, S, OPEN>(src, kp, fl, cidx, a, compact ? nodew[a] : nodew[i1], x + dx, y + dy, z + dz, i1, ha);
+        if (second)
+            inflow_state<L, S, OPEN>(src, kp, fl, cidx, b, compact ? nodew[b] : nodew[i2], x + 2 * dx, y + 2 * dy,
+                                     z + 2 * dz, i2, hb);
+        switch (cls) {
+        case PRES_E: extrap_mix<L, T, 0, -1>(f, ha, hb, second); break;
+        case OUT_W: extrap_mix<L, T, 0, 1>(f, ha, hb, second); break;
+        case OUT_S: extrap_mix<L, T, 1, 1>(f, ha, hb, second); break;
+        case OUT_N: extrap_mix<L, T, 1, -1>(f, ha, hb, second); break;
+        case OUT_B: extrap_mix<L, T, 2, 1>(f, ha, hb, second); break;
+        default: extrap_mix<L, T, 2, -1>(f, ha, hb, second); break;
+        }
+    } else {
+        switch (cls) {
+        case PRES_E: outlet3_face<L, S, OPEN, 0, -1>(kp, fl, idx, f); break;
+        case OUT_W: outlet3_face<L, S, OPEN, 0, 1>(kp, fl, idx, f); break;
+        case OUT_S: outlet3_face<L, S, OPEN, 1, 1>(kp, fl, idx, f); break;
+        case OUT_N: outlet3_face<L, S, OPEN, 1, -1>(kp, fl, idx, f); break;
+        case OUT_B: outlet3_face<L, S, OPEN, 2, 1>(kp, fl, idx, f); break;
+        default: outlet3_face<L, S, OPEN, 2, -1>(kp, fl, idx, f); break;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) h[i] = f[i];
 }
 
 __device__ inline double recip(double x) { return 1.0 / x; }

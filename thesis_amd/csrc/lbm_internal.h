@@ -6,11 +6,18 @@
 
 namespace lbm {
 
-enum Cls : uint32_t { SOLID = 0, FLUID = 1, VEL_W = 2, PRES_E = 3, VEL_N = 4 };
+enum Cls : uint32_t {
+    SOLID = 0, FLUID = 1, VEL_W = 2, PRES_E = 3, VEL_N = 4,
+    VEL_E = 5, VEL_S = 6, VEL_B = 7, VEL_T = 8,
+    OUT_W = 9, OUT_S = 10, OUT_N = 11, OUT_B = 12, OUT_T = 13,
+    SYM_W = 14, SYM_E = 15, SYM_S = 16, SYM_N = 17, SYM_B = 18, SYM_T = 19
+};
 enum Mode : int { INIT_EQ = 0, INIT_ZERO = 1, INIT_FROM_F = 2 };
 
-// Per-node word: bits 0..18 bounce-back links, bits 24..31 node class.
+// Per-node word: bits 0..18 bounce-back links, bits 19..23 the wet-node wall
+// closure id (3D), bits 24..31 node class.
 constexpr uint32_t kBBMask = 0x7FFFFu;
+constexpr int kHHShift = 19;
 constexpr int kClsShift = 24;
 
 // Everything a kernel needs, passed by value as a kernel argument.
@@ -23,9 +30,9 @@ struct KParams {
     double s[19];            // MRT diag(S)
     double g[3];
     int force;               // 0 none, 1 shift, 2 guo
-    int wall;                // 0 halfway, 1 on-node
+    int wall;                // 0 halfway, 1 on-node, 2 wet-node (3D)
     int inlet;               // 0 zou-he, 1 reference 2D (west inlet and north lid)
-    int outlet;              // 0 zou-he pressure, 1 reference velocity, 2/3 extrapolation
+    int outlet;              // 0 pressure, 1 reference velocity (2D), 2/3 extrapolation, 4 f_eq (3D)
     uint32_t parity;
     double rho_out;
     uint32_t nf;             // non-solid nodes (compact population count)
@@ -51,6 +58,10 @@ struct Fields {              // FP64 side arrays (N each)
     double *rho, *u, *v, *w; // macroscopic output / input
     const double *ubc, *vbc, *wbc;
     double* fout;            // Q*N full populations (reconstruct) or input (INIT_FROM_F)
+    // f_eq outlet: the macros of the previous step (rho, u, v, w blocks of N)
+    // read by this step, and the block this step writes (null: none)
+    const double* om_in;
+    double* om_out;
 };
 
 // Launchers; q in {9,19}; storage 0..3; collision 0..2.

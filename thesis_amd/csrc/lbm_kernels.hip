@@ -77,15 +77,24 @@ __global__ __launch_bounds__(256) void k_step(const typename Io<L, S>::raw* __re
     const uint32_t cls = nd >> kClsShift;
     T h[L::Q];
     inflow_state<L, S, OPEN>(src, kp, fl, cidx, own, nd, x, y, z, idx, h);
-    if (OPEN && cls == PRES_E) {
+    if (OPEN && L::Q == 9 && cls == PRES_E) {
         zou_outlet<L, T>(kp, fl, idx, h);
         outlet_extrap<L, S, OPEN>(src, kp, fl, cidx, nodew, true, x, y, z, idx, h);
+    }
+    if (OPEN && L::Q == 19) {
+        if (is_outlet3(cls)) outlet3d<L, S, OPEN>(src, kp, fl, cidx, nodew, true, x, y, z, idx, cls, h);
+        sym3d<L, T>(cls, h);
     }
     T drho, rho, u[3];
     moments<L, T>(kp, cls, h, drho, rho, u);
     if (write_macros) {
         fl.rho[idx] = (double)rho; fl.u[idx] = (double)u[0];
         fl.v[idx] = (double)u[1]; fl.w[idx] = (double)u[2];
+    }
+    if (OPEN && L::Q == 19 && fl.om_out && is_outlet3(cls)) {
+        const size_t n = (size_t)kp.nx * kp.ny * kp.nz;
+        fl.om_out[idx] = (double)rho; fl.om_out[n + idx] = (double)u[0];
+        fl.om_out[2 * n + idx] = (double)u[1]; fl.om_out[3 * n + idx] = (double)u[2];
     }
     T out[L::Q];
     collide<L, T, C>(kp, h, drho, rho, u, out);
@@ -113,7 +122,8 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
     using T = typename Io<L, S>::T;
     extern __shared__ uint2 smeta[];
     const int W = kp.words, lg = kp.lgw, nx = kp.nx, ny = kp.ny, nz = kp.nz;
-    const uint2 bi = blk[blockIdx.x];
+    const uint32_t bid = xcd_block(blockIdx.x, gridDim.x, kp.nxcd);
+    const uint2 bi = blk[bid];
     const uint32_t r0 = bi.x;
     const int R = (int)bi.y;
     // Stage the words of rows r0..r0+R-1 and of their (dy, dz) neighbours at
@@ -140,7 +150,7 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
     // Position of every node of this workgroup: the threads owning a staged own
     // row word walk its fluid bits and write (row, x) at the node's slot, so no
     // thread searches for its own node.
-    const uint32_t own0 = blockIdx.x * blockDim.x;
+    const uint32_t own0 = bid * blockDim.x;
     uint32_t* pos = (uint32_t*)(smeta + ((kp.rmax * 9) << lg));
     for (int t = threadIdx.x; t < (R << lg); t += blockDim.x) {
         const int k = t >> lg, w = t & ((1 << lg) - 1);
@@ -197,11 +207,12 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
         }
         h[i] = Io<L, S>::ld(src, tslot<L>(pl, j), i);
     }
-    // the extrapolating outlets need upstream nodes: the host keeps them off this kernel
-    if (OPEN && cls == VEL_W) zou_inlet<L, T>(kp, fl, idx, nd, h);
-    if (OPEN && cls == VEL_N) zou_lid<L, T>(kp, fl, idx, nd, h);
+    // 2D only when open: the host keeps extrapolating outlets and every 3D
+    // face or wet-node wall on k_step
+    if (OPEN && L::Q == 9 && cls == VEL_W) zou_inlet<L, T>(kp, fl, idx, nd, h);
+    if (OPEN && L::Q == 9 && cls == VEL_N) zou_lid<L, T>(kp, fl, idx, nd, h);
     if (kp.wall == 1 && (nd & kBBMask)) onnode_walls<L, T>(nd, h);
-    if (OPEN && cls == PRES_E) zou_outlet<L, T>(kp, fl, idx, h);
+    if (OPEN && L::Q == 9 && cls == PRES_E) zou_outlet<L, T>(kp, fl, idx, h);
     T drho, rho, u[3];
     moments<L, T>(kp, cls, h, drho, rho, u);
     if (write_macros) {
@@ -234,9 +245,13 @@ __global__ __launch_bounds__(256) void k_recon(const typename Io<L, S>::raw* __r
     const uint32_t cls = nd >> kClsShift;
     T h[L::Q];
     inflow_state<L, S, OPEN>(src, kp, fl, cidx, own, nd, x, y, z, idx, h);
-    if (OPEN && cls == PRES_E) {
+    if (OPEN && L::Q == 9 && cls == PRES_E) {
         zou_outlet<L, T>(kp, fl, idx, h);
         outlet_extrap<L, S, OPEN>(src, kp, fl, cidx, node, false, x, y, z, idx, h);
+    }
+    if (OPEN && L::Q == 19) {
+        if (is_outlet3(cls)) outlet3d<L, S, OPEN>(src, kp, fl, cidx, node, false, x, y, z, idx, cls, h);
+        sym3d<L, T>(cls, h);
     }
     T drho, rho, u[3];
     moments<L, T>(kp, cls, h, drho, rho, u);

@@ -15,11 +15,13 @@
 //   storage = f64|f32|f16|bf16    init = zero|equilibrium
 //   walls = onnode|halfway        trt_magic = <Lambda>
 //   force_model = shift|guo       rho_out = <density>
-//   inlet = zouhe|reference       outlet = pressure|velocity|extrap2|extrap1
+//   inlet = zouhe|reference       outlet = pressure|velocity|extrap2|extrap1|feq
 // BCoutlet maps as the reference reads it (S/FilesReading.cu:180-190 with
-// S/include/Arguments.h:22-28): Vout -> velocity outlet, 1stE -> second-order
-// extrapolation, 2ndE -> first-order extrapolation, HEmodel -> the Zou-He
-// pressure outlet (the reference's 2D solver applies no outlet for it).
+// S/include/Arguments.h:22-28): in 2D Vout -> velocity outlet, 1stE ->
+// second-order extrapolation, 2ndE -> first-order extrapolation, HEmodel ->
+// the Zou-He pressure outlet (the reference's 2D solver applies no outlet for
+// it); in 3D Vout and 1stE -> second-order, 2ndE -> first-order, HEmodel ->
+// the f_eq outlet.  BCwallModel = HHmodel gives the 3D wet-node walls.
 #include "lbm.h"
 
 #include <hip/hip_runtime_api.h>
@@ -48,6 +50,7 @@ struct Args {
     std::string residual_model = "L2";
     int boundary_id = 0, update_inlet_outlet = 1;
     int inlet_model = LBM_INLET_ZOUHE;
+    int walls_set = 0;               // [mi355x] walls given: overrides BCwallModel
     double stop[4] = {0, 0, 0, 0};
     int format = 3;                  // 1 CSV, 2 DAT (Tecplot), 3 VTI
     // new knobs
@@ -117,14 +120,17 @@ bool read_ini(const std::string& path, Args& a) {
         a.storage = s == "f32" ? LBM_STORE_F32 : s == "f16" ? LBM_STORE_F16 : s == "bf16" ? LBM_STORE_BF16 : LBM_STORE_F64;
     }
     if (kv.count("init")) a.init_eq = kv["init"] == "equilibrium";
-    if (kv.count("walls")) a.wall_model = kv["walls"] == "halfway" ? LBM_WALL_HALFWAY : LBM_WALL_ONNODE;
+    if (kv.count("walls")) {
+        a.wall_model = kv["walls"] == "halfway" ? LBM_WALL_HALFWAY : LBM_WALL_ONNODE;
+        a.walls_set = 1;
+    }
     if (kv.count("force_model")) a.force_model = kv["force_model"] == "guo" ? LBM_FORCE_GUO : LBM_FORCE_SHIFT;
     if (kv.count("trt_magic")) a.trt_magic = atof(kv["trt_magic"].c_str());
     if (kv.count("rho_out")) a.rho_out = atof(kv["rho_out"].c_str());
     if (kv.count("inlet")) a.inlet_model = kv["inlet"] == "reference" ? LBM_INLET_REF2D : LBM_INLET_ZOUHE;
     if (kv.count("outlet")) {
         const std::string& o = kv["outlet"];
-        a.outlet = o == "velocity" ? 1 : o == "extrap2" ? 2 : o == "extrap1" ? 3 : 4;
+        a.outlet = o == "velocity" ? 1 : o == "extrap2" ? 2 : o == "extrap1" ? 3 : o == "feq" ? 4 : 5;
     }
     return true;
 }
@@ -392,7 +398,6 @@ int main(int argc, char** argv) {
         a.bc_file = "Mesh/" + a.lattice + "_BC.dat";
     }
     if (a.curved) std::fprintf(stderr, "note: CURVED (Bouzidi) walls are not available; straight walls used\n");
-    if (a.hh_walls) std::fprintf(stderr, "note: HHmodel wet-node walls are not available; bounce-back used\n");
 
     int32_t dims[3];
     int64_t nlinks = 0;
@@ -454,6 +459,37 @@ int main(int argc, char** argv) {
                 vbc[i] = 0.0;
             }
     }
+    std::vector<double> rho(NS, a.rho), u(NS, 0.0), v(NS, 0.0), w(NS, 0.0);
+    for (size_t i = 0; i < NS; ++i) { u[i] = scls[i] == LBM_VEL_W ? ubc[i] : a.u; v[i] = a.v; w[i] = a.w; }
+    bool from_file = false;
+    if (a.use_init_file) {
+        std::vector<double> fu, fv, fw, fr;
+        if (read_initial_conditions(a.init_file, N, fu, fv, fw, fr)) {
+            for (int z = 0; z < sim[2]; ++z)
+                for (int y = 0; y < sim[1]; ++y)
+                    for (int x = 0; x < sim[0]; ++x) {
+                        const size_t s = ((size_t)z * sim[1] + y) * sim[0] + x, d = ((size_t)z * dims[1] + y) * dims[0] + x;
+                        u[s] = fu[d]; v[s] = fv[d]; w[s] = fw[d]; rho[s] = fr[d];
+                    }
+            from_file = true;
+            std::printf("Initial conditions loaded from file %s\n", a.init_file.c_str());
+        } else {
+            std::fprintf(stderr, "wrong or missing initial condition file %s: uniform start\n", a.init_file.c_str());
+        }
+    }
+    // 3D velocity faces move at (U, V, W), or at the loaded velocities
+    // (S/Iterate3D.cu:186-205: u1 = the start fields; the parabolic 3D profile
+    // is not available in the reference either)
+    std::vector<double> wbc(NS, 0.0);
+    if (three_d)
+        for (size_t i = 0; i < NS; ++i) {
+            const uint8_t c = scls[i];
+            if (c == LBM_VEL_W || c == LBM_VEL_N || (c >= LBM_VEL_E && c <= LBM_VEL_T)) {
+                ubc[i] = from_file ? u[i] : a.u;
+                vbc[i] = from_file ? v[i] : a.v;
+                wbc[i] = from_file ? w[i] : a.w;
+            }
+        }
     lbm_params p;
     std::memset(&p, 0, sizeof p);
     p.abi_version = LBM_ABI_VERSION;
@@ -466,30 +502,24 @@ int main(int argc, char** argv) {
     p.force_model = a.g != 0 ? a.force_model : LBM_FORCE_NONE;
     p.force[0] = a.g;
     p.storage = a.storage;
-    p.wall_model = a.wall_model;
+    // BCwallModel = HHmodel selects the wet-node walls in 3D (gpuBcComplexWall3D);
+    // the reference's 2D walls are bounce-back whatever the setting
+    p.wall_model = three_d && a.hh_walls && !a.walls_set ? LBM_WALL_HH : a.wall_model;
     p.inlet_model = a.inlet_model;
-    p.outlet_model = a.outlet == 1 ? LBM_OUTLET_REF_VELOCITY : a.outlet == 2 ? LBM_OUTLET_EXTRAP2
-                   : a.outlet == 3 ? LBM_OUTLET_EXTRAP1 : LBM_OUTLET_PRESSURE;
+    if (!three_d) {
+        p.outlet_model = a.outlet == 1 ? LBM_OUTLET_REF_VELOCITY : a.outlet == 2 ? LBM_OUTLET_EXTRAP2
+                       : a.outlet == 3 ? LBM_OUTLET_EXTRAP1 : LBM_OUTLET_PRESSURE;
+    } else {
+        // 3D (S/GpuBoundaries.cu:1666-1852): OUTLET only prints "not implemented,
+        // go for open boundary 2nd order", which is what it gets here; HEmodel
+        // is the f_eq outlet; [mi355x] outlet = pressure the pressure face
+        p.outlet_model = a.outlet == 1 || a.outlet == 2 ? LBM_OUTLET_EXTRAP2 : a.outlet == 3 ? LBM_OUTLET_EXTRAP1
+                       : a.outlet == 5 ? LBM_OUTLET_PRESSURE : LBM_OUTLET_FEQ;
+    }
     p.rho_out = a.rho_out > 0 ? a.rho_out : a.rho;
-    lbm_lattice lat = {scls.data(), sbb.data(), ubc.data(), vbc.data(), nullptr};
+    lbm_lattice lat = {scls.data(), sbb.data(), ubc.data(), vbc.data(), wbc.data()};
     lbm_ctx* ctx = nullptr;
     if (lbm_create(&p, &lat, &ctx) != LBM_OK) { std::fprintf(stderr, "lbm_create failed\n"); return 2; }
-    std::vector<double> rho(NS, a.rho), u(NS, 0.0), v(NS, 0.0), w(NS, 0.0);
-    for (size_t i = 0; i < NS; ++i) { u[i] = scls[i] == LBM_VEL_W ? ubc[i] : a.u; v[i] = a.v; w[i] = a.w; }
-    if (a.use_init_file) {
-        std::vector<double> fu, fv, fw, fr;
-        if (read_initial_conditions(a.init_file, N, fu, fv, fw, fr)) {
-            for (int z = 0; z < sim[2]; ++z)
-                for (int y = 0; y < sim[1]; ++y)
-                    for (int x = 0; x < sim[0]; ++x) {
-                        const size_t s = ((size_t)z * sim[1] + y) * sim[0] + x, d = ((size_t)z * dims[1] + y) * dims[0] + x;
-                        u[s] = fu[d]; v[s] = fv[d]; w[s] = fw[d]; rho[s] = fr[d];
-                    }
-            std::printf("Initial conditions loaded from file %s\n", a.init_file.c_str());
-        } else {
-            std::fprintf(stderr, "wrong or missing initial condition file %s: uniform start\n", a.init_file.c_str());
-        }
-    }
     if (lbm_init(ctx, a.init_eq ? 0 : 1, rho.data(), u.data(), v.data(), w.data()) != LBM_OK) {
         std::fprintf(stderr, "lbm_init: %s\n", lbm_last_error(ctx));
         lbm_destroy(ctx);

@@ -101,6 +101,29 @@ def cavity2d(n, u_lid):
     return dict(node_class=cls, bb=bb, ubc=ubc, vbc=np.zeros((n, n)), periodic=(0, 0, 0))
 
 
+def duct3d(nx, n, u_in):
+    """D3Q19 square duct along x with wet-node walls on the planes y, z = 0
+    and n-1 (the reference's HHmodel layout: every node fluid, the wall
+    nodes carry a wall link on every link leaving the box, so faces have 5,
+    edges 9 and corners 12 of them), a plug velocity inlet (VEL_W, u_in along
+    x) on the inner nodes of x = 0 and an outlet (PRES_E) on the inner nodes of
+    x = nx-1.  The duct side is n - 1."""
+    c = velocities(19)
+    zz, yy, xx = np.mgrid[0:n, 0:n, 0:nx]
+    wall = (yy == 0) | (yy == n - 1) | (zz == 0) | (zz == n - 1)
+    bb = np.zeros((n, n, nx), np.uint32)
+    for d in range(1, 19):
+        sx, sy, sz = xx + c[d, 0], yy + c[d, 1], zz + c[d, 2]
+        out = (sx < 0) | (sx >= nx) | (sy < 0) | (sy >= n) | (sz < 0) | (sz >= n)
+        bb |= np.where(wall & out, np.uint32(1 << d), np.uint32(0))
+    cls = np.full((n, n, nx), FLUID, np.uint8)
+    cls[~wall & (xx == 0)] = VEL_W
+    cls[~wall & (xx == nx - 1)] = PRES_E
+    ubc = np.where(cls == VEL_W, float(u_in), 0.0)
+    zero = np.zeros((n, n, nx))
+    return dict(node_class=cls, bb=bb, ubc=ubc, vbc=zero, wbc=zero.copy(), periodic=(0, 0, 0))
+
+
 def channel_sweep(count=64, seed=7, nx_range=(128, 1024), ny_range=(32, 256), re_range=(10.0, 100.0),
                   obstacles=(1, 5), r_range=(4.0, 20.0), trt_magic=3.0 / 16.0, u_cap=0.1, drho=0.03):
     """Seeded family of D2Q9 channels for a batched parametric sweep.

@@ -16,13 +16,16 @@ LIB_PATH = os.path.join(HERE, "liblbm.so")
 ABI_VERSION = 2
 
 SOLID, FLUID, VEL_W, PRES_E, VEL_N = 0, 1, 2, 3, 4
+VEL_E, VEL_S, VEL_B, VEL_T = 5, 6, 7, 8
+OUT_W, OUT_S, OUT_N, OUT_B, OUT_T = 9, 10, 11, 12, 13
+SYM_W, SYM_E, SYM_S, SYM_N, SYM_B, SYM_T = 14, 15, 16, 17, 18, 19
 BGK, TRT, MRT = 0, 1, 2
 STORE_F64, STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2, 3
 FORCE_NONE, FORCE_SHIFT, FORCE_GUO = 0, 1, 2
-WALL_HALFWAY, WALL_ONNODE = 0, 1
+WALL_HALFWAY, WALL_ONNODE, WALL_HH = 0, 1, 2
 INLET_ZOUHE, INLET_REF2D = 0, 1
-OUTLET_PRESSURE, OUTLET_REF_VELOCITY, OUTLET_EXTRAP2, OUTLET_EXTRAP1 = 0, 1, 2, 3
-PARITY_MRT_RHO2, PARITY_MRT_NO_F18, PARITY_OUTLET_V0 = 0x1, 0x2, 0x8
+OUTLET_PRESSURE, OUTLET_REF_VELOCITY, OUTLET_EXTRAP2, OUTLET_EXTRAP1, OUTLET_FEQ = 0, 1, 2, 3, 4
+PARITY_MRT_RHO2, PARITY_MRT_NO_F18, PARITY_OUTLET_V0, PARITY_INLET3D_FLIP = 0x1, 0x2, 0x8, 0x10
 
 STATUS = {0: "OK", 1: "DIVERGED", 2: "EIO", 3: "EINVAL", 4: "EDEVICE", 5: "ENOMEM", 6: "EUNSUPPORTED"}
 STORAGE_NAMES = {"f64": STORE_F64, "f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16}
