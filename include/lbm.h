@@ -155,6 +155,18 @@ lbm_status lbm_run_to_residual(lbm_ctx* ctx, double tol, int64_t check_every, in
  * u_prev is the field at the previous call (zero before the first). */
 lbm_status lbm_reduce(lbm_ctx* ctx, double out[4]);
 
+/* lbm_reduce plus the reference's MacroDiff residual (S/Iterate3D.cu:696-756,
+ * S/GpuSum.cu:10-23): out[4..7] = max |u - u_prev|, max |v - v_prev|,
+ * max |w - w_prev|, max |rho - rho_prev| over non-solid nodes, the fields of
+ * the previous reduction (either call) as reference. */
+lbm_status lbm_reduce_all(lbm_ctx* ctx, double out[8]);
+
+/* The reference's L2 residual (S/ComputeResiduals.cu:13-27, S/Iterate.cu:551):
+ * sqrt of the sum over non-solid nodes and populations of (f - fColl)^2, with
+ * f the post-boundary populations of the last step and fColl the
+ * post-collision populations that step streamed.  Needs at least one step. */
+lbm_status lbm_residual_l2(lbm_ctx* ctx, double* out);
+
 /* Macroscopic fields of the current state (N doubles each; any may be NULL).
  * The velocity is the stored velocity of the force model in use. */
 lbm_status lbm_get_macros(lbm_ctx* ctx, double* rho, double* u, double* v, double* w);

@@ -781,6 +781,19 @@ void orc_get_macros(const orc_sim* s, double* rho, double* u, double* v, double*
     if (w) memcpy(w, s->w3, b);
 }
 
+/* The reference's L2 residual: sqrt(sum (f - fColl)^2) over all nodes and
+ * populations of the last step (S/ComputeResiduals.cu:13-27; solid nodes hold
+ * zeros in both here). */
+double orc_residual_l2(const orc_sim* s)
+{
+    long double acc = 0;
+    for (size_t k = 0; k < (size_t)s->n * s->Q; ++k) {
+        double d = s->f[k] - s->fcoll[k];
+        acc += (long double)d * d;
+    }
+    return sqrt((double)acc);
+}
+
 double orc_mass(const orc_sim* s)
 {
     long double acc = 0;

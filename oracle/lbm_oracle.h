@@ -96,6 +96,7 @@ void orc_get_f(const orc_sim* s, double* f);         /* post-boundary f (Q*N) */
 void orc_set_f(orc_sim* s, const double* f);         /* also recomputes the stored macros */
 void orc_get_macros(const orc_sim* s, double* rho, double* u, double* v, double* w);
 double orc_mass(const orc_sim* s);                   /* sum of f over non-solid nodes */
+double orc_residual_l2(const orc_sim* s);            /* sqrt(sum (f - fColl)^2) of the last step */
 
 /* Kernels exposed for closed-form tests. */
 void orc_collide_node(const orc_sim* s, const double* f, double rho, const double* uvw, double* out);

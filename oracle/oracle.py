@@ -72,6 +72,8 @@ def lib():
         L.orc_get_macros.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.orc_mass.argtypes = [C.c_void_p]
         L.orc_mass.restype = C.c_double
+        L.orc_residual_l2.argtypes = [C.c_void_p]
+        L.orc_residual_l2.restype = C.c_double
         L.orc_collide_node.argtypes = [C.c_void_p, dp, C.c_double, dp, dp]
         L.orc_open_boundary.argtypes = [C.c_void_p, C.c_long, dp]
         L.orc_wall_node.argtypes = [C.c_void_p, C.c_long, dp]
@@ -181,6 +183,9 @@ class Sim:
 
     def mass(self):
         return lib().orc_mass(self.h)
+
+    def residual_l2(self):
+        return lib().orc_residual_l2(self.h)
 
     def collide_node(self, f, rho, uvw):
         f = _f64(f); u = np.array(list(uvw) + [0.0] * (3 - len(uvw)))

@@ -135,3 +135,44 @@ def test_lbmsolver_restart_from_results_file(tmp_path, fmt, ext):
     r = subprocess.run([CLI, "-f", "bad.ini", "-o", "Run4/"], cwd=tmp_path, capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0 and "uniform start" in r.stderr
+
+
+def test_lbmsolver_macro_diff_stop_condition(tmp_path):
+    """ResidualModel = MaxMacroDiff with per-field thresholds
+    (S/Iterate3D.cu:696-756): lbmsolver stops at the first check where the
+    largest changes of u, v, w and rho since the previous check are all below
+    them, logs every check in MacroDiffs.txt (S/LogWriter.cu:215-221), and the
+    state it writes equals the ABI run of that many steps."""
+    print("case: lbmsolver MaxMacroDiff", flush=True)
+    nx, ny, U, nu = 64, 16, 0.02, 0.1
+    ch = LT.channel2d(nx, ny, 1.5 * U, obstacles=((20, 8, 3),))
+    os.makedirs(tmp_path / "Mesh")
+    LT.write_latticegen2(str(tmp_path / "Mesh" / "chan"), ch["node_class"], ch["bb"])
+    ini = (INI.format(U=U, nu=nu, iters=40000)
+           .replace("MinMacroDiffs(u_v_w_rho)(StopCond) = 0.0 0.0 0.0 0.0",
+                    "MinMacroDiffs(u_v_w_rho)(StopCond) = 1.0e-7 1.0e-7 1.0e-7 1.0e-7")
+           .replace("ResidualModel = L2", "ResidualModel = MaxMacroDiff"))
+    (tmp_path / "SetUpData.ini").write_text(ini)
+    r = subprocess.run([CLI, "-f", "SetUpData.ini"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-300:], r.stderr[-300:])
+    assert r.returncode == 0 and "simulation converged!" in r.stdout
+    lines = (tmp_path / "Results" / "MacroDiffs.txt").read_text().splitlines()
+    import re
+    assert all(re.fullmatch(r"\d+\t\t\d+\.\d{10}\t\d+\.\d{10}\t\d+\.\d{10}\t\d+\.\d{10}", ln) for ln in lines)
+    last = lines[-1].split()
+    steps = int(last[0])
+    assert steps < 40000 and steps % 100 == 0
+    assert all(float(x) < 1e-7 for x in last[1:])
+    assert any(float(x) >= 1e-7 for x in lines[-2].split()[1:])
+    data = np.loadtxt(tmp_path / "Results" / "FinalData.csv", delimiter=",", skiprows=1)
+    lat = L.read_lattice(str(tmp_path / "Mesh" / "chan_Node.dat"), str(tmp_path / "Mesh" / "chan_BC.dat"))
+    cls = lat["node_class"][0]
+    ys = np.arange(ny)
+    ubc = np.zeros((ny, nx))
+    ubc[:, 0] = 6 * U * (ys + 0.5) * (ny - 0.5 - ys) / ny ** 2
+    s = L.Solver(9, (ny, nx), cls, lat["bb_links"][0], collision=L.TRT, omega=1 / (3 * nu + 0.5),
+                 ubc=np.where(cls == L.VEL_W, ubc, 0.0), vbc=np.zeros((ny, nx)), rho_out=1.0)
+    s.init(0, rho=np.ones((ny, nx)), u=np.where(cls == L.VEL_W, ubc, U))
+    s.step(steps)
+    rho, u, v, w = s.macros()
+    assert np.abs(data[:, 3] - u.ravel()).max() < 1e-9

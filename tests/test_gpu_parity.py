@@ -239,3 +239,38 @@ def test_c2_full_size_256_trt_f64_two_steps(oracle):
     g.init(0)
     o.init(0)
     compare(g, o, 2, check_f=False)
+
+
+@pytest.mark.parametrize("case", ["channel2d", "porous3d"])
+def test_reference_residuals_match_oracle(oracle, case):
+    """lbm_residual_l2 equals the oracle's sqrt(sum (f - fColl)^2) to 1e-12,
+    and lbm_reduce_all's largest field changes equal those of the two
+    fields read back around the steps."""
+    print(f"case: residuals {case}", flush=True)
+    if case == "channel2d":
+        ch = channel()
+        shape, q = (16, 32), 9
+        kw = dict(collision=L.TRT, omega=1 / 0.8, ubc=ch["ubc"], vbc=ch["vbc"], rho_out=1.0)
+        cls, bb = ch["node_class"], ch["bb"]
+    else:
+        cls, bb, _ = LT.porous3d(16, seed=5, rmin=2.0, rmax=3.5)
+        shape, q = (16, 16, 16), 19
+        kw = dict(collision=L.TRT, omega=1 / 0.9, force_model=L.FORCE_GUO, g=(1e-4, 0, 0), periodic=(1, 1, 1))
+    g, o = pair(oracle, q, shape, cls, bb, **kw)
+    g.init(0)
+    o.init(0)
+    g.step(30)
+    o.step(30)
+    r = g.residual_l2()
+    ro = o.residual_l2()
+    print(f"L2 {r:.6e} oracle {ro:.6e}", flush=True)
+    assert abs(r - ro) <= 1e-12 * ro
+    g.reduce_all()
+    m0 = g.macros()
+    g.step(7)
+    m1 = g.macros()
+    red = g.reduce_all()
+    fl = cls != L.SOLID
+    want = [np.abs(m1[k] - m0[k])[fl].max() for k in (1, 2, 3, 0)]
+    got = [red["max_du"], red["max_dv"], red["max_dw"], red["max_drho"]]
+    assert np.allclose(got, want, rtol=0, atol=1e-17), (got, want)

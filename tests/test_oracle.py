@@ -325,3 +325,33 @@ def test_cg_gradient_exact_for_linear_field(oracle, ho):
     sl = slice(3, n - 3)
     assert np.abs(gx[sl, sl] - 0.01).max() < 1e-15
     assert np.abs(gy[sl, sl] - 0.02).max() < 1e-15
+
+
+def test_reference_l2_residual_is_the_streaming_difference(oracle):
+    """orc_residual_l2 = sqrt(sum (f - fColl)^2) of the last step
+    (S/ComputeResiduals.cu:13-27), restated here from the node collision and
+    periodic streaming: fColl from collide_node on the state before the step,
+    f from get_f after it."""
+    ny, nx = 5, 6
+    cls = np.ones((ny, nx), np.uint8)
+    rng = np.random.default_rng(4)
+    u = 0.05 * rng.standard_normal((ny, nx))
+    v = 0.05 * rng.standard_normal((ny, nx))
+    rho = 1 + 0.02 * rng.standard_normal((ny, nx))
+    s = oracle.Sim(9, (ny, nx), cls, None, collision=1, omega=1 / 0.8, periodic=(1, 1, 0))
+    s.init(0, rho=rho, u=u, v=v)
+    s.step(2)
+    f0 = s.get_f()[:, 0]
+    r0, u0, v0, _ = (a[0] for a in s.macros())
+    fc = np.zeros_like(f0)
+    for y in range(ny):
+        for x in range(nx):
+            fc[:, y, x] = s.collide_node(f0[:, y, x], r0[y, x], (u0[y, x], v0[y, x]))
+    s.step(1)
+    f1 = s.get_f()[:, 0]
+    c = LT.velocities(9)
+    for i in range(9):                      # streaming: f1_i(x) = fc_i(x - c_i)
+        assert np.allclose(f1[i], np.roll(np.roll(fc[i], c[i, 0], axis=1), c[i, 1], axis=0), atol=1e-15)
+    want = np.sqrt(((f1 - fc) ** 2).sum())
+    assert abs(s.residual_l2() - want) < 1e-14 * want
+

@@ -41,12 +41,12 @@ struct lbm_ctx {
     uint32_t* blk = nullptr;      // per workgroup: first row, rows spanned
     size_t nf = 0;                // non-solid nodes
     double *rho = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
-    double *up = nullptr, *vp = nullptr, *wp = nullptr;
+    double *up = nullptr, *vp = nullptr, *wp = nullptr, *rp = nullptr;
     double *ubc = nullptr, *vbc = nullptr, *wbc = nullptr;
     double* fout = nullptr;
     double* om[2] = {nullptr, nullptr};   // f_eq outlet: macros of even / odd steps (4 N each)
     double* partials = nullptr;
-    double* out4 = nullptr;
+    double* out4 = nullptr;       // 8 doubles: the last reduction
     int64_t bytes = 0;
     std::string err;
 };
@@ -414,7 +414,7 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     if (st == LBM_OK) st = dalloc(c, (void**)&c->nodew, sizeof(uint32_t) * (nf > 0 ? nf : 1));
     if (st == LBM_OK) st = dalloc(c, (void**)&c->rowmeta, sizeof(uint32_t) * rowmeta.size());
     if (st == LBM_OK) st = dalloc(c, (void**)&c->blk, sizeof(uint32_t) * blk.size());
-    double** fl[] = {&c->rho, &c->u, &c->v, &c->w, &c->up, &c->vp, &c->wp};
+    double** fl[] = {&c->rho, &c->u, &c->v, &c->w, &c->up, &c->vp, &c->wp, &c->rp};
     for (double** a : fl)
         if (st == LBM_OK) st = dalloc(c, (void**)a, sizeof(double) * n);
     if (st == LBM_OK && any_open_cls) {
@@ -424,8 +424,8 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     }
     if (st == LBM_OK && any_outlet && p->q == 19 && p->outlet_model == LBM_OUTLET_FEQ)
         for (int b = 0; b < 2 && st == LBM_OK; ++b) st = dalloc(c, (void**)&c->om[b], sizeof(double) * 4 * n);
-    if (st == LBM_OK) st = dalloc(c, (void**)&c->partials, sizeof(double) * 4 * reduce_blocks());
-    if (st == LBM_OK) st = dalloc(c, (void**)&c->out4, sizeof(double) * 4);
+    if (st == LBM_OK) st = dalloc(c, (void**)&c->partials, sizeof(double) * 8 * reduce_blocks());
+    if (st == LBM_OK) st = dalloc(c, (void**)&c->out4, sizeof(double) * 8);
     if (st != LBM_OK) {
         g_create_err = c->err;
         return cleanup(st);
@@ -464,7 +464,7 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
             return cleanup(LBM_EDEVICE);
         if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(LBM_EDEVICE);
     }
-    double* zs[] = {c->up, c->vp, c->wp};
+    double* zs[] = {c->up, c->vp, c->wp, c->rp};
     for (double* a : zs)
         if (hipMemsetAsync(a, 0, 8 * n, c->stream) != hipSuccess) return cleanup(LBM_EDEVICE);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(LBM_EDEVICE);
@@ -476,7 +476,7 @@ void lbm_destroy(lbm_ctx* c) {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
     void* bufs[] = {c->Palloc[0], c->Palloc[1], c->node, c->cidx, c->didx, c->nodew, c->rowmeta, c->blk, c->rho, c->u, c->v, c->w, c->up, c->vp, c->wp,
-                    c->ubc, c->vbc, c->wbc, c->fout, c->om[0], c->om[1], c->partials, c->out4};
+                    c->rp, c->ubc, c->vbc, c->wbc, c->fout, c->om[0], c->om[1], c->partials, c->out4};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -518,13 +518,41 @@ lbm_status lbm_step(lbm_ctx* c, int64_t nsteps) {
     return LBM_OK;
 }
 
-lbm_status lbm_reduce(lbm_ctx* c, double out[4]) {
+lbm_status lbm_reduce_all(lbm_ctx* c, double out[8]) {
     if (!c || !out) return LBM_EINVAL;
     if (c->steps < 0) return fail(c, LBM_EINVAL, "state not initialised");
-    HIPCHK(c, launch_reduce(c->kp, c->node, fields(c), c->up, c->vp, c->wp, c->partials, reduce_blocks(),
+    HIPCHK(c, launch_reduce(c->kp, c->node, fields(c), c->up, c->vp, c->wp, c->rp, c->partials, reduce_blocks(),
                             c->out4, c->stream));
-    HIPCHK(c, hipMemcpyAsync(out, c->out4, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out, c->out4, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+lbm_status lbm_reduce(lbm_ctx* c, double out[4]) {
+    if (!c || !out) return LBM_EINVAL;
+    double r8[8];
+    lbm_status s = lbm_reduce_all(c, r8);
+    if (s != LBM_OK) return s;
+    for (int k = 0; k < 4; ++k) out[k] = r8[k];
+    return LBM_OK;
+}
+
+lbm_status lbm_residual_l2(lbm_ctx* c, double* out) {
+    if (!c || !out) return LBM_EINVAL;
+    if (c->steps < 1) return fail(c, LBM_EINVAL, "the L2 residual needs one step");
+    lbm_status s = ensure_fout(c);
+    if (s != LBM_OK) return s;
+    Fields fl = fields(c);
+    if (c->om[0]) fl.om_in = c->om[(c->steps - 1) & 1];
+    HIPCHK(c, launch_recon(c->q, c->p.storage, c->open, c->kp, c->P[c->cur ^ 1], c->node, c->cidx, fl, c->stream));
+    // the reference's fColl of step s is the collision output that step s
+    // streamed, i.e. the buffer k_recon just read
+    HIPCHK(c, launch_fdiff2(c->q, c->p.storage, c->kp, c->P[c->cur ^ 1], c->cidx, c->fout, c->partials,
+                            reduce_blocks(), c->out4, c->stream));
+    double sum = 0;
+    HIPCHK(c, hipMemcpyAsync(&sum, c->out4, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = std::sqrt(sum);
     return LBM_OK;
 }
 
@@ -532,8 +560,8 @@ lbm_status lbm_run_to_residual(lbm_ctx* c, double tol, int64_t every, int64_t ma
                                double* final_res) {
     if (!c) return LBM_EINVAL;
     if (every < 1 || max_steps < 0) return fail(c, LBM_EINVAL, "bad check interval or step limit");
-    double r4[4];
-    lbm_status s = lbm_reduce(c, r4);   // seeds u_prev with the current field
+    double r4[8];
+    lbm_status s = lbm_reduce_all(c, r4);   // seeds the previous fields with the current ones
     if (s != LBM_OK) return s;
     int64_t done = 0;
     double res = INFINITY;
@@ -541,7 +569,7 @@ lbm_status lbm_run_to_residual(lbm_ctx* c, double tol, int64_t every, int64_t ma
         const int64_t k = (max_steps - done) < every ? (max_steps - done) : every;
         if ((s = run_steps(c, k)) != LBM_OK) return s;
         done += k;
-        if ((s = lbm_reduce(c, r4)) != LBM_OK) return s;
+        if ((s = lbm_reduce_all(c, r4)) != LBM_OK) return s;
         if (r4[3] > 0) {
             if (steps_done) *steps_done = done;
             if (final_res) *final_res = NAN;

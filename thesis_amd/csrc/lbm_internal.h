@@ -72,10 +72,15 @@ hipError_t launch_recon(int q, int storage, bool open, const KParams& kp, const 
                         const uint32_t* node, const uint32_t* cidx, Fields fl, hipStream_t st);
 hipError_t launch_init(int q, int storage, int collision, int mode, const KParams& kp, void* dst,
                        const uint32_t* node, const uint32_t* cidx, Fields fl, hipStream_t st);
-// FP64 reductions over non-solid nodes: out4 (device) = {sum du^2, sum u^2, sum rho, nonfinite}.
-hipError_t launch_reduce(const KParams& kp, const uint32_t* node, Fields fl, double* uprev,
-                         double* vprev, double* wprev, double* partials, int nblocks, double* out4,
+// FP64 reductions over non-solid nodes: out8 (device) = {sum du^2, sum u^2,
+// sum rho, nonfinite, max |du|, max |dv|, max |dw|, max |drho|}; partials
+// holds 8 doubles per block.
+hipError_t launch_reduce(const KParams& kp, const uint32_t* node, Fields fl, double* uprev, double* vprev,
+                         double* wprev, double* rprev, double* partials, int nblocks, double* out8,
                          hipStream_t st);
+// out (device) = sum over non-solid nodes of (f - fColl)^2, f dense FP64 in fout.
+hipError_t launch_fdiff2(int q, int storage, const KParams& kp, const void* pc, const uint32_t* cidx,
+                         const double* fout, double* partials, int nblocks, double* out, hipStream_t st);
 int reduce_blocks();
 
 }  // namespace lbm

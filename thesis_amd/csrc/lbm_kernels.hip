@@ -318,45 +318,109 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// FP64 reductions over non-solid nodes, per block: sums of |du|^2, |u|^2, rho
+// and the count of non-finite values, then the largest |du|, |dv|, |dw|,
+// |drho| against the previous reduction's fields (the reference's MacroDiff
+// residual, S/Iterate3D.cu:696-756, with a true maximum where
+// S/GpuSum.cu:152-187 adds the two loads); the previous fields are updated.
 __global__ __launch_bounds__(kRedThreads) void k_reduce_partial(const uint32_t* __restrict__ node,
                                                                  Fields fl, double* uprev, double* vprev,
-                                                                 double* wprev, size_t n, double* partials) {
-    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+                                                                 double* wprev, double* rprev, size_t n,
+                                                                 double* partials) {
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         if ((node[i] >> kClsShift) == SOLID) continue;
         const double u = fl.u[i], v = fl.v[i], w = fl.w[i], r = fl.rho[i];
-        const double du = u - uprev[i], dv = v - vprev[i], dw = w - wprev[i];
+        const double du = u - uprev[i], dv = v - vprev[i], dw = w - wprev[i], dr = r - rprev[i];
         const double uu = u * u + v * v + w * w;
-        a0 += du * du + dv * dv + dw * dw;
-        a1 += uu;
-        a2 += r;
-        if (!(uu == uu) || !(r == r) || isinf(uu) || isinf(r)) a3 += 1.0;
-        uprev[i] = u; vprev[i] = v; wprev[i] = w;
+        a[0] += du * du + dv * dv + dw * dw;
+        a[1] += uu;
+        a[2] += r;
+        if (!(uu == uu) || !(r == r) || isinf(uu) || isinf(r)) a[3] += 1.0;
+        a[4] = fmax(a[4], fabs(du)); a[5] = fmax(a[5], fabs(dv));
+        a[6] = fmax(a[6], fabs(dw)); a[7] = fmax(a[7], fabs(dr));
+        uprev[i] = u; vprev[i] = v; wprev[i] = w; rprev[i] = r;
     }
-    __shared__ double sh[4][kRedThreads / 64];
-    a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2); a3 = wave_sum(a3);
+    __shared__ double sh[8][kRedThreads / 64];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = k < 4 ? wave_sum(a[k]) : wave_max(a[k]);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) { sh[0][wv] = a0; sh[1][wv] = a1; sh[2][wv] = a2; sh[3][wv] = a3; }
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) sh[k][wv] = a[k];
     __syncthreads();
-    if (threadIdx.x < 4) {
+    if (threadIdx.x < 8) {
         double s = 0;
-        for (int k = 0; k < kRedThreads / 64; ++k) s += sh[threadIdx.x][k];
-        partials[(size_t)blockIdx.x * 4 + threadIdx.x] = s;
+        for (int k = 0; k < kRedThreads / 64; ++k)
+            s = threadIdx.x < 4 ? s + sh[threadIdx.x][k] : fmax(s, sh[threadIdx.x][k]);
+        partials[(size_t)blockIdx.x * 8 + threadIdx.x] = s;
     }
 }
 
-__global__ __launch_bounds__(256) void k_reduce_final(const double* __restrict__ partials, int nb, double* out4) {
-    __shared__ double sh[4][256];
-    for (int k = 0; k < 4; ++k) {
+__global__ __launch_bounds__(256) void k_reduce_final(const double* __restrict__ partials, int nb, double* out8) {
+    __shared__ double sh[8][256];
+    for (int k = 0; k < 8; ++k) {
         double s = 0;
-        for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partials[(size_t)b * 4 + k];
+        for (int b = threadIdx.x; b < nb; b += blockDim.x)
+            s = k < 4 ? s + partials[(size_t)b * 8 + k] : fmax(s, partials[(size_t)b * 8 + k]);
         sh[k][threadIdx.x] = s;
     }
     __syncthreads();
-    if (threadIdx.x < 4) {
+    if (threadIdx.x < 8) {
         double s = 0;
-        for (int t = 0; t < 256; ++t) s += sh[threadIdx.x][t];
-        out4[threadIdx.x] = s;
+        for (int t = 0; t < 256; ++t) s = threadIdx.x < 4 ? s + sh[threadIdx.x][t] : fmax(s, sh[threadIdx.x][t]);
+        out8[threadIdx.x] = s;
+    }
+}
+
+// Sum over non-solid nodes and populations of (f - fColl)^2 per block, with f
+// the post-boundary populations (dense FP64, from k_recon) and fColl the
+// post-collision populations they were streamed from (the reference's L2
+// residual, S/ComputeResiduals.cu:13-27).
+template <class L, int S>
+__global__ __launch_bounds__(kRedThreads) void k_fdiff2(const typename Io<L, S>::raw* __restrict__ pc,
+                                                        const uint32_t* __restrict__ cidx, KParams kp,
+                                                        const double* __restrict__ fout, size_t n,
+                                                        double* partials) {
+    using T = typename Io<L, S>::T;
+    double acc = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t own = cidx[i];
+        if (own == kNone) continue;
+#pragma unroll
+        for (int q = 0; q < L::Q; ++q) {
+            const T hc = Io<L, S>::ld(pc, paddr<L>(kp, q, own), q);
+            const double d = fout[q * n + i] - ((double)hc + L::w(q));
+            acc += d * d;
+        }
+    }
+    acc = wave_sum(acc);
+    __shared__ double sh[kRedThreads / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int k = 0; k < kRedThreads / 64; ++k) s += sh[k];
+        partials[blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sum_final(const double* __restrict__ partials, int nb, double* out) {
+    __shared__ double sh[256];
+    double s = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partials[b];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int k = 0; k < 256; ++k) t += sh[k];
+        out[0] = t;
     }
 }
 
@@ -481,12 +545,37 @@ hipError_t launch_init(int q, int storage, int collision, int mode, const KParam
 }
 
 hipError_t launch_reduce(const KParams& kp, const uint32_t* node, Fields fl, double* uprev, double* vprev,
-                         double* wprev, double* partials, int nblocks, double* out4, hipStream_t st) {
+                         double* wprev, double* rprev, double* partials, int nblocks, double* out8, hipStream_t st) {
     const size_t n = (size_t)kp.nx * kp.ny * kp.nz;
     hipLaunchKernelGGL(k_reduce_partial, dim3(nblocks), dim3(kRedThreads), 0, st, node, fl, uprev, vprev, wprev,
-                       n, partials);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, (const double*)partials, nblocks, out4);
+                       rprev, n, partials);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, (const double*)partials, nblocks, out8);
     return hipGetLastError();
+}
+
+template <class L, int S>
+static hipError_t fdiff2_t(const KParams& kp, const void* pc, const uint32_t* cidx, const double* fout,
+                           double* partials, int nblocks, double* out, hipStream_t st) {
+    const size_t n = (size_t)kp.nx * kp.ny * kp.nz;
+    hipLaunchKernelGGL((k_fdiff2<L, S>), dim3(nblocks), dim3(kRedThreads), 0, st,
+                       (const typename Io<L, S>::raw*)pc, cidx, kp, fout, n, partials);
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, st, (const double*)partials, nblocks, out);
+    return hipGetLastError();
+}
+template <class L>
+static hipError_t fdiff2_s(int s, const KParams& kp, const void* pc, const uint32_t* cidx, const double* fout,
+                           double* partials, int nblocks, double* out, hipStream_t st) {
+    switch (s) {
+    case 0: return fdiff2_t<L, 0>(kp, pc, cidx, fout, partials, nblocks, out, st);
+    case 1: return fdiff2_t<L, 1>(kp, pc, cidx, fout, partials, nblocks, out, st);
+    case 2: return fdiff2_t<L, 2>(kp, pc, cidx, fout, partials, nblocks, out, st);
+    default: return fdiff2_t<L, 3>(kp, pc, cidx, fout, partials, nblocks, out, st);
+    }
+}
+hipError_t launch_fdiff2(int q, int storage, const KParams& kp, const void* pc, const uint32_t* cidx,
+                         const double* fout, double* partials, int nblocks, double* out, hipStream_t st) {
+    return q == 9 ? fdiff2_s<D2Q9>(storage, kp, pc, cidx, fout, partials, nblocks, out, st)
+                  : fdiff2_s<D3Q19>(storage, kp, pc, cidx, fout, partials, nblocks, out, st);
 }
 
 }  // namespace lbm

@@ -557,8 +557,17 @@ int main(int argc, char** argv) {
     const double t_loop = now_s();
     FILE* rf = std::fopen((a.out_dir + "residuals.dat").c_str(), "w");
     if (rf) std::fprintf(rf, "Iter L2_norm L2_norm_weighted Drag Lift\n");
-    double r4[4];
-    lbm_reduce(ctx, r4);
+    // ResidualModel (ini line 19): MacroDiff compares the largest change of u,
+    // v, w, rho since the last check with the four stop values and logs them to
+    // MacroDiffs.txt (S/Iterate3D.cu:696-756, S/LogWriter.cu:215-221); L2 is
+    // sqrt(sum (f - fColl)^2) against the first stop value (S/Iterate.cu:551).
+    // FdRelDiff (which keeps two more copies of f) falls back to L2.
+    const bool macro_diff = a.residual_model.find("MacroDiff") != std::string::npos;
+    if (a.residual_model.find("FdRelDiff") != std::string::npos)
+        std::fprintf(stderr, "note: FdRelDiff residuals are not available; L2 used\n");
+    FILE* mf = macro_diff ? std::fopen((a.out_dir + "MacroDiffs.txt").c_str(), "w") : nullptr;
+    double r4[8];
+    lbm_reduce_all(ctx, r4);
     const long every = a.residual_every > 0 ? a.residual_every : a.iterations;
     long done = 0;
     int code = 0;
@@ -570,14 +579,25 @@ int main(int argc, char** argv) {
         if (st != LBM_OK) { std::fprintf(stderr, "step: %s\n", lbm_last_error(ctx)); code = 2; break; }
         done += k;
         t0 = now_s();
-        lbm_reduce(ctx, r4);
+        lbm_reduce_all(ctx, r4);
+        double l2 = 0;
+        if (!macro_diff) lbm_residual_l2(ctx, &l2);
         tm.resi += now_s() - t0;
-        const double res = r4[1] > 0 ? std::sqrt(r4[0] / r4[1]) : std::sqrt(r4[0]);
-        if (rf) std::fprintf(rf, "%ld %5.4e %5.4e %f %f\n", done, std::sqrt(r4[0]), res, 0.0, 0.0);
-        std::printf("iteration %ld residual %.4e mass %.10f\n", done, res, r4[2]);
-        if (r4[3] > 0 || res != res) { std::fprintf(stderr, "\nDIVERGENCE!\n"); code = 1; break; }
-        if (a.stop[0] > 0 && res < a.stop[0]) { std::printf("simulation converged!\n"); break; }
+        if (rf) std::fprintf(rf, "%ld %5.4e %5.4e %f %f\n", done, l2, l2 / (double)NS, 0.0, 0.0);
+        if (r4[3] > 0 || l2 != l2) { std::fprintf(stderr, "\nDIVERGENCE!\n"); code = 1; break; }
+        if (macro_diff) {
+            if (mf) std::fprintf(mf, "%ld\t\t%.10f\t%.10f\t%.10f\t%.10f\n", done, r4[4], r4[5], r4[6], r4[7]);
+            std::printf("iteration %ld max diffs u %.4e v %.4e w %.4e rho %.4e\n", done, r4[4], r4[5], r4[6], r4[7]);
+            if (r4[4] < a.stop[0] && r4[5] < a.stop[1] && r4[6] < a.stop[2] && r4[7] < a.stop[3]) {
+                std::printf("simulation converged!\n");
+                break;
+            }
+        } else {
+            std::printf("iteration %ld L2 residual %.4e mass %.10f\n", done, l2, r4[2]);
+            if (l2 < a.stop[0]) { std::printf("simulation converged!\n"); break; }
+        }
     }
+    if (mf) std::fclose(mf);
     if (rf) std::fclose(rf);
     tm.iter = now_s() - t_loop;
     const double t_write = now_s();

@@ -32,7 +32,7 @@ STORAGE_NAMES = {"f64": STORE_F64, "f32": STORE_F32, "f16": STORE_F16, "bf16": S
 
 EXPORTED = [
     "lbm_abi_version", "lbm_device_count", "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_init",
-    "lbm_step", "lbm_run_to_residual", "lbm_reduce", "lbm_get_macros", "lbm_get_f", "lbm_set_f",
+    "lbm_step", "lbm_run_to_residual", "lbm_reduce", "lbm_reduce_all", "lbm_residual_l2", "lbm_get_macros", "lbm_get_f", "lbm_set_f",
     "lbm_time_steps", "lbm_device_bytes", "lbm_read_lattice",
     "lbm_cg_create", "lbm_cg_destroy", "lbm_cg_last_error", "lbm_cg_init", "lbm_cg_step", "lbm_cg_get_macros",
     "lbm_cg_get_f", "lbm_cg_reduce", "lbm_cg_time_steps",
@@ -115,6 +115,8 @@ def load():
     L.lbm_step.argtypes = [vp, C.c_int64]
     L.lbm_run_to_residual.argtypes = [vp, C.c_double, C.c_int64, C.c_int64, P(C.c_int64), dp]
     L.lbm_reduce.argtypes = [vp, dp]
+    L.lbm_reduce_all.argtypes = [vp, dp]
+    L.lbm_residual_l2.argtypes = [vp, dp]
     L.lbm_get_macros.argtypes = [vp, dp, dp, dp, dp]
     L.lbm_get_f.argtypes = [vp, dp]
     L.lbm_set_f.argtypes = [vp, dp]
@@ -275,6 +277,19 @@ class Solver:
         out = np.zeros(4)
         self._check(self._L.lbm_reduce(self.h, _dp(out)))
         return dict(du2=out[0], u2=out[1], mass=out[2], nonfinite=out[3])
+
+    def reduce_all(self):
+        """lbm_reduce plus the largest |du|, |dv|, |dw|, |drho| since the last reduction."""
+        out = np.zeros(8)
+        self._check(self._L.lbm_reduce_all(self.h, _dp(out)))
+        return dict(du2=out[0], u2=out[1], mass=out[2], nonfinite=out[3], max_du=out[4], max_dv=out[5],
+                    max_dw=out[6], max_drho=out[7])
+
+    def residual_l2(self):
+        """The reference's L2 residual sqrt(sum (f - fColl)^2) of the last step."""
+        out = C.c_double()
+        self._check(self._L.lbm_residual_l2(self.h, C.byref(out)))
+        return out.value
 
     def macros(self):
         out = [np.zeros(self.n) for _ in range(4)]
