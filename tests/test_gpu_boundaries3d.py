@@ -190,13 +190,14 @@ UpdateInltOutl = yes
 
 def test_lbmsolver_reference_lid_cavity(tmp_path):
     """The reference's 3D validation lattice through lbmsolver with its
-    SetUpData keys (HHmodel walls, U = 0.05 on the lid) equals the ABI run."""
+    SetUpData keys (HHmodel walls, U = 0.05 on the lid) equals the ABI run,
+    and autosaves every 100 iterations."""
     print("case: lbmsolver lidvalidation HHmodel", flush=True)
     cls, bb, U, paths = lid_cavity()
     os.makedirs(tmp_path / "Mesh")
     for p in paths:
         os.replace(p, tmp_path / "Mesh" / os.path.basename(p))
-    (tmp_path / "SetUpData.ini").write_text(INI3)
+    (tmp_path / "SetUpData.ini").write_text(INI3.replace("AutosaveEach = 0", "AutosaveEach = 100"))
     r = subprocess.run([CLI, "-f", "SetUpData.ini"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     print(r.stdout[-300:], r.stderr[-300:])
     assert r.returncode == 0
@@ -211,3 +212,8 @@ def test_lbmsolver_reference_lid_cavity(tmp_path):
     assert data.shape[0] == n
     assert np.abs(data[:, 3] - u.ravel()).max() < 1e-9
     assert np.abs(data[:, 7] - rho.ravel()).max() < 1e-9
+    # 3D autosave files (S/Iterate3D.cu:803-837): every 100 iterations
+    saves = sorted(f for f in os.listdir(tmp_path / "Results") if f.startswith("autosave_iter"))
+    assert saves == ["autosave_iter00100.csv", "autosave_iter00200.csv", "autosave_iter00300.csv"]
+    last = np.loadtxt(tmp_path / "Results" / saves[-1], delimiter=",", skiprows=1)
+    assert (last == data).all()

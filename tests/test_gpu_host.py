@@ -176,3 +176,64 @@ def test_lbmsolver_macro_diff_stop_condition(tmp_path):
     s.step(steps)
     rho, u, v, w = s.macros()
     assert np.abs(data[:, 3] - u.ravel()).max() < 1e-9
+
+
+def read_appended_vti(path, n):
+    """Velocity (n x 3) and density (n) of a binary-appended VTI file."""
+    raw = open(path, "rb").read()
+    p = raw.index(b'<AppendedData encoding="raw">')
+    p = raw.index(b"_", p) + 1
+    out = []
+    for count in (3 * n, n, n):
+        nb = int(np.frombuffer(raw[p:p + 4], np.uint32)[0])
+        assert nb == 4 * count
+        out.append(np.frombuffer(raw[p + 4:p + 4 + nb], np.float32).astype(np.float64))
+        p += 4 + nb
+    return out[0].reshape(n, 3), out[1]
+
+
+def test_lbmsolver_autosave_and_appended_vti(tmp_path):
+    """AutosaveAfter / AutosaveEach (S/Iterate.cu:590-621): a 2D ParaView run
+    writes Video/FinalData_<k>.vti at every multiple of AutosaveEach beyond
+    AutosaveAfter; with vti = appended all VTI files carry Float32 binary
+    data, the final one equals the ABI run to Float32 rounding, and a restart
+    from it with no further iterations writes the same fields back."""
+    print("case: lbmsolver autosave + appended VTI", flush=True)
+    nx, ny, U, nu = 64, 16, 0.02, 0.1
+    ch = LT.channel2d(nx, ny, 1.5 * U, obstacles=((20, 8, 3),))
+    os.makedirs(tmp_path / "Mesh")
+    LT.write_latticegen2(str(tmp_path / "Mesh" / "chan"), ch["node_class"], ch["bb"])
+    ini = (INI.format(U=U, nu=nu, iters=600).replace("OutputFormat = CSV", "OutputFormat = VTI")
+           .replace("AutosaveAfter = 0", "AutosaveAfter = 100").replace("AutosaveEach = 0", "AutosaveEach = 200")
+           + "vti = appended\n")
+    (tmp_path / "SetUpData.ini").write_text(ini)
+    r = subprocess.run([CLI, "-f", "SetUpData.ini"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-300:], r.stderr[-300:])
+    assert r.returncode == 0 and r.stdout.count("autosave") == 3
+    video = sorted(os.listdir(tmp_path / "Results" / "Video"))
+    assert video == ["FinalData_2.vti", "FinalData_3.vti", "FinalData_4.vti"]
+    n = nx * ny
+    vel, rho = read_appended_vti(tmp_path / "Results" / "FinalData.vti", n)
+    vel4, rho4 = read_appended_vti(tmp_path / "Results" / "Video" / "FinalData_4.vti", n)
+    assert (vel == vel4).all() and (rho == rho4).all()
+    lat = L.read_lattice(str(tmp_path / "Mesh" / "chan_Node.dat"), str(tmp_path / "Mesh" / "chan_BC.dat"))
+    cls = lat["node_class"][0]
+    ys = np.arange(ny)
+    ubc = np.zeros((ny, nx))
+    ubc[:, 0] = 6 * U * (ys + 0.5) * (ny - 0.5 - ys) / ny ** 2
+    s = L.Solver(9, (ny, nx), cls, lat["bb_links"][0], collision=L.TRT, omega=1 / (3 * nu + 0.5),
+                 ubc=np.where(cls == L.VEL_W, ubc, 0.0), vbc=np.zeros((ny, nx)), rho_out=1.0)
+    s.init(0, rho=np.ones((ny, nx)), u=np.where(cls == L.VEL_W, ubc, U))
+    s.step(600)
+    rr, uu, vv, ww = s.macros()
+    assert np.abs(vel[:, 0] - uu.ravel()).max() < 1e-8 and np.abs(rho - rr.ravel()).max() < 1e-6
+    first = tmp_path / "Results" / "FinalData.vti"
+    ini2 = (ini.replace("InitialConditionFromFile = no", "InitialConditionFromFile = yes")
+               .replace('InitialFile = "none"', f'InitialFile = "{first}"')
+               .replace("Number of iterations = 600", "Number of iterations = 0"))
+    (tmp_path / "restart.ini").write_text(ini2)
+    r = subprocess.run([CLI, "-f", "restart.ini", "-o", "Run2/"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0 and "Initial conditions loaded" in r.stdout, r.stderr
+    vel2, rho2 = read_appended_vti(tmp_path / "Run2" / "FinalData.vti", n)
+    assert (vel2 == vel).all() and (rho2 == rho).all()

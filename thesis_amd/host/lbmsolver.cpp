@@ -16,6 +16,7 @@
 //   walls = onnode|halfway        trt_magic = <Lambda>
 //   force_model = shift|guo       rho_out = <density>
 //   inlet = zouhe|reference       outlet = pressure|velocity|extrap2|extrap1|feq
+//   vti = appended                (binary-appended VTI output, also read back on restart)
 // BCoutlet maps as the reference reads it (S/FilesReading.cu:180-190 with
 // S/include/Arguments.h:22-28): in 2D Vout -> velocity outlet, 1stE ->
 // second-order extrapolation, 2ndE -> first-order extrapolation, HEmodel ->
@@ -52,7 +53,7 @@ struct Args {
     int inlet_model = LBM_INLET_ZOUHE;
     int walls_set = 0;               // [mi355x] walls given: overrides BCwallModel
     double stop[4] = {0, 0, 0, 0};
-    int format = 3;                  // 1 CSV, 2 DAT (Tecplot), 3 VTI
+    int format = 3;                  // 1 CSV, 2 DAT (Tecplot), 3 VTI, 4 VTI with binary-appended data
     // new knobs
     bool use_init_file = false;      // ini line 17: start from the macroscopic fields of a results file
     std::string init_file;           // ini line 18
@@ -127,6 +128,7 @@ bool read_ini(const std::string& path, Args& a) {
     if (kv.count("force_model")) a.force_model = kv["force_model"] == "guo" ? LBM_FORCE_GUO : LBM_FORCE_SHIFT;
     if (kv.count("trt_magic")) a.trt_magic = atof(kv["trt_magic"].c_str());
     if (kv.count("rho_out")) a.rho_out = atof(kv["rho_out"].c_str());
+    if (kv.count("vti") && kv["vti"] == "appended" && a.format == 3) a.format = 4;
     if (kv.count("inlet")) a.inlet_model = kv["inlet"] == "reference" ? LBM_INLET_REF2D : LBM_INLET_ZOUHE;
     if (kv.count("outlet")) {
         const std::string& o = kv["outlet"];
@@ -139,7 +141,7 @@ void usage() {
     std::printf(
         "usage: lbmsolver [-f SetUpData.ini] [-n node.dat -b bc.dat] [-o outdir] [-u U] [-v V] [-r rho]\n"
         "                 [-s viscosity] [-c BGKW|TRT|MRT] [-i iterations] [--every N]\n"
-        "                 [--format paraview|tecplot|csv] [--storage f64|f32|f16|bf16]\n"
+        "                 [--format paraview|tecplot|csv|vti-appended] [--storage f64|f32|f16|bf16]\n"
         "                 [-l|--outlet yes|second|first]\n"
         "       lbmsolver compare <file1> <file2>\n");
 }
@@ -196,8 +198,43 @@ int compare_files(const char* f1, const char* f2) {
 // two more lines) or ASCII VTI ("<?" header, 3N + 18 lines: velocity
 // vectors, then the density).  The node count must match; columns are
 // x, y, z, u, v, w, |u|, rho, p.  Returns false (uniform start) otherwise.
+// Binary-appended VTI (written with [mi355x] vti = appended): after the "_"
+// that opens <AppendedData encoding="raw">, each array is a UInt32 byte count
+// and its Float32 values: Velocity (3 N), Density (N), Pressure (N).
+bool read_appended_vti(const std::string& path, size_t N, std::vector<double>& u, std::vector<double>& v,
+                       std::vector<double>& w, std::vector<double>& rho) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) return false;
+    std::string all((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    const std::string tag = "<AppendedData encoding=\"raw\">";
+    size_t p = all.find(tag);
+    if (p == std::string::npos) return false;
+    p = all.find('_', p + tag.size());
+    if (p == std::string::npos) return false;
+    ++p;
+    auto block = [&](size_t count, std::vector<float>& out) {
+        uint32_t nb = 0;
+        if (p + 4 > all.size()) return false;
+        std::memcpy(&nb, all.data() + p, 4);
+        p += 4;
+        if (nb != count * 4 || p + nb > all.size()) return false;
+        out.resize(count);
+        std::memcpy(out.data(), all.data() + p, nb);
+        p += nb;
+        return true;
+    };
+    std::vector<float> vel, den;
+    if (!block(3 * N, vel) || !block(N, den)) return false;
+    u.resize(N); v.resize(N); w.resize(N); rho.resize(N);
+    for (size_t i = 0; i < N; ++i) {
+        u[i] = vel[3 * i]; v[i] = vel[3 * i + 1]; w[i] = vel[3 * i + 2]; rho[i] = den[i];
+    }
+    return true;
+}
+
 bool read_initial_conditions(const std::string& path, size_t N, std::vector<double>& u, std::vector<double>& v,
                              std::vector<double>& w, std::vector<double>& rho) {
+    if (read_appended_vti(path, N, u, v, w, rho)) return true;
     std::ifstream in(path);
     if (!in) return false;
     std::vector<std::string> lines;
@@ -230,6 +267,8 @@ bool read_initial_conditions(const std::string& path, size_t N, std::vector<doub
     return false;
 }
 
+const char* ext_of(int fmt) { return fmt == 1 ? "csv" : fmt == 2 ? "dat" : "vti"; }
+
 void write_results(const std::string& path, int fmt, int n, int m, int h, double dx, const std::vector<double>& rho,
                    const std::vector<double>& u, const std::vector<double>& v, const std::vector<double>& w) {
     FILE* fp = std::fopen(path.c_str(), "w");
@@ -246,6 +285,31 @@ void write_results(const std::string& path, int fmt, int n, int m, int h, double
             std::fprintf(fp, f, x * dx, y * dx, z * dx, u[i], v[i], w[i],
                          std::sqrt(u[i] * u[i] + v[i] * v[i] + w[i] * w[i]), rho[i], rho[i] / 3.0);
         }
+    } else if (fmt == 4) {
+        // binary-appended VTI: the same arrays as the ASCII file, Float32 raw
+        std::fprintf(fp, "<?xml version=\"1.0\"?>\n"
+                         "<VTKFile type=\"ImageData\" version=\"0.1\" byte_order=\"LittleEndian\" header_type=\"UInt32\">\n"
+                         "  <ImageData WholeExtent=\"0 %d 0 %d 0 %d\" Origin=\"0 0 0\" Spacing=\"1 1 1\">\n"
+                         "  <Piece Extent=\"0 %d 0 %d 0 %d\">\n    <PointData Scalars=\"scalars\">\n"
+                         "      <DataArray type=\"Float32\" Name=\"Velocity\" NumberOfComponents=\"3\" format=\"appended\" offset=\"0\"/>\n"
+                         "      <DataArray type=\"Float32\" Name=\"Density\" NumberOfComponents=\"1\" format=\"appended\" offset=\"%zu\"/>\n"
+                         "      <DataArray type=\"Float32\" Name=\"Pressure\" NumberOfComponents=\"1\" format=\"appended\" offset=\"%zu\"/>\n"
+                         "    </PointData>\n    <CellData>\n    </CellData>\n  </Piece>\n  </ImageData>\n"
+                         "  <AppendedData encoding=\"raw\">\n_",
+                     n - 1, m - 1, h - 1, n - 1, m - 1, h - 1, 4 + 12 * N, 8 + 16 * N);
+        std::vector<float> buf(3 * N);
+        auto put = [&](size_t count) {
+            const uint32_t nb = (uint32_t)(count * 4);
+            std::fwrite(&nb, 4, 1, fp);
+            std::fwrite(buf.data(), 4, count, fp);
+        };
+        for (size_t i = 0; i < N; ++i) { buf[3 * i] = (float)u[i]; buf[3 * i + 1] = (float)v[i]; buf[3 * i + 2] = (float)w[i]; }
+        put(3 * N);
+        for (size_t i = 0; i < N; ++i) buf[i] = (float)rho[i];
+        put(N);
+        for (size_t i = 0; i < N; ++i) buf[i] = (float)(rho[i] / 3.0);
+        put(N);
+        std::fprintf(fp, "\n  </AppendedData>\n</VTKFile>\n");
     } else {
         std::fprintf(fp, "<?xml version=\"1.0\"?>\n<!-- 3D LBM Solver -->\n"
                          "<VTKFile type=\"ImageData\" version=\"0.1\" byte_order=\"LittleEndian\">\n"
@@ -384,7 +448,10 @@ int main(int argc, char** argv) {
             else std::fprintf(stderr, "Invalid outlet profile (%s), going with default\n", o.c_str());
         }
         else if (s == "-c") { std::string c = next(); a.collision = c == "TRT" ? LBM_TRT : c == "MRT" ? LBM_MRT : LBM_BGK; from_cli = true; }
-        else if (s == "--format") { std::string f = next(); a.format = f == "tecplot" ? 2 : f == "csv" ? 1 : 3; }
+        else if (s == "--format") {
+            std::string f = next();
+            a.format = f == "tecplot" ? 2 : f == "csv" ? 1 : f == "vti-appended" ? 4 : 3;
+        }
         else if (s == "--storage") { std::string f = next(); a.storage = f == "f32" ? LBM_STORE_F32 : f == "f16" ? LBM_STORE_F16 : f == "bf16" ? LBM_STORE_BF16 : LBM_STORE_F64; }
         else { std::fprintf(stderr, "unknown option %s\n", s.c_str()); usage(); return 2; }
     }
@@ -571,13 +638,52 @@ int main(int argc, char** argv) {
     const long every = a.residual_every > 0 ? a.residual_every : a.iterations;
     long done = 0;
     int code = 0;
+    // Autosave (S/Iterate3D.cu:803-837, S/Iterate.cu:590-621): at every
+    // multiple of AutosaveEach beyond AutosaveAfter, the fields go to
+    // autosave_iter%05d.<ext> (3D) or over FinalData (2D; ParaView output to
+    // Video/FinalData_<k>.vti)
+    const long save_every = a.autosave_every > 0 ? a.autosave_every : 0;
+    long save_k = 1;
+    auto autosave = [&]() {
+        std::vector<double> sr(NS), su(NS), sv(NS), sw(NS);
+        lbm_get_macros(ctx, sr.data(), su.data(), sv.data(), sw.data());
+        std::vector<double> R(N), U(N), V(N), W(N);
+        for (int z = 0; z < dims[2]; ++z)
+            for (int y = 0; y < dims[1]; ++y)
+                for (int x = 0; x < dims[0]; ++x) {
+                    const size_t d = ((size_t)z * dims[1] + y) * dims[0] + x;
+                    const size_t q = ((size_t)(z % sim[2]) * sim[1] + (y % sim[1])) * sim[0] + (x % sim[0]);
+                    R[d] = sr[q]; U[d] = su[q]; V[d] = sv[q]; W[d] = sw[q];
+                }
+        char name[768];
+        if (three_d) {
+            std::snprintf(name, sizeof name, "%sautosave_iter%05ld.%s", a.out_dir.c_str(), done, ext_of(a.format));
+        } else if (a.format >= 3) {
+            mkdir((a.out_dir + "Video").c_str(), 0755);
+            std::snprintf(name, sizeof name, "%sVideo/FinalData_%ld.vti", a.out_dir.c_str(), save_k);
+        } else {
+            std::snprintf(name, sizeof name, "%sFinalData.%s", a.out_dir.c_str(), ext_of(a.format));
+        }
+        std::printf("autosave\n\n");
+        write_results(name, a.format, dims[0], dims[1], dims[2], dx, R, U, V, W);
+    };
     while (done < a.iterations) {
-        const long k = std::min(every, a.iterations - done);
+        long k = std::min(every - done % every, a.iterations - done);
+        if (save_every) k = std::min(k, save_every * save_k - done);
         double t0 = now_s();
         lbm_status st = lbm_step(ctx, k);
         tm.coll += now_s() - t0;
         if (st != LBM_OK) { std::fprintf(stderr, "step: %s\n", lbm_last_error(ctx)); code = 2; break; }
         done += k;
+        if (save_every && done == save_every * save_k) {
+            ++save_k;
+            if (done > a.autosave_after) {
+                t0 = now_s();
+                autosave();
+                tm.writ += now_s() - t0;
+            }
+        }
+        if (done % every != 0 && done != a.iterations) continue;
         t0 = now_s();
         lbm_reduce_all(ctx, r4);
         double l2 = 0;
@@ -612,10 +718,10 @@ int main(int argc, char** argv) {
                 const size_t s = ((size_t)(z % sim[2]) * sim[1] + (y % sim[1])) * sim[0] + (x % sim[0]);
                 R[d] = rho[s]; U[d] = u[s]; V[d] = v[s]; W[d] = w[s];
             }
-    const char* ext = a.format == 1 ? "csv" : a.format == 2 ? "dat" : "vti";
+    const char* ext = ext_of(a.format);
     write_results(a.out_dir + "FinalData." + ext, a.format, dims[0], dims[1], dims[2], dx, R, U, V, W);
     std::printf("results written to %sFinalData.%s\n", a.out_dir.c_str(), ext);
-    tm.writ = now_s() - t_write;
+    tm.writ += now_s() - t_write;
     tm.overall = now_s() - t_start;
     write_end_log(a.out_dir + "lbmsolver.log", tm);
     write_times(a.out_dir + "runtimes.dat", tm);
