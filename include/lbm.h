@@ -65,7 +65,11 @@ enum { LBM_FORCE_NONE = 0, LBM_FORCE_SHIFT = 1, LBM_FORCE_GUO = 2 };
  * S/GpuBoundaries.cu:492-1560), D3Q19: nodes with 5, 9 or 12 wall links are
  * closed as a face, an edge or a corner of a box; other wall links keep the
  * node's own population, as in the reference */
-enum { LBM_WALL_HALFWAY = 0, LBM_WALL_ONNODE = 1, LBM_WALL_HH = 2 };
+enum { LBM_WALL_HALFWAY = 0, LBM_WALL_ONNODE = 1, LBM_WALL_HH = 2, LBM_WALL_CURVED = 3 };
+/* LBM_WALL_CURVED: Bouzidi linear interpolation (Bouzidi, Firdaouss &
+ * Lallemand 2001) with the wall fraction q of each wall link
+ * (lbm_lattice.wall_q; 1/2 is half-way bounce-back), the textbook form of the
+ * reference's CURVED walls (S/GpuBoundaries.cu:414-431, :457-471). */
 /* velocity faces: textbook Zou-He, or the reference's 2D forms without
  * transverse terms, corners skipped (west S/GpuBoundaries.cu:45-64; north
  * lid :21-41); the reference forms impose u0 exactly on equilibrium states */
@@ -113,14 +117,17 @@ typedef struct {
 /* Lattice description; node_class is required, the rest may be NULL.
  * bb_links[i] bit d (1 <= d < q): the link from node i in direction d
  * crosses a wall (the _BC.dat line "i j k d WALL").
- * u_bc/v_bc/w_bc: prescribed velocity on LBM_VEL_W and LBM_VEL_N nodes, and
- * u0 of the reference velocity outlet on LBM_PRES_E nodes. */
+ * u_bc/v_bc/w_bc: prescribed velocity on velocity-face nodes, and u0 of the
+ * reference velocity outlet on LBM_PRES_E nodes.
+ * wall_q[i*19 + d]: fraction of the link from node i in direction d at which
+ * it meets the wall (LBM_WALL_CURVED; entries <= 0, or NULL, mean 1/2). */
 typedef struct {
     const uint8_t* node_class;
     const uint32_t* bb_links;
     const double* u_bc;
     const double* v_bc;
     const double* w_bc;
+    const double* wall_q;
 } lbm_lattice;
 
 int lbm_abi_version(void);
@@ -198,6 +205,12 @@ lbm_status lbm_read_lattice(const char* node_path, const char* bc_path, int32_t 
                             int64_t* nlinks, uint8_t* node_class, uint32_t* bb_links,
                             int64_t type_counts[6], int64_t dir_counts[19], double* delta,
                             int32_t cyclic[3]);
+
+/* Wall fractions of the WALL links of a latticegen2 lattice for
+ * LBM_WALL_CURVED: wall_q[i*19 + d] = |x_bc - x_i| / (delta |c_d|) from the BC
+ * line's intersection point and the node's coordinates (S/GpuInit.cu:1088,
+ * :1205); 0 for links without a WALL line.  wall_q holds nx*ny*nz*19 doubles. */
+lbm_status lbm_read_wall_q(const char* node_path, const char* bc_path, double* wall_q);
 
 /* ---- colour-gradient two-phase model (D2Q9, periodic box) ---------------- */
 /* Replaces the reference's multiphase path: gpuCollBgkwGC2D + gradient

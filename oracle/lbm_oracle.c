@@ -120,6 +120,7 @@ struct orc_sim {
     uint32_t* bb;
     double *ubc, *vbc, *wbc;
     double *f, *fcoll, *rho, *u, *v, *w3;
+    double* wq;                     /* wall fractions (N x 19) or NULL */
 };
 
 /* f_eq = rho w (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u.u)  (S/GpuCollision.cu:21-32) */
@@ -309,9 +310,16 @@ orc_sim* orc_create(const orc_params* p, const uint8_t* node_class, const uint32
     return s;
 }
 
+void orc_set_wall_q(orc_sim* s, const double* q)
+{
+    if (!s->wq) s->wq = (double*)malloc(sizeof(double) * 19 * (size_t)s->n);
+    memcpy(s->wq, q, sizeof(double) * 19 * (size_t)s->n);
+}
+
 void orc_destroy(orc_sim* s)
 {
     if (!s) return;
+    free(s->wq);
     free(s->cls); free(s->bb); free(s->ubc); free(s->vbc); free(s->wbc);
     free(s->f); free(s->fcoll); free(s->rho); free(s->u); free(s->v); free(s->w3);
     free(s);
@@ -713,6 +721,33 @@ void orc_step(orc_sim* s, int nsteps)
                 int outside;
                 long src = wrap_src(s, x, y, z, i, &outside);
                 f[i] = outside ? s->fcoll[(size_t)i * n + ind] : s->fcoll[(size_t)i * n + src];
+            }
+            if (s->p.wall_model == ORC_WALL_CURVED && bb) {
+                /* Bouzidi, Firdaouss & Lallemand (2001), linear: for a wall
+                 * link d at fraction q the unknown opp(d) is
+                 *   q < 1/2:  2q f*_d(x) + (1 - 2q) f*_d(x - c_d)
+                 *   q >= 1/2: f*_d(x) / 2q + (2q - 1)/2q f*_opp(d)(x)
+                 * (the textbook form of S/GpuBoundaries.cu:414-431, whose
+                 * q index and second node are off by one link); q < 1/2 with
+                 * no fluid node at x - c_d falls back to bounce-back. */
+                double g[19];
+                for (int i = 0; i < Q; ++i) g[i] = f[i];
+                for (int d = 1; d < Q; ++d) {
+                    if (!((bb >> d) & 1u)) continue;
+                    const int o = s->opp[d];
+                    double q = s->wq ? s->wq[(size_t)ind * 19 + d] : 0.5;
+                    if (q <= 0) q = 0.5;
+                    const double fd = s->fcoll[(size_t)d * n + ind];
+                    if (q < 0.5) {
+                        int outside;
+                        long src = wrap_src(s, x, y, z, d, &outside);
+                        if (((bb >> o) & 1u) || outside) { g[o] = fd; continue; }
+                        g[o] = 2.0 * q * fd + (1.0 - 2.0 * q) * s->fcoll[(size_t)d * n + src];
+                    } else {
+                        g[o] = fd / (2.0 * q) + (2.0 * q - 1.0) / (2.0 * q) * s->fcoll[(size_t)o * n + ind];
+                    }
+                }
+                for (int i = 0; i < Q; ++i) f[i] = g[i];
             }
             /* inlet -> wall -> outlet order of S/Iterate.cu:468-473 and
              * S/Iterate3D.cu:602-621 (then periodic, here by wrap, and symmetry) */

@@ -38,7 +38,9 @@ enum { ORC_BGK = 0, ORC_TRT = 1, ORC_MRT = 2 };
 enum { ORC_FORCE_NONE = 0, ORC_FORCE_SHIFT = 1, ORC_FORCE_GUO = 2 };
 /* ORC_WALL_HH: the reference's wet-node walls (HHmodel, S/GpuBoundaries.cu:492-1560),
  * 3D, on nodes with 5 (face), 9 (edge) or 12 (corner) wall links */
-enum { ORC_WALL_HALFWAY = 0, ORC_WALL_ONNODE = 1, ORC_WALL_HH = 2 };
+/* ORC_WALL_CURVED: Bouzidi linear interpolation with the wall fraction q of
+ * each wall link (orc_set_wall_q; 1/2 is half-way bounce-back) */
+enum { ORC_WALL_HALFWAY = 0, ORC_WALL_ONNODE = 1, ORC_WALL_HH = 2, ORC_WALL_CURVED = 3 };
 enum { ORC_INLET_ZOUHE = 0, ORC_INLET_REF2D = 1 };
 /* east outlet: Zou-He pressure, the reference's velocity outlet, its second-
  * and first-order extrapolations (S/GpuBoundaries.cu:1562-1653) */
@@ -86,6 +88,9 @@ int orc_mrt_matrices(int q, double omega, double* M, double* Minv, double* s);
 orc_sim* orc_create(const orc_params* p, const uint8_t* node_class, const uint32_t* bb,
                     const double* ubc, const double* vbc, const double* wbc);
 void orc_destroy(orc_sim* s);
+/* q[ind*19 + d]: fraction of the link from node ind in direction d at which it
+ * meets the wall (values <= 0 mean 1/2) */
+void orc_set_wall_q(orc_sim* s, const double* q);
 
 /* mode 0: f = f_eq(rho,u); mode 1: f = 0 (the reference's start, S/Iterate3D.cu:301-302).
  * rho/u/v/w may be NULL (1 and 0). Stored macros are set to the given fields. */

@@ -20,7 +20,7 @@ OUT_W, OUT_S, OUT_N, OUT_B, OUT_T = 9, 10, 11, 12, 13
 SYM_W, SYM_E, SYM_S, SYM_N, SYM_B, SYM_T = 14, 15, 16, 17, 18, 19
 BGK, TRT, MRT = 0, 1, 2
 FORCE_NONE, FORCE_SHIFT, FORCE_GUO = 0, 1, 2
-WALL_HALFWAY, WALL_ONNODE, WALL_HH = 0, 1, 2
+WALL_HALFWAY, WALL_ONNODE, WALL_HH, WALL_CURVED = 0, 1, 2, 3
 INLET_ZOUHE, INLET_REF2D = 0, 1
 OUTLET_PRESSURE, OUTLET_REF_VELOCITY, OUTLET_EXTRAP2, OUTLET_EXTRAP1, OUTLET_FEQ = 0, 1, 2, 3, 4
 PARITY_MRT_RHO2, PARITY_MRT_NO_F18, PARITY_INLET_FROZEN, PARITY_OUTLET_V0 = 1, 2, 4, 8
@@ -77,6 +77,7 @@ def lib():
         L.orc_collide_node.argtypes = [C.c_void_p, dp, C.c_double, dp, dp]
         L.orc_open_boundary.argtypes = [C.c_void_p, C.c_long, dp]
         L.orc_wall_node.argtypes = [C.c_void_p, C.c_long, dp]
+        L.orc_set_wall_q.argtypes = [C.c_void_p, dp]
         L.orc_equilibrium.argtypes = [C.c_int, C.c_double, dp, dp]
         L.orc_lattice.argtypes = [C.c_int, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), dp]
         L.orc_mrt_matrices.argtypes = [C.c_int, C.c_double, dp, dp, dp]
@@ -198,6 +199,12 @@ class Sim:
         g = np.array(f, dtype=np.float64)
         lib().orc_open_boundary(self.h, int(ind), _dp(g))
         return g
+
+    def set_wall_q(self, q):
+        """q (N x 19, or (..., 19) in node order): wall fraction of each wall link."""
+        self._wq = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        assert self._wq.size == 19 * self.n
+        lib().orc_set_wall_q(self.h, _dp(self._wq))
 
     def wall_node(self, ind, f):
         """Wall condition (on-node or wet-node) of dense node ind on streamed f[Q]."""

@@ -92,3 +92,36 @@ def square_duct_profile(n, terms=99):
     Sf = square_duct_series(n, terms, fine)
     mean = np.trapezoid(np.trapezoid(Sf, fine, axis=1), fine) / (2 * a) ** 2
     return square_duct_series(n, terms) / mean
+
+
+def sphere_wall_q(shape, centre, r, q=9):
+    """Solid nodes inside a circle (2D, shape (ny, nx)) or sphere (3D, shape
+    (nz, ny, nx)) of radius r, the wall links of the fluid nodes into them and
+    each link's wall fraction: the smaller root t in (0, 1] of |p + t c| = r
+    with p the node's offset from the centre.  Returns (solid, bb, wall_q)
+    with wall_q shaped shape + (19,)."""
+    c = LT.velocities(q).astype(float)
+    grids = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
+    pos = [g.astype(float) for g in grids[::-1]]          # x, y[, z]
+    ctr = list(centre)
+    rel = [pos[a] - ctr[a] for a in range(len(shape))]
+    solid = sum(p * p for p in rel) <= r * r
+    bb = np.zeros(shape, np.uint32)
+    wq = np.zeros(shape + (19,))
+    for d in range(1, q):
+        cd = c[d, :len(shape)]
+        nb = tuple((grids[::-1][a] + int(cd[a])) for a in range(len(shape)))
+        inside = np.zeros(shape, bool)
+        ok = np.ones(shape, bool)
+        for a in range(len(shape)):
+            ok &= (nb[a] >= 0) & (nb[a] < shape[::-1][a])
+        idx = tuple(np.clip(nb[a], 0, shape[::-1][a] - 1) for a in range(len(shape)))[::-1]
+        inside[ok] = solid[idx][ok]
+        link = ~solid & inside
+        cc = float(cd @ cd)
+        pc = sum(rel[a] * cd[a] for a in range(len(shape)))
+        pp = sum(rel[a] * rel[a] for a in range(len(shape)))
+        t = (-pc - np.sqrt(np.maximum(pc * pc - cc * (pp - r * r), 0.0))) / cc
+        bb |= np.where(link, np.uint32(1 << d), np.uint32(0))
+        wq[..., d] = np.where(link, t, 0.0)
+    return solid, bb, wq

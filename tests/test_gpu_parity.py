@@ -25,7 +25,9 @@ def rel(a, b):
 def pair(oracle, q, shape, cls, bb=None, storage=L.STORE_F64, **kw):
     print(f"case q={q} shape={shape} storage={storage} {kw.get('collision')} {kw.get('force_model')}", flush=True)
     g = L.Solver(q, shape, cls, bb, storage=storage, **kw)
-    o = oracle.Sim(q, shape, cls, bb, **kw)
+    o = oracle.Sim(q, shape, cls, bb, **{k: v for k, v in kw.items() if k != "wall_q"})
+    if kw.get("wall_q") is not None:
+        o.set_wall_q(kw["wall_q"])
     return g, o
 
 

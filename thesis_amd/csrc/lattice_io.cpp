@@ -229,3 +229,65 @@ extern "C" lbm_status lbm_read_lattice(const char* node_path, const char* bc_pat
     if (nlinks) *nlinks = cnt;
     return LBM_OK;
 }
+
+// Wall fractions of the WALL links: q = |x_bc - x_node| / (delta |c_d|), the
+// BC line's point being where the link meets the wall (S/GpuInit.cu:1088, :1205).
+extern "C" lbm_status lbm_read_wall_q(const char* node_path, const char* bc_path, double* wall_q) {
+    if (!node_path || !bc_path || !wall_q) return LBM_EINVAL;
+    int32_t dims[3];
+    double delta = 1.0;
+    lbm_status st = lbm_read_lattice(node_path, nullptr, dims, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                     nullptr);
+    if (st != LBM_OK) return st;
+    const int64_t nx = dims[0], ny = dims[1], n = nx * ny * dims[2];
+    std::string txt;
+    if (!slurp(node_path, txt)) return LBM_EIO;
+    std::vector<double> xyz((size_t)n * 3, 0.0);
+    const char* p = txt.data();
+    const char* end = p + txt.size();
+    double v[16];
+    int cols = 0;
+    while (p < end) {
+        int c = parse_line(p, end, v, 16);
+        if (c == 0) continue;
+        if (cols == 0) cols = c;
+        if (c != cols || (cols != 5 && cols != 7)) return LBM_EIO;
+        const bool t3 = cols == 7;
+        const int64_t i = (int64_t)v[0], j = (int64_t)v[1], k = t3 ? (int64_t)v[2] : 0;
+        const int64_t at = i + j * nx + k * nx * ny;
+        if (at < 0 || at >= n) return LBM_EIO;
+        xyz[3 * at] = v[t3 ? 3 : 2];
+        xyz[3 * at + 1] = v[t3 ? 4 : 3];
+        xyz[3 * at + 2] = t3 ? v[5] : 0.0;
+    }
+    const bool three_d = cols == 7;
+    {   // grid spacing as lbm_read_lattice reports it
+        std::vector<uint8_t> cls((size_t)n);
+        int64_t nl = 0;
+        st = lbm_read_lattice(node_path, nullptr, dims, &nl, cls.data(), nullptr, nullptr, nullptr, &delta, nullptr);
+        if (st != LBM_OK) return st;
+        if (!(delta > 0)) delta = 1.0;
+    }
+    std::memset(wall_q, 0, sizeof(double) * 19 * (size_t)n);
+    std::string bct;
+    if (!slurp(bc_path, bct)) return LBM_EIO;
+    p = bct.data();
+    end = p + bct.size();
+    const int want = three_d ? 9 : 7;
+    while (p < end) {
+        int c = parse_line(p, end, v, 16);
+        if (c == 0) continue;
+        if (c != want) return LBM_EIO;
+        const int64_t i = (int64_t)v[0], j = (int64_t)v[1], k = three_d ? (int64_t)v[2] : 0;
+        const int dir = (int)v[three_d ? 3 : 2], type = (int)v[three_d ? 4 : 3];
+        if (type != 1) continue;
+        const int64_t at = i + j * nx + k * nx * ny;
+        if (at < 0 || at >= n || dir < 1 || dir >= (three_d ? 19 : 9)) return LBM_EIO;
+        const double bx = v[three_d ? 5 : 4], by = v[three_d ? 6 : 5], bz = three_d ? v[7] : 0.0;
+        const double dx = bx - xyz[3 * at], dy = by - xyz[3 * at + 1], dz = bz - xyz[3 * at + 2];
+        // |c_d|: 1 on the axes, sqrt(2) on the diagonals (D2Q9 5-8, D3Q19 7-18)
+        const double len = (three_d ? dir >= 7 : dir >= 5) ? std::sqrt(2.0) : 1.0;
+        wall_q[at * 19 + dir] = std::sqrt(dx * dx + dy * dy + dz * dz) / (delta * len);
+    }
+    return LBM_OK;
+}

@@ -39,6 +39,8 @@ struct lbm_ctx {
     uint32_t* nodew = nullptr;    // compact node words
     uint32_t* rowmeta = nullptr;  // per row and word: fluid mask, first compact index
     uint32_t* blk = nullptr;      // per workgroup: first row, rows spanned
+    uint32_t* qslot = nullptr;    // curved walls: per compact node, row of qv
+    double* qv = nullptr;         // curved walls: 19 wall fractions per row
     size_t nf = 0;                // non-solid nodes
     double *rho = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
     double *up = nullptr, *vp = nullptr, *wp = nullptr, *rp = nullptr;
@@ -174,7 +176,7 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     if (p->collision < 0 || p->collision > 2) return reject("bad collision model");
     if (p->storage < 0 || p->storage > 3) return reject("bad storage");
     if (p->force_model < 0 || p->force_model > 2) return reject("bad force model");
-    if (p->wall_model < 0 || p->wall_model > 2) return reject("bad wall model");
+    if (p->wall_model < 0 || p->wall_model > 3) return reject("bad wall model");
     if (p->wall_model == LBM_WALL_HH && p->q != 19) return reject("wet-node walls are D3Q19 only");
     if (p->inlet_model < 0 || p->inlet_model > 1) return reject("bad inlet model");
     if (p->outlet_model < 0 || p->outlet_model > 4) return reject("bad outlet model");
@@ -285,6 +287,24 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
             didx.push_back((uint32_t)i);
             nodew.push_back(node[i]);
         }
+    }
+    // Curved walls: the wall fractions of every node with wall links, one row
+    // of 19 per such node (entries <= 0 or no table: 1/2)
+    std::vector<uint32_t> qslot;
+    std::vector<double> qv;
+    if (p->wall_model == LBM_WALL_CURVED) {
+        qslot.assign(nf > 0 ? nf : 1, 0xFFFFFFFFu);
+        for (size_t j = 0; j < nf; ++j) {
+            const size_t i = didx[j];
+            if (!(node[i] & kBBMask)) continue;
+            qslot[j] = (uint32_t)(qv.size() / 19);
+            for (int d = 0; d < 19; ++d) {
+                const double q = (lat->wall_q && d < p->q) ? lat->wall_q[i * 19 + d] : 0.0;
+                if (q > 1.0) return reject("wall fractions must lie in (0, 1]");
+                qv.push_back(q);
+            }
+        }
+        if (qv.empty()) qv.assign(19, 0.0);
     }
     // Row metadata of the rank step: per row and 32-node word, the fluid mask
     // and the compact index of the word's first fluid node; per 256-node
@@ -414,6 +434,10 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     if (st == LBM_OK) st = dalloc(c, (void**)&c->nodew, sizeof(uint32_t) * (nf > 0 ? nf : 1));
     if (st == LBM_OK) st = dalloc(c, (void**)&c->rowmeta, sizeof(uint32_t) * rowmeta.size());
     if (st == LBM_OK) st = dalloc(c, (void**)&c->blk, sizeof(uint32_t) * blk.size());
+    if (st == LBM_OK && !qslot.empty()) {
+        st = dalloc(c, (void**)&c->qslot, sizeof(uint32_t) * qslot.size());
+        if (st == LBM_OK) st = dalloc(c, (void**)&c->qv, sizeof(double) * qv.size());
+    }
     double** fl[] = {&c->rho, &c->u, &c->v, &c->w, &c->up, &c->vp, &c->wp, &c->rp};
     for (double** a : fl)
         if (st == LBM_OK) st = dalloc(c, (void**)a, sizeof(double) * n);
@@ -439,10 +463,12 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     k.rblock = (int)rblock;
     k.rmax = rmax;
     k.simple = simple ? 1 : 0;
+    k.qslot = c->qslot;
+    k.qv = c->qv;
     // the rank step addresses the tile layout only, and has no extrapolating
     // outlet, 3D face or wet-node wall
     const bool rank_fits = rank_ok && k.tile && !(any_outlet && p->outlet_model >= LBM_OUTLET_EXTRAP2) &&
-                           !any_face3 && p->wall_model != LBM_WALL_HH;
+                           !any_face3 && p->wall_model != LBM_WALL_HH && p->wall_model != LBM_WALL_CURVED;
     k.rank = rank_fits ? 1 : 0;
     if (const char* e = std::getenv("LBM_RANK")) k.rank = rank_fits && e[0] == '1';
     k.sr = 1;
@@ -453,7 +479,9 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
         (nf > 0 && h2d(c->didx, didx.data(), sizeof(uint32_t) * nf) != hipSuccess) ||
         (nf > 0 && h2d(c->nodew, nodew.data(), sizeof(uint32_t) * nf) != hipSuccess) ||
         h2d(c->rowmeta, rowmeta.data(), sizeof(uint32_t) * rowmeta.size()) != hipSuccess ||
-        h2d(c->blk, blk.data(), sizeof(uint32_t) * blk.size()) != hipSuccess)
+        h2d(c->blk, blk.data(), sizeof(uint32_t) * blk.size()) != hipSuccess ||
+        (c->qslot && h2d(c->qslot, qslot.data(), sizeof(uint32_t) * qslot.size()) != hipSuccess) ||
+        (c->qv && h2d(c->qv, qv.data(), sizeof(double) * qv.size()) != hipSuccess))
         return cleanup(LBM_EDEVICE);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(LBM_EDEVICE);
     if (c->ubc) {
@@ -476,7 +504,7 @@ void lbm_destroy(lbm_ctx* c) {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
     void* bufs[] = {c->Palloc[0], c->Palloc[1], c->node, c->cidx, c->didx, c->nodew, c->rowmeta, c->blk, c->rho, c->u, c->v, c->w, c->up, c->vp, c->wp,
-                    c->rp, c->ubc, c->vbc, c->wbc, c->fout, c->om[0], c->om[1], c->partials, c->out4};
+                    c->rp, c->qslot, c->qv, c->ubc, c->vbc, c->wbc, c->fout, c->om[0], c->om[1], c->partials, c->out4};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->ev0) hipEventDestroy(c->ev0);

@@ -253,7 +253,7 @@ __device__ inline void gather(const typename Io<L, S>::raw* __restrict__ src, co
         int pl = i;
         if (((nd >> o) & 1u) || j == kNone) {
             j = own;
-            pl = ((nd >> o) & 1u) && kp.wall == 0 ? o : i;
+            pl = ((nd >> o) & 1u) && (kp.wall == 0 || kp.wall == 3) ? o : i;
         }
         h[i] = Io<L, S>::ld(src, paddr<L>(kp, pl, j), i);
     }
@@ -554,6 +554,41 @@ __device__ inline void sym3d(uint32_t cls, T* h) {
 
 __device__ inline bool is_outlet3(uint32_t cls) { return cls == PRES_E || (cls >= OUT_W && cls <= OUT_T); }
 
+// Curved walls (Bouzidi, Firdaouss & Lallemand 2001, linear): after a
+// half-way gather, h[opp d] = f*_d(x) for every wall link d; with the link's
+// wall fraction q the unknown becomes
+//   q < 1/2:  2q f*_d(x) + (1 - 2q) f*_d(x - c_d)   (f*_d(x - c_d) is the pulled h[d]),
+//   q >= 1/2: f*_d(x) / 2q + (2q - 1)/2q f*_opp(d)(x),
+// the textbook form of S/GpuBoundaries.cu:414-431 (whose q index and second
+// node are off by one link).  Coefficients sum to one, so deviations combine
+// directly.  q < 1/2 without a fluid node at x - c_d keeps bounce-back.
+template <class L, int S>
+__device__ inline void bouzidi(const typename Io<L, S>::raw* __restrict__ src, const KParams& kp, uint32_t own,
+                               uint32_t nd, const uint32_t* cn, typename Io<L, S>::T* h) {
+    using T = typename Io<L, S>::T;
+    const uint32_t slot = kp.qslot[own];
+    if (slot == kNone) return;
+    const double* q = kp.qv + (size_t)slot * 19;
+    T g[L::Q];
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) g[i] = h[i];
+#pragma unroll
+    for (int d = 1; d < L::Q; ++d) {
+        const int o = L::opp(d);
+        if (!((nd >> d) & 1u)) continue;
+        double qd = q[d];
+        if (qd <= 0.0) qd = 0.5;
+        const T fd = g[o];                       // f*_d(x), gathered for the unknown
+        if (qd < 0.5) {
+            if (((nd >> o) & 1u) || cn[d] == kNone) continue;
+            h[o] = (T)(2.0 * qd) * fd + (T)(1.0 - 2.0 * qd) * g[d];
+        } else {
+            const T fo = Io<L, S>::ld(src, paddr<L>(kp, o, own), o);
+            h[o] = (T)(0.5 / qd) * fd + (T)((2.0 * qd - 1.0) / (2.0 * qd)) * fo;
+        }
+    }
+}
+
 // Populations of a node after streaming and its inlet and wall conditions,
 // i.e. everything before the outlet (the reference orders inlet, wall,
 // outlet: S/Iterate.cu:468-473, S/Iterate3D.cu:602-621).
@@ -565,6 +600,7 @@ __device__ inline void inflow_state(const typename Io<L, S>::raw* __restrict__ s
     uint32_t cn[L::Q];
     source_index<L, OPEN>(cidx, kp, x, y, z, cn);
     gather<L, S, OPEN>(src, kp, own, nd, cn, h);
+    if (kp.wall == 3 && (nd & kBBMask)) bouzidi<L, S>(src, kp, own, nd, cn, h);
     const uint32_t cls = nd >> kClsShift;
     if (OPEN && L::Q == 9 && cls == VEL_W) zou_inlet<L, T>(kp, fl, idx, nd, h);
     if (OPEN && L::Q == 9 && cls == VEL_N) zou_lid<L, T>(kp, fl, idx, nd, h);

@@ -30,7 +30,7 @@ struct KParams {
     double s[19];            // MRT diag(S)
     double g[3];
     int force;               // 0 none, 1 shift, 2 guo
-    int wall;                // 0 halfway, 1 on-node, 2 wet-node (3D)
+    int wall;                // 0 halfway, 1 on-node, 2 wet-node (3D), 3 curved (Bouzidi)
     int inlet;               // 0 zou-he, 1 reference 2D (west inlet and north lid)
     int outlet;              // 0 pressure, 1 reference velocity (2D), 2/3 extrapolation, 4 f_eq (3D)
     uint32_t parity;
@@ -52,6 +52,8 @@ struct KParams {
     int rmax;                // rows staged per workgroup (LDS sizing)
     const uint2* rowmeta;    // per row and word: (fluid mask, compact index of first fluid node)
     const uint2* blk;        // per rblock-node workgroup: (first row, rows spanned)
+    const uint32_t* qslot;   // curved walls: per compact node, its row of qv (0xFFFFFFFF: none)
+    const double* qv;        // curved walls: 19 wall fractions per row
 };
 
 struct Fields {              // FP64 side arrays (N each)

@@ -464,7 +464,6 @@ int main(int argc, char** argv) {
         a.node_file = "Mesh/" + a.lattice + "_Node.dat";   // S/main.cu:97-100
         a.bc_file = "Mesh/" + a.lattice + "_BC.dat";
     }
-    if (a.curved) std::fprintf(stderr, "note: CURVED (Bouzidi) walls are not available; straight walls used\n");
 
     int32_t dims[3];
     int64_t nlinks = 0;
@@ -571,7 +570,25 @@ int main(int argc, char** argv) {
     p.storage = a.storage;
     // BCwallModel = HHmodel selects the wet-node walls in 3D (gpuBcComplexWall3D);
     // the reference's 2D walls are bounce-back whatever the setting
-    p.wall_model = three_d && a.hh_walls && !a.walls_set ? LBM_WALL_HH : a.wall_model;
+    // Walls = CURVED selects Bouzidi walls with the BC file's wall fractions,
+    // unless the 3D wet-node walls take the wall links (S/Iterate3D.cu:604-614)
+    p.wall_model = three_d && a.hh_walls && !a.walls_set ? LBM_WALL_HH
+                 : a.curved && !a.walls_set ? LBM_WALL_CURVED : a.wall_model;
+    std::vector<double> sq;
+    if (p.wall_model == LBM_WALL_CURVED) {
+        std::vector<double> fq((size_t)N * 19);
+        if (lbm_read_wall_q(a.node_file.c_str(), a.bc_file.c_str(), fq.data()) != LBM_OK) {
+            std::fprintf(stderr, "cannot read wall fractions from %s\n", a.bc_file.c_str());
+            return 2;
+        }
+        sq.assign(NS * 19, 0.0);
+        for (int z = 0; z < sim[2]; ++z)
+            for (int y = 0; y < sim[1]; ++y)
+                for (int x = 0; x < sim[0]; ++x) {
+                    const size_t s2 = ((size_t)z * sim[1] + y) * sim[0] + x, d = ((size_t)z * dims[1] + y) * dims[0] + x;
+                    for (int k = 0; k < 19; ++k) sq[s2 * 19 + k] = fq[d * 19 + k];
+                }
+    }
     p.inlet_model = a.inlet_model;
     if (!three_d) {
         p.outlet_model = a.outlet == 1 ? LBM_OUTLET_REF_VELOCITY : a.outlet == 2 ? LBM_OUTLET_EXTRAP2
@@ -584,7 +601,7 @@ int main(int argc, char** argv) {
                        : a.outlet == 5 ? LBM_OUTLET_PRESSURE : LBM_OUTLET_FEQ;
     }
     p.rho_out = a.rho_out > 0 ? a.rho_out : a.rho;
-    lbm_lattice lat = {scls.data(), sbb.data(), ubc.data(), vbc.data(), wbc.data()};
+    lbm_lattice lat = {scls.data(), sbb.data(), ubc.data(), vbc.data(), wbc.data(), sq.empty() ? nullptr : sq.data()};
     lbm_ctx* ctx = nullptr;
     if (lbm_create(&p, &lat, &ctx) != LBM_OK) { std::fprintf(stderr, "lbm_create failed\n"); return 2; }
     if (lbm_init(ctx, a.init_eq ? 0 : 1, rho.data(), u.data(), v.data(), w.data()) != LBM_OK) {

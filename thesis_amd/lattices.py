@@ -268,13 +268,14 @@ def droplet2d(nx, ny, radius, cx=None, cy=None):
     return r, 1.0 - r
 
 
-def write_latticegen2(path_prefix, node_class, bb=None, delta=1.0, inlet_dirs=None, outlet_dirs=None):
+def write_latticegen2(path_prefix, node_class, bb=None, delta=1.0, inlet_dirs=None, outlet_dirs=None, wall_q=None):
     """Write <prefix>_Node.dat and <prefix>_BC.dat in the latticegen2 layout
     (L/main.cpp:59-66, L/FluidTracking.cpp:229-236): 2D lines "i j x y fluid"
     and "i j dir type x y id", 3D with the k column.  WALL links come from
     `bb`; VEL_W / PRES_E / VEL_N nodes of a 2D lattice get INLE / OUTL / INLE
     links in the directions leaving the west / east / north face (wall links
-    excepted)."""
+    excepted).  A WALL line's point lies at the wall fraction wall_q[..., d]
+    of the link (1/2 when wall_q is None or 0 there)."""
     cls = np.asarray(node_class)
     two_d = cls.ndim == 2
     c3 = cls[None] if two_d else cls
@@ -294,8 +295,13 @@ def write_latticegen2(path_prefix, node_class, bb=None, delta=1.0, inlet_dirs=No
                     else:
                         fh.write(f"{i}\t{j}\t{k}\t{i * delta:.6f}\t{j * delta:.6f}\t{k * delta:.6f}\t{fl}\n")
     with open(path_prefix + "_BC.dat", "w") as fh:
+        wq = None if wall_q is None else (np.asarray(wall_q)[None] if two_d else np.asarray(wall_q))
+
         def link(i, j, k, d, t):
-            x, y, z = (i + 0.5 * c[d, 0]) * delta, (j + 0.5 * c[d, 1]) * delta, (k + 0.5 * c[d, 2]) * delta
+            f = 0.5
+            if t == 1 and wq is not None and wq[k, j, i, d] > 0:
+                f = float(wq[k, j, i, d])
+            x, y, z = (i + f * c[d, 0]) * delta, (j + f * c[d, 1]) * delta, (k + f * c[d, 2]) * delta
             if two_d:
                 fh.write(f"{i}\t{j}\t{d}\t{t}\t{x:.6f}\t{y:.6f}\t1\n")
             else:

@@ -22,7 +22,7 @@ SYM_W, SYM_E, SYM_S, SYM_N, SYM_B, SYM_T = 14, 15, 16, 17, 18, 19
 BGK, TRT, MRT = 0, 1, 2
 STORE_F64, STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2, 3
 FORCE_NONE, FORCE_SHIFT, FORCE_GUO = 0, 1, 2
-WALL_HALFWAY, WALL_ONNODE, WALL_HH = 0, 1, 2
+WALL_HALFWAY, WALL_ONNODE, WALL_HH, WALL_CURVED = 0, 1, 2, 3
 INLET_ZOUHE, INLET_REF2D = 0, 1
 OUTLET_PRESSURE, OUTLET_REF_VELOCITY, OUTLET_EXTRAP2, OUTLET_EXTRAP1, OUTLET_FEQ = 0, 1, 2, 3, 4
 PARITY_MRT_RHO2, PARITY_MRT_NO_F18, PARITY_OUTLET_V0, PARITY_INLET3D_FLIP = 0x1, 0x2, 0x8, 0x10
@@ -33,7 +33,7 @@ STORAGE_NAMES = {"f64": STORE_F64, "f32": STORE_F32, "f16": STORE_F16, "bf16": S
 EXPORTED = [
     "lbm_abi_version", "lbm_device_count", "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_init",
     "lbm_step", "lbm_run_to_residual", "lbm_reduce", "lbm_reduce_all", "lbm_residual_l2", "lbm_get_macros", "lbm_get_f", "lbm_set_f",
-    "lbm_time_steps", "lbm_device_bytes", "lbm_read_lattice",
+    "lbm_time_steps", "lbm_device_bytes", "lbm_read_lattice", "lbm_read_wall_q",
     "lbm_cg_create", "lbm_cg_destroy", "lbm_cg_last_error", "lbm_cg_init", "lbm_cg_step", "lbm_cg_get_macros",
     "lbm_cg_get_f", "lbm_cg_reduce", "lbm_cg_time_steps",
     "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_last_error", "lbm_batch_init", "lbm_batch_run",
@@ -71,6 +71,7 @@ class Lattice(C.Structure):
     _fields_ = [
         ("node_class", C.POINTER(C.c_uint8)), ("bb_links", C.POINTER(C.c_uint32)),
         ("u_bc", C.POINTER(C.c_double)), ("v_bc", C.POINTER(C.c_double)), ("w_bc", C.POINTER(C.c_double)),
+        ("wall_q", C.POINTER(C.c_double)),
     ]
 
 
@@ -125,6 +126,7 @@ def load():
     L.lbm_device_bytes.restype = C.c_int64
     L.lbm_read_lattice.argtypes = [C.c_char_p, C.c_char_p, P(C.c_int32), P(C.c_int64), P(C.c_uint8),
                                    P(C.c_uint32), P(C.c_int64), P(C.c_int64), P(C.c_double), P(C.c_int32)]
+    L.lbm_read_wall_q.argtypes = [C.c_char_p, C.c_char_p, dp]
     L.lbm_cg_create.argtypes = [P(CgParams), P(vp)]
     L.lbm_cg_destroy.argtypes = [vp]
     L.lbm_cg_destroy.restype = None
@@ -174,6 +176,19 @@ def device_count():
     return load().lbm_device_count()
 
 
+def read_wall_q(node_path, bc_path):
+    """Wall fractions (nz, ny, nx, 19) of the WALL links of a latticegen2 pair
+    (S/GpuInit.cu:1088, :1205); 0 where there is no WALL link."""
+    L = load()
+    r = read_lattice(node_path, None)
+    nx, ny, nz = r["dims"]
+    q = np.zeros(nx * ny * nz * 19)
+    st = L.lbm_read_wall_q(node_path.encode(), bc_path.encode(), _dp(q))
+    if st:
+        raise LbmError(st, f"cannot read {node_path} / {bc_path}")
+    return q.reshape(nz, ny, nx, 19)
+
+
 def read_lattice(node_path, bc_path=None):
     """Read a latticegen2 pair (S/FilesReading.cu:270-347).
 
@@ -212,7 +227,7 @@ class Solver:
     def __init__(self, q, shape, node_class, bb=None, *, collision=BGK, omega=1.0, trt_magic=0.0,
                  mrt_s=None, force_model=FORCE_NONE, g=(0.0, 0.0, 0.0), wall_model=WALL_HALFWAY,
                  inlet_model=INLET_ZOUHE, parity=0, rho_out=1.0, periodic=(0, 0, 0), ubc=None,
-                 vbc=None, wbc=None, storage=STORE_F64, outlet_model=OUTLET_PRESSURE):
+                 vbc=None, wbc=None, storage=STORE_F64, outlet_model=OUTLET_PRESSURE, wall_q=None):
         L = load()
         if isinstance(storage, str):
             storage = STORAGE_NAMES[storage]
@@ -236,11 +251,11 @@ class Solver:
         if cls.size != self.n:
             raise ValueError("node_class size does not match the lattice shape")
         bbv = None if bb is None else np.ascontiguousarray(bb, dtype=np.uint32).ravel()
-        self._keep = [cls, bbv, _f64(ubc, self.n), _f64(vbc, self.n), _f64(wbc, self.n)]
+        self._keep = [cls, bbv, _f64(ubc, self.n), _f64(vbc, self.n), _f64(wbc, self.n), _f64(wall_q, 19 * self.n)]
         lat = Lattice()
         lat.node_class = cls.ctypes.data_as(C.POINTER(C.c_uint8))
         lat.bb_links = bbv.ctypes.data_as(C.POINTER(C.c_uint32)) if bbv is not None else None
-        lat.u_bc, lat.v_bc, lat.w_bc = (_dp(a) for a in self._keep[2:])
+        lat.u_bc, lat.v_bc, lat.w_bc, lat.wall_q = (_dp(a) for a in self._keep[2:])
         h = C.c_void_p()
         st = L.lbm_create(C.byref(p), C.byref(lat), C.byref(h))
         if st:
