@@ -212,12 +212,16 @@ lbm_status lbm_read_lattice(const char* node_path, const char* bc_path, int32_t 
  * :1205); 0 for links without a WALL line.  wall_q holds nx*ny*nz*19 doubles. */
 lbm_status lbm_read_wall_q(const char* node_path, const char* bc_path, double* wall_q);
 
-/* ---- colour-gradient two-phase model (D2Q9, periodic box) ---------------- */
-/* Replaces the reference's multiphase path: gpuCollBgkwGC2D + gradient
- * (S/GpuCollision.cu:112-281, :552-622), gpuStreaming2DCG (S/GpuStream.cu:30-158),
- * gpuUpdateMacro2DCG (S/GpuUpdateMacro.cu:43-135) and the Laplace sums of
- * S/Iterate.cu:495-500, driven by S/Iterate.cu:392-500.  Colour parameters as
- * in S/main.cu:52-72; no body force. */
+/* ---- colour-gradient two-phase model (D2Q9 or D3Q19, periodic box) ------- */
+/* Replaces the reference's multiphase path.  D2Q9: gpuCollBgkwGC2D +
+ * gradient (S/GpuCollision.cu:112-281, :552-622), gpuStreaming2DCG
+ * (S/GpuStream.cu:30-158), gpuUpdateMacro2DCG (S/GpuUpdateMacro.cu:43-135) and
+ * the Laplace sums of S/Iterate.cu:495-500, driven by S/Iterate.cu:392-500.
+ * D3Q19 (nz >= 3): gpuCollBgkwGC3D + gradient (S/GpuCollision.cu:283-420,
+ * :709-779), gpuStreaming3DCG (S/GpuStream.cu:196-285), gpuUpdateMacro3DCG
+ * (S/GpuUpdateMacro.cu:215-361) and the sums of S/Iterate3D.cu:637-645,
+ * driven by S/Iterate3D.cu:495-800.  Colour parameters as in S/main.cu:52-72;
+ * no body force. */
 typedef struct lbm_cg_ctx lbm_cg_ctx;
 typedef struct {
     uint32_t abi_version;
@@ -227,9 +231,10 @@ typedef struct {
     double beta;              /* recolouring parameter */
     double A;                 /* surface-tension parameter: sigma = 4 A / (9 omega) */
     double g_limit;           /* gradient threshold */
-    int32_t high_order;       /* 0: 8-neighbour gradient, 1: 24-neighbour */
+    int32_t high_order;       /* 0: 8 / 18-neighbour gradient, 1: 24 / 104-neighbour */
     double kappa;             /* Laplace sampling threshold (control_param) */
     int32_t storage;          /* LBM_STORE_F64, _F32 or _F16 */
+    int32_t nz;               /* 0 or 1: D2Q9 box nx x ny; >= 3: D3Q19 box nx x ny x nz */
 } lbm_cg_params;
 
 lbm_status lbm_cg_create(const lbm_cg_params* p, lbm_cg_ctx** out);
@@ -237,14 +242,20 @@ void lbm_cg_destroy(lbm_cg_ctx* ctx);
 const char* lbm_cg_last_error(const lbm_cg_ctx* ctx);
 /* rest-state colours from colour densities (S/GpuInit.cu:323-352) */
 lbm_status lbm_cg_init(lbm_cg_ctx* ctx, const double* r_rho, const double* b_rho);
+/* start from colour populations r_f[q*N + i], b_f[q*N + i] (q = 9 or 19),
+ * e.g. the reference's steady bubble of initCGBubble3D (S/GpuInit.cu:543-592) */
+lbm_status lbm_cg_init_f(lbm_cg_ctx* ctx, const double* r_f, const double* b_f);
 lbm_status lbm_cg_step(lbm_cg_ctx* ctx, int64_t nsteps);
 lbm_status lbm_cg_get_macros(lbm_cg_ctx* ctx, double* r_rho, double* b_rho, double* rho, double* u, double* v);
+/* the same with the z velocity (zero for D2Q9); any pointer may be NULL */
+lbm_status lbm_cg_get_macros3d(lbm_cg_ctx* ctx, double* r_rho, double* b_rho, double* rho, double* u, double* v,
+                               double* w);
 /* post-streaming colour populations r_f[q*N + i], b_f[q*N + i] (needs one step) */
 lbm_status lbm_cg_get_f(lbm_cg_ctx* ctx, double* r_f, double* b_f);
 /* FP64 sums: 0 sum rho_r, 1 sum rho_b, 2-5 the reference's Laplace sums
  * (rho_r over chi >= kappa and count, rho_b over chi <= -kappa and count;
- * S/GpuUpdateMacro.cu:124-133), 6-9 bulk rho over chi > 0.99 and < -0.99 with
- * counts, 10 max |u| (spurious currents), 11 non-finite count. */
+ * S/GpuUpdateMacro.cu:124-133, :347-357), 6-9 bulk rho over chi > 0.99 and
+ * < -0.99 with counts, 10 max |u| (spurious currents), 11 non-finite count. */
 lbm_status lbm_cg_reduce(lbm_cg_ctx* ctx, double out[12]);
 lbm_status lbm_cg_time_steps(lbm_cg_ctx* ctx, int64_t nsteps, double* ms);
 

@@ -840,24 +840,29 @@ double orc_mass(const orc_sim* s)
 }
 
 /* ======================================================================== */
-/* colour-gradient two-phase model, D2Q9, periodic box.
- * Follows the GPU formulation gpuCollBgkwGC2D (S/GpuCollision.cu:552-622),
- * gradient calculateColorGradient / calculateHOColorGradient case 0
- * (S/GpuCollision.cu:112-281), streaming gpuStreaming2DCG case 0
- * (S/GpuStream.cu:30-158) and moments gpuUpdateMacro2DCG
- * (S/GpuUpdateMacro.cu:43-135), with periodic wrap instead of the reference's
- * edge-orientation stencils.                                                */
+/* colour-gradient two-phase model, D2Q9 or D3Q19, periodic box.
+ * Follows the GPU formulation: gpuCollBgkwGC2D / gpuCollBgkwGC3D
+ * (S/GpuCollision.cu:552-622, :709-779) with the gradients
+ * calculateColorGradient / calculateHOColorGradient (2D, :112-281) and
+ * calculateColorGradient3D / calculateHOColorGradient3D case 0 (3D, :283-420),
+ * streaming gpuStreaming2DCG / gpuStreaming3DCG (S/GpuStream.cu:30-158,
+ * :196-285) and moments gpuUpdateMacro2DCG / gpuUpdateMacro3DCG
+ * (S/GpuUpdateMacro.cu:43-135, :215-361), with periodic wrap instead of the
+ * reference's edge-orientation stencils.                                    */
 struct orc_cg {
     orc_cg_params p;
     long n;
+    int q, nz;
+    int cx[19], cy[19], cz[19];
+    double w[19], phi[19], teta[19], bpert[19], cn[19];
     double *r, *b, *rc, *bc, *f;          /* colour populations, post-collision, total */
-    double *rr, *br, *rho, *u, *v;        /* stored macros */
-    int hx[25], hy[25];
-    double hw[25];
+    double *rr, *br, *rho, *u, *v, *w3;   /* stored macros */
+    int hx[105], hy[105], hz[105];
+    double hw[105];
     int nst;
 };
 
-/* CG constants (S/GpuInit.cu:150-186) */
+/* 2D constants (S/GpuInit.cu:150-186) */
 static const double CG_PHI[9] = {0.0, 0.2, 0.2, 0.2, 0.2, 0.05, 0.05, 0.05, 0.05};
 static const double CG_TETA[9] = {1.0, -0.2, -0.2, -0.2, -0.2, -0.05, -0.05, -0.05, -0.05};
 static const double CG_BPERT[9] = {-4.0 / 27.0, 2.0 / 27.0, 2.0 / 27.0, 2.0 / 27.0, 2.0 / 27.0,
@@ -869,32 +874,65 @@ orc_cg* orc_cg_create(const orc_cg_params* p)
 {
     orc_cg* s = (orc_cg*)calloc(1, sizeof *s);
     s->p = *p;
-    s->n = (long)p->nx * p->ny;
-    size_t n9 = (size_t)s->n * 9;
-    s->r = (double*)calloc(n9, 8); s->b = (double*)calloc(n9, 8);
-    s->rc = (double*)calloc(n9, 8); s->bc = (double*)calloc(n9, 8);
-    s->f = (double*)calloc(n9, 8);
+    s->nz = p->nz > 1 ? p->nz : 1;
+    s->q = s->nz > 1 ? 19 : 9;
+    s->n = (long)p->nx * p->ny * s->nz;
+    const int q = s->q;
+    for (int i = 0; i < q; ++i) {
+        if (q == 9) {
+            s->cx[i] = CX9[i]; s->cy[i] = CY9[i]; s->cz[i] = 0;
+            s->w[i] = W9[i]; s->phi[i] = CG_PHI[i]; s->teta[i] = CG_TETA[i]; s->bpert[i] = CG_BPERT[i];
+        } else {
+            /* 3D constants (S/GpuInit.cu:900-935): phi, theta and B by shell */
+            s->cx[i] = CX19[i]; s->cy[i] = CY19[i]; s->cz[i] = CZ19[i];
+            s->w[i] = i == 0 ? 1.0 / 3.0 : (i <= 6 ? 1.0 / 18.0 : 1.0 / 36.0);
+            s->phi[i] = i == 0 ? 0.0 : (i <= 6 ? 1.0 / 12.0 : 1.0 / 24.0);
+            s->teta[i] = i == 0 ? 1.0 : (i <= 6 ? -1.0 / 12.0 : -1.0 / 24.0);
+            s->bpert[i] = i == 0 ? -2.0 / 9.0 : (i <= 6 ? 1.0 / 54.0 : 1.0 / 27.0);
+        }
+        s->cn[i] = sqrt((double)(s->cx[i] * s->cx[i] + s->cy[i] * s->cy[i] + s->cz[i] * s->cz[i]));
+    }
+    size_t nq = (size_t)s->n * q;
+    s->r = (double*)calloc(nq, 8); s->b = (double*)calloc(nq, 8);
+    s->rc = (double*)calloc(nq, 8); s->bc = (double*)calloc(nq, 8);
+    s->f = (double*)calloc(nq, 8);
     s->rr = (double*)calloc(s->n, 8); s->br = (double*)calloc(s->n, 8);
     s->rho = (double*)calloc(s->n, 8); s->u = (double*)calloc(s->n, 8); s->v = (double*)calloc(s->n, 8);
-    if (p->high_order) {
+    s->w3 = (double*)calloc(s->n, 8);
+    int k = 0;
+    if (p->high_order && q == 9) {
         /* 24-neighbour isotropic stencil; weights by |c|^2 over 5040
          * (S/GpuInit.cu:186-199): 1 -> 960, 2 -> 448, 4 -> 84, 5 -> 32, 8 -> 1 */
-        int k = 0;
         for (int dy = -2; dy <= 2; ++dy)
             for (int dx = -2; dx <= 2; ++dx) {
                 int c2 = dx * dx + dy * dy;
                 if (c2 == 0) continue;
                 double wt = c2 == 1 ? 960 : c2 == 2 ? 448 : c2 == 4 ? 84 : c2 == 5 ? 32 : 1;
-                s->hx[k] = dx; s->hy[k] = dy; s->hw[k] = wt / 5040.0; ++k;
+                s->hx[k] = dx; s->hy[k] = dy; s->hz[k] = 0; s->hw[k] = wt / 5040.0; ++k;
             }
-        s->nst = k;
+    } else if (p->high_order) {
+        /* 104-neighbour stencil of S/GpuInit.cu:950-990: the points of
+         * [-2,2]^3 with |c|^2 in {1,2,3,4,5,6,9} (the (2,2,1) shell, not
+         * (2,2,0)), weights 4/45, 1/21, 2/105, 5/504, 1/315, 1/630, 1/5040 */
+        for (int dz = -2; dz <= 2; ++dz)
+            for (int dy = -2; dy <= 2; ++dy)
+                for (int dx = -2; dx <= 2; ++dx) {
+                    int c2 = dx * dx + dy * dy + dz * dz;
+                    double wt = c2 == 1 ? 4.0 / 45 : c2 == 2 ? 1.0 / 21 : c2 == 3 ? 2.0 / 105
+                              : c2 == 4 ? 5.0 / 504 : c2 == 5 ? 1.0 / 315 : c2 == 6 ? 1.0 / 630
+                              : c2 == 9 ? 1.0 / 5040 : 0.0;
+                    if (wt == 0.0) continue;
+                    s->hx[k] = dx; s->hy[k] = dy; s->hz[k] = dz; s->hw[k] = wt; ++k;
+                }
     } else {
-        for (int i = 1; i < 9; ++i) {
-            s->hx[i - 1] = CX9[i]; s->hy[i - 1] = CY9[i];
-            s->hw[i - 1] = (i < 5) ? 4.0 / 12.0 : 1.0 / 12.0;   /* cg_w, S/GpuInit.cu:186 */
+        for (int i = 1; i < q; ++i) {
+            s->hx[i - 1] = s->cx[i]; s->hy[i - 1] = s->cy[i]; s->hz[i - 1] = s->cz[i];
+            /* cg_w: 2D 4/12, 1/12 (S/GpuInit.cu:186); 3D 1/6, 1/12 (:928) */
+            s->hw[i - 1] = q == 9 ? ((i < 5) ? 4.0 / 12.0 : 1.0 / 12.0) : ((i <= 6) ? 1.0 / 6.0 : 1.0 / 12.0);
         }
-        s->nst = 8;
+        k = q - 1;
     }
+    s->nst = k;
     return s;
 }
 
@@ -902,8 +940,22 @@ void orc_cg_destroy(orc_cg* s)
 {
     if (!s) return;
     free(s->r); free(s->b); free(s->rc); free(s->bc); free(s->f);
-    free(s->rr); free(s->br); free(s->rho); free(s->u); free(s->v);
+    free(s->rr); free(s->br); free(s->rho); free(s->u); free(s->v); free(s->w3);
     free(s);
+}
+
+/* macros of the populations in r, b (velocity without the force shift) */
+static void cg_moments(orc_cg* s, long ind)
+{
+    const long n = s->n;
+    double R = 0, B = 0, jx = 0, jy = 0, jz = 0;
+    for (int k = 0; k < s->q; ++k) {
+        double rv = s->r[k * n + ind], bv = s->b[k * n + ind];
+        R += rv; B += bv;
+        jx += s->cx[k] * (rv + bv); jy += s->cy[k] * (rv + bv); jz += s->cz[k] * (rv + bv);
+    }
+    s->rr[ind] = R; s->br[ind] = B; s->rho[ind] = R + B;
+    s->u[ind] = jx / (R + B); s->v[ind] = jy / (R + B); s->w3[ind] = jz / (R + B);
 }
 
 void orc_cg_init(orc_cg* s, const double* r_rho, const double* b_rho)
@@ -911,66 +963,83 @@ void orc_cg_init(orc_cg* s, const double* r_rho, const double* b_rho)
     const long n = s->n;
     for (long ind = 0; ind < n; ++ind) {
         double rr = r_rho[ind], br = b_rho[ind];
-        for (int i = 0; i < 9; ++i) {
-            s->r[i * n + ind] = rr * (CG_PHI[i] + CG_TETA[i] * s->p.r_alpha);
-            s->b[i * n + ind] = br * (CG_PHI[i] + CG_TETA[i] * s->p.b_alpha);
+        for (int i = 0; i < s->q; ++i) {
+            s->r[i * n + ind] = rr * (s->phi[i] + s->teta[i] * s->p.r_alpha);
+            s->b[i * n + ind] = br * (s->phi[i] + s->teta[i] * s->p.b_alpha);
             s->f[i * n + ind] = s->r[i * n + ind] + s->b[i * n + ind];
         }
         s->rr[ind] = rr; s->br[ind] = br; s->rho[ind] = rr + br;
-        s->u[ind] = 0; s->v[ind] = 0;
+        s->u[ind] = 0; s->v[ind] = 0; s->w3[ind] = 0;
     }
 }
 
-static void cg_grad_node(const orc_cg* s, int x, int y, double* gx, double* gy)
+void orc_cg_init_f(orc_cg* s, const double* r_f, const double* b_f)
 {
-    const int nx = s->p.nx, ny = s->p.ny;
-    double ax = 0, ay = 0;
+    const long n = s->n;
+    for (long j = 0; j < n * s->q; ++j) {
+        s->r[j] = r_f[j]; s->b[j] = b_f[j]; s->f[j] = r_f[j] + b_f[j];
+    }
+    for (long ind = 0; ind < n; ++ind) cg_moments(s, ind);
+}
+
+static int cg_wrap(int a, int n) { a %= n; return a < 0 ? a + n : a; }
+
+static void cg_grad_node(const orc_cg* s, int x, int y, int z, double* g)
+{
+    const int nx = s->p.nx, ny = s->p.ny, nz = s->nz;
+    double ax = 0, ay = 0, az = 0;
     for (int k = 0; k < s->nst; ++k) {
-        int xx = (x + s->hx[k] + nx) % nx, yy = (y + s->hy[k] + ny) % ny;
-        long j = (long)yy * nx + xx;
+        int xx = cg_wrap(x + s->hx[k], nx), yy = cg_wrap(y + s->hy[k], ny), zz = cg_wrap(z + s->hz[k], nz);
+        long j = ((long)zz * ny + yy) * nx + xx;
         double psi = s->hw[k] * (s->rr[j] - s->br[j]) / s->rho[j];
         ax += psi * s->hx[k];
         ay += psi * s->hy[k];
+        az += psi * s->hz[k];
     }
-    *gx = ax; *gy = ay;
+    g[0] = ax; g[1] = ay; g[2] = az;
 }
 
-void orc_cg_gradient(const orc_cg* s, double* cgx, double* cgy)
+void orc_cg_gradient(const orc_cg* s, double* cgx, double* cgy, double* cgz)
 {
-    for (long ind = 0; ind < s->n; ++ind)
-        cg_grad_node(s, (int)(ind % s->p.nx), (int)(ind / s->p.nx), &cgx[ind], &cgy[ind]);
+    const long nx = s->p.nx, ny = s->p.ny;
+    for (long ind = 0; ind < s->n; ++ind) {
+        double g[3];
+        cg_grad_node(s, (int)(ind % nx), (int)(ind / nx % ny), (int)(ind / (nx * ny)), g);
+        cgx[ind] = g[0]; cgy[ind] = g[1];
+        if (cgz) cgz[ind] = g[2];
+    }
 }
 
 void orc_cg_step(orc_cg* s, int nsteps)
 {
     const long n = s->n;
-    const int nx = s->p.nx, ny = s->p.ny;
+    const int nx = s->p.nx, ny = s->p.ny, nz = s->nz, q = s->q;
     const orc_cg_params* P = &s->p;
     set_threads(P->nthreads);
     for (int it = 0; it < nsteps; ++it) {
 #pragma omp parallel for
         for (long ind = 0; ind < n; ++ind) {
-            int x = (int)(ind % nx), y = (int)(ind / nx);
-            double cgx, cgy;
-            cg_grad_node(s, x, y, &cgx, &cgy);
+            int x = (int)(ind % nx), y = (int)(ind / nx % ny), z = (int)(ind / ((long)nx * ny));
+            double g[3];
+            cg_grad_node(s, x, y, z, g);
             double rr = s->rr[ind], br = s->br[ind], r = s->rho[ind];
-            double u = s->u[ind], v = s->v[ind];
+            double u = s->u[ind], v = s->v[ind], w = s->w3[ind];
             double mean_nu = 1.0 / (rr / (r * P->r_nu) + br / (r * P->b_nu));
             double om = 1.0 / (3.0 * mean_nu + 0.5);
             double alpha = P->r_alpha * rr / r + P->b_alpha * br / r;
-            double gn = sqrt(cgx * cgx + cgy * cgy);
+            double gn = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
             double kr = rr / r, kb = br / r, kk = P->beta * rr * br / r;
-            double uu = u * u + v * v;
-            for (int k = 0; k < 9; ++k) {
+            double uu = u * u + v * v + w * w;
+            for (int k = 0; k < q; ++k) {
                 double cs = 0, pert = 0;
                 if (gn > P->g_limit) {
-                    double cg = CX9[k] * cgx + CY9[k] * cgy;
-                    if (k) cs = cg / (gn * sqrt((double)(CX9[k] * CX9[k] + CY9[k] * CY9[k])));
-                    pert = P->A * gn * (W9[k] * cg * cg / (gn * gn) - CG_BPERT[k]);
+                    double cg = s->cx[k] * g[0] + s->cy[k] * g[1] + s->cz[k] * g[2];
+                    if (k) cs = cg / (gn * s->cn[k]);
+                    pert = P->A * gn * (s->w[k] * cg * cg / (gn * gn) - s->bpert[k]);
                 }
-                double cu = u * CX9[k] + v * CY9[k];
-                double rest = CG_PHI[k] + CG_TETA[k] * alpha;
-                double feq = r * (rest + W9[k] * (3.0 * cu + 4.5 * cu * cu - 1.5 * uu));
+                double cu = u * s->cx[k] + v * s->cy[k] + w * s->cz[k];
+                double rest = s->phi[k] + s->teta[k] * alpha;
+                double feq = r * (rest + s->w[k] * (3.0 * cu + 4.5 * cu * cu - 1.5 * uu));
                 double fs = om * feq + (1.0 - om) * s->f[k * n + ind] + pert;
                 s->rc[k * n + ind] = kr * fs + kk * cs * rest;
                 s->bc[k * n + ind] = kb * fs - kk * cs * rest;
@@ -978,31 +1047,33 @@ void orc_cg_step(orc_cg* s, int nsteps)
         }
 #pragma omp parallel for
         for (long ind = 0; ind < n; ++ind) {
-            int x = (int)(ind % nx), y = (int)(ind / nx);
+            int x = (int)(ind % nx), y = (int)(ind / nx % ny), z = (int)(ind / ((long)nx * ny));
             double rr0 = s->rr[ind], br0 = s->br[ind], r0 = s->rho[ind];
-            double R = 0, B = 0, jx = 0, jy = 0;
-            for (int k = 0; k < 9; ++k) {
-                int sx = (x - CX9[k] + nx) % nx, sy = (y - CY9[k] + ny) % ny;
-                long src = (long)sy * nx + sx;
+            double R = 0, B = 0, jx = 0, jy = 0, jz = 0;
+            for (int k = 0; k < q; ++k) {
+                int sx = cg_wrap(x - s->cx[k], nx), sy = cg_wrap(y - s->cy[k], ny), sz = cg_wrap(z - s->cz[k], nz);
+                long src = ((long)sz * ny + sy) * nx + sx;
                 double rv = s->rc[k * n + src], bv = s->bc[k * n + src];
                 s->r[k * n + ind] = rv;
                 s->b[k * n + ind] = bv;
                 s->f[k * n + ind] = rv + bv;
                 R += rv; B += bv;
-                jx += CX9[k] * (rv + bv);
-                jy += CY9[k] * (rv + bv);
+                jx += s->cx[k] * (rv + bv);
+                jy += s->cy[k] * (rv + bv);
+                jz += s->cz[k] * (rv + bv);
             }
-            /* omega_eff from the previous densities (S/GpuUpdateMacro.cu:62-64) */
+            /* omega_eff from the previous densities (S/GpuUpdateMacro.cu:62-64, :233-235) */
             double mean_nu = 1.0 / (rr0 / (r0 * P->r_nu) + br0 / (r0 * P->b_nu));
             double om = 1.0 / (3.0 * mean_nu + 0.5);
             double r = R + B;
             s->u[ind] = jx / r + (P->external_force ? P->g / (r * om) : 0.0);
             s->v[ind] = jy / r + (P->external_force ? 0.0 : P->g / om);
+            s->w3[ind] = jz / r;
         }
 #pragma omp parallel for
         for (long ind = 0; ind < n; ++ind) {
             double R = 0, B = 0;
-            for (int k = 0; k < 9; ++k) { R += s->r[k * n + ind]; B += s->b[k * n + ind]; }
+            for (int k = 0; k < q; ++k) { R += s->r[k * n + ind]; B += s->b[k * n + ind]; }
             s->rr[ind] = R; s->br[ind] = B; s->rho[ind] = R + B;
         }
     }
@@ -1011,9 +1082,9 @@ void orc_cg_step(orc_cg* s, int nsteps)
 void orc_cg_get(const orc_cg* s, double* r_f, double* b_f, double* r_rho, double* b_rho,
                 double* rho, double* u, double* v)
 {
-    size_t n = (size_t)s->n, b9 = n * 9 * 8, b1 = n * 8;
-    if (r_f) memcpy(r_f, s->r, b9);
-    if (b_f) memcpy(b_f, s->b, b9);
+    size_t n = (size_t)s->n, bq = n * s->q * 8, b1 = n * 8;
+    if (r_f) memcpy(r_f, s->r, bq);
+    if (b_f) memcpy(b_f, s->b, bq);
     if (r_rho) memcpy(r_rho, s->rr, b1);
     if (b_rho) memcpy(b_rho, s->br, b1);
     if (rho) memcpy(rho, s->rho, b1);
@@ -1021,7 +1092,9 @@ void orc_cg_get(const orc_cg* s, double* r_f, double* b_f, double* r_rho, double
     if (v) memcpy(v, s->v, b1);
 }
 
-/* Laplace sampling masks (S/GpuUpdateMacro.cu:124-133) and max |u|. */
+void orc_cg_get_w(const orc_cg* s, double* w) { memcpy(w, s->w3, (size_t)s->n * 8); }
+
+/* Laplace sampling masks (S/GpuUpdateMacro.cu:124-133, :347-357) and max |u|. */
 void orc_cg_laplace(const orc_cg* s, double* out)
 {
     double pin = 0, nin = 0, pout = 0, nout = 0, umax = 0;
@@ -1029,7 +1102,7 @@ void orc_cg_laplace(const orc_cg* s, double* out)
         double chi = (s->rr[ind] - s->br[ind]) / s->rho[ind];
         if (chi >= s->p.kappa) { pin += s->rr[ind]; nin += 1; }
         else if (chi <= -s->p.kappa) { pout += s->br[ind]; nout += 1; }
-        double um = sqrt(s->u[ind] * s->u[ind] + s->v[ind] * s->v[ind]);
+        double um = sqrt(s->u[ind] * s->u[ind] + s->v[ind] * s->v[ind] + s->w3[ind] * s->w3[ind]);
         if (um > umax) umax = um;
     }
     out[0] = pin; out[1] = nin; out[2] = pout; out[3] = nout; out[4] = umax;

@@ -113,7 +113,7 @@ void orc_open_boundary(const orc_sim* s, long ind, double* f);
 /* The wall condition of node ind (on-node bounce-back or wet-node walls). */
 void orc_wall_node(const orc_sim* s, long ind, double* f);
 
-/* ---- colour-gradient two-phase model (D2Q9) ----------------------------- */
+/* ---- colour-gradient two-phase model (D2Q9, or D3Q19 when nz > 1) ------ */
 typedef struct {
     int nx, ny;
     double r_alpha, b_alpha;     /* rest fractions alpha_k */
@@ -125,6 +125,7 @@ typedef struct {
     double g; int external_force;/* force (S/GpuUpdateMacro.cu:118-119) */
     double kappa;                /* Laplace sampling threshold (control_param) */
     int nthreads;
+    int nz;                      /* > 1: D3Q19 box nx x ny x nz */
 } orc_cg_params;
 
 typedef struct orc_cg orc_cg;
@@ -132,13 +133,17 @@ orc_cg* orc_cg_create(const orc_cg_params* p);
 void orc_cg_destroy(orc_cg* s);
 /* rest-state colours from the given density fields (S/GpuInit.cu:323-352) */
 void orc_cg_init(orc_cg* s, const double* r_rho, const double* b_rho);
+/* colour populations r_f[q*N + i], b_f[q*N + i] as the state before the
+ * first collision, e.g. the reference's steady bubble of initCGBubble3D (S/GpuInit.cu:543-592) */
+void orc_cg_init_f(orc_cg* s, const double* r_f, const double* b_f);
 void orc_cg_step(orc_cg* s, int nsteps);
 void orc_cg_get(const orc_cg* s, double* r_f, double* b_f, double* r_rho, double* b_rho,
                 double* rho, double* u, double* v);
+void orc_cg_get_w(const orc_cg* s, double* w);
 /* out[0]=sum p_in, out[1]=n_in, out[2]=sum p_out, out[3]=n_out, out[4]=max|u| */
 void orc_cg_laplace(const orc_cg* s, double* out);
-/* gradient field (cg_x, cg_y) for tests of the stencil */
-void orc_cg_gradient(const orc_cg* s, double* cgx, double* cgy);
+/* gradient field (cg_x, cg_y, cg_z) for tests of the stencil; cgz may be NULL */
+void orc_cg_gradient(const orc_cg* s, double* cgx, double* cgy, double* cgz);
 
 #ifdef __cplusplus
 }

@@ -43,7 +43,7 @@ class OrcCgParams(C.Structure):
         ("nx", C.c_int), ("ny", C.c_int), ("r_alpha", C.c_double), ("b_alpha", C.c_double),
         ("r_nu", C.c_double), ("b_nu", C.c_double), ("beta", C.c_double), ("A", C.c_double),
         ("g_limit", C.c_double), ("high_order", C.c_int), ("g", C.c_double),
-        ("external_force", C.c_int), ("kappa", C.c_double), ("nthreads", C.c_int),
+        ("external_force", C.c_int), ("kappa", C.c_double), ("nthreads", C.c_int), ("nz", C.c_int),
     ]
 
 
@@ -85,10 +85,12 @@ def lib():
         L.orc_cg_create.argtypes = [P(OrcCgParams)]
         L.orc_cg_destroy.argtypes = [C.c_void_p]
         L.orc_cg_init.argtypes = [C.c_void_p, dp, dp]
+        L.orc_cg_init_f.argtypes = [C.c_void_p, dp, dp]
+        L.orc_cg_get_w.argtypes = [C.c_void_p, dp]
         L.orc_cg_step.argtypes = [C.c_void_p, C.c_int]
         L.orc_cg_get.argtypes = [C.c_void_p] + [dp] * 7
         L.orc_cg_laplace.argtypes = [C.c_void_p, dp]
-        L.orc_cg_gradient.argtypes = [C.c_void_p, dp, dp]
+        L.orc_cg_gradient.argtypes = [C.c_void_p, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -214,16 +216,20 @@ class Sim:
 
 
 class CgSim:
-    """Colour-gradient D2Q9 oracle on a periodic nx x ny box; arrays are (ny, nx)."""
+    """Colour-gradient oracle on a periodic box: D2Q9 for CgSim(nx, ny) with
+    arrays (ny, nx), D3Q19 for CgSim(nx, ny, nz=...) with arrays (nz, ny, nx)."""
 
-    def __init__(self, nx, ny, *, r_alpha=4 / 9, b_alpha=4 / 9, r_nu=1 / 6, b_nu=1 / 6, beta=0.7,
+    def __init__(self, nx, ny, *, nz=1, r_alpha=4 / 9, b_alpha=4 / 9, r_nu=1 / 6, b_nu=1 / 6, beta=0.7,
                  A=0.01125, g_limit=0.0, high_order=0, g=0.0, external_force=0, kappa=0.9,
                  nthreads=0):
         p = OrcCgParams()
         p.nx, p.ny, p.r_alpha, p.b_alpha, p.r_nu, p.b_nu = nx, ny, r_alpha, b_alpha, r_nu, b_nu
         p.beta, p.A, p.g_limit, p.high_order, p.g = beta, A, g_limit, high_order, g
-        p.external_force, p.kappa, p.nthreads = external_force, kappa, nthreads
-        self.params, self.nx, self.ny, self.n = p, nx, ny, nx * ny
+        p.external_force, p.kappa, p.nthreads, p.nz = external_force, kappa, nthreads, nz
+        self.params, self.nx, self.ny, self.nz = p, nx, ny, nz
+        self.n = nx * ny * nz
+        self.q = 19 if nz > 1 else 9
+        self.shape = (nz, ny, nx) if nz > 1 else (ny, nx)
         self.h = lib().orc_cg_create(C.byref(p))
 
     def __del__(self):
@@ -235,16 +241,24 @@ class CgSim:
         r = _f64(r_rho); b = _f64(b_rho)
         lib().orc_cg_init(self.h, _dp(r), _dp(b))
 
+    def init_f(self, r_f, b_f):
+        r = _f64(r_f); b = _f64(b_f)
+        lib().orc_cg_init_f(self.h, _dp(r), _dp(b))
+
     def step(self, n=1):
         lib().orc_cg_step(self.h, int(n))
 
     def get(self):
-        n = self.n
-        arrs = [np.zeros(9 * n), np.zeros(9 * n)] + [np.zeros(n) for _ in range(5)]
+        n, q = self.n, self.q
+        arrs = [np.zeros(q * n), np.zeros(q * n)] + [np.zeros(n) for _ in range(5)]
         lib().orc_cg_get(self.h, *[_dp(a) for a in arrs])
-        sh9, sh = (9, self.ny, self.nx), (self.ny, self.nx)
+        shq = (q,) + self.shape
         keys = ["r_f", "b_f", "r_rho", "b_rho", "rho", "u", "v"]
-        return {k: a.reshape(sh9 if i < 2 else sh) for i, (k, a) in enumerate(zip(keys, arrs))}
+        out = {k: a.reshape(shq if i < 2 else self.shape) for i, (k, a) in enumerate(zip(keys, arrs))}
+        w = np.zeros(n)
+        lib().orc_cg_get_w(self.h, _dp(w))
+        out["w"] = w.reshape(self.shape)
+        return out
 
     def laplace(self):
         out = np.zeros(5)
@@ -252,9 +266,11 @@ class CgSim:
         return out
 
     def gradient(self):
-        gx = np.zeros(self.n); gy = np.zeros(self.n)
-        lib().orc_cg_gradient(self.h, _dp(gx), _dp(gy))
-        return gx.reshape(self.ny, self.nx), gy.reshape(self.ny, self.nx)
+        g = [np.zeros(self.n) for _ in range(3)]
+        lib().orc_cg_gradient(self.h, *[_dp(a) for a in g])
+        if self.nz > 1:
+            return tuple(a.reshape(self.shape) for a in g)
+        return g[0].reshape(self.shape), g[1].reshape(self.shape)
 
 
 def ref_mrt_tables(q, omega):

@@ -83,3 +83,28 @@ def test_cg_static_droplet_config():
         assert r["u64"] < 1e-3 and r["u16"] < 1e-3          # spurious currents
         assert r["dmass"] < 1e-10                             # red mass, FP64 storage
         assert r["err_dp"] < 0.1                              # FP16 storage vs FP64
+
+
+def test_cg_init_f_f64_matches_oracle(oracle):
+    """Start from given colour populations (perturbed rest states)."""
+    nx, ny = 40, 36
+    print(f"case: cg init_f parity {nx}x{ny}", flush=True)
+    r, b = LT.droplet2d(nx, ny, 9, cx=20.2, cy=17.6)
+    phi = np.array([0.0, 0.2, 0.2, 0.2, 0.2, 0.05, 0.05, 0.05, 0.05])
+    teta = np.array([1.0, -0.2, -0.2, -0.2, -0.2, -0.05, -0.05, -0.05, -0.05])
+    rest = phi + teta * 0.3
+    rng = np.random.default_rng(7)
+    rf = rest[:, None, None] * r[None] * (1 + 0.05 * rng.random((9, ny, nx)))
+    bf = rest[:, None, None] * b[None] * (1 + 0.05 * rng.random((9, ny, nx)))
+    kw = dict(beta=0.7, A=0.01125, r_nu=1 / 6, b_nu=0.1, r_alpha=0.3, b_alpha=0.3, high_order=1)
+    g = L.CgSolver(nx, ny, storage=L.STORE_F64, **kw)
+    o = oracle.CgSim(nx, ny, **kw)
+    g.init_f(rf, bf)
+    o.init_f(rf, bf)
+    g.step(50)
+    o.step(50)
+    gm, om = g.macros(), o.get()
+    for k in ("r_rho", "b_rho", "rho", "u", "v"):
+        assert np.abs(gm[k] - om[k]).max() <= 1e-12 * max(np.abs(om[k]).max(), 1e-3), k
+    gr, gb = g.get_f()
+    assert np.abs(gr - om["r_f"]).max() <= 1e-12 and np.abs(gb - om["b_f"]).max() <= 1e-12

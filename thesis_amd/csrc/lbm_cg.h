@@ -1,0 +1,45 @@
+// Shared declarations of the colour-gradient kernels (D2Q9 and D3Q19,
+// periodic box) and their C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace lbm {
+namespace cg {
+
+struct CgParams {
+    int nx, ny;
+    double r_alpha, b_alpha, r_nu, b_nu, beta, A, g_limit, kappa;
+    uint32_t plane;          // elements per plane (2Q planes: Q deviations h = f - w, Q red)
+    int nz;                  // 1 for D2Q9
+};
+
+constexpr int NRED = 12;     // reductions of k_cg_reduce
+
+}  // namespace cg
+
+// D2Q9 (lbm_cg.hip)
+hipError_t cg_launch_step(int storage, int ho, const void* src, void* dst, const cg::CgParams& cp, double* mac,
+                          bool wm, hipStream_t st);
+hipError_t cg_launch_init(int storage, int ho, void* dst, const cg::CgParams& cp, const double* rr,
+                          const double* rb, const double* rf, const double* bf, double* mac, hipStream_t st);
+hipError_t cg_launch_recon(int storage, const void* src, const cg::CgParams& cp, double* rf, double* bf,
+                           hipStream_t st);
+// colour densities rr, rb of FP64 populations rf, bf (q planes of n)
+hipError_t cg_launch_dens(int q, const double* rf, const double* bf, size_t n, double* rr, double* rb,
+                          hipStream_t st);
+// mac: rho_r, rho_b, rho, u, v, w (n each; w ignored when nz == 1)
+hipError_t cg_launch_reduce(const double* mac, size_t n, bool three_d, double kappa, double* partials,
+                            int nblocks, hipStream_t st);
+
+// D3Q19 (lbm_cg3.hip): phase field of the post-streaming state, then the step
+hipError_t cg3_launch_step(int storage, int ho, const void* src, void* dst, void* psi, const cg::CgParams& cp,
+                           double* mac, bool wm, hipStream_t st);
+hipError_t cg3_launch_init(int storage, int ho, void* dst, void* psi, const cg::CgParams& cp, const double* rr,
+                           const double* rb, const double* rf, const double* bf, double* mac, hipStream_t st);
+hipError_t cg3_launch_recon(int storage, const void* src, const cg::CgParams& cp, double* rf, double* bf,
+                            hipStream_t st);
+
+}  // namespace lbm

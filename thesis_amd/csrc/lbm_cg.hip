@@ -14,7 +14,7 @@
 // red population r (b = f - r); both colours are pulled, the phase field of
 // the 32x16 node tile and a 1- or 2-node halo goes to LDS, then every node
 // collides and recolours in registers.
-#include "lbm_internal.h"
+#include "lbm_cg.h"
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -36,12 +36,6 @@ __host__ __device__ constexpr double CNORM(int i) { return i == 0 ? 1.0 : (i <= 
 __host__ __device__ constexpr double HOW(int c2) {
     return c2 == 1 ? 960.0 / 5040 : c2 == 2 ? 448.0 / 5040 : c2 == 4 ? 84.0 / 5040 : c2 == 5 ? 32.0 / 5040 : 1.0 / 5040;
 }
-
-struct CgParams {
-    int nx, ny;
-    double r_alpha, b_alpha, r_nu, b_nu, beta, A, g_limit, kappa;
-    uint32_t plane;          // elements per plane (18 planes: 9 deviations, 9 red)
-};
 
 // storage: FP64 / FP32 raw; FP16 scaled by 1024/w (deviation) and 1/w (red)
 template <int S> struct CIo;
@@ -232,10 +226,13 @@ __global__ __launch_bounds__(THREADS) void k_cg_step(const typename CIo<S>::raw*
 }
 
 // First post-collision state from colour densities (rest equilibria,
-// S/GpuInit.cu:323-352): gradient straight from the density arrays.
+// S/GpuInit.cu:323-352), or from given colour populations rf, bf (FP64,
+// 9 planes of N) when rf is not null: gradient straight from the density
+// arrays rr0, rb0 (the densities of rf, bf in the second case).
 template <int S, int H>
 __global__ __launch_bounds__(256) void k_cg_init(typename CIo<S>::raw* __restrict__ dst, CgParams cp,
                                                  const double* __restrict__ rr0, const double* __restrict__ rb0,
+                                                 const double* __restrict__ rf, const double* __restrict__ bf,
                                                  double* __restrict__ mac) {
     using T = typename CIo<S>::T;
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
@@ -254,13 +251,19 @@ __global__ __launch_bounds__(256) void k_cg_init(typename CIo<S>::raw* __restric
             gy += (T)(wk * dy * p);
         }
     const size_t idx = (size_t)y * nx + x;
+    const size_t n = (size_t)nx * ny;
     const T rr = (T)rr0[idx], rb = (T)rb0[idx];
     T h[9], r[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-        const T rr_i = rr * (T)(PHI(i) + TETA(i) * cp.r_alpha), rb_i = rb * (T)(PHI(i) + TETA(i) * cp.b_alpha);
-        r[i] = rr_i;
-        h[i] = rr_i + rb_i - (T)W9(i);
+        if (rf) {
+            r[i] = (T)rf[i * n + idx];
+            h[i] = (T)(rf[i * n + idx] + bf[i * n + idx] - W9(i));
+        } else {
+            const T rr_i = rr * (T)(PHI(i) + TETA(i) * cp.r_alpha), rb_i = rb * (T)(PHI(i) + TETA(i) * cp.b_alpha);
+            r[i] = rr_i;
+            h[i] = rr_i + rb_i - (T)W9(i);
+        }
     }
     T ho[9], ro[9], m[5];
     cg_collide<T>(cp, h, r, gx, gy, ho, ro, m);
@@ -269,9 +272,20 @@ __global__ __launch_bounds__(256) void k_cg_init(typename CIo<S>::raw* __restric
         CIo<S>::st(dst, (uint32_t)i * cp.plane + (uint32_t)idx, ho[i], i, false);
         CIo<S>::st(dst, (uint32_t)(9 + i) * cp.plane + (uint32_t)idx, ro[i], i, true);
     }
-    const size_t n = (size_t)nx * ny;
     mac[idx] = (double)rr; mac[n + idx] = (double)rb; mac[2 * n + idx] = (double)(rr + rb);
-    mac[3 * n + idx] = 0.0; mac[4 * n + idx] = 0.0;
+    mac[3 * n + idx] = rf ? (double)m[3] : 0.0;
+    mac[4 * n + idx] = rf ? (double)m[4] : 0.0;
+}
+
+// Colour densities of FP64 populations (q planes of n).
+__global__ __launch_bounds__(256) void k_cg_dens(int q, const double* __restrict__ rf, const double* __restrict__ bf,
+                                                 size_t n, double* __restrict__ rr, double* __restrict__ rb) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a = 0, b = 0;
+    for (int k = 0; k < q; ++k) { a += rf[k * n + i]; b += bf[k * n + i]; }
+    rr[i] = a;
+    rb[i] = b;
 }
 
 // Post-stream colour populations of the state in src (FP64, full).
@@ -292,12 +306,11 @@ __global__ __launch_bounds__(256) void k_cg_recon(const typename CIo<S>::raw* __
 
 // FP64 reductions over the macroscopic fields: 0 sum rho_r, 1 sum rho_b,
 // 2/3 reference Laplace sums over chi >= kappa (rho_r, count), 4/5 over
-// chi <= -kappa (rho_b, count) (S/GpuUpdateMacro.cu:124-133), 6/7 bulk
-// rho over chi > 0.99 (sum, count), 8/9 over chi < -0.99, 10 max |u|,
-// 11 non-finite count.
-constexpr int NRED = 12;
-__global__ __launch_bounds__(256) void k_cg_reduce(const double* __restrict__ mac, size_t n, double kappa,
-                                                   double* __restrict__ partials) {
+// chi <= -kappa (rho_b, count) (S/GpuUpdateMacro.cu:124-133, :347-357),
+// 6/7 bulk rho over chi > 0.99 (sum, count), 8/9 over chi < -0.99,
+// 10 max |u|, 11 non-finite count.  w is null for D2Q9.
+__global__ __launch_bounds__(256) void k_cg_reduce(const double* __restrict__ mac, const double* __restrict__ wz,
+                                                   size_t n, double kappa, double* __restrict__ partials) {
     double a[NRED];
     for (int k = 0; k < NRED; ++k) a[k] = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -308,7 +321,8 @@ __global__ __launch_bounds__(256) void k_cg_reduce(const double* __restrict__ ma
         else if (chi <= -kappa) { a[4] += rb; a[5] += 1; }
         if (chi > 0.99) { a[6] += rho; a[7] += 1; }
         if (chi < -0.99) { a[8] += rho; a[9] += 1; }
-        const double um = sqrt(u * u + v * v);
+        const double w = wz ? wz[i] : 0.0;
+        const double um = sqrt(u * u + v * v + w * w);
         a[10] = fmax(a[10], um);
         if (!(um == um) || !(rho == rho)) a[11] += 1;
     }
@@ -334,10 +348,9 @@ __global__ __launch_bounds__(256) void k_cg_reduce(const double* __restrict__ ma
 }  // namespace cg
 
 // ---------------------------------------------------------------------------
-// launchers (declared in lbm_cg.h-free form for the ABI file)
-hipError_t cg_launch_step(int storage, int ho, const void* src, void* dst, const void* cpv, double* mac, bool wm,
-                          hipStream_t st) {
-    const cg::CgParams& cp = *(const cg::CgParams*)cpv;
+// launchers (lbm_cg.h)
+hipError_t cg_launch_step(int storage, int ho, const void* src, void* dst, const cg::CgParams& cp, double* mac,
+                          bool wm, hipStream_t st) {
     dim3 grid((cp.nx + cg::TX - 1) / cg::TX, (cp.ny + cg::TY - 1) / cg::TY), block(cg::TX, 8);
     const int w = wm ? 1 : 0;
 #define CG_STEP(S_, H_)                                                                                     \
@@ -350,13 +363,12 @@ hipError_t cg_launch_step(int storage, int ho, const void* src, void* dst, const
     return hipGetLastError();
 }
 
-hipError_t cg_launch_init(int storage, int ho, void* dst, const void* cpv, const double* rr, const double* rb,
-                          double* mac, hipStream_t st) {
-    const cg::CgParams& cp = *(const cg::CgParams*)cpv;
+hipError_t cg_launch_init(int storage, int ho, void* dst, const cg::CgParams& cp, const double* rr, const double* rb,
+                          const double* rf, const double* bf, double* mac, hipStream_t st) {
     dim3 grid((cp.nx + 255) / 256, cp.ny), block(256);
 #define CG_INIT(S_, H_)                                                                                       \
     hipLaunchKernelGGL((cg::k_cg_init<S_, H_>), grid, block, 0, st, (typename cg::CIo<S_>::raw*)dst, cp, rr, \
-                       rb, mac)
+                       rb, rf, bf, mac)
     if (storage == 0) { if (ho) CG_INIT(0, 2); else CG_INIT(0, 1); }
     else if (storage == 1) { if (ho) CG_INIT(1, 2); else CG_INIT(1, 1); }
     else { if (ho) CG_INIT(2, 2); else CG_INIT(2, 1); }
@@ -364,8 +376,8 @@ hipError_t cg_launch_init(int storage, int ho, void* dst, const void* cpv, const
     return hipGetLastError();
 }
 
-hipError_t cg_launch_recon(int storage, const void* src, const void* cpv, double* rf, double* bf, hipStream_t st) {
-    const cg::CgParams& cp = *(const cg::CgParams*)cpv;
+hipError_t cg_launch_recon(int storage, const void* src, const cg::CgParams& cp, double* rf, double* bf,
+                           hipStream_t st) {
     dim3 grid((cp.nx + 255) / 256, cp.ny), block(256);
     if (storage == 0) hipLaunchKernelGGL((cg::k_cg_recon<0>), grid, block, 0, st, (const double*)src, cp, rf, bf);
     else if (storage == 1) hipLaunchKernelGGL((cg::k_cg_recon<1>), grid, block, 0, st, (const float*)src, cp, rf, bf);
@@ -373,13 +385,17 @@ hipError_t cg_launch_recon(int storage, const void* src, const void* cpv, double
     return hipGetLastError();
 }
 
-hipError_t cg_launch_reduce(const double* mac, size_t n, double kappa, double* partials, int nblocks,
-                            hipStream_t st) {
-    hipLaunchKernelGGL(cg::k_cg_reduce, dim3(nblocks), dim3(256), 0, st, mac, n, kappa, partials);
+hipError_t cg_launch_dens(int q, const double* rf, const double* bf, size_t n, double* rr, double* rb,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(cg::k_cg_dens, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, q, rf, bf, n, rr, rb);
     return hipGetLastError();
 }
 
-size_t cg_params_size() { return sizeof(cg::CgParams); }
-int cg_nred() { return cg::NRED; }
+hipError_t cg_launch_reduce(const double* mac, size_t n, bool three_d, double kappa, double* partials, int nblocks,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(cg::k_cg_reduce, dim3(nblocks), dim3(256), 0, st, mac, three_d ? mac + 5 * n : nullptr, n,
+                       kappa, partials);
+    return hipGetLastError();
+}
 
 }  // namespace lbm

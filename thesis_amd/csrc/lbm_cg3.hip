@@ -1,0 +1,331 @@
+// Colour-gradient two-phase step for D3Q19 (periodic box).
+//
+// Follows the reference's GPU formulation in 3D: gradient of the phase field
+// (rho_r - rho_b)/rho with the 18-neighbour stencil or the 104-neighbour one
+// (calculateColorGradient3D / calculateHOColorGradient3D case 0,
+// S/GpuCollision.cu:283-420), BGK collision towards the two-phase
+// equilibrium with effective viscosity and rest fraction, perturbation and
+// recolouring (gpuCollBgkwGC3D, S/GpuCollision.cu:709-779), pull streaming of
+// both colours (gpuStreaming3DCG, S/GpuStream.cu:196-285) and moments
+// (gpuUpdateMacro3DCG, S/GpuUpdateMacro.cu:215-361).  The stored state is the
+// post-collision colours, as in the single-phase step.
+//
+// Per link the storage holds h = f - w and the red population r (b = f - r),
+// 38 planes.  A step is two launches: k_cg3_psi pulls each node's
+// post-streaming colours and stores the phase field psi; k_cg3_step pulls
+// them again, takes the gradient of psi over its neighbours (served from
+// L1/L2: each psi value is read by 18 or 104 nodes), collides and recolours
+// in registers and stores the new post-collision state.
+#include "lbm_cg.h"
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace lbm {
+namespace cg3 {
+
+using cg::CgParams;
+
+constexpr int BX = 64, BY = 4;   // one wavefront per x row segment, 4 rows per block
+__device__ constexpr int CX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+__device__ constexpr int CY[19] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
+__device__ constexpr int CZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+__host__ __device__ constexpr double W19(int i) { return i == 0 ? 1.0 / 3.0 : (i <= 6 ? 1.0 / 18.0 : 1.0 / 36.0); }
+// phi_k, theta_k, B_k by shell (S/GpuInit.cu:900-915)
+__host__ __device__ constexpr double PHI(int i) { return i == 0 ? 0.0 : (i <= 6 ? 1.0 / 12.0 : 1.0 / 24.0); }
+__host__ __device__ constexpr double TETA(int i) { return i == 0 ? 1.0 : (i <= 6 ? -1.0 / 12.0 : -1.0 / 24.0); }
+__host__ __device__ constexpr double BPERT(int i) { return i == 0 ? -2.0 / 9.0 : (i <= 6 ? 1.0 / 54.0 : 1.0 / 27.0); }
+__host__ __device__ constexpr double CNORM(int i) { return i <= 6 ? 1.0 : 1.4142135623730950488; }
+// gradient weights by |c|^2: 18-neighbour stencil 1/6, 1/12 (S/GpuInit.cu:928);
+// 104-neighbour stencil (S/GpuInit.cu:950-990): shells 1, 2, 3, 4, 5, 6 and 9
+// (the (2,2,1) points), 0 for the points of [-2,2]^3 it leaves out
+__host__ __device__ constexpr double LOW(int c2) { return c2 == 1 ? 1.0 / 6.0 : c2 == 2 ? 1.0 / 12.0 : 0.0; }
+__host__ __device__ constexpr double HOW(int c2) {
+    return c2 == 1 ? 4.0 / 45 : c2 == 2 ? 1.0 / 21 : c2 == 3 ? 2.0 / 105 : c2 == 4 ? 5.0 / 504
+         : c2 == 5 ? 1.0 / 315 : c2 == 6 ? 1.0 / 630 : c2 == 9 ? 1.0 / 5040 : 0.0;
+}
+
+// storage: FP64 / FP32 raw; FP16 scaled by 1024/w (deviation) and 1/w (red)
+template <int S> struct CIo;
+template <> struct CIo<0> {
+    using raw = double; using T = double;
+    __device__ static T ld(const raw* p, uint32_t k, int, bool) { return p[k]; }
+    __device__ static void st(raw* p, uint32_t k, T v, int, bool) { __builtin_nontemporal_store(v, p + k); }
+};
+template <> struct CIo<1> {
+    using raw = float; using T = float;
+    __device__ static T ld(const raw* p, uint32_t k, int, bool) { return p[k]; }
+    __device__ static void st(raw* p, uint32_t k, T v, int, bool) { __builtin_nontemporal_store(v, p + k); }
+};
+template <> struct CIo<2> {
+    using raw = uint16_t; using T = float;
+    __device__ static T ld(const raw* p, uint32_t k, int i, bool red) {
+        const float s = red ? (float)W19(i) : (float)(W19(i) / 1024.0);
+        return (float)__builtin_bit_cast(_Float16, p[k]) * s;
+    }
+    __device__ static void st(raw* p, uint32_t k, T v, int i, bool red) {
+        const float s = red ? (float)(1.0 / W19(i)) : (float)(1024.0 / W19(i));
+        __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, (_Float16)(v * s)), p + k);
+    }
+};
+
+__device__ inline int wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+// Post-streaming colours of node (x, y, z): pull from the post-collision
+// buffer with periodic wrap.
+template <int S>
+__device__ inline void pull(const typename CIo<S>::raw* __restrict__ src, const CgParams& cp, int x, int y, int z,
+                            typename CIo<S>::T* h, typename CIo<S>::T* r) {
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        const int sx = wrap(x - CX[i], cp.nx), sy = wrap(y - CY[i], cp.ny), sz = wrap(z - CZ[i], cp.nz);
+        const uint32_t k = ((uint32_t)sz * (uint32_t)cp.ny + (uint32_t)sy) * (uint32_t)cp.nx + (uint32_t)sx;
+        h[i] = CIo<S>::ld(src, (uint32_t)i * cp.plane + k, i, false);
+        r[i] = CIo<S>::ld(src, (uint32_t)(19 + i) * cp.plane + k, i, true);
+    }
+}
+
+template <class T>
+__device__ inline void densities(const T* h, const T* r, T& rho, T& rr) {
+    T d = 0, s = 0;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) { d += h[i]; s += r[i]; }
+    rho = (T)1 + d;
+    rr = s;
+}
+
+// Collision, perturbation and recolouring of one node (gpuCollBgkwGC3D):
+// new deviation h' = f* - w and red r'; mac = rho_r, rho_b, rho, u, v, w of
+// the incoming state.
+template <class T>
+__device__ inline void collide(const CgParams& cp, const T* h, const T* r, T gx, T gy, T gz, T* ho, T* ro, T* mac) {
+    T rho, rr;
+    densities<T>(h, r, rho, rr);
+    const T rb = rho - rr;
+    T jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int i = 1; i < 19; ++i) { jx += (T)CX[i] * h[i]; jy += (T)CY[i] * h[i]; jz += (T)CZ[i] * h[i]; }
+    const T ir = (T)1 / rho;
+    const T u = jx * ir, v = jy * ir, w = jz * ir;
+    const T mean_nu = (T)1 / (rr / (rho * (T)cp.r_nu) + rb / (rho * (T)cp.b_nu));
+    const T om = (T)1 / ((T)3 * mean_nu + (T)0.5);
+    const T alpha = (T)cp.r_alpha * rr * ir + (T)cp.b_alpha * rb * ir;
+    const T gn = sqrt(gx * gx + gy * gy + gz * gz);
+    const bool on = gn > (T)cp.g_limit;
+    const T ign = on ? (T)1 / gn : (T)0;
+    const T kr = rr * ir, kk = (T)cp.beta * rr * rb * ir;
+    const T uu = u * u + v * v + w * w;
+#pragma unroll
+    for (int k = 0; k < 19; ++k) {
+        const T cg = (T)CX[k] * gx + (T)CY[k] * gy + (T)CZ[k] * gz;
+        const T cs = (on && k) ? cg * ign * (T)(1.0 / CNORM(k)) : (T)0;
+        const T pert = on ? (T)cp.A * gn * ((T)W19(k) * cg * cg * ign * ign - (T)BPERT(k)) : (T)0;
+        const T rest = (T)PHI(k) + (T)TETA(k) * alpha;
+        const T cu = (T)CX[k] * u + (T)CY[k] * v + (T)CZ[k] * w;
+        // f_eq - w = (rho - 1) rest + (rest - w) + rho w (3cu + 4.5cu^2 - 1.5uu)
+        const T feqd = (rho - (T)1) * rest + (rest - (T)W19(k)) +
+                       rho * (T)W19(k) * ((T)3 * cu + (T)4.5 * cu * cu - (T)1.5 * uu);
+        const T fsd = om * feqd + ((T)1 - om) * h[k] + pert;       // f* - w
+        ho[k] = fsd;
+        ro[k] = kr * (fsd + (T)W19(k)) + kk * cs * rest;
+    }
+    mac[0] = rr; mac[1] = rb; mac[2] = rho; mac[3] = u; mac[4] = v; mac[5] = w;
+}
+
+// Gradient of the phase field at (x, y, z): psi(i) returns the phase field of
+// linear node index i.
+template <int H, class T, class Psi>
+__device__ inline void gradient(const CgParams& cp, int x, int y, int z, Psi psi, T& gx, T& gy, T& gz) {
+    int xs[2 * H + 1], ys[2 * H + 1], zs[2 * H + 1];
+#pragma unroll
+    for (int d = -H; d <= H; ++d) {
+        xs[d + H] = wrap(x + d, cp.nx);
+        ys[d + H] = wrap(y + d, cp.ny);
+        zs[d + H] = wrap(z + d, cp.nz);
+    }
+    T ax = 0, ay = 0, az = 0;
+#pragma unroll
+    for (int dz = -H; dz <= H; ++dz)
+#pragma unroll
+        for (int dy = -H; dy <= H; ++dy)
+#pragma unroll
+            for (int dx = -H; dx <= H; ++dx) {
+                const int c2 = dx * dx + dy * dy + dz * dz;
+                const double wk = H == 1 ? LOW(c2) : HOW(c2);
+                if (wk == 0.0) continue;
+                const uint32_t j = ((uint32_t)zs[dz + H] * (uint32_t)cp.ny + (uint32_t)ys[dy + H]) * (uint32_t)cp.nx +
+                                   (uint32_t)xs[dx + H];
+                const T p = (T)wk * psi(j);
+                ax += (T)dx * p;
+                ay += (T)dy * p;
+                az += (T)dz * p;
+            }
+    gx = ax; gy = ay; gz = az;
+}
+
+__device__ inline bool node_of(const CgParams& cp, int& x, int& y, int& z) {
+    x = blockIdx.x * BX + threadIdx.x;
+    y = blockIdx.y * BY + threadIdx.y;
+    z = blockIdx.z;
+    return x < cp.nx && y < cp.ny;
+}
+
+// Phase field of the post-streaming state.
+template <int S>
+__global__ __launch_bounds__(BX * BY) void k_cg3_psi(const typename CIo<S>::raw* __restrict__ src, CgParams cp,
+                                                     typename CIo<S>::T* __restrict__ psi) {
+    using T = typename CIo<S>::T;
+    int x, y, z;
+    if (!node_of(cp, x, y, z)) return;
+    T h[19], r[19], rho, rr;
+    pull<S>(src, cp, x, y, z, h, r);
+    densities<T>(h, r, rho, rr);
+    psi[((uint32_t)z * (uint32_t)cp.ny + (uint32_t)y) * (uint32_t)cp.nx + (uint32_t)x] = ((T)2 * rr - rho) / rho;
+}
+
+template <int S, int H>
+__global__ __launch_bounds__(BX * BY) void k_cg3_step(const typename CIo<S>::raw* __restrict__ src,
+                                                      typename CIo<S>::raw* __restrict__ dst,
+                                                      const typename CIo<S>::T* __restrict__ psi, CgParams cp,
+                                                      double* __restrict__ mac, int write_macros) {
+    using T = typename CIo<S>::T;
+    int x, y, z;
+    if (!node_of(cp, x, y, z)) return;
+    T h[19], r[19];
+    pull<S>(src, cp, x, y, z, h, r);
+    T gx, gy, gz;
+    gradient<H, T>(cp, x, y, z, [psi](uint32_t j) { return psi[j]; }, gx, gy, gz);
+    T ho[19], ro[19], m[6];
+    collide<T>(cp, h, r, gx, gy, gz, ho, ro, m);
+    const uint32_t idx = ((uint32_t)z * (uint32_t)cp.ny + (uint32_t)y) * (uint32_t)cp.nx + (uint32_t)x;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        CIo<S>::st(dst, (uint32_t)i * cp.plane + idx, ho[i], i, false);
+        CIo<S>::st(dst, (uint32_t)(19 + i) * cp.plane + idx, ro[i], i, true);
+    }
+    if (write_macros) {
+        const size_t n = (size_t)cp.nx * cp.ny * cp.nz;
+        for (int a = 0; a < 6; ++a) mac[a * n + idx] = (double)m[a];
+    }
+}
+
+// Phase field of colour densities.
+template <class T>
+__global__ __launch_bounds__(BX * BY) void k_cg3_psi0(const double* __restrict__ rr0, const double* __restrict__ rb0,
+                                                      CgParams cp, T* __restrict__ psi) {
+    int x, y, z;
+    if (!node_of(cp, x, y, z)) return;
+    const uint32_t j = ((uint32_t)z * (uint32_t)cp.ny + (uint32_t)y) * (uint32_t)cp.nx + (uint32_t)x;
+    psi[j] = (T)((rr0[j] - rb0[j]) / (rr0[j] + rb0[j]));
+}
+
+// First post-collision state from colour densities rr0, rb0 (rest
+// equilibria phi + theta alpha) or, when rf is not null, from the colour
+// populations rf, bf (FP64, 19 planes of N; rr0, rb0 their densities);
+// psi holds the phase field of rr0, rb0.
+template <int S, int H>
+__global__ __launch_bounds__(BX * BY) void k_cg3_init(typename CIo<S>::raw* __restrict__ dst, CgParams cp,
+                                                      const double* __restrict__ rr0, const double* __restrict__ rb0,
+                                                      const double* __restrict__ rf, const double* __restrict__ bf,
+                                                      const typename CIo<S>::T* __restrict__ psi,
+                                                      double* __restrict__ mac) {
+    using T = typename CIo<S>::T;
+    int x, y, z;
+    if (!node_of(cp, x, y, z)) return;
+    T gx, gy, gz;
+    gradient<H, T>(cp, x, y, z, [psi](uint32_t j) { return psi[j]; }, gx, gy, gz);
+    const size_t n = (size_t)cp.nx * cp.ny * cp.nz;
+    const uint32_t idx = ((uint32_t)z * (uint32_t)cp.ny + (uint32_t)y) * (uint32_t)cp.nx + (uint32_t)x;
+    const T rr = (T)rr0[idx], rb = (T)rb0[idx];
+    T h[19], r[19];
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        if (rf) {
+            r[i] = (T)rf[i * n + idx];
+            h[i] = (T)(rf[i * n + idx] + bf[i * n + idx] - W19(i));
+        } else {
+            const T rr_i = rr * (T)(PHI(i) + TETA(i) * cp.r_alpha), rb_i = rb * (T)(PHI(i) + TETA(i) * cp.b_alpha);
+            r[i] = rr_i;
+            h[i] = rr_i + rb_i - (T)W19(i);
+        }
+    }
+    T ho[19], ro[19], m[6];
+    collide<T>(cp, h, r, gx, gy, gz, ho, ro, m);
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        CIo<S>::st(dst, (uint32_t)i * cp.plane + idx, ho[i], i, false);
+        CIo<S>::st(dst, (uint32_t)(19 + i) * cp.plane + idx, ro[i], i, true);
+    }
+    mac[idx] = (double)rr; mac[n + idx] = (double)rb; mac[2 * n + idx] = (double)(rr + rb);
+    for (int a = 3; a < 6; ++a) mac[a * n + idx] = rf ? (double)m[a] : 0.0;
+}
+
+// Post-streaming colour populations of the state in src (FP64).
+template <int S>
+__global__ __launch_bounds__(BX * BY) void k_cg3_recon(const typename CIo<S>::raw* __restrict__ src, CgParams cp,
+                                                       double* __restrict__ rf, double* __restrict__ bf) {
+    using T = typename CIo<S>::T;
+    int x, y, z;
+    if (!node_of(cp, x, y, z)) return;
+    T h[19], r[19];
+    pull<S>(src, cp, x, y, z, h, r);
+    const size_t n = (size_t)cp.nx * cp.ny * cp.nz;
+    const size_t idx = ((size_t)z * cp.ny + y) * cp.nx + x;
+    for (int i = 0; i < 19; ++i) {
+        rf[i * n + idx] = (double)r[i];
+        bf[i * n + idx] = (double)h[i] + W19(i) - (double)r[i];
+    }
+}
+
+inline dim3 grid_of(const CgParams& cp) { return dim3((cp.nx + BX - 1) / BX, (cp.ny + BY - 1) / BY, cp.nz); }
+
+}  // namespace cg3
+
+// ---------------------------------------------------------------------------
+// launchers (lbm_cg.h)
+hipError_t cg3_launch_step(int storage, int ho, const void* src, void* dst, void* psi, const cg::CgParams& cp,
+                           double* mac, bool wm, hipStream_t st) {
+    const dim3 grid = cg3::grid_of(cp), block(cg3::BX, cg3::BY);
+    const int w = wm ? 1 : 0;
+#define CG3_STEP(S_, H_)                                                                                       \
+    do {                                                                                                       \
+        using R_ = typename cg3::CIo<S_>::raw;                                                                 \
+        using T_ = typename cg3::CIo<S_>::T;                                                                   \
+        hipLaunchKernelGGL((cg3::k_cg3_psi<S_>), grid, block, 0, st, (const R_*)src, cp, (T_*)psi);            \
+        hipLaunchKernelGGL((cg3::k_cg3_step<S_, H_>), grid, block, 0, st, (const R_*)src, (R_*)dst,            \
+                           (const T_*)psi, cp, mac, w);                                                        \
+    } while (0)
+    if (storage == 0) { if (ho) CG3_STEP(0, 2); else CG3_STEP(0, 1); }
+    else if (storage == 1) { if (ho) CG3_STEP(1, 2); else CG3_STEP(1, 1); }
+    else { if (ho) CG3_STEP(2, 2); else CG3_STEP(2, 1); }
+#undef CG3_STEP
+    return hipGetLastError();
+}
+
+hipError_t cg3_launch_init(int storage, int ho, void* dst, void* psi, const cg::CgParams& cp, const double* rr,
+                           const double* rb, const double* rf, const double* bf, double* mac, hipStream_t st) {
+    const dim3 grid = cg3::grid_of(cp), block(cg3::BX, cg3::BY);
+#define CG3_INIT(S_, H_)                                                                                           \
+    do {                                                                                                           \
+        using T_ = typename cg3::CIo<S_>::T;                                                                       \
+        hipLaunchKernelGGL((cg3::k_cg3_psi0<T_>), grid, block, 0, st, rr, rb, cp, (T_*)psi);                       \
+        hipLaunchKernelGGL((cg3::k_cg3_init<S_, H_>), grid, block, 0, st, (typename cg3::CIo<S_>::raw*)dst, cp, rr, \
+                           rb, rf, bf, (const T_*)psi, mac);                                                       \
+    } while (0)
+    if (storage == 0) { if (ho) CG3_INIT(0, 2); else CG3_INIT(0, 1); }
+    else if (storage == 1) { if (ho) CG3_INIT(1, 2); else CG3_INIT(1, 1); }
+    else { if (ho) CG3_INIT(2, 2); else CG3_INIT(2, 1); }
+#undef CG3_INIT
+    return hipGetLastError();
+}
+
+hipError_t cg3_launch_recon(int storage, const void* src, const cg::CgParams& cp, double* rf, double* bf,
+                            hipStream_t st) {
+    const dim3 grid = cg3::grid_of(cp), block(cg3::BX, cg3::BY);
+    if (storage == 0) hipLaunchKernelGGL((cg3::k_cg3_recon<0>), grid, block, 0, st, (const double*)src, cp, rf, bf);
+    else if (storage == 1) hipLaunchKernelGGL((cg3::k_cg3_recon<1>), grid, block, 0, st, (const float*)src, cp, rf, bf);
+    else hipLaunchKernelGGL((cg3::k_cg3_recon<2>), grid, block, 0, st, (const uint16_t*)src, cp, rf, bf);
+    return hipGetLastError();
+}
+
+}  // namespace lbm

@@ -268,6 +268,28 @@ def droplet2d(nx, ny, radius, cx=None, cy=None):
     return r, 1.0 - r
 
 
+def droplet3d(nx, ny, nz, radius, cx=None, cy=None, cz=None):
+    """Colour densities (nz, ny, nx) of a sphere of radius `radius` (red
+    inside, blue outside) in an nx x ny x nz periodic box."""
+    cx = (nx - 1) / 2.0 if cx is None else cx
+    cy = (ny - 1) / 2.0 if cy is None else cy
+    cz = (nz - 1) / 2.0 if cz is None else cz
+    zz, yy, xx = np.mgrid[0:nz, 0:ny, 0:nx]
+    inside = (xx - cx) ** 2 + (yy - cy) ** 2 + (zz - cz) ** 2 <= radius ** 2
+    r = inside.astype(np.float64)
+    return r, 1.0 - r
+
+
+def bubble3d_populations(r_rho, b_rho):
+    """Colour populations (19, nz, ny, nx) of the reference's steady-bubble
+    start (S/GpuInit.cu:543-592): f_q = rho_k phi_q with phi = 1/12 on the
+    axes, 1/24 on the diagonals and f_0 = 0, whatever the rest fraction."""
+    phi = np.array([0.0] + [1 / 12] * 6 + [1 / 24] * 12)
+    r = np.asarray(r_rho, np.float64)
+    b = np.asarray(b_rho, np.float64)
+    return phi[:, None, None, None] * r[None], phi[:, None, None, None] * b[None]
+
+
 def write_latticegen2(path_prefix, node_class, bb=None, delta=1.0, inlet_dirs=None, outlet_dirs=None, wall_q=None):
     """Write <prefix>_Node.dat and <prefix>_BC.dat in the latticegen2 layout
     (L/main.cpp:59-66, L/FluidTracking.cpp:229-236): 2D lines "i j x y fluid"

@@ -35,7 +35,7 @@ EXPORTED = [
     "lbm_step", "lbm_run_to_residual", "lbm_reduce", "lbm_reduce_all", "lbm_residual_l2", "lbm_get_macros", "lbm_get_f", "lbm_set_f",
     "lbm_time_steps", "lbm_device_bytes", "lbm_read_lattice", "lbm_read_wall_q",
     "lbm_cg_create", "lbm_cg_destroy", "lbm_cg_last_error", "lbm_cg_init", "lbm_cg_step", "lbm_cg_get_macros",
-    "lbm_cg_get_f", "lbm_cg_reduce", "lbm_cg_time_steps",
+    "lbm_cg_get_f", "lbm_cg_reduce", "lbm_cg_time_steps", "lbm_cg_init_f", "lbm_cg_get_macros3d",
     "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_last_error", "lbm_batch_init", "lbm_batch_run",
     "lbm_batch_get_macros", "lbm_batch_get_f", "lbm_batch_nodes",
 ]
@@ -63,7 +63,7 @@ class CgParams(C.Structure):
         ("abi_version", C.c_uint32), ("nx", C.c_int32), ("ny", C.c_int32), ("r_alpha", C.c_double),
         ("b_alpha", C.c_double), ("r_nu", C.c_double), ("b_nu", C.c_double), ("beta", C.c_double),
         ("A", C.c_double), ("g_limit", C.c_double), ("high_order", C.c_int32), ("kappa", C.c_double),
-        ("storage", C.c_int32),
+        ("storage", C.c_int32), ("nz", C.c_int32),
     ]
 
 
@@ -135,6 +135,8 @@ def load():
     L.lbm_cg_init.argtypes = [vp, dp, dp]
     L.lbm_cg_step.argtypes = [vp, C.c_int64]
     L.lbm_cg_get_macros.argtypes = [vp, dp, dp, dp, dp, dp]
+    L.lbm_cg_get_macros3d.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+    L.lbm_cg_init_f.argtypes = [vp, dp, dp]
     L.lbm_cg_get_f.argtypes = [vp, dp, dp]
     L.lbm_cg_reduce.argtypes = [vp, dp]
     L.lbm_cg_time_steps.argtypes = [vp, C.c_int64, dp]
@@ -331,19 +333,23 @@ class Solver:
 
 
 class CgSolver:
-    """Colour-gradient D2Q9 two-phase step on a periodic nx x ny box (GPU).
-    Same keyword arguments as the oracle's CgSim; arrays are (ny, nx)."""
+    """Colour-gradient two-phase step on a periodic box (GPU): D2Q9 for
+    CgSolver(nx, ny) with arrays (ny, nx), D3Q19 for CgSolver(nx, ny, nz=...)
+    with arrays (nz, ny, nx).  Same keyword arguments as the oracle's CgSim."""
 
-    def __init__(self, nx, ny, *, r_alpha=4 / 9, b_alpha=4 / 9, r_nu=1 / 6, b_nu=1 / 6, beta=0.7, A=0.01125,
+    def __init__(self, nx, ny, *, nz=1, r_alpha=4 / 9, b_alpha=4 / 9, r_nu=1 / 6, b_nu=1 / 6, beta=0.7, A=0.01125,
                  g_limit=0.0, high_order=0, kappa=0.9, storage=STORE_F64):
         L = load()
         if isinstance(storage, str):
             storage = STORAGE_NAMES[storage]
         p = CgParams()
-        p.abi_version, p.nx, p.ny = ABI_VERSION, nx, ny
+        p.abi_version, p.nx, p.ny, p.nz = ABI_VERSION, nx, ny, nz
         p.r_alpha, p.b_alpha, p.r_nu, p.b_nu, p.beta, p.A = r_alpha, b_alpha, r_nu, b_nu, beta, A
         p.g_limit, p.high_order, p.kappa, p.storage = g_limit, high_order, kappa, storage
-        self.nx, self.ny, self.n = nx, ny, nx * ny
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.n = nx * ny * max(nz, 1)
+        self.q = 19 if nz > 1 else 9
+        self.shape = (nz, ny, nx) if nz > 1 else (ny, nx)
         h = C.c_void_p()
         st = L.lbm_cg_create(C.byref(p), C.byref(h))
         if st:
@@ -366,18 +372,27 @@ class CgSolver:
         r, b = _f64(r_rho, self.n), _f64(b_rho, self.n)
         self._check(self._L.lbm_cg_init(self.h, _dp(r), _dp(b)))
 
+    def init_f(self, r_f, b_f):
+        """Start from colour populations (q, *shape)."""
+        r, b = _f64(r_f, self.q * self.n), _f64(b_f, self.q * self.n)
+        self._check(self._L.lbm_cg_init_f(self.h, _dp(r), _dp(b)))
+
     def step(self, n=1):
         self._check(self._L.lbm_cg_step(self.h, int(n)))
 
     def macros(self):
-        out = [np.zeros(self.n) for _ in range(5)]
-        self._check(self._L.lbm_cg_get_macros(self.h, *[_dp(a) for a in out]))
-        return dict(zip(["r_rho", "b_rho", "rho", "u", "v"], [a.reshape(self.ny, self.nx) for a in out]))
+        out = [np.zeros(self.n) for _ in range(6)]
+        self._check(self._L.lbm_cg_get_macros3d(self.h, *[_dp(a) for a in out]))
+        keys = ["r_rho", "b_rho", "rho", "u", "v", "w"]
+        d = dict(zip(keys, [a.reshape(self.shape) for a in out]))
+        if self.q == 9:
+            del d["w"]
+        return d
 
     def get_f(self):
-        r, b = np.zeros(9 * self.n), np.zeros(9 * self.n)
+        r, b = np.zeros(self.q * self.n), np.zeros(self.q * self.n)
         self._check(self._L.lbm_cg_get_f(self.h, _dp(r), _dp(b)))
-        return r.reshape(9, self.ny, self.nx), b.reshape(9, self.ny, self.nx)
+        return r.reshape((self.q,) + self.shape), b.reshape((self.q,) + self.shape)
 
     def reduce(self):
         out = np.zeros(12)
