@@ -235,6 +235,8 @@ typedef struct {
     double kappa;             /* Laplace sampling threshold (control_param) */
     int32_t storage;          /* LBM_STORE_F64, _F32 or _F16 */
     int32_t nz;               /* 0 or 1: D2Q9 box nx x ny; >= 3: D3Q19 box nx x ny x nz */
+    int32_t enhanced;         /* 1: f_eq += nu (chi u.grad rho + psi G:cc), the reference's enhanced_distrib
+                                 (gpuCollEnhancedBgkwGC2D/3D, S/GpuCollision.cu:624-707, :781-880) */
 } lbm_cg_params;
 
 lbm_status lbm_cg_create(const lbm_cg_params* p, lbm_cg_ctx** out);

@@ -842,7 +842,8 @@ double orc_mass(const orc_sim* s)
 /* ======================================================================== */
 /* colour-gradient two-phase model, D2Q9 or D3Q19, periodic box.
  * Follows the GPU formulation: gpuCollBgkwGC2D / gpuCollBgkwGC3D
- * (S/GpuCollision.cu:552-622, :709-779) with the gradients
+ * (S/GpuCollision.cu:552-622, :709-779), or with enhanced = 1
+ * gpuCollEnhancedBgkwGC2D / 3D (:624-707, :781-880), with the gradients
  * calculateColorGradient / calculateHOColorGradient (2D, :112-281) and
  * calculateColorGradient3D / calculateHOColorGradient3D case 0 (3D, :283-420),
  * streaming gpuStreaming2DCG / gpuStreaming3DCG (S/GpuStream.cu:30-158,
@@ -854,7 +855,7 @@ struct orc_cg {
     long n;
     int q, nz;
     int cx[19], cy[19], cz[19];
-    double w[19], phi[19], teta[19], bpert[19], cn[19];
+    double w[19], phi[19], teta[19], bpert[19], cn[19], chi[19], psi[19];
     double *r, *b, *rc, *bc, *f;          /* colour populations, post-collision, total */
     double *rr, *br, *rho, *u, *v, *w3;   /* stored macros */
     int hx[105], hy[105], hz[105];
@@ -882,6 +883,9 @@ orc_cg* orc_cg_create(const orc_cg_params* p)
         if (q == 9) {
             s->cx[i] = CX9[i]; s->cy[i] = CY9[i]; s->cz[i] = 0;
             s->w[i] = W9[i]; s->phi[i] = CG_PHI[i]; s->teta[i] = CG_TETA[i]; s->bpert[i] = CG_BPERT[i];
+            /* enhanced-model chi, psi (S/GpuInit.cu:158-165) */
+            s->chi[i] = i == 0 ? -8.0 / 3.0 : (i <= 4 ? -1.0 / 6.0 : 1.0 / 12.0);
+            s->psi[i] = i == 0 ? 0.0 : (i <= 4 ? 0.5 : 0.125);
         } else {
             /* 3D constants (S/GpuInit.cu:900-935): phi, theta and B by shell */
             s->cx[i] = CX19[i]; s->cy[i] = CY19[i]; s->cz[i] = CZ19[i];
@@ -889,6 +893,9 @@ orc_cg* orc_cg_create(const orc_cg_params* p)
             s->phi[i] = i == 0 ? 0.0 : (i <= 6 ? 1.0 / 12.0 : 1.0 / 24.0);
             s->teta[i] = i == 0 ? 1.0 : (i <= 6 ? -1.0 / 12.0 : -1.0 / 24.0);
             s->bpert[i] = i == 0 ? -2.0 / 9.0 : (i <= 6 ? 1.0 / 54.0 : 1.0 / 27.0);
+            /* enhanced-model chi3D, psi3D (S/GpuInit.cu:916-926) */
+            s->chi[i] = i == 0 ? -2.5 : (i <= 6 ? -1.0 / 6.0 : 1.0 / 24.0);
+            s->psi[i] = i == 0 ? 0.0 : (i <= 6 ? 0.25 : 0.125);
         }
         s->cn[i] = sqrt((double)(s->cx[i] * s->cx[i] + s->cy[i] * s->cy[i] + s->cz[i] * s->cz[i]));
     }
@@ -984,26 +991,29 @@ void orc_cg_init_f(orc_cg* s, const double* r_f, const double* b_f)
 
 static int cg_wrap(int a, int n) { a %= n; return a < 0 ? a + n : a; }
 
+/* colour gradient g[0..2] and density gradient g[3..5] (the stencil's
+ * second accumulator, S/GpuCollision.cu:291-300) */
 static void cg_grad_node(const orc_cg* s, int x, int y, int z, double* g)
 {
     const int nx = s->p.nx, ny = s->p.ny, nz = s->nz;
-    double ax = 0, ay = 0, az = 0;
+    double ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
     for (int k = 0; k < s->nst; ++k) {
         int xx = cg_wrap(x + s->hx[k], nx), yy = cg_wrap(y + s->hy[k], ny), zz = cg_wrap(z + s->hz[k], nz);
         long j = ((long)zz * ny + yy) * nx + xx;
         double psi = s->hw[k] * (s->rr[j] - s->br[j]) / s->rho[j];
-        ax += psi * s->hx[k];
-        ay += psi * s->hy[k];
-        az += psi * s->hz[k];
+        double rw = s->hw[k] * s->rho[j];
+        ax += psi * s->hx[k]; ay += psi * s->hy[k]; az += psi * s->hz[k];
+        bx += rw * s->hx[k]; by += rw * s->hy[k]; bz += rw * s->hz[k];
     }
     g[0] = ax; g[1] = ay; g[2] = az;
+    g[3] = bx; g[4] = by; g[5] = bz;
 }
 
 void orc_cg_gradient(const orc_cg* s, double* cgx, double* cgy, double* cgz)
 {
     const long nx = s->p.nx, ny = s->p.ny;
     for (long ind = 0; ind < s->n; ++ind) {
-        double g[3];
+        double g[6];
         cg_grad_node(s, (int)(ind % nx), (int)(ind / nx % ny), (int)(ind / (nx * ny)), g);
         cgx[ind] = g[0]; cgy[ind] = g[1];
         if (cgz) cgz[ind] = g[2];
@@ -1020,7 +1030,7 @@ void orc_cg_step(orc_cg* s, int nsteps)
 #pragma omp parallel for
         for (long ind = 0; ind < n; ++ind) {
             int x = (int)(ind % nx), y = (int)(ind / nx % ny), z = (int)(ind / ((long)nx * ny));
-            double g[3];
+            double g[6];
             cg_grad_node(s, x, y, z, g);
             double rr = s->rr[ind], br = s->br[ind], r = s->rho[ind];
             double u = s->u[ind], v = s->v[ind], w = s->w3[ind];
@@ -1030,6 +1040,7 @@ void orc_cg_step(orc_cg* s, int nsteps)
             double gn = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
             double kr = rr / r, kb = br / r, kk = P->beta * rr * br / r;
             double uu = u * u + v * v + w * w;
+            double ugr = u * g[3] + v * g[4] + w * g[5];
             for (int k = 0; k < q; ++k) {
                 double cs = 0, pert = 0;
                 if (gn > P->g_limit) {
@@ -1040,6 +1051,12 @@ void orc_cg_step(orc_cg* s, int nsteps)
                 double cu = u * s->cx[k] + v * s->cy[k] + w * s->cz[k];
                 double rest = s->phi[k] + s->teta[k] * alpha;
                 double feq = r * (rest + s->w[k] * (3.0 * cu + 4.5 * cu * cu - 1.5 * uu));
+                if (P->enhanced) {
+                    /* nu (chi_k u.grad rho + psi_k G:c c), G = u grad rho + grad rho u,
+                     * so G:c c = 2 (c.u)(c.grad rho) */
+                    double cgr = s->cx[k] * g[3] + s->cy[k] * g[4] + s->cz[k] * g[5];
+                    feq += mean_nu * (s->chi[k] * ugr + s->psi[k] * 2.0 * cu * cgr);
+                }
                 double fs = om * feq + (1.0 - om) * s->f[k * n + ind] + pert;
                 s->rc[k * n + ind] = kr * fs + kk * cs * rest;
                 s->bc[k * n + ind] = kb * fs - kk * cs * rest;

@@ -126,6 +126,7 @@ typedef struct {
     double kappa;                /* Laplace sampling threshold (control_param) */
     int nthreads;
     int nz;                      /* > 1: D3Q19 box nx x ny x nz */
+    int enhanced;                /* 1: density-gradient correction of f_eq (gpuCollEnhancedBgkwGC2D/3D) */
 } orc_cg_params;
 
 typedef struct orc_cg orc_cg;

@@ -12,12 +12,14 @@ from thesis_amd import lbm as L
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("ho", [0, 1])
-def test_cg_f64_matches_oracle(oracle, ho):
+@pytest.mark.parametrize("ho,enh", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_cg_f64_matches_oracle(oracle, ho, enh):
     nx, ny = 70, 45          # partial 32x16 tiles on both axes
-    print(f"case: cg parity {nx}x{ny} high_order={ho}", flush=True)
+    print(f"case: cg parity {nx}x{ny} high_order={ho} enhanced={enh}", flush=True)
     r, b = LT.droplet2d(nx, ny, 12, cx=30.3, cy=20.7)
-    kw = dict(beta=0.7, A=0.01125, r_nu=1 / 6, b_nu=0.1, r_alpha=4 / 9, b_alpha=0.2, high_order=ho)
+    if enh:
+        b = 0.6 * b          # density contrast for the enhanced terms
+    kw = dict(beta=0.7, A=0.01125, r_nu=1 / 6, b_nu=0.1, r_alpha=4 / 9, b_alpha=0.2, high_order=ho, enhanced=enh)
     g = L.CgSolver(nx, ny, storage=L.STORE_F64, **kw)
     o = oracle.CgSim(nx, ny, **kw)
     g.init(r, b)

@@ -14,6 +14,7 @@ struct CgParams {
     double r_alpha, b_alpha, r_nu, b_nu, beta, A, g_limit, kappa;
     uint32_t plane;          // elements per plane (2Q planes: Q deviations h = f - w, Q red)
     int nz;                  // 1 for D2Q9
+    int enhanced;            // 1: density-gradient terms in f_eq (enhanced model)
 };
 
 constexpr int NRED = 12;     // reductions of k_cg_reduce

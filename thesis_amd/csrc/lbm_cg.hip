@@ -32,6 +32,9 @@ __host__ __device__ constexpr double TETA(int i) { return i == 0 ? 1.0 : (i <= 4
 // perturbation constants B_k (S/GpuInit.cu:170-171)
 __host__ __device__ constexpr double BPERT(int i) { return i == 0 ? -4.0 / 27.0 : (i <= 4 ? 2.0 / 27.0 : 5.0 / 108.0); }
 __host__ __device__ constexpr double CNORM(int i) { return i == 0 ? 1.0 : (i <= 4 ? 1.0 : 1.4142135623730950488); }
+// enhanced-model chi_k, psi_k (S/GpuInit.cu:158-165)
+__host__ __device__ constexpr double CHI(int i) { return i == 0 ? -8.0 / 3.0 : (i <= 4 ? -1.0 / 6.0 : 1.0 / 12.0); }
+__host__ __device__ constexpr double PSI(int i) { return i == 0 ? 0.0 : (i <= 4 ? 0.5 : 0.125); }
 // 24-neighbour stencil weight by |c|^2 (S/GpuInit.cu:186-199)
 __host__ __device__ constexpr double HOW(int c2) {
     return c2 == 1 ? 960.0 / 5040 : c2 == 2 ? 448.0 / 5040 : c2 == 4 ? 84.0 / 5040 : c2 == 5 ? 32.0 / 5040 : 1.0 / 5040;
@@ -88,10 +91,13 @@ __device__ inline void densities(const T* h, const T* r, T& rho, T& rr) {
     rr = s;
 }
 
-// Collision, perturbation and recolouring of one node (gpuCollBgkwGC2D).
+// Collision, perturbation and recolouring of one node (gpuCollBgkwGC2D;
+// with E the density-gradient terms of gpuCollEnhancedBgkwGC2D,
+// S/GpuCollision.cu:624-707, from the density gradient (dx, dy)).
 // Writes the new deviation h' = f* - w and red r'.
-template <class T>
-__device__ inline void cg_collide(const CgParams& cp, const T* h, const T* r, T gx, T gy, T* ho, T* ro, T* mac) {
+template <class T, bool E>
+__device__ inline void cg_collide(const CgParams& cp, const T* h, const T* r, T gx, T gy, T dx, T dy, T* ho, T* ro,
+                                  T* mac) {
     T rho, rr;
     densities<T>(h, r, rho, rr);
     const T rb = rho - rr;
@@ -108,6 +114,7 @@ __device__ inline void cg_collide(const CgParams& cp, const T* h, const T* r, T 
     const T ign = on ? (T)1 / gn : (T)0;
     const T kr = rr * ir, kk = (T)cp.beta * rr * rb * ir;
     const T uu = u * u + v * v;
+    const T ugr = u * dx + v * dy;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const T cg = (T)CX[k] * gx + (T)CY[k] * gy;
@@ -116,8 +123,12 @@ __device__ inline void cg_collide(const CgParams& cp, const T* h, const T* r, T 
         const T rest = (T)PHI(k) + (T)TETA(k) * alpha;
         const T cu = (T)CX[k] * u + (T)CY[k] * v;
         // f_eq - w = (rho - 1) rest + (rest - w) + rho w (3cu + 4.5cu^2 - 1.5uu)
-        const T feqd = (rho - (T)1) * rest + (rest - (T)W9(k)) +
-                       rho * (T)W9(k) * ((T)3 * cu + (T)4.5 * cu * cu - (T)1.5 * uu);
+        T feqd = (rho - (T)1) * rest + (rest - (T)W9(k)) +
+                 rho * (T)W9(k) * ((T)3 * cu + (T)4.5 * cu * cu - (T)1.5 * uu);
+        if (E) {   // nu (chi u.grad rho + psi G:cc), G:cc = 2 (c.u)(c.grad rho)
+            const T cgr = (T)CX[k] * dx + (T)CY[k] * dy;
+            feqd += mean_nu * ((T)CHI(k) * ugr + (T)PSI(k) * (T)2 * cu * cgr);
+        }
         const T fsd = om * feqd + ((T)1 - om) * h[k] + pert;       // f* - w
         const T fs = fsd + (T)W9(k);
         ho[k] = fsd;
@@ -163,14 +174,16 @@ __device__ inline void gradient(const double* __restrict__ psi, int lx, int ly, 
 }
 
 // One fused step.  Block (32, 8): thread (tx, ty) owns interior nodes
-// (tx, ty) and (tx, ty + 8) of the 32 x 16 tile.
-template <int S, int H>
+// (tx, ty) and (tx, ty + 8) of the 32 x 16 tile.  With E the density goes to
+// a second LDS tile for the density gradient of the enhanced model.
+template <int S, int H, bool E>
 __global__ __launch_bounds__(THREADS) void k_cg_step(const typename CIo<S>::raw* __restrict__ src,
                                                      typename CIo<S>::raw* __restrict__ dst, CgParams cp,
                                                      double* __restrict__ mac, int write_macros) {
     using T = typename CIo<S>::T;
     constexpr int RW = TX + 2 * H, RH = TY + 2 * H;
     __shared__ T psi[RW * RH];
+    __shared__ T rt[E ? RW * RH : 1];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
     const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
     const int nx = cp.nx, ny = cp.ny;
@@ -185,6 +198,7 @@ __global__ __launch_bounds__(THREADS) void k_cg_step(const typename CIo<S>::raw*
         T rho, rr;
         densities<T>(h[k], r[k], rho, rr);
         psi[(ly + H) * RW + (lx + H)] = ((T)2 * rr - rho) / rho;
+        if (E) rt[(ly + H) * RW + (lx + H)] = rho;
     }
     // halo ring (positions of the padded tile outside the interior)
     constexpr int NHALO = RW * RH - TX * TY;
@@ -202,16 +216,18 @@ __global__ __launch_bounds__(THREADS) void k_cg_step(const typename CIo<S>::raw*
         T rho, rr;
         densities<T>(hh, rr9, rho, rr);
         psi[py * RW + px] = ((T)2 * rr - rho) / rho;
+        if (E) rt[py * RW + px] = rho;
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         if (!inside[k]) continue;
         const int lx = tx, ly = ty + 8 * k, x = x0 + lx, y = y0 + ly;
-        T gx, gy;
+        T gx, gy, dx = 0, dy = 0;
         gradient<H>(psi, lx, ly, gx, gy);
+        if (E) gradient<H>(rt, lx, ly, dx, dy);
         T ho[9], ro[9], m[5];
-        cg_collide<T>(cp, h[k], r[k], gx, gy, ho, ro, m);
+        cg_collide<T, E>(cp, h[k], r[k], gx, gy, dx, dy, ho, ro, m);
         const uint32_t idx = (uint32_t)y * (uint32_t)nx + (uint32_t)x;
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
@@ -229,7 +245,7 @@ __global__ __launch_bounds__(THREADS) void k_cg_step(const typename CIo<S>::raw*
 // S/GpuInit.cu:323-352), or from given colour populations rf, bf (FP64,
 // 9 planes of N) when rf is not null: gradient straight from the density
 // arrays rr0, rb0 (the densities of rf, bf in the second case).
-template <int S, int H>
+template <int S, int H, bool E>
 __global__ __launch_bounds__(256) void k_cg_init(typename CIo<S>::raw* __restrict__ dst, CgParams cp,
                                                  const double* __restrict__ rr0, const double* __restrict__ rb0,
                                                  const double* __restrict__ rf, const double* __restrict__ bf,
@@ -238,7 +254,7 @@ __global__ __launch_bounds__(256) void k_cg_init(typename CIo<S>::raw* __restric
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     const int nx = cp.nx, ny = cp.ny;
     if (x >= nx) return;
-    T gx = 0, gy = 0;
+    T gx = 0, gy = 0, ex = 0, ey = 0;
     for (int dy = -H; dy <= H; ++dy)
         for (int dx = -H; dx <= H; ++dx) {
             const int c2 = dx * dx + dy * dy;
@@ -249,6 +265,10 @@ __global__ __launch_bounds__(256) void k_cg_init(typename CIo<S>::raw* __restric
             const double p = (rr0[j] - rb0[j]) / (rr0[j] + rb0[j]);
             gx += (T)(wk * dx * p);
             gy += (T)(wk * dy * p);
+            if (E) {
+                ex += (T)(wk * dx * (rr0[j] + rb0[j]));
+                ey += (T)(wk * dy * (rr0[j] + rb0[j]));
+            }
         }
     const size_t idx = (size_t)y * nx + x;
     const size_t n = (size_t)nx * ny;
@@ -266,7 +286,7 @@ __global__ __launch_bounds__(256) void k_cg_init(typename CIo<S>::raw* __restric
         }
     }
     T ho[9], ro[9], m[5];
-    cg_collide<T>(cp, h, r, gx, gy, ho, ro, m);
+    cg_collide<T, E>(cp, h, r, gx, gy, ex, ey, ho, ro, m);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         CIo<S>::st(dst, (uint32_t)i * cp.plane + (uint32_t)idx, ho[i], i, false);
@@ -353,12 +373,14 @@ hipError_t cg_launch_step(int storage, int ho, const void* src, void* dst, const
                           bool wm, hipStream_t st) {
     dim3 grid((cp.nx + cg::TX - 1) / cg::TX, (cp.ny + cg::TY - 1) / cg::TY), block(cg::TX, 8);
     const int w = wm ? 1 : 0;
-#define CG_STEP(S_, H_)                                                                                     \
-    hipLaunchKernelGGL((cg::k_cg_step<S_, H_>), grid, block, 0, st, (const typename cg::CIo<S_>::raw*)src, \
+#define CG_STEP(S_, H_, E_)                                                                                     \
+    hipLaunchKernelGGL((cg::k_cg_step<S_, H_, E_>), grid, block, 0, st, (const typename cg::CIo<S_>::raw*)src, \
                        (typename cg::CIo<S_>::raw*)dst, cp, mac, w)
-    if (storage == 0) { if (ho) CG_STEP(0, 2); else CG_STEP(0, 1); }
-    else if (storage == 1) { if (ho) CG_STEP(1, 2); else CG_STEP(1, 1); }
-    else { if (ho) CG_STEP(2, 2); else CG_STEP(2, 1); }
+#define CG_STEP_E(S_, H_) do { if (cp.enhanced) CG_STEP(S_, H_, true); else CG_STEP(S_, H_, false); } while (0)
+    if (storage == 0) { if (ho) CG_STEP_E(0, 2); else CG_STEP_E(0, 1); }
+    else if (storage == 1) { if (ho) CG_STEP_E(1, 2); else CG_STEP_E(1, 1); }
+    else { if (ho) CG_STEP_E(2, 2); else CG_STEP_E(2, 1); }
+#undef CG_STEP_E
 #undef CG_STEP
     return hipGetLastError();
 }
@@ -366,12 +388,14 @@ hipError_t cg_launch_step(int storage, int ho, const void* src, void* dst, const
 hipError_t cg_launch_init(int storage, int ho, void* dst, const cg::CgParams& cp, const double* rr, const double* rb,
                           const double* rf, const double* bf, double* mac, hipStream_t st) {
     dim3 grid((cp.nx + 255) / 256, cp.ny), block(256);
-#define CG_INIT(S_, H_)                                                                                       \
-    hipLaunchKernelGGL((cg::k_cg_init<S_, H_>), grid, block, 0, st, (typename cg::CIo<S_>::raw*)dst, cp, rr, \
+#define CG_INIT(S_, H_, E_)                                                                                       \
+    hipLaunchKernelGGL((cg::k_cg_init<S_, H_, E_>), grid, block, 0, st, (typename cg::CIo<S_>::raw*)dst, cp, rr, \
                        rb, rf, bf, mac)
-    if (storage == 0) { if (ho) CG_INIT(0, 2); else CG_INIT(0, 1); }
-    else if (storage == 1) { if (ho) CG_INIT(1, 2); else CG_INIT(1, 1); }
-    else { if (ho) CG_INIT(2, 2); else CG_INIT(2, 1); }
+#define CG_INIT_E(S_, H_) do { if (cp.enhanced) CG_INIT(S_, H_, true); else CG_INIT(S_, H_, false); } while (0)
+    if (storage == 0) { if (ho) CG_INIT_E(0, 2); else CG_INIT_E(0, 1); }
+    else if (storage == 1) { if (ho) CG_INIT_E(1, 2); else CG_INIT_E(1, 1); }
+    else { if (ho) CG_INIT_E(2, 2); else CG_INIT_E(2, 1); }
+#undef CG_INIT_E
 #undef CG_INIT
     return hipGetLastError();
 }

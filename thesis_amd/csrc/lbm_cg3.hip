@@ -12,7 +12,8 @@
 //
 // Per link the storage holds h = f - w and the red population r (b = f - r),
 // 38 planes.  A step is two launches: k_cg3_psi pulls each node's
-// post-streaming colours and stores the phase field psi; k_cg3_step pulls
+// post-streaming colours and stores the phase field psi (and the density,
+// for the enhanced model's density gradient); k_cg3_step pulls
 // them again, takes the gradient of psi over its neighbours (served from
 // L1/L2: each psi value is read by 18 or 104 nodes), collides and recolours
 // in registers and stores the new post-collision state.
@@ -36,6 +37,9 @@ __host__ __device__ constexpr double PHI(int i) { return i == 0 ? 0.0 : (i <= 6 
 __host__ __device__ constexpr double TETA(int i) { return i == 0 ? 1.0 : (i <= 6 ? -1.0 / 12.0 : -1.0 / 24.0); }
 __host__ __device__ constexpr double BPERT(int i) { return i == 0 ? -2.0 / 9.0 : (i <= 6 ? 1.0 / 54.0 : 1.0 / 27.0); }
 __host__ __device__ constexpr double CNORM(int i) { return i <= 6 ? 1.0 : 1.4142135623730950488; }
+// enhanced-model chi_k, psi_k (S/GpuInit.cu:916-926)
+__host__ __device__ constexpr double CHI(int i) { return i == 0 ? -2.5 : (i <= 6 ? -1.0 / 6.0 : 1.0 / 24.0); }
+__host__ __device__ constexpr double PSI(int i) { return i == 0 ? 0.0 : (i <= 6 ? 0.25 : 0.125); }
 // gradient weights by |c|^2: 18-neighbour stencil 1/6, 1/12 (S/GpuInit.cu:928);
 // 104-neighbour stencil (S/GpuInit.cu:950-990): shells 1, 2, 3, 4, 5, 6 and 9
 // (the (2,2,1) points), 0 for the points of [-2,2]^3 it leaves out
@@ -94,11 +98,14 @@ __device__ inline void densities(const T* h, const T* r, T& rho, T& rr) {
     rr = s;
 }
 
-// Collision, perturbation and recolouring of one node (gpuCollBgkwGC3D):
-// new deviation h' = f* - w and red r'; mac = rho_r, rho_b, rho, u, v, w of
-// the incoming state.
-template <class T>
-__device__ inline void collide(const CgParams& cp, const T* h, const T* r, T gx, T gy, T gz, T* ho, T* ro, T* mac) {
+// Collision, perturbation and recolouring of one node (gpuCollBgkwGC3D;
+// with E the density-gradient terms of gpuCollEnhancedBgkwGC3D,
+// S/GpuCollision.cu:781-880, from the density gradient d): new deviation
+// h' = f* - w and red r'; mac = rho_r, rho_b, rho, u, v, w of the incoming
+// state.
+template <class T, bool E>
+__device__ inline void collide(const CgParams& cp, const T* h, const T* r, T gx, T gy, T gz, const T* d, T* ho, T* ro,
+                               T* mac) {
     T rho, rr;
     densities<T>(h, r, rho, rr);
     const T rb = rho - rr;
@@ -115,6 +122,7 @@ __device__ inline void collide(const CgParams& cp, const T* h, const T* r, T gx,
     const T ign = on ? (T)1 / gn : (T)0;
     const T kr = rr * ir, kk = (T)cp.beta * rr * rb * ir;
     const T uu = u * u + v * v + w * w;
+    const T ugr = E ? u * d[0] + v * d[1] + w * d[2] : (T)0;
 #pragma unroll
     for (int k = 0; k < 19; ++k) {
         const T cg = (T)CX[k] * gx + (T)CY[k] * gy + (T)CZ[k] * gz;
@@ -123,8 +131,12 @@ __device__ inline void collide(const CgParams& cp, const T* h, const T* r, T gx,
         const T rest = (T)PHI(k) + (T)TETA(k) * alpha;
         const T cu = (T)CX[k] * u + (T)CY[k] * v + (T)CZ[k] * w;
         // f_eq - w = (rho - 1) rest + (rest - w) + rho w (3cu + 4.5cu^2 - 1.5uu)
-        const T feqd = (rho - (T)1) * rest + (rest - (T)W19(k)) +
-                       rho * (T)W19(k) * ((T)3 * cu + (T)4.5 * cu * cu - (T)1.5 * uu);
+        T feqd = (rho - (T)1) * rest + (rest - (T)W19(k)) +
+                 rho * (T)W19(k) * ((T)3 * cu + (T)4.5 * cu * cu - (T)1.5 * uu);
+        if (E) {   // nu (chi u.grad rho + psi G:cc), G:cc = 2 (c.u)(c.grad rho)
+            const T cgr = (T)CX[k] * d[0] + (T)CY[k] * d[1] + (T)CZ[k] * d[2];
+            feqd += mean_nu * ((T)CHI(k) * ugr + (T)PSI(k) * (T)2 * cu * cgr);
+        }
         const T fsd = om * feqd + ((T)1 - om) * h[k] + pert;       // f* - w
         ho[k] = fsd;
         ro[k] = kr * (fsd + (T)W19(k)) + kk * cs * rest;
@@ -170,8 +182,9 @@ __device__ inline bool node_of(const CgParams& cp, int& x, int& y, int& z) {
     return x < cp.nx && y < cp.ny;
 }
 
-// Phase field of the post-streaming state.
-template <int S>
+// Phase field of the post-streaming state (and with E its density in the
+// second half of psi).
+template <int S, bool E>
 __global__ __launch_bounds__(BX * BY) void k_cg3_psi(const typename CIo<S>::raw* __restrict__ src, CgParams cp,
                                                      typename CIo<S>::T* __restrict__ psi) {
     using T = typename CIo<S>::T;
@@ -180,10 +193,12 @@ __global__ __launch_bounds__(BX * BY) void k_cg3_psi(const typename CIo<S>::raw*
     T h[19], r[19], rho, rr;
     pull<S>(src, cp, x, y, z, h, r);
     densities<T>(h, r, rho, rr);
-    psi[((uint32_t)z * (uint32_t)cp.ny + (uint32_t)y) * (uint32_t)cp.nx + (uint32_t)x] = ((T)2 * rr - rho) / rho;
+    const uint32_t j = ((uint32_t)z * (uint32_t)cp.ny + (uint32_t)y) * (uint32_t)cp.nx + (uint32_t)x;
+    psi[j] = ((T)2 * rr - rho) / rho;
+    if (E) psi[(size_t)cp.nx * cp.ny * cp.nz + j] = rho;
 }
 
-template <int S, int H>
+template <int S, int H, bool E>
 __global__ __launch_bounds__(BX * BY) void k_cg3_step(const typename CIo<S>::raw* __restrict__ src,
                                                       typename CIo<S>::raw* __restrict__ dst,
                                                       const typename CIo<S>::T* __restrict__ psi, CgParams cp,
@@ -193,10 +208,14 @@ __global__ __launch_bounds__(BX * BY) void k_cg3_step(const typename CIo<S>::raw
     if (!node_of(cp, x, y, z)) return;
     T h[19], r[19];
     pull<S>(src, cp, x, y, z, h, r);
-    T gx, gy, gz;
+    T gx, gy, gz, d[3] = {0, 0, 0};
     gradient<H, T>(cp, x, y, z, [psi](uint32_t j) { return psi[j]; }, gx, gy, gz);
+    if (E) {
+        const T* rh = psi + (size_t)cp.nx * cp.ny * cp.nz;
+        gradient<H, T>(cp, x, y, z, [rh](uint32_t j) { return rh[j]; }, d[0], d[1], d[2]);
+    }
     T ho[19], ro[19], m[6];
-    collide<T>(cp, h, r, gx, gy, gz, ho, ro, m);
+    collide<T, E>(cp, h, r, gx, gy, gz, d, ho, ro, m);
     const uint32_t idx = ((uint32_t)z * (uint32_t)cp.ny + (uint32_t)y) * (uint32_t)cp.nx + (uint32_t)x;
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
@@ -209,7 +228,7 @@ __global__ __launch_bounds__(BX * BY) void k_cg3_step(const typename CIo<S>::raw
     }
 }
 
-// Phase field of colour densities.
+// Phase field and density of colour densities.
 template <class T>
 __global__ __launch_bounds__(BX * BY) void k_cg3_psi0(const double* __restrict__ rr0, const double* __restrict__ rb0,
                                                       CgParams cp, T* __restrict__ psi) {
@@ -217,13 +236,14 @@ __global__ __launch_bounds__(BX * BY) void k_cg3_psi0(const double* __restrict__
     if (!node_of(cp, x, y, z)) return;
     const uint32_t j = ((uint32_t)z * (uint32_t)cp.ny + (uint32_t)y) * (uint32_t)cp.nx + (uint32_t)x;
     psi[j] = (T)((rr0[j] - rb0[j]) / (rr0[j] + rb0[j]));
+    psi[(size_t)cp.nx * cp.ny * cp.nz + j] = (T)(rr0[j] + rb0[j]);
 }
 
 // First post-collision state from colour densities rr0, rb0 (rest
 // equilibria phi + theta alpha) or, when rf is not null, from the colour
 // populations rf, bf (FP64, 19 planes of N; rr0, rb0 their densities);
 // psi holds the phase field of rr0, rb0.
-template <int S, int H>
+template <int S, int H, bool E>
 __global__ __launch_bounds__(BX * BY) void k_cg3_init(typename CIo<S>::raw* __restrict__ dst, CgParams cp,
                                                       const double* __restrict__ rr0, const double* __restrict__ rb0,
                                                       const double* __restrict__ rf, const double* __restrict__ bf,
@@ -232,9 +252,13 @@ __global__ __launch_bounds__(BX * BY) void k_cg3_init(typename CIo<S>::raw* __re
     using T = typename CIo<S>::T;
     int x, y, z;
     if (!node_of(cp, x, y, z)) return;
-    T gx, gy, gz;
-    gradient<H, T>(cp, x, y, z, [psi](uint32_t j) { return psi[j]; }, gx, gy, gz);
     const size_t n = (size_t)cp.nx * cp.ny * cp.nz;
+    T gx, gy, gz, d[3] = {0, 0, 0};
+    gradient<H, T>(cp, x, y, z, [psi](uint32_t j) { return psi[j]; }, gx, gy, gz);
+    if (E) {
+        const T* rh = psi + n;
+        gradient<H, T>(cp, x, y, z, [rh](uint32_t j) { return rh[j]; }, d[0], d[1], d[2]);
+    }
     const uint32_t idx = ((uint32_t)z * (uint32_t)cp.ny + (uint32_t)y) * (uint32_t)cp.nx + (uint32_t)x;
     const T rr = (T)rr0[idx], rb = (T)rb0[idx];
     T h[19], r[19];
@@ -250,7 +274,7 @@ __global__ __launch_bounds__(BX * BY) void k_cg3_init(typename CIo<S>::raw* __re
         }
     }
     T ho[19], ro[19], m[6];
-    collide<T>(cp, h, r, gx, gy, gz, ho, ro, m);
+    collide<T, E>(cp, h, r, gx, gy, gz, d, ho, ro, m);
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
         CIo<S>::st(dst, (uint32_t)i * cp.plane + idx, ho[i], i, false);
@@ -287,35 +311,39 @@ hipError_t cg3_launch_step(int storage, int ho, const void* src, void* dst, void
                            double* mac, bool wm, hipStream_t st) {
     const dim3 grid = cg3::grid_of(cp), block(cg3::BX, cg3::BY);
     const int w = wm ? 1 : 0;
-#define CG3_STEP(S_, H_)                                                                                       \
+#define CG3_STEP_(S_, H_, E_)                                                                                  \
     do {                                                                                                       \
         using R_ = typename cg3::CIo<S_>::raw;                                                                 \
         using T_ = typename cg3::CIo<S_>::T;                                                                   \
-        hipLaunchKernelGGL((cg3::k_cg3_psi<S_>), grid, block, 0, st, (const R_*)src, cp, (T_*)psi);            \
-        hipLaunchKernelGGL((cg3::k_cg3_step<S_, H_>), grid, block, 0, st, (const R_*)src, (R_*)dst,            \
+        hipLaunchKernelGGL((cg3::k_cg3_psi<S_, E_>), grid, block, 0, st, (const R_*)src, cp, (T_*)psi);        \
+        hipLaunchKernelGGL((cg3::k_cg3_step<S_, H_, E_>), grid, block, 0, st, (const R_*)src, (R_*)dst,        \
                            (const T_*)psi, cp, mac, w);                                                        \
     } while (0)
+#define CG3_STEP(S_, H_) do { if (cp.enhanced) CG3_STEP_(S_, H_, true); else CG3_STEP_(S_, H_, false); } while (0)
     if (storage == 0) { if (ho) CG3_STEP(0, 2); else CG3_STEP(0, 1); }
     else if (storage == 1) { if (ho) CG3_STEP(1, 2); else CG3_STEP(1, 1); }
     else { if (ho) CG3_STEP(2, 2); else CG3_STEP(2, 1); }
 #undef CG3_STEP
+#undef CG3_STEP_
     return hipGetLastError();
 }
 
 hipError_t cg3_launch_init(int storage, int ho, void* dst, void* psi, const cg::CgParams& cp, const double* rr,
                            const double* rb, const double* rf, const double* bf, double* mac, hipStream_t st) {
     const dim3 grid = cg3::grid_of(cp), block(cg3::BX, cg3::BY);
-#define CG3_INIT(S_, H_)                                                                                           \
+#define CG3_INIT_(S_, H_, E_)                                                                                      \
     do {                                                                                                           \
         using T_ = typename cg3::CIo<S_>::T;                                                                       \
         hipLaunchKernelGGL((cg3::k_cg3_psi0<T_>), grid, block, 0, st, rr, rb, cp, (T_*)psi);                       \
-        hipLaunchKernelGGL((cg3::k_cg3_init<S_, H_>), grid, block, 0, st, (typename cg3::CIo<S_>::raw*)dst, cp, rr, \
-                           rb, rf, bf, (const T_*)psi, mac);                                                       \
+        hipLaunchKernelGGL((cg3::k_cg3_init<S_, H_, E_>), grid, block, 0, st, (typename cg3::CIo<S_>::raw*)dst, cp, \
+                           rr, rb, rf, bf, (const T_*)psi, mac);                                                   \
     } while (0)
+#define CG3_INIT(S_, H_) do { if (cp.enhanced) CG3_INIT_(S_, H_, true); else CG3_INIT_(S_, H_, false); } while (0)
     if (storage == 0) { if (ho) CG3_INIT(0, 2); else CG3_INIT(0, 1); }
     else if (storage == 1) { if (ho) CG3_INIT(1, 2); else CG3_INIT(1, 1); }
     else { if (ho) CG3_INIT(2, 2); else CG3_INIT(2, 1); }
 #undef CG3_INIT
+#undef CG3_INIT_
     return hipGetLastError();
 }
 

@@ -28,7 +28,7 @@ struct lbm_cg_ctx {
     int64_t steps = -1;
     double* mac = nullptr;       // 6 N: rho_r, rho_b, rho, u, v, w
     double* rr0 = nullptr;       // 2 N initial densities
-    void* psi = nullptr;         // N phase field (D3Q19)
+    void* psi = nullptr;         // 2 N phase field and density (D3Q19)
     double* fout = nullptr;      // 2 q N: populations in and out
     double* partials = nullptr;
     std::string err;
@@ -61,6 +61,7 @@ lbm_status lbm_cg_create(const lbm_cg_params* p, lbm_cg_ctx** out) {
     if (p->nz < 0 || p->nz == 2) return LBM_EINVAL;
     if (p->storage < LBM_STORE_F64 || p->storage > LBM_STORE_F16) return LBM_EUNSUPPORTED;
     if (!(p->r_nu > 0 && p->b_nu > 0)) return LBM_EINVAL;
+    if (p->enhanced != 0 && p->enhanced != 1) return LBM_EINVAL;
     const bool three_d = p->nz >= 3;
     const int nz = three_d ? p->nz : 1, q = three_d ? 19 : 9;
     const size_t n = (size_t)p->nx * p->ny * nz;
@@ -78,13 +79,14 @@ lbm_status lbm_cg_create(const lbm_cg_params* p, lbm_cg_ctx** out) {
     k.r_alpha = p->r_alpha; k.b_alpha = p->b_alpha; k.r_nu = p->r_nu; k.b_nu = p->b_nu;
     k.beta = p->beta; k.A = p->A; k.g_limit = p->g_limit; k.kappa = p->kappa;
     k.plane = (uint32_t)plane;
+    k.enhanced = p->enhanced;
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
     const size_t pb = 2 * (size_t)q * plane * sbytes(p->storage);
     ok = ok && hipMalloc(&c->P[0], pb) == hipSuccess && hipMalloc(&c->P[1], pb) == hipSuccess &&
          hipMalloc((void**)&c->mac, 6 * n * 8) == hipSuccess && hipMalloc((void**)&c->rr0, 2 * n * 8) == hipSuccess &&
          hipMalloc((void**)&c->partials, (size_t)kRedBlocks * lbm::cg::NRED * 8) == hipSuccess &&
-         (!three_d || hipMalloc(&c->psi, n * 8) == hipSuccess) &&
+         (!three_d || hipMalloc(&c->psi, 2 * n * 8) == hipSuccess) &&
          hipMemset(c->mac + 5 * n, 0, n * 8) == hipSuccess;
     if (!ok) {
         lbm_cg_destroy(c);

@@ -63,7 +63,7 @@ class CgParams(C.Structure):
         ("abi_version", C.c_uint32), ("nx", C.c_int32), ("ny", C.c_int32), ("r_alpha", C.c_double),
         ("b_alpha", C.c_double), ("r_nu", C.c_double), ("b_nu", C.c_double), ("beta", C.c_double),
         ("A", C.c_double), ("g_limit", C.c_double), ("high_order", C.c_int32), ("kappa", C.c_double),
-        ("storage", C.c_int32), ("nz", C.c_int32),
+        ("storage", C.c_int32), ("nz", C.c_int32), ("enhanced", C.c_int32),
     ]
 
 
@@ -338,12 +338,12 @@ class CgSolver:
     with arrays (nz, ny, nx).  Same keyword arguments as the oracle's CgSim."""
 
     def __init__(self, nx, ny, *, nz=1, r_alpha=4 / 9, b_alpha=4 / 9, r_nu=1 / 6, b_nu=1 / 6, beta=0.7, A=0.01125,
-                 g_limit=0.0, high_order=0, kappa=0.9, storage=STORE_F64):
+                 g_limit=0.0, high_order=0, kappa=0.9, storage=STORE_F64, enhanced=0):
         L = load()
         if isinstance(storage, str):
             storage = STORAGE_NAMES[storage]
         p = CgParams()
-        p.abi_version, p.nx, p.ny, p.nz = ABI_VERSION, nx, ny, nz
+        p.abi_version, p.nx, p.ny, p.nz, p.enhanced = ABI_VERSION, nx, ny, nz, enhanced
         p.r_alpha, p.b_alpha, p.r_nu, p.b_nu, p.beta, p.A = r_alpha, b_alpha, r_nu, b_nu, beta, A
         p.g_limit, p.high_order, p.kappa, p.storage = g_limit, high_order, kappa, storage
         self.nx, self.ny, self.nz = nx, ny, nz
