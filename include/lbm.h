@@ -173,6 +173,13 @@ lbm_status lbm_reduce_all(lbm_ctx* ctx, double out[8]);
  * f the post-boundary populations of the last step and fColl the
  * post-collision populations that step streamed.  Needs at least one step. */
 lbm_status lbm_residual_l2(lbm_ctx* ctx, double* out);
+/* The reference's FdRelDiff residual (computeNewResidual3D + gpu_NewResidual,
+ * S/ComputeResiduals.cu:28-36, :55-63; driven by S/Iterate3D.cu:677-693):
+ * max over non-solid links of |f - f_prev| / |f1|, f the populations after the
+ * last step, f_prev those of the previous call, f1 those of the first call
+ * since lbm_init / lbm_set_f.  The first call returns 1, the reference's
+ * comparison with its zero initial populations.  Keeps two FP64 copies of f. */
+lbm_status lbm_residual_fdrel(lbm_ctx* ctx, double* out);
 
 /* Macroscopic fields of the current state (N doubles each; any may be NULL).
  * The velocity is the stored velocity of the force model in use. */

@@ -178,6 +178,42 @@ def test_lbmsolver_macro_diff_stop_condition(tmp_path):
     assert np.abs(data[:, 3] - u.ravel()).max() < 1e-9
 
 
+def test_lbmsolver_fdreldiff_stop_condition(tmp_path):
+    """ResidualModel = FdRelDiff (S/Iterate3D.cu:677-693): lbmsolver logs
+    max |f - f_prev| / f1 at every check in residuals.dat (1 at the first
+    check), stops when it falls below the first stop value, and the values
+    equal those of the ABI run checked at the same steps."""
+    print("case: lbmsolver FdRelDiff", flush=True)
+    nx, ny, U, nu = 64, 16, 0.02, 0.1
+    ch = LT.channel2d(nx, ny, 1.5 * U, obstacles=((20, 8, 3),))
+    os.makedirs(tmp_path / "Mesh")
+    LT.write_latticegen2(str(tmp_path / "Mesh" / "chan"), ch["node_class"], ch["bb"])
+    ini = (INI.format(U=U, nu=nu, iters=40000)
+           .replace("MinMacroDiffs(u_v_w_rho)(StopCond) = 0.0 0.0 0.0 0.0",
+                    "MinMacroDiffs(u_v_w_rho)(StopCond) = 1.0e-6 0.0 0.0 0.0")
+           .replace("ResidualModel = L2", "ResidualModel = FdRelDiff"))
+    (tmp_path / "SetUpData.ini").write_text(ini)
+    r = subprocess.run([CLI, "-f", "SetUpData.ini"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-300:], r.stderr[-300:])
+    assert r.returncode == 0 and "simulation converged!" in r.stdout
+    rows = np.loadtxt(tmp_path / "Results" / "residuals.dat", skiprows=1)
+    assert rows[0, 1] == 1.0 and rows[-1, 1] < 1e-6 <= rows[-2, 1]
+    lat = L.read_lattice(str(tmp_path / "Mesh" / "chan_Node.dat"), str(tmp_path / "Mesh" / "chan_BC.dat"))
+    cls = lat["node_class"][0]
+    ys = np.arange(ny)
+    ubc = np.zeros((ny, nx))
+    ubc[:, 0] = 6 * U * (ys + 0.5) * (ny - 0.5 - ys) / ny ** 2
+    s = L.Solver(9, (ny, nx), cls, lat["bb_links"][0], collision=L.TRT, omega=1 / (3 * nu + 0.5),
+                 ubc=np.where(cls == L.VEL_W, ubc, 0.0), vbc=np.zeros((ny, nx)), rho_out=1.0)
+    s.init(0, rho=np.ones((ny, nx)), u=np.where(cls == L.VEL_W, ubc, U))
+    every = int(rows[1, 0] - rows[0, 0])
+    got = []
+    for _ in range(len(rows)):
+        s.step(every)
+        got.append(s.residual_fdrel())
+    assert np.allclose(rows[:, 1], got, rtol=1e-4, atol=0)
+
+
 def read_appended_vti(path, n):
     """Velocity (n x 3) and density (n) of a binary-appended VTI file."""
     raw = open(path, "rb").read()

@@ -267,12 +267,27 @@ def test_reference_residuals_match_oracle(oracle, case):
     ro = o.residual_l2()
     print(f"L2 {r:.6e} oracle {ro:.6e}", flush=True)
     assert abs(r - ro) <= 1e-12 * ro
+    # FdRelDiff (S/ComputeResiduals.cu:28-36): max |f - f_prev| / |f1| over
+    # non-solid links, f1 the first check's populations
+    fl = cls != L.SOLID
+    assert g.residual_fdrel() == 1.0
+    f1 = o.get_f()
+    fprev = f1
+    for k in range(2):
+        g.step(5)
+        o.step(5)
+        f = o.get_f()
+        m = fl.ravel()
+        want = (np.abs(f - fprev).reshape(q, -1)[:, m] / np.abs(f1).reshape(q, -1)[:, m]).max()
+        got = g.residual_fdrel()
+        print(f"FdRelDiff {got:.6e} oracle {want:.6e}", flush=True)
+        assert abs(got - want) <= 1e-11 * want
+        fprev = f
     g.reduce_all()
     m0 = g.macros()
     g.step(7)
     m1 = g.macros()
     red = g.reduce_all()
-    fl = cls != L.SOLID
     want = [np.abs(m1[k] - m0[k])[fl].max() for k in (1, 2, 3, 0)]
     got = [red["max_du"], red["max_dv"], red["max_dw"], red["max_drho"]]
     assert np.allclose(got, want, rtol=0, atol=1e-17), (got, want)

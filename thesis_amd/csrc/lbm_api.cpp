@@ -46,6 +46,8 @@ struct lbm_ctx {
     double *up = nullptr, *vp = nullptr, *wp = nullptr, *rp = nullptr;
     double *ubc = nullptr, *vbc = nullptr, *wbc = nullptr;
     double* fout = nullptr;
+    double *f1 = nullptr, *fprev = nullptr;   // FdRelDiff: f at the first check and at the last one
+    bool fdrel_started = false;
     double* om[2] = {nullptr, nullptr};   // f_eq outlet: macros of even / odd steps (4 N each)
     double* partials = nullptr;
     double* out4 = nullptr;       // 8 doubles: the last reduction
@@ -504,7 +506,7 @@ void lbm_destroy(lbm_ctx* c) {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
     void* bufs[] = {c->Palloc[0], c->Palloc[1], c->node, c->cidx, c->didx, c->nodew, c->rowmeta, c->blk, c->rho, c->u, c->v, c->w, c->up, c->vp, c->wp,
-                    c->rp, c->qslot, c->qv, c->ubc, c->vbc, c->wbc, c->fout, c->om[0], c->om[1], c->partials, c->out4};
+                    c->rp, c->qslot, c->qv, c->ubc, c->vbc, c->wbc, c->fout, c->f1, c->fprev, c->om[0], c->om[1], c->partials, c->out4};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -530,6 +532,7 @@ lbm_status lbm_init(lbm_ctx* c, int mode, const double* rho, const double* u, co
                           c->P[0], c->node, c->cidx, fields(c), c->stream));
     c->cur = 0;
     c->steps = 0;
+    c->fdrel_started = false;
     lbm_status s = seed_outlet_macros(c);
     if (s != LBM_OK) return s;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -581,6 +584,34 @@ lbm_status lbm_residual_l2(lbm_ctx* c, double* out) {
     HIPCHK(c, hipMemcpyAsync(&sum, c->out4, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     *out = std::sqrt(sum);
+    return LBM_OK;
+}
+
+lbm_status lbm_residual_fdrel(lbm_ctx* c, double* out) {
+    if (!c || !out) return LBM_EINVAL;
+    if (c->steps < 1) return fail(c, LBM_EINVAL, "the FdRelDiff residual needs one step");
+    lbm_status s = ensure_fout(c);
+    if (s == LBM_OK && !c->f1) s = dalloc(c, (void**)&c->f1, sizeof(double) * c->n * c->q);
+    if (s == LBM_OK && !c->fprev) s = dalloc(c, (void**)&c->fprev, sizeof(double) * c->n * c->q);
+    if (s != LBM_OK) return s;
+    Fields fl = fields(c);
+    if (c->om[0]) fl.om_in = c->om[(c->steps - 1) & 1];
+    HIPCHK(c, launch_recon(c->q, c->p.storage, c->open, c->kp, c->P[c->cur ^ 1], c->node, c->cidx, fl, c->stream));
+    const size_t bytes = sizeof(double) * c->n * c->q;
+    double r = 1.0;
+    if (!c->fdrel_started) {
+        // the reference compares the first check with its zero initial
+        // populations: |f - 0| / f = 1 on every link
+        HIPCHK(c, hipMemcpyAsync(c->f1, c->fout, bytes, hipMemcpyDeviceToDevice, c->stream));
+        c->fdrel_started = true;
+    } else {
+        HIPCHK(c, launch_fdrel(c->q, c->fout, c->f1, c->fprev, c->cidx, c->n, c->partials, reduce_blocks(), c->out4,
+                               c->stream));
+        HIPCHK(c, hipMemcpyAsync(&r, c->out4, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipMemcpyAsync(c->fprev, c->fout, bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = r;
     return LBM_OK;
 }
 
@@ -645,6 +676,7 @@ lbm_status lbm_set_f(lbm_ctx* c, const double* f) {
                           c->stream));
     c->cur = 0;
     c->steps = 0;
+    c->fdrel_started = false;
     s = seed_outlet_macros(c);
     if (s != LBM_OK) return s;
     HIPCHK(c, hipStreamSynchronize(c->stream));

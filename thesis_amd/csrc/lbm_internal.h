@@ -83,6 +83,9 @@ hipError_t launch_reduce(const KParams& kp, const uint32_t* node, Fields fl, dou
 // out (device) = sum over non-solid nodes of (f - fColl)^2, f dense FP64 in fout.
 hipError_t launch_fdiff2(int q, int storage, const KParams& kp, const void* pc, const uint32_t* cidx,
                          const double* fout, double* partials, int nblocks, double* out, hipStream_t st);
+// out (device) = max over non-solid links of |f - fprev| / |f1| (dense FP64).
+hipError_t launch_fdrel(int q, const double* f, const double* f1, const double* fprev, const uint32_t* cidx,
+                        size_t n, double* partials, int nblocks, double* out, hipStream_t st);
 int reduce_blocks();
 
 }  // namespace lbm

@@ -411,6 +411,43 @@ __global__ __launch_bounds__(kRedThreads) void k_fdiff2(const typename Io<L, S>:
     }
 }
 
+// FdRelDiff residual: max over non-solid links of |f - f_prev| / |f1|
+// (gpu_NewResidual + gpu_max_h, S/ComputeResiduals.cu:28-36, :55-63), f, f1,
+// f_prev dense FP64 (q planes of n); a NaN ratio wins the maximum.
+__global__ __launch_bounds__(kRedThreads) void k_fdrel(const double* __restrict__ f, const double* __restrict__ f1,
+                                                       const double* __restrict__ fp, const uint32_t* __restrict__ cidx,
+                                                       size_t n, int q, double* partials) {
+    double m = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (cidx[i] == kNone) continue;
+        for (int k = 0; k < q; ++k) {
+            const size_t j = (size_t)k * n + i;
+            const double r = fabs(f[j] - fp[j]) / fabs(f1[j]);
+            m = (r > m || r != r) ? r : m;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64);
+        m = (o > m || o != o) ? o : m;
+    }
+    __shared__ double sh[kRedThreads / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = sh[0];
+        for (int k = 1; k < kRedThreads / 64; ++k) t = (sh[k] > t || sh[k] != sh[k]) ? sh[k] : t;
+        partials[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_max_final(const double* __restrict__ partials, int nb, double* out) {
+    if (threadIdx.x != 0) return;
+    double t = 0;
+    for (int b = 0; b < nb; ++b) t = (partials[b] > t || partials[b] != partials[b]) ? partials[b] : t;
+    out[0] = t;
+}
+
 __global__ __launch_bounds__(256) void k_sum_final(const double* __restrict__ partials, int nb, double* out) {
     __shared__ double sh[256];
     double s = 0;
@@ -571,6 +608,12 @@ static hipError_t fdiff2_s(int s, const KParams& kp, const void* pc, const uint3
     case 2: return fdiff2_t<L, 2>(kp, pc, cidx, fout, partials, nblocks, out, st);
     default: return fdiff2_t<L, 3>(kp, pc, cidx, fout, partials, nblocks, out, st);
     }
+}
+hipError_t launch_fdrel(int q, const double* f, const double* f1, const double* fprev, const uint32_t* cidx,
+                        size_t n, double* partials, int nblocks, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_fdrel, dim3(nblocks), dim3(kRedThreads), 0, st, f, f1, fprev, cidx, n, q, partials);
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(64), 0, st, (const double*)partials, nblocks, out);
+    return hipGetLastError();
 }
 hipError_t launch_fdiff2(int q, int storage, const KParams& kp, const void* pc, const uint32_t* cidx,
                          const double* fout, double* partials, int nblocks, double* out, hipStream_t st) {

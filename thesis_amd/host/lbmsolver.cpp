@@ -644,11 +644,11 @@ int main(int argc, char** argv) {
     // ResidualModel (ini line 19): MacroDiff compares the largest change of u,
     // v, w, rho since the last check with the four stop values and logs them to
     // MacroDiffs.txt (S/Iterate3D.cu:696-756, S/LogWriter.cu:215-221); L2 is
-    // sqrt(sum (f - fColl)^2) against the first stop value (S/Iterate.cu:551).
-    // FdRelDiff (which keeps two more copies of f) falls back to L2.
+    // sqrt(sum (f - fColl)^2) against the first stop value (S/Iterate.cu:551);
+    // FdRelDiff is max |f - f_prev| / f1 against the first stop value
+    // (S/Iterate3D.cu:677-693, S/ComputeResiduals.cu:28-36).
     const bool macro_diff = a.residual_model.find("MacroDiff") != std::string::npos;
-    if (a.residual_model.find("FdRelDiff") != std::string::npos)
-        std::fprintf(stderr, "note: FdRelDiff residuals are not available; L2 used\n");
+    const bool fd_rel = a.residual_model.find("FdRelDiff") != std::string::npos;
     FILE* mf = macro_diff ? std::fopen((a.out_dir + "MacroDiffs.txt").c_str(), "w") : nullptr;
     double r4[8];
     lbm_reduce_all(ctx, r4);
@@ -704,7 +704,8 @@ int main(int argc, char** argv) {
         t0 = now_s();
         lbm_reduce_all(ctx, r4);
         double l2 = 0;
-        if (!macro_diff) lbm_residual_l2(ctx, &l2);
+        if (fd_rel) lbm_residual_fdrel(ctx, &l2);
+        else if (!macro_diff) lbm_residual_l2(ctx, &l2);
         tm.resi += now_s() - t0;
         if (rf) std::fprintf(rf, "%ld %5.4e %5.4e %f %f\n", done, l2, l2 / (double)NS, 0.0, 0.0);
         if (r4[3] > 0 || l2 != l2) { std::fprintf(stderr, "\nDIVERGENCE!\n"); code = 1; break; }
@@ -716,7 +717,7 @@ int main(int argc, char** argv) {
                 break;
             }
         } else {
-            std::printf("iteration %ld L2 residual %.4e mass %.10f\n", done, l2, r4[2]);
+            std::printf("iteration %ld %s residual %.4e mass %.10f\n", done, fd_rel ? "FdRelDiff" : "L2", l2, r4[2]);
             if (l2 < a.stop[0]) { std::printf("simulation converged!\n"); break; }
         }
     }

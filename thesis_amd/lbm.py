@@ -32,7 +32,7 @@ STORAGE_NAMES = {"f64": STORE_F64, "f32": STORE_F32, "f16": STORE_F16, "bf16": S
 
 EXPORTED = [
     "lbm_abi_version", "lbm_device_count", "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_init",
-    "lbm_step", "lbm_run_to_residual", "lbm_reduce", "lbm_reduce_all", "lbm_residual_l2", "lbm_get_macros", "lbm_get_f", "lbm_set_f",
+    "lbm_step", "lbm_run_to_residual", "lbm_reduce", "lbm_reduce_all", "lbm_residual_l2", "lbm_residual_fdrel", "lbm_get_macros", "lbm_get_f", "lbm_set_f",
     "lbm_time_steps", "lbm_device_bytes", "lbm_read_lattice", "lbm_read_wall_q",
     "lbm_cg_create", "lbm_cg_destroy", "lbm_cg_last_error", "lbm_cg_init", "lbm_cg_step", "lbm_cg_get_macros",
     "lbm_cg_get_f", "lbm_cg_reduce", "lbm_cg_time_steps", "lbm_cg_init_f", "lbm_cg_get_macros3d",
@@ -118,6 +118,7 @@ def load():
     L.lbm_reduce.argtypes = [vp, dp]
     L.lbm_reduce_all.argtypes = [vp, dp]
     L.lbm_residual_l2.argtypes = [vp, dp]
+    L.lbm_residual_fdrel.argtypes = [vp, dp]
     L.lbm_get_macros.argtypes = [vp, dp, dp, dp, dp]
     L.lbm_get_f.argtypes = [vp, dp]
     L.lbm_set_f.argtypes = [vp, dp]
@@ -306,6 +307,14 @@ class Solver:
         """The reference's L2 residual sqrt(sum (f - fColl)^2) of the last step."""
         out = C.c_double()
         self._check(self._L.lbm_residual_l2(self.h, C.byref(out)))
+        return out.value
+
+    def residual_fdrel(self):
+        """The reference's FdRelDiff residual max |f - f_prev| / |f1| (f_prev:
+        the previous call's populations, f1: the first call's; the first call
+        returns 1)."""
+        out = C.c_double()
+        self._check(self._L.lbm_residual_fdrel(self.h, C.byref(out)))
         return out.value
 
     def macros(self):
