@@ -219,6 +219,22 @@ lbm_status lbm_read_lattice(const char* node_path, const char* bc_path, int32_t 
  * :1205); 0 for links without a WALL line.  wall_q holds nx*ny*nz*19 doubles. */
 lbm_status lbm_read_wall_q(const char* node_path, const char* bc_path, double* wall_q);
 
+/* Lattice generator compatible with the reference's latticegen2
+ * (L/main.cpp:17-70, L/GeomRead.cpp:14-189, L/MeshGen.cpp:19-80,
+ * L/FluidTracking.cpp:24-236, L/Intersect.cpp:43-528): reads a STAR-CD
+ * vertex file (lines "id x y z") and boundary file (lines "n p1 p2 p3 p4 id
+ * n type", type WALL / INLE / OUTL / CYCL / SYMP), meshes the vertices'
+ * bounding box with nx nodes along x, fills the fluid from the node nearest
+ * seed[3] and writes the node and BC files (byte-identical to the reference
+ * generator's, complexity 1: links written during the fill, 2: after it) and,
+ * when mesh_out is not NULL, the padded mesh listing (its xtecplot.dat).
+ * dims receives n, m, h; nlinks the number of BC lines. */
+lbm_status lbm_latticegen(const char* vrt_path, const char* bnd_path, int32_t nx, const double seed[3],
+                          int32_t complexity, const char* node_out, const char* bc_out, const char* mesh_out,
+                          int32_t dims[3], int64_t* nlinks);
+/* Message of the calling thread's last lbm_latticegen failure. */
+const char* lbm_latticegen_last_error(void);
+
 /* ---- colour-gradient two-phase model (D2Q9 or D3Q19, periodic box) ------- */
 /* Replaces the reference's multiphase path.  D2Q9: gpuCollBgkwGC2D +
  * gradient (S/GpuCollision.cu:112-281, :552-622), gpuStreaming2DCG

@@ -15,6 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblbm.so")
 CLI = os.path.join(HERE, "lbmsolver")
+LATTICEGEN = os.path.join(HERE, "latticegen")
 ARCH = os.environ.get("LBM_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -52,11 +53,11 @@ def build_lib(jobs=None):
 
 
 def build_cli():
-    src = os.path.join(HERE, "host", "lbmsolver.cpp")
-    if not os.path.exists(src):
-        return
-    _run([HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-o", CLI,
-          "-L", HERE, "-llbm", "-Wl,-rpath,$ORIGIN"])
+    for name, out in (("lbmsolver.cpp", CLI), ("latticegen.cpp", LATTICEGEN)):
+        src = os.path.join(HERE, "host", name)
+        if os.path.exists(src):
+            _run([HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-o", out,
+                  "-L", HERE, "-llbm", "-Wl,-rpath,$ORIGIN"])
 
 
 def build_calib():

@@ -33,7 +33,8 @@ STORAGE_NAMES = {"f64": STORE_F64, "f32": STORE_F32, "f16": STORE_F16, "bf16": S
 EXPORTED = [
     "lbm_abi_version", "lbm_device_count", "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_init",
     "lbm_step", "lbm_run_to_residual", "lbm_reduce", "lbm_reduce_all", "lbm_residual_l2", "lbm_residual_fdrel", "lbm_get_macros", "lbm_get_f", "lbm_set_f",
-    "lbm_time_steps", "lbm_device_bytes", "lbm_read_lattice", "lbm_read_wall_q",
+    "lbm_time_steps", "lbm_device_bytes", "lbm_read_lattice", "lbm_read_wall_q", "lbm_latticegen",
+    "lbm_latticegen_last_error",
     "lbm_cg_create", "lbm_cg_destroy", "lbm_cg_last_error", "lbm_cg_init", "lbm_cg_step", "lbm_cg_get_macros",
     "lbm_cg_get_f", "lbm_cg_reduce", "lbm_cg_time_steps", "lbm_cg_init_f", "lbm_cg_get_macros3d",
     "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_last_error", "lbm_batch_init", "lbm_batch_run",
@@ -128,6 +129,9 @@ def load():
     L.lbm_read_lattice.argtypes = [C.c_char_p, C.c_char_p, P(C.c_int32), P(C.c_int64), P(C.c_uint8),
                                    P(C.c_uint32), P(C.c_int64), P(C.c_int64), P(C.c_double), P(C.c_int32)]
     L.lbm_read_wall_q.argtypes = [C.c_char_p, C.c_char_p, dp]
+    L.lbm_latticegen.argtypes = [C.c_char_p, C.c_char_p, C.c_int32, dp, C.c_int32, C.c_char_p, C.c_char_p,
+                                 C.c_char_p, P(C.c_int32), P(C.c_int64)]
+    L.lbm_latticegen_last_error.restype = C.c_char_p
     L.lbm_cg_create.argtypes = [P(CgParams), P(vp)]
     L.lbm_cg_destroy.argtypes = [vp]
     L.lbm_cg_destroy.restype = None
@@ -220,6 +224,21 @@ def read_lattice(node_path, bc_path=None):
         raise LbmError(st, f"cannot read {node_path} / {bc_path}")
     return dict(dims=(nx, ny, nz), node_class=cls.reshape(nz, ny, nx), bb_links=bb.reshape(nz, ny, nx),
                 nlinks=nl.value, type_counts=tc, dir_counts=dc, delta=delta.value, cyclic=tuple(cyc[:]))
+
+
+def latticegen(vrt_path, bnd_path, nx, seed, out_prefix, complexity=1, mesh_path=None):
+    """Generate <out_prefix>_Node.dat / _BC.dat from a STAR-CD geometry, as the
+    reference's latticegen2 does (L/main.cpp:17-70).  Returns (dims, nlinks)."""
+    L = load()
+    dims = (C.c_int32 * 3)()
+    nl = C.c_int64()
+    sd = np.ascontiguousarray(seed, dtype=np.float64)
+    st = L.lbm_latticegen(vrt_path.encode(), bnd_path.encode(), int(nx), _dp(sd), int(complexity),
+                          (out_prefix + "_Node.dat").encode(), (out_prefix + "_BC.dat").encode(),
+                          mesh_path.encode() if mesh_path else None, dims, C.byref(nl))
+    if st:
+        raise LbmError(st, L.lbm_latticegen_last_error().decode())
+    return tuple(dims[:]), nl.value
 
 
 class Solver:
