@@ -219,6 +219,12 @@ lbm_status lbm_read_lattice(const char* node_path, const char* bc_path, int32_t 
  * :1205); 0 for links without a WALL line.  wall_q holds nx*ny*nz*19 doubles. */
 lbm_status lbm_read_wall_q(const char* node_path, const char* bc_path, double* wall_q);
 
+/* Binary copy of a latticegen2 pair: node rows (indices, coordinates, fluid
+ * flag) and BC rows (indices, direction, type, point, id) as arrays after an
+ * "LBMLAT01" header.  lbm_read_lattice and lbm_read_wall_q take such a file as
+ * node_path (bc_path is then ignored) and return what the text pair gives. */
+lbm_status lbm_write_lattice_binary(const char* node_path, const char* bc_path, const char* out_path);
+
 /* Lattice generator compatible with the reference's latticegen2
  * (L/main.cpp:17-70, L/GeomRead.cpp:14-189, L/MeshGen.cpp:19-80,
  * L/FluidTracking.cpp:24-236, L/Intersect.cpp:43-528): reads a STAR-CD

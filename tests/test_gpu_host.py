@@ -214,6 +214,29 @@ def test_lbmsolver_fdreldiff_stop_condition(tmp_path):
     assert np.allclose(rows[:, 1], got, rtol=1e-4, atol=0)
 
 
+def test_lbmsolver_binary_lattice(tmp_path):
+    """With Mesh/<Lattice>.lbl present lbmsolver reads the binary copy; the
+    results equal those from the text pair."""
+    print("case: lbmsolver binary lattice", flush=True)
+    nx, ny, U, nu = 48, 14, 0.02, 0.1
+    ch = LT.channel2d(nx, ny, 1.5 * U, obstacles=((15, 7, 2.5),))
+    out = {}
+    for mode in ("text", "binary"):
+        d = tmp_path / mode
+        os.makedirs(d / "Mesh")
+        LT.write_latticegen2(str(d / "Mesh" / "chan"), ch["node_class"], ch["bb"])
+        if mode == "binary":
+            L.write_lattice_binary(str(d / "Mesh" / "chan_Node.dat"), str(d / "Mesh" / "chan_BC.dat"),
+                                   str(d / "Mesh" / "chan.lbl"))
+            os.remove(d / "Mesh" / "chan_BC.dat")      # the binary copy alone must do
+        (d / "SetUpData.ini").write_text(INI.format(U=U, nu=nu, iters=200))
+        r = subprocess.run([CLI, "-f", "SetUpData.ini"], cwd=d, capture_output=True, text=True, timeout=120)
+        print(r.stdout[-200:], r.stderr[-300:])
+        assert r.returncode == 0
+        out[mode] = (d / "Results" / "FinalData.csv").read_bytes()
+    assert out["text"] == out["binary"]
+
+
 def read_appended_vti(path, n):
     """Velocity (n x 3) and density (n) of a binary-appended VTI file."""
     raw = open(path, "rb").read()

@@ -134,3 +134,31 @@ def test_obstacle_box_fill_and_boundary_links(tmp_path):
     # the obstacle are cut at the node itself (on-node walls)
     ax = rows[(rows[:, 3] == 1) & (rows[:, 0] == 4)]
     assert len(ax) > 0 and np.allclose(ax[:, 5], 0.25)
+
+
+def test_binary_lattice_round_trip(tmp_path):
+    """A binary copy reads back to the same classes, wall links, counts,
+    spacing, cyclic axes and wall fractions as the text pair (the reference's
+    lid cavity, and a 2D channel with curved-wall points)."""
+    node = str(tmp_path / "lid_Node.dat")
+    bc = str(tmp_path / "lid_BC.dat")
+    for kind, path in (("Node", node), ("BC", bc)):
+        open(path, "wb").write(gzip.open(os.path.join(GOLDEN, f"lidvalidation_{kind}.dat.gz"), "rb").read())
+    ny, nx = 12, 20
+    ch = LT.channel2d(nx, ny, 0.03, obstacles=((8, 6, 2.5),))
+    q = np.zeros((ny, nx, 19))
+    q[..., 1:9] = np.where((ch["bb"][..., None] >> np.arange(1, 9)) & 1, 0.3, 0.0)
+    LT.write_latticegen2(str(tmp_path / "ch"), ch["node_class"], ch["bb"], wall_q=q)
+    for np_, bp in ((node, bc), (str(tmp_path / "ch_Node.dat"), str(tmp_path / "ch_BC.dat"))):
+        bin_path = np_ + ".lbl"
+        L.write_lattice_binary(np_, bp, bin_path)
+        a, b = L.read_lattice(np_, bp), L.read_lattice(bin_path)
+        for k in ("dims", "nlinks", "delta", "cyclic"):
+            assert a[k] == b[k], k
+        for k in ("node_class", "bb_links", "type_counts", "dir_counts"):
+            assert (a[k] == b[k]).all(), k
+        assert (L.read_wall_q(np_, bp) == L.read_wall_q(bin_path, None)).all()
+    with open(tmp_path / "bad.lbl", "wb") as f:
+        f.write(b"LBMLAT01" + b"\0" * 10)
+    with pytest.raises(L.LbmError):
+        L.read_lattice(str(tmp_path / "bad.lbl"))

@@ -463,6 +463,13 @@ int main(int argc, char** argv) {
         if (keep.residual_every != 100) a.residual_every = keep.residual_every;
         a.node_file = "Mesh/" + a.lattice + "_Node.dat";   // S/main.cu:97-100
         a.bc_file = "Mesh/" + a.lattice + "_BC.dat";
+        // a binary copy of the lattice (lbm_write_lattice_binary), when there is
+        // one, is read instead of the text pair
+        const std::string bin = "Mesh/" + a.lattice + ".lbl";
+        if (FILE* fb = std::fopen(bin.c_str(), "rb")) {
+            std::fclose(fb);
+            a.node_file = bin;
+        }
     }
 
     int32_t dims[3];

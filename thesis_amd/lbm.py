@@ -34,7 +34,7 @@ EXPORTED = [
     "lbm_abi_version", "lbm_device_count", "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_init",
     "lbm_step", "lbm_run_to_residual", "lbm_reduce", "lbm_reduce_all", "lbm_residual_l2", "lbm_residual_fdrel", "lbm_get_macros", "lbm_get_f", "lbm_set_f",
     "lbm_time_steps", "lbm_device_bytes", "lbm_read_lattice", "lbm_read_wall_q", "lbm_latticegen",
-    "lbm_latticegen_last_error",
+    "lbm_latticegen_last_error", "lbm_write_lattice_binary",
     "lbm_cg_create", "lbm_cg_destroy", "lbm_cg_last_error", "lbm_cg_init", "lbm_cg_step", "lbm_cg_get_macros",
     "lbm_cg_get_f", "lbm_cg_reduce", "lbm_cg_time_steps", "lbm_cg_init_f", "lbm_cg_get_macros3d",
     "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_last_error", "lbm_batch_init", "lbm_batch_run",
@@ -132,6 +132,7 @@ def load():
     L.lbm_latticegen.argtypes = [C.c_char_p, C.c_char_p, C.c_int32, dp, C.c_int32, C.c_char_p, C.c_char_p,
                                  C.c_char_p, P(C.c_int32), P(C.c_int64)]
     L.lbm_latticegen_last_error.restype = C.c_char_p
+    L.lbm_write_lattice_binary.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p]
     L.lbm_cg_create.argtypes = [P(CgParams), P(vp)]
     L.lbm_cg_destroy.argtypes = [vp]
     L.lbm_cg_destroy.restype = None
@@ -190,7 +191,7 @@ def read_wall_q(node_path, bc_path):
     r = read_lattice(node_path, None)
     nx, ny, nz = r["dims"]
     q = np.zeros(nx * ny * nz * 19)
-    st = L.lbm_read_wall_q(node_path.encode(), bc_path.encode(), _dp(q))
+    st = L.lbm_read_wall_q(node_path.encode(), bc_path.encode() if bc_path else None, _dp(q))
     if st:
         raise LbmError(st, f"cannot read {node_path} / {bc_path}")
     return q.reshape(nz, ny, nx, 19)
@@ -239,6 +240,13 @@ def latticegen(vrt_path, bnd_path, nx, seed, out_prefix, complexity=1, mesh_path
     if st:
         raise LbmError(st, L.lbm_latticegen_last_error().decode())
     return tuple(dims[:]), nl.value
+
+
+def write_lattice_binary(node_path, bc_path, out_path):
+    """Binary copy of a latticegen2 pair, readable by read_lattice(out_path)."""
+    st = load().lbm_write_lattice_binary(node_path.encode(), bc_path.encode() if bc_path else None, out_path.encode())
+    if st:
+        raise LbmError(st, f"cannot convert {node_path} / {bc_path}")
 
 
 class Solver:
