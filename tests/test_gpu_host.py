@@ -157,6 +157,8 @@ def test_lbmsolver_macro_diff_stop_condition(tmp_path):
     print(r.stdout[-300:], r.stderr[-300:])
     assert r.returncode == 0 and "simulation converged!" in r.stdout
     lines = (tmp_path / "Results" / "MacroDiffs.txt").read_text().splitlines()
+    assert lines[0] == "iteration\tMax u diff\tMax v diff\tMax w diff\tMax rho diff"   # S/main.cu:125-127
+    lines = lines[1:]
     import re
     assert all(re.fullmatch(r"\d+\t\t\d+\.\d{10}\t\d+\.\d{10}\t\d+\.\d{10}\t\d+\.\d{10}", ln) for ln in lines)
     last = lines[-1].split()
@@ -296,3 +298,67 @@ def test_lbmsolver_autosave_and_appended_vti(tmp_path):
     assert r.returncode == 0 and "Initial conditions loaded" in r.stdout, r.stderr
     vel2, rho2 = read_appended_vti(tmp_path / "Run2" / "FinalData.vti", n)
     assert (vel2 == vel).all() and (rho2 == rho).all()
+
+
+INI_MP = INI.replace('Lattice = "chan"', 'Lattice = "box"').replace("BCoutlet = Vout", "BCoutlet = 1stE") + \
+    "multiphase = yes\ncg_bubble_radius = {R}\n"
+
+
+@pytest.mark.parametrize("three_d", [True, False])
+def test_lbmsolver_multiphase_bubble(tmp_path, three_d):
+    """[mi355x] multiphase = yes: the reference's default colour-gradient
+    steady bubble (S/main.cu:52-72) on the lattice's periodic box equals the
+    ABI run from the same start, and "surface tension.txt" holds one error
+    value per iteration."""
+    print(f"case: lbmsolver multiphase 3D={three_d}", flush=True)
+    os.makedirs(tmp_path / "Mesh")
+    iters, R = 120, 5.0
+    if three_d:
+        # the reference's periodic cube (CYCL on all faces), 17^3 nodes: period 16
+        star = os.path.join(ROOT, "tests", "golden", "starcd")
+        dims, _ = L.latticegen(os.path.join(star, "periodicCube.vrt"), os.path.join(star, "periodicCube.bnd"), 17,
+                               (0.5, 0.5, 0.5), str(tmp_path / "Mesh" / "box"))
+        n = 16
+        shape = (n, n, n)
+    else:
+        n = 32
+        LT.write_latticegen2(str(tmp_path / "Mesh" / "box"), np.ones((n + 1, n + 1), np.uint8),
+                             np.zeros((n + 1, n + 1), np.uint32), delta=1.0 / n)
+        shape = (n, n)
+    (tmp_path / "SetUpData.ini").write_text(INI_MP.format(U=0.0, nu=0.1, iters=iters, R=R)
+                                            .replace("OutputFormat = CSV", "OutputFormat = CSV"))
+    r = subprocess.run([CLI, "-f", "SetUpData.ini"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-400:], r.stderr[-300:])
+    assert r.returncode == 0 and "Suface tension error" in r.stdout
+    data = np.loadtxt(tmp_path / "Results" / "FinalData.csv", delimiter=",", skiprows=1)
+    err = np.loadtxt(tmp_path / "Results" / "surface tension.txt")
+    assert err.shape == (iters,) and np.isfinite(err).all()
+    # the same run through the C ABI
+    grids = np.meshgrid(*[np.arange(k) / n for k in shape], indexing="ij")
+    d2 = sum((g - 0.5) ** 2 for g in grids)
+    inside = np.sqrt(d2) <= R / (n + 1)
+    r_rho = np.where(inside, 1.0, 0.0)
+    b_rho = 1.0 - r_rho
+    kw = dict(r_alpha=4 / 9, b_alpha=4 / 9, r_nu=0.5, b_nu=0.5, beta=0.99, A=1e-4, high_order=1, kappa=0.9)
+    if three_d:
+        g = L.CgSolver(n, n, nz=n, **kw)
+        g.init_f(*LT.bubble3d_populations(r_rho, b_rho))
+    else:
+        g = L.CgSolver(n, n, **kw)
+        g.init(r_rho, b_rho)
+    g.step(iters)
+    m = g.macros()
+    red = g.reduce()
+    g.close()
+    full = tuple(k + 1 for k in shape)
+    idx = np.ix_(*[np.arange(k) % (k - 1) for k in full])
+    rho = m["rho"][idx].ravel()
+    u = m["u"][idx].ravel()
+    assert np.abs(data[:, 7] - rho).max() < 1e-9
+    assert np.abs(data[:, 3] - u).max() < 1e-9
+    om = 1 / (3 * 0.5 + 0.5)
+    sigma = 4 * 1e-4 / 9 / om
+    pf = 0.5 if three_d else 0.6
+    pin = pf * (5 / 9) * red["pin_sum"] / red["pin_n"]
+    pout = pf * (5 / 9) * red["pout_sum"] / red["pout_n"]
+    assert abs(err[-1] - abs(sigma - R * (pin - pout)) / sigma * 100) < 1e-6

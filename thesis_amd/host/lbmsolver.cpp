@@ -17,6 +17,10 @@
 //   force_model = shift|guo       rho_out = <density>
 //   inlet = zouhe|reference       outlet = pressure|velocity|extrap2|extrap1|feq
 //   vti = appended                (binary-appended VTI output, also read back on restart)
+//   multiphase = yes              (the reference's default colour-gradient steady bubble; its
+//                                  parameters by cg_beta, cg_A, cg_b_alpha, cg_gamma, cg_r_density,
+//                                  cg_r_viscosity, cg_b_viscosity, cg_g_limit, cg_high_order,
+//                                  cg_enhanced, cg_bubble_radius, cg_control_param)
 // BCoutlet maps as the reference reads it (S/FilesReading.cu:180-190 with
 // S/include/Arguments.h:22-28): in 2D Vout -> velocity outlet, 1stE ->
 // second-order extrapolation, 2ndE -> first-order extrapolation, HEmodel ->
@@ -59,6 +63,12 @@ struct Args {
     std::string init_file;           // ini line 18
     int storage = LBM_STORE_F64, init_eq = 0, wall_model = LBM_WALL_ONNODE, force_model = LBM_FORCE_SHIFT;
     double trt_magic = 0, rho_out = -1;
+    // colour-gradient two-phase run ([mi355x] multiphase = yes): the steady
+    // bubble with the reference's hard-coded parameters (S/main.cu:52-72)
+    bool multiphase = false;
+    double cg_r_density = 1.0, cg_gamma = 1.0, cg_b_alpha = 4.0 / 9.0, cg_r_nu = 0.5, cg_b_nu = 0.5;
+    double cg_beta = 0.99, cg_A = 1e-4, cg_kappa = 0.9, cg_g_limit = 0.0, cg_radius = 18.0;
+    int cg_high_order = 1, cg_enhanced = 0;
 };
 
 std::string trim(const std::string& s) {
@@ -134,6 +144,14 @@ bool read_ini(const std::string& path, Args& a) {
         const std::string& o = kv["outlet"];
         a.outlet = o == "velocity" ? 1 : o == "extrap2" ? 2 : o == "extrap1" ? 3 : o == "feq" ? 4 : 5;
     }
+    if (kv.count("multiphase")) a.multiphase = kv["multiphase"] == "yes";
+    auto num = [&](const char* k, double& dst) { if (kv.count(k)) dst = atof(kv[k].c_str()); };
+    num("cg_r_density", a.cg_r_density); num("cg_gamma", a.cg_gamma); num("cg_b_alpha", a.cg_b_alpha);
+    num("cg_r_viscosity", a.cg_r_nu); num("cg_b_viscosity", a.cg_b_nu); num("cg_beta", a.cg_beta);
+    num("cg_A", a.cg_A); num("cg_control_param", a.cg_kappa); num("cg_g_limit", a.cg_g_limit);
+    num("cg_bubble_radius", a.cg_radius);
+    if (kv.count("cg_high_order")) a.cg_high_order = atoi(kv["cg_high_order"].c_str());
+    if (kv.count("cg_enhanced")) a.cg_enhanced = atoi(kv["cg_enhanced"].c_str());
     return true;
 }
 
@@ -270,7 +288,8 @@ bool read_initial_conditions(const std::string& path, size_t N, std::vector<doub
 const char* ext_of(int fmt) { return fmt == 1 ? "csv" : fmt == 2 ? "dat" : "vti"; }
 
 void write_results(const std::string& path, int fmt, int n, int m, int h, double dx, const std::vector<double>& rho,
-                   const std::vector<double>& u, const std::vector<double>& v, const std::vector<double>& w) {
+                   const std::vector<double>& u, const std::vector<double>& v, const std::vector<double>& w,
+                   const std::vector<double>* rr = nullptr, const std::vector<double>* rb = nullptr) {
     FILE* fp = std::fopen(path.c_str(), "w");
     if (!fp) return;
     const size_t N = (size_t)n * m * h;
@@ -321,6 +340,14 @@ void write_results(const std::string& path, int fmt, int n, int m, int h, double
         std::fprintf(fp, "      </DataArray>\n"
                          "      <DataArray type=\"Float32\" Name=\"Density\" NumberOfComponents=\"1\" format=\"ascii\">\n");
         for (size_t i = 0; i < N; ++i) std::fprintf(fp, "%.10f\n", rho[i]);
+        if (rr && rb) {   // two-phase output (S/FilesWriting.cu:91-112)
+            std::fprintf(fp, "      </DataArray>\n"
+                             "      <DataArray type=\"Float32\" Name=\"Red Density\" NumberOfComponents=\"1\" format=\"ascii\">\n");
+            for (size_t i = 0; i < N; ++i) std::fprintf(fp, "%.10f\n", (*rr)[i]);
+            std::fprintf(fp, "      </DataArray>\n"
+                             "      <DataArray type=\"Float32\" Name=\"Blue Density\" NumberOfComponents=\"1\" format=\"ascii\">\n");
+            for (size_t i = 0; i < N; ++i) std::fprintf(fp, "%.10f\n", (*rb)[i]);
+        }
         std::fprintf(fp, "      </DataArray>\n"
                          "      <DataArray type=\"Float32\" Name=\"Pressure\" NumberOfComponents=\"1\" format=\"ascii\">\n");
         for (size_t i = 0; i < N; ++i) std::fprintf(fp, "%.10f\n", rho[i] / 3.0);
@@ -415,6 +442,111 @@ double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Colour-gradient steady bubble on the lattice's box (the reference's
+// default run: multiPhase = 1, test_case = 1, S/main.cu:52-72, driven by
+// S/Iterate.cu:392-500 and S/Iterate3D.cu:495-800).  The box is periodic:
+// in 2D always (gpuBcPeriodic2D, period n - 1), in 3D along the axes whose
+// faces carry CYCL links (any other boundary link is refused).  The bubble
+// holds the nodes within bubble_radius / n of (0.5, 0.5[, 0.5]) in the
+// lattice's coordinates, taken as i * delta from the origin as latticegen2
+// writes them (initCGBubble / initCGBubble3D, S/GpuInit.cu:323-592; the 3D
+// start has f_q = rho phi_q and f_0 = 0).  Every step the Laplace sums give
+// the surface-tension error |sigma - R (p_in - p_out)| / sigma in %, with
+// p = (3/5)(1 - alpha) rho in 2D and (1/2)(1 - alpha) rho in 3D and
+// sigma = 4 A / (9 omega_eff) (S/Multiphase.cu:1082-1102,
+// S/Iterate3D.cu:222-230); it goes to "surface tension.txt", one line per
+// iteration (S/FilesWriting.cu:338-357).
+int run_multiphase(const Args& a, const int dims[3], const int sim[3], double dx, Times& tm, double t_start) {
+    const bool three_d = dims[2] > 1;
+    const double b_density = a.cg_r_density / a.cg_gamma;
+    const double r_alpha = 1.0 - (1.0 - a.cg_b_alpha) / a.cg_gamma;
+    lbm_cg_params p{};
+    p.abi_version = LBM_ABI_VERSION;
+    p.nx = sim[0]; p.ny = sim[1]; p.nz = three_d ? sim[2] : 1;
+    p.r_alpha = r_alpha; p.b_alpha = a.cg_b_alpha;
+    p.r_nu = a.cg_r_nu; p.b_nu = a.cg_b_nu;
+    p.beta = a.cg_beta; p.A = a.cg_A; p.g_limit = a.cg_g_limit;
+    p.high_order = a.cg_high_order; p.kappa = a.cg_kappa;
+    p.storage = a.storage == LBM_STORE_BF16 ? LBM_STORE_F16 : a.storage;
+    p.enhanced = a.cg_enhanced;
+    lbm_cg_ctx* cg = nullptr;
+    lbm_status st = lbm_cg_create(&p, &cg);
+    if (st != LBM_OK) { std::fprintf(stderr, "colour-gradient set-up failed (%d)\n", (int)st); return 2; }
+    const size_t NS = (size_t)sim[0] * sim[1] * sim[2];
+    const int q = three_d ? 19 : 9;
+    const double rad = a.cg_radius / dims[0];
+    std::vector<double> rr(NS, 0.0), rb(NS, 0.0);
+    for (size_t i = 0; i < NS; ++i) {
+        const double x = (double)(i % sim[0]) * dx, y = (double)((i / sim[0]) % sim[1]) * dx;
+        const double z = (double)(i / ((size_t)sim[0] * sim[1])) * dx;
+        const double d2 = (x - 0.5) * (x - 0.5) + (y - 0.5) * (y - 0.5) + (three_d ? (z - 0.5) * (z - 0.5) : 0.0);
+        if (std::sqrt(d2) <= rad) rr[i] = a.cg_r_density;
+        else rb[i] = b_density;
+    }
+    if (three_d) {
+        std::vector<double> rf(q * NS), bf(q * NS);
+        for (int k = 0; k < q; ++k) {
+            const double phi = k == 0 ? 0.0 : (k <= 6 ? 1.0 / 12.0 : 1.0 / 24.0);
+            for (size_t i = 0; i < NS; ++i) { rf[k * NS + i] = rr[i] * phi; bf[k * NS + i] = rb[i] * phi; }
+        }
+        st = lbm_cg_init_f(cg, rf.data(), bf.data());
+    } else {
+        st = lbm_cg_init(cg, rr.data(), rb.data());
+    }
+    if (st != LBM_OK) { std::fprintf(stderr, "init: %s\n", lbm_cg_last_error(cg)); lbm_cg_destroy(cg); return 2; }
+    const double aux = a.cg_r_density / ((a.cg_r_density + b_density) * a.cg_r_nu) +
+                       b_density / ((a.cg_r_density + b_density) * a.cg_b_nu);
+    const double om = 1.0 / (3.0 / aux + 0.5);
+    const double sigma = 4.0 * a.cg_A / 9.0 / om;
+    const double pf = three_d ? 0.5 : 0.6;
+    tm.init = now_s() - t_start;
+    const double t_loop = now_s();
+    std::vector<double> st_err((size_t)a.iterations, 0.0);
+    int code = 0;
+    for (long it = 0; it < a.iterations; ++it) {
+        double t0 = now_s();
+        st = lbm_cg_step(cg, 1);
+        tm.coll += now_s() - t0;
+        if (st != LBM_OK) { std::fprintf(stderr, "step: %s\n", lbm_cg_last_error(cg)); code = 2; break; }
+        t0 = now_s();
+        double red[12];
+        lbm_cg_reduce(cg, red);
+        tm.resi += now_s() - t0;
+        if (red[11] > 0) { std::fprintf(stderr, "\nDIVERGENCE!\n"); code = 1; break; }
+        const double pin = pf * (1.0 - r_alpha) * red[2] / red[3], pout = pf * (1.0 - a.cg_b_alpha) * red[4] / red[5];
+        st_err[it] = std::fabs(sigma - a.cg_radius * (pin - pout)) / sigma * 100.0;
+        if ((it + 1) % (a.residual_every > 0 ? a.residual_every : a.iterations) == 0)
+            std::printf("iteration %ld surface tension error %.4f %%, max |u| %.3e\n", it + 1, st_err[it], red[10]);
+    }
+    tm.iter = now_s() - t_loop;
+    const double t_write = now_s();
+    if (a.iterations > 0) std::printf("Suface tension error: %.10f\n", st_err[(size_t)a.iterations - 1]);
+    if (FILE* f = std::fopen((a.out_dir + "surface tension.txt").c_str(), "w")) {
+        for (double e : st_err) std::fprintf(f, "%.10f\n", e);
+        std::fclose(f);
+    }
+    std::vector<double> sr(NS), sb(NS), sp(NS), su(NS), sv(NS), sw(NS);
+    lbm_cg_get_macros3d(cg, sr.data(), sb.data(), sp.data(), su.data(), sv.data(), sw.data());
+    lbm_cg_destroy(cg);
+    const size_t N = (size_t)dims[0] * dims[1] * dims[2];
+    std::vector<double> R(N), U(N), V(N), W(N), RR(N), RB(N);
+    for (int z = 0; z < dims[2]; ++z)
+        for (int y = 0; y < dims[1]; ++y)
+            for (int x = 0; x < dims[0]; ++x) {
+                const size_t d = ((size_t)z * dims[1] + y) * dims[0] + x;
+                const size_t s = ((size_t)(z % sim[2]) * sim[1] + (y % sim[1])) * sim[0] + (x % sim[0]);
+                R[d] = sp[s]; U[d] = su[s]; V[d] = sv[s]; W[d] = sw[s]; RR[d] = sr[s]; RB[d] = sb[s];
+            }
+    const char* ext = ext_of(a.format);
+    write_results(a.out_dir + "FinalData." + ext, a.format, dims[0], dims[1], dims[2], dx, R, U, V, W, &RR, &RB);
+    std::printf("results written to %sFinalData.%s\n", a.out_dir.c_str(), ext);
+    tm.writ += now_s() - t_write;
+    tm.overall = now_s() - t_start;
+    write_end_log(a.out_dir + "lbmsolver.log", tm);
+    write_times(a.out_dir + "runtimes.dat", tm);
+    return code;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -497,6 +629,23 @@ int main(int argc, char** argv) {
     int sim[3] = {dims[0], dims[1], dims[2]};
     for (int ax = 0; ax < 3; ++ax)
         if (cyc[ax] && dims[ax] > 1) sim[ax] = dims[ax] - 1;
+    // the reference writes the MacroDiffs.txt header at start-up (S/main.cu:125-127)
+    mkdir(a.out_dir.c_str(), 0755);
+    if (FILE* hf = std::fopen((a.out_dir + "MacroDiffs.txt").c_str(), "w")) {
+        std::fprintf(hf, "iteration\tMax u diff\tMax v diff\tMax w diff\tMax rho diff\n");
+        std::fclose(hf);
+    }
+    if (a.multiphase) {
+        // 2D: periodic box (period n - 1 on both axes); 3D: CYCL faces only
+        if (!three_d) { sim[0] = dims[0] - 1; sim[1] = dims[1] - 1; }
+        else if (tc[1] + tc[2] + tc[3] + tc[5] > 0) {
+            std::fprintf(stderr, "the colour-gradient run needs a periodic lattice (CYCL links only)\n");
+            return 2;
+        }
+        write_init_log(a.out_dir + "lbmsolver.log", a, dx, dims[0], dims[1], dims[2], 0, 0.0, 0.0);
+        Times tmp;
+        return run_multiphase(a, dims, sim, dx, tmp, t_start);
+    }
     const size_t NS = (size_t)sim[0] * sim[1] * sim[2];
     std::vector<uint8_t> scls(NS);
     std::vector<uint32_t> sbb(NS);
@@ -656,7 +805,7 @@ int main(int argc, char** argv) {
     // (S/Iterate3D.cu:677-693, S/ComputeResiduals.cu:28-36).
     const bool macro_diff = a.residual_model.find("MacroDiff") != std::string::npos;
     const bool fd_rel = a.residual_model.find("FdRelDiff") != std::string::npos;
-    FILE* mf = macro_diff ? std::fopen((a.out_dir + "MacroDiffs.txt").c_str(), "w") : nullptr;
+    FILE* mf = macro_diff ? std::fopen((a.out_dir + "MacroDiffs.txt").c_str(), "a") : nullptr;
     double r4[8];
     lbm_reduce_all(ctx, r4);
     const long every = a.residual_every > 0 ? a.residual_every : a.iterations;
