@@ -217,3 +217,14 @@ def test_lbmsolver_reference_lid_cavity(tmp_path):
     assert saves == ["autosave_iter00100.csv", "autosave_iter00200.csv", "autosave_iter00300.csv"]
     last = np.loadtxt(tmp_path / "Results" / saves[-1], delimiter=",", skiprows=1)
     assert (last == data).all()
+    # the 3D line and plane files (S/FilesWriting.cu:260-337)
+    lines = (tmp_path / "Results" / "midLines.txt").read_text().splitlines()
+    assert lines[0] == "CASE: Ulid = 0.050000 Nx = 30" and lines[1] == "U/Ulid\tZ\tX\tW/Ulid"
+    ml = np.loadtxt(lines[2:])
+    u3 = data[:, 3].reshape(cls.shape)
+    w3 = data[:, 5].reshape(cls.shape)
+    assert ml.shape == (30, 4)
+    assert np.abs(ml[:, 0] - u3[:, 15, 15] / 0.05).max() < 1e-7
+    assert np.abs(ml[:, 3] - w3[15, 15, :] / 0.05).max() < 1e-7
+    plane = np.loadtxt(tmp_path / "Results" / "channelPlane.txt", skiprows=2)
+    assert plane.shape == (900, 6) and np.abs(plane[:, 3] - u3[:, :, 22].ravel()).max() < 1e-9

@@ -357,6 +357,39 @@ void write_results(const std::string& path, int fmt, int n, int m, int h, double
     std::fclose(fp);
 }
 
+// The 3D runs' line and plane files (S/FilesWriting.cu:260-337, written at
+// the end of every 3D run, S/Iterate3D.cu:911-912): midLines.txt holds u/U
+// on the vertical line x = n/2, y = m/2 against z and w/U on the line
+// y = m/2, z = h/2 against x (one row per z; the reference's arrays have n
+// rows, so rows beyond n carry zeros there); channelPlane.txt holds the plane
+// x = floor(3n/4).  Coordinates are i * delta as latticegen2 writes them.
+void write_3d_lines(const std::string& dir, int n, int m, int h, double dx, const std::vector<double>& U,
+                    const std::vector<double>& V, const std::vector<double>& W, double ulid) {
+    const size_t nm = (size_t)n * m;
+    if (FILE* f = std::fopen((dir + "midLines.txt").c_str(), "w")) {
+        std::fprintf(f, "CASE: Ulid = %f Nx = %d\n", ulid, n);
+        std::fprintf(f, "U/Ulid\tZ\tX\tW/Ulid\n");
+        for (int k = 0; k < h; ++k) {
+            const double uz = U[(size_t)(n / 2) + (size_t)(m / 2) * n + k * nm] / ulid;
+            const double xx = k < n ? k * dx : 0.0;
+            const double wx = k < n ? W[(size_t)k + (size_t)(m / 2) * n + (size_t)(h / 2) * nm] / ulid : 0.0;
+            std::fprintf(f, "%.8f\t%.8f\t%.8f\t%.8f\n", uz, k * dx, xx, wx);
+        }
+        std::fclose(f);
+    }
+    if (FILE* f = std::fopen((dir + "channelPlane.txt").c_str(), "w")) {
+        std::fprintf(f, "CASE: Uin = %f Ny = %d\n", ulid, m);
+        std::fprintf(f, "X\tY\tZ\tU\tV\tW\n");
+        const int x = (int)((3. / 4.) * n);
+        for (int z = 0; z < h; ++z)
+            for (int y = 0; y < m; ++y) {
+                const size_t i = (size_t)x + (size_t)y * n + z * nm;
+                std::fprintf(f, "%.10f %.10f %.10f %.10f %.10f %.10f\n", x * dx, y * dx, z * dx, U[i], V[i], W[i]);
+            }
+        std::fclose(f);
+    }
+}
+
 // Results/lbmsolver.log in the layout of the reference's writeInitLog,
 // writeNodeNumbers and writeEndLog (S/LogWriter.cu:5-174).
 void write_init_log(const std::string& path, const Args& a, double delta, int n, int m, int h, long inlet_nodes,
@@ -539,6 +572,7 @@ int run_multiphase(const Args& a, const int dims[3], const int sim[3], double dx
             }
     const char* ext = ext_of(a.format);
     write_results(a.out_dir + "FinalData." + ext, a.format, dims[0], dims[1], dims[2], dx, R, U, V, W, &RR, &RB);
+    if (three_d) write_3d_lines(a.out_dir, dims[0], dims[1], dims[2], dx, U, V, W, a.u);
     std::printf("results written to %sFinalData.%s\n", a.out_dir.c_str(), ext);
     tm.writ += now_s() - t_write;
     tm.overall = now_s() - t_start;
@@ -894,6 +928,7 @@ int main(int argc, char** argv) {
             }
     const char* ext = ext_of(a.format);
     write_results(a.out_dir + "FinalData." + ext, a.format, dims[0], dims[1], dims[2], dx, R, U, V, W);
+    if (three_d) write_3d_lines(a.out_dir, dims[0], dims[1], dims[2], dx, U, V, W, a.u);
     std::printf("results written to %sFinalData.%s\n", a.out_dir.c_str(), ext);
     tm.writ += now_s() - t_write;
     tm.overall = now_s() - t_start;
