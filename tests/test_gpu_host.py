@@ -308,7 +308,7 @@ INI_MP = INI.replace('Lattice = "chan"', 'Lattice = "box"').replace("BCoutlet = 
 def test_lbmsolver_multiphase_bubble(tmp_path, three_d):
     """[mi355x] multiphase = yes: the reference's default colour-gradient
     steady bubble (S/main.cu:52-72) on the lattice's periodic box equals the
-    ABI run from the same start, and "surface tension.txt" holds one error
+    ABI run from the same start, and the surface-tension error file holds one
     value per iteration."""
     print(f"case: lbmsolver multiphase 3D={three_d}", flush=True)
     os.makedirs(tmp_path / "Mesh")
@@ -331,7 +331,13 @@ def test_lbmsolver_multiphase_bubble(tmp_path, three_d):
     print(r.stdout[-400:], r.stderr[-300:])
     assert r.returncode == 0 and "Suface tension error" in r.stdout
     data = np.loadtxt(tmp_path / "Results" / "FinalData.csv", delimiter=",", skiprows=1)
-    err = np.loadtxt(tmp_path / "Results" / "surface tension.txt")
+    if three_d:      # WriteArray: one value per line
+        err = np.loadtxt(tmp_path / "Results" / "surface tension.txt")
+    else:            # writeOscilatingSolution: "i value" per line
+        rows = np.loadtxt(tmp_path / "Results" / "Oscilating_frequency.txt")
+        assert (rows[:, 0] == np.arange(iters)).all()
+        err = rows[:, 1]
+        assert "Pressure inside" in r.stdout
     assert err.shape == (iters,) and np.isfinite(err).all()
     # the same run through the C ABI
     grids = np.meshgrid(*[np.arange(k) / n for k in shape], indexing="ij")

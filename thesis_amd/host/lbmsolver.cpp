@@ -487,8 +487,9 @@ double now_s() {
 // the surface-tension error |sigma - R (p_in - p_out)| / sigma in %, with
 // p = (3/5)(1 - alpha) rho in 2D and (1/2)(1 - alpha) rho in 3D and
 // sigma = 4 A / (9 omega_eff) (S/Multiphase.cu:1082-1102,
-// S/Iterate3D.cu:222-230); it goes to "surface tension.txt", one line per
-// iteration (S/FilesWriting.cu:338-357).
+// S/Iterate3D.cu:222-230); it goes to "surface tension.txt" in 3D, one line
+// per iteration (S/FilesWriting.cu:338-357), and "Oscilating_frequency.txt"
+// in 2D, "i value" per line (S/FilesWriting.cu:384-394).
 int run_multiphase(const Args& a, const int dims[3], const int sim[3], double dx, Times& tm, double t_start) {
     const bool three_d = dims[2] > 1;
     const double b_density = a.cg_r_density / a.cg_gamma;
@@ -535,6 +536,7 @@ int run_multiphase(const Args& a, const int dims[3], const int sim[3], double dx
     tm.init = now_s() - t_start;
     const double t_loop = now_s();
     std::vector<double> st_err((size_t)a.iterations, 0.0);
+    double pin_mean = 0, pout_mean = 0;
     int code = 0;
     for (long it = 0; it < a.iterations; ++it) {
         double t0 = now_s();
@@ -546,17 +548,27 @@ int run_multiphase(const Args& a, const int dims[3], const int sim[3], double dx
         lbm_cg_reduce(cg, red);
         tm.resi += now_s() - t0;
         if (red[11] > 0) { std::fprintf(stderr, "\nDIVERGENCE!\n"); code = 1; break; }
-        const double pin = pf * (1.0 - r_alpha) * red[2] / red[3], pout = pf * (1.0 - a.cg_b_alpha) * red[4] / red[5];
+        pin_mean = red[2] / red[3];
+        pout_mean = red[4] / red[5];
+        const double pin = pf * (1.0 - r_alpha) * pin_mean, pout = pf * (1.0 - a.cg_b_alpha) * pout_mean;
         st_err[it] = std::fabs(sigma - a.cg_radius * (pin - pout)) / sigma * 100.0;
         if ((it + 1) % (a.residual_every > 0 ? a.residual_every : a.iterations) == 0)
             std::printf("iteration %ld surface tension error %.4f %%, max |u| %.3e\n", it + 1, st_err[it], red[10]);
     }
     tm.iter = now_s() - t_loop;
     const double t_write = now_s();
-    if (a.iterations > 0) std::printf("Suface tension error: %.10f\n", st_err[(size_t)a.iterations - 1]);
-    if (FILE* f = std::fopen((a.out_dir + "surface tension.txt").c_str(), "w")) {
-        for (double e : st_err) std::fprintf(f, "%.10f\n", e);
-        std::fclose(f);
+    if (a.iterations > 0) std::printf("Suface tension error: %lf\n", st_err[(size_t)a.iterations - 1]);
+    if (three_d) {   // WriteArray (S/Iterate3D.cu:871-872)
+        if (FILE* f = std::fopen((a.out_dir + "surface tension.txt").c_str(), "w")) {
+            for (double e : st_err) std::fprintf(f, "%.10f\n", e);
+            std::fclose(f);
+        }
+    } else {         // writeOscilatingSolution (S/Iterate.cu:667-673, S/FilesWriting.cu:384-394)
+        std::printf("Pressure inside %lf Pressure outside %lf ST_predicted %lf\n", pin_mean, pout_mean, sigma);
+        if (FILE* f = std::fopen((a.out_dir + "Oscilating_frequency.txt").c_str(), "w")) {
+            for (size_t i = 0; i < st_err.size(); ++i) std::fprintf(f, "%zu %.10f\n", i, st_err[i]);
+            std::fclose(f);
+        }
     }
     std::vector<double> sr(NS), sb(NS), sp(NS), su(NS), sv(NS), sw(NS);
     lbm_cg_get_macros3d(cg, sr.data(), sb.data(), sp.data(), su.data(), sv.data(), sw.data());
