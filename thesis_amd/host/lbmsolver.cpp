@@ -675,8 +675,10 @@ int main(int argc, char** argv) {
     int sim[3] = {dims[0], dims[1], dims[2]};
     for (int ax = 0; ax < 3; ++ax)
         if (cyc[ax] && dims[ax] > 1) sim[ax] = dims[ax] - 1;
-    // the reference writes the MacroDiffs.txt header at start-up (S/main.cu:125-127)
+    // the reference empties AlertLog.txt and writes the MacroDiffs.txt header
+    // at start-up (S/main.cu:25, :125-127, S/LogWriter.cu:222-226)
     mkdir(a.out_dir.c_str(), 0755);
+    if (FILE* af = std::fopen((a.out_dir + "AlertLog.txt").c_str(), "w")) std::fclose(af);
     if (FILE* hf = std::fopen((a.out_dir + "MacroDiffs.txt").c_str(), "w")) {
         std::fprintf(hf, "iteration\tMax u diff\tMax v diff\tMax w diff\tMax rho diff\n");
         std::fclose(hf);
