@@ -162,3 +162,25 @@ def test_binary_lattice_round_trip(tmp_path):
         f.write(b"LBMLAT01" + b"\0" * 10)
     with pytest.raises(L.LbmError):
         L.read_lattice(str(tmp_path / "bad.lbl"))
+
+
+def test_generator_rejects_bad_input(tmp_path):
+    vrt, bnd = stage(str(tmp_path), "lid3D")
+    out = str(tmp_path / "x")
+    with pytest.raises(L.LbmError) as e:
+        L.latticegen(str(tmp_path / "missing.vrt"), bnd, 10, (0.5, 0.5, 0.5), out)
+    assert e.value.status == 2                       # EIO
+    with pytest.raises(L.LbmError) as e:
+        L.latticegen(vrt, bnd, 1, (0.5, 0.5, 0.5), out)
+    assert e.value.status == 3                       # EINVAL: one node along x
+    with pytest.raises(L.LbmError) as e:
+        L.latticegen(vrt, bnd, 10, (0.5, 0.5, 0.5), out, complexity=3)
+    assert e.value.status == 3
+    with pytest.raises(L.LbmError) as e:             # no node nearer the seed than the origin is
+        L.latticegen(vrt, bnd, 10, (0.0, 0.0, 0.0), out)
+    assert e.value.status == 3 and "seed" in str(e.value)
+    bad = tmp_path / "bad.bnd"
+    bad.write_text("1 1 2 3 99 1 0 WALL\n")           # vertex 99 does not exist
+    with pytest.raises(L.LbmError) as e:
+        L.latticegen(vrt, str(bad), 10, (0.5, 0.5, 0.5), out)
+    assert e.value.status == 2 and "vertex" in str(e.value)
