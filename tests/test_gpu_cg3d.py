@@ -107,13 +107,15 @@ def test_cg3d_laplace_law_sphere():
     assert abs(red["mass_r"] - m0["mass_r"]) < 1e-4 * m0["mass_r"]
 
 
-def test_cg3d_throughput_256():
-    """Throughput of the two-launch step at 256^3 (FP32 and FP16 storage)."""
+@pytest.mark.parametrize("ho", [0, 1])
+def test_cg3d_throughput_256(ho):
+    """Throughput of the two-launch step at 256^3 (FP32 and FP16 storage;
+    18- and 104-neighbour gradients)."""
     n = 256
     r, b = LT.droplet3d(n, n, n, 40)
     for st in (L.STORE_F32, L.STORE_F16):
-        print(f"case: cg3d throughput {n}^3 storage {st}", flush=True)
-        g = L.CgSolver(n, n, nz=n, storage=st, A=0.01125, r_alpha=1 / 3, b_alpha=1 / 3)
+        print(f"case: cg3d throughput {n}^3 storage {st} high_order {ho}", flush=True)
+        g = L.CgSolver(n, n, nz=n, storage=st, A=0.01125, r_alpha=1 / 3, b_alpha=1 / 3, high_order=ho)
         g.init(r, b)
         g.step(5)
         ms = g.time_steps(20)
