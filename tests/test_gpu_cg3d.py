@@ -26,9 +26,9 @@ def compare(g, o, tol=1e-12):
 
 @pytest.mark.parametrize("ho,enh", [(0, 0), (1, 0), (0, 1), (1, 1)])
 def test_cg3d_f64_matches_oracle(oracle, ho, enh):
-    nx, ny, nz = 70, 13, 11            # partial 64 x 4 blocks
+    nx, ny, nz = 70, 13, 37            # partial 64 x 4 blocks
     print(f"case: cg3d parity {nx}x{ny}x{nz} high_order={ho} enhanced={enh}", flush=True)
-    r, b = LT.droplet3d(nx, ny, nz, 4.6, cx=30.3, cy=6.7, cz=5.2)
+    r, b = LT.droplet3d(nx, ny, nz, 4.6, cx=30.3, cy=6.7, cz=17.2)
     b = 0.6 * b                        # density contrast for the enhanced terms
     kw = dict(beta=0.7, A=0.01, r_nu=1 / 6, b_nu=0.1, r_alpha=0.3, b_alpha=0.4, high_order=ho, enhanced=enh)
     g = L.CgSolver(nx, ny, nz=nz, storage=L.STORE_F64, **kw)
