@@ -12,6 +12,8 @@ populations resident in HBM.  Prints one JSON line:
   cpu_baseline  the FP64 CPU oracle on the same lattice, a few steps
   warm_start    wall time to a 1e-6 residual from rest vs from the learned start
                 on held-out 512 x 128 channels (inference included)
+  f16_storage   the same step with FP16 deviation storage (FP32 compute), its
+                MLUPS and ratio to the FP32 line
 """
 import argparse
 import json
@@ -121,6 +123,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variants", action="store_true", help="also time f16/bf16 storage, MRT, BGK and the batch")
     ap.add_argument("--no-warm-start", action="store_true")
+    ap.add_argument("--no-f16", action="store_true", help="skip the FP16-storage line of the f32 run")
     ap.add_argument("--batch-only", action="store_true", help="only the 64-channel batch line (for profiling)")
     args = ap.parse_args()
 
@@ -201,6 +204,17 @@ def main():
     }
     if variants:
         out["variants"] = variants
+    if args.storage == "f32" and not args.no_f16:
+        # the same step with FP16 deviation storage (the 16-bit target is stated against FP32)
+        s16 = make("f16", coll)
+        s16.step(args.warmup)
+        ms16 = s16.time_steps(args.steps)
+        s16.close()
+        b16 = algorithmic_bytes(n_nodes, n_fluid, 19, 2)
+        mlups16 = n_nodes * args.steps / (ms16 / 1e3) / 1e6
+        out["f16_storage"] = {"mlups": round(mlups16, 1), "ms_per_step": round(ms16 / args.steps, 5),
+                              "vs_f32": round(mlups16 / mlups, 3),
+                              "gbs": round(b16 / (ms16 / args.steps / 1e3) / 1e9, 1)}
     if not args.no_warm_start:
         out["warm_start"] = warm_start()
     if not args.no_cpu_baseline:
