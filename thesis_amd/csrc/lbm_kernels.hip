@@ -127,9 +127,11 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
     const uint32_t r0 = bi.x;
     const int R = (int)bi.y;
     // Stage the words of rows r0..r0+R-1 and of their (dy, dz) neighbours at
-    // smeta[((k*9 + nb) << lg) + w], nb = (dz+1)*3 + (dy+1), rows padded to a
-    // power of two; rows outside a non-periodic face read as all-solid
-    // (outside sources are resolved by coordinates).
+    // smeta[((k << lg) + w)*9 + nb], nb = (dz+1)*3 + (dy+1), rows padded to a
+    // power of two: the nine neighbour words of one word are adjacent, so a
+    // node reaches all of them from three base addresses with constant
+    // offsets.  Rows outside a non-periodic face read as all-solid (outside
+    // sources are resolved by coordinates).
     const int tot = (R * 9) << lg;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         const int w = e & ((1 << lg) - 1), slot = e >> lg;
@@ -144,7 +146,7 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
             if (z < 0) { z += nz; out |= !kp.per[2]; } else if (z >= nz) { z -= nz; out |= !kp.per[2]; }
             if (!out) val = rowmeta[((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)W + (uint32_t)w];
         }
-        smeta[e] = val;
+        smeta[((k << lg) + w) * 9 + nb] = val;
     }
     __syncthreads();
     // Position of every node of this workgroup: the threads owning a staged own
@@ -155,7 +157,7 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
     for (int t = threadIdx.x; t < (R << lg); t += blockDim.x) {
         const int k = t >> lg, w = t & ((1 << lg) - 1);
         if (w >= W) continue;
-        const uint2 me = smeta[((k * 9 + 4) << lg) + w];
+        const uint2 me = smeta[((k << lg) + w) * 9 + 4];
         uint32_t m = me.x, rank = me.y;
         while (m) {
             const uint32_t slot = rank - own0;
@@ -183,29 +185,84 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
     }
     const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
     T h[L::Q];
-    h[0] = Io<L, S>::ld(src, tslot<L>(0, own), 0);
-    const int kb = (k * 9) << lg;
+    const uint32_t oslot = tslot<L>(0, own);
+    h[0] = Io<L, S>::ld(src, oslot, 0);
+    // per source column (x+1, x, x-1 for cx = -1, 0, 1): the LDS base of its
+    // word's nine neighbour words, its bit, and the mask of the bits below it
+    uint32_t cb[3], cbitm[3], clow[3];
+    {
+        const int sxs[3] = {xm, x, xp};
 #pragma unroll
-    for (int i = 1; i < L::Q; ++i) {
-        const int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i), o = L::opp(i);
-        const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
-        bool out = false;
-        if (OPEN)
-            out = (cx > 0 && xm_o) || (cx < 0 && xp_o) || (cy > 0 && ym_o) || (cy < 0 && yp_o) ||
-                  (cz > 0 && zm_o) || (cz < 0 && zp_o);
-        uint32_t j = kNone;
-        if (!out) {
-            const uint2 e = smeta[kb + (((1 - cz) * 3 + (1 - cy)) << lg) + (sx >> 5)];
-            const uint32_t bit = (uint32_t)sx & 31u;
-            j = ((e.x >> bit) & 1u) ? e.y + __popc(e.x & ((1u << bit) - 1u)) : kSolidSrc;
+        for (int c = 0; c < 3; ++c) {
+            cb[c] = (((uint32_t)k << lg) + ((uint32_t)sxs[c] >> 5)) * 9u;
+            cbitm[c] = 1u << ((uint32_t)sxs[c] & 31u);
+            clow[c] = cbitm[c] - 1u;
         }
-        const bool wall = ((nd >> o) & 1u) || j == kSolidSrc;
-        int pl = i;
-        if (wall || j == kNone) {
-            j = own;
-            pl = wall && kp.wall == 0 ? o : i;
+    }
+    // a source outside a non-periodic face (resolved by the face condition)
+    auto outside = [&](int cx, int cy, int cz) {
+        return OPEN && ((cx > 0 && xm_o) || (cx < 0 && xp_o) || (cy > 0 && ym_o) || (cy < 0 && yp_o) ||
+                        (cz > 0 && zm_o) || (cz < 0 && zp_o));
+    };
+    if constexpr (sizeof(typename Io<L, S>::raw) == 2) {
+        // 16-bit storage is bound by instruction issue, not by HBM: first every
+        // direction's LDS word and count, with no branch around the reads so
+        // they are all in flight together (the count is the source's compact
+        // index, meaningful when it is fluid), then the slot of every direction
+        // as a select.  Measured at 256^3: 7 % faster than the per-direction
+        // form below for FP16, 2.5 % slower for FP32.
+        uint32_t cnt[L::Q], word[L::Q];
+#pragma unroll
+        for (int i = 1; i < L::Q; ++i) {
+            const int c = 1 - L::cx(i);
+            const uint2 e = smeta[cb[c] + (uint32_t)((1 - L::cz(i)) * 3 + (1 - L::cy(i)))];
+            cnt[i] = e.y + __popc(e.x & clow[c]);
+            word[i] = e.x;
         }
-        h[i] = Io<L, S>::ld(src, tslot<L>(pl, j), i);
+        // keeps the counts ahead of the selects below: otherwise the compiler
+        // sinks each read and count into a branch of its own direction, and
+        // every direction then waits for its own LDS round trip
+#pragma unroll
+        for (int i = 1; i < L::Q; ++i) asm volatile("" : "+v"(cnt[i]));
+#pragma unroll
+        for (int i = 1; i < L::Q; ++i) {
+            const int cx = L::cx(i), o = L::opp(i);
+            const bool out = outside(cx, L::cy(i), L::cz(i));
+            const bool solid = !out && !(word[i] & cbitm[1 - cx]);
+            const bool wall = ((nd >> o) & 1u) || solid;
+            // a wall reflects the node's own opposite population (half-way) or
+            // keeps population i for the on-node closure (SIMPLE lattices have
+            // half-way walls only); a source outside a non-periodic face reads
+            // the node's own population i, replaced by the face condition later
+            const uint32_t wsl = SIMPLE || kp.wall == 0 ? 64u * (uint32_t)o : 64u * (uint32_t)i;
+            const uint32_t self = oslot + (wall ? wsl : 64u * (uint32_t)i);
+            // both slots computed in straight-line code, so the choice is a
+            // select rather than a branch per direction
+            const uint32_t there = tslot<L>(i, cnt[i]);
+            const bool mine = wall | out;
+            h[i] = Io<L, S>::ld(src, mine ? self : there, i);
+        }
+    } else {
+#pragma unroll
+        for (int i = 1; i < L::Q; ++i) {
+            const int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i), o = L::opp(i);
+            const int sx = cx == 0 ? x : (cx > 0 ? xm : xp);
+            const bool out = outside(cx, cy, cz);
+            uint32_t j = kNone;
+            if (!out) {
+                const uint2 e = smeta[(((uint32_t)k << lg) + ((uint32_t)sx >> 5)) * 9u +
+                                      (uint32_t)((1 - cz) * 3 + (1 - cy))];
+                const uint32_t bit = (uint32_t)sx & 31u;
+                j = ((e.x >> bit) & 1u) ? e.y + __popc(e.x & ((1u << bit) - 1u)) : kSolidSrc;
+            }
+            const bool wall = ((nd >> o) & 1u) || j == kSolidSrc;
+            int pl = i;
+            if (wall || j == kNone) {
+                j = own;
+                pl = wall && kp.wall == 0 ? o : i;
+            }
+            h[i] = Io<L, S>::ld(src, tslot<L>(pl, j), i);
+        }
     }
     // 2D only when open: the host keeps extrapolating outlets and every 3D
     // face or wet-node wall on k_step
