@@ -767,15 +767,33 @@ __device__ inline void moments(const KParams& kp, uint32_t cls, const T* h, T& d
     if ((kp.parity & 0x8u) && cls == PRES_E) u[1] = 0;
 }
 
+// c_i . (a, b, c) with the zero components of c_i left out: a product with
+// zero is not folded away by the compiler (0 * x is not 0 for every float x),
+// and the result is the same for finite operands.
+template <class L, class T>
+__device__ inline T cdot(int i, T a, T b, T c) {
+    T r = 0;
+    bool any = false;
+    const int k[3] = {L::cx(i), L::cy(i), L::cz(i)};
+    const T v[3] = {a, b, c};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (k[d] == 0) continue;
+        const T t = k[d] > 0 ? v[d] : -v[d];
+        r = any ? r + t : t;
+        any = true;
+    }
+    return r;
+}
+
 template <class L, class T>
 __device__ inline void guo_terms(const KParams& kp, T rho, const T* u, T* S) {
     const T Fx = rho * (T)kp.g[0], Fy = rho * (T)kp.g[1], Fz = rho * (T)kp.g[2];
     const T uF = u[0] * Fx + u[1] * Fy + u[2] * Fz;
 #pragma unroll
     for (int i = 0; i < L::Q; ++i) {
-        const T cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
-        const T cF = cx * Fx + cy * Fy + cz * Fz;
-        const T cu = cx * u[0] + cy * u[1] + cz * u[2];
+        const T cF = cdot<L, T>(i, Fx, Fy, Fz);
+        const T cu = cdot<L, T>(i, u[0], u[1], u[2]);
         S[i] = (T)L::w(i) * ((T)3 * (cF - uF) + (T)9 * cu * cF);
     }
 }
@@ -861,16 +879,15 @@ __device__ inline void collide(const KParams& kp, const T* h, T drho, T rho, con
         const int o = L::opp(i);
         if (i < o) {
             const T w = (T)L::w(i);
-            const T cu = (T)L::cx(i) * u[0] + (T)L::cy(i) * u[1] + (T)L::cz(i) * u[2];
+            const T cu = cdot<L, T>(i, u[0], u[1], u[2]);
             const T ep = w * (drho + rho * ((T)4.5 * cu * cu - (T)1.5 * uu));
             const T em = (T)3 * w * rho * cu;
-            T dp = op * ((T)0.5 * (h[i] + h[o]) - ep);
-            T dm = om * ((T)0.5 * (h[i] - h[o]) - em);
-            if (guo) {
-                const T cF = (T)L::cx(i) * Fx + (T)L::cy(i) * Fy + (T)L::cz(i) * Fz;
-                dp -= gp * w * ((T)9 * cu * cF - (T)3 * uF);
-                dm -= gm * (T)3 * w * cF;
-            }
+            // the source terms are applied unconditionally (zero without
+            // forcing, which leaves dp and dm unchanged): a test per link
+            // becomes a select per link
+            const T cF = cdot<L, T>(i, Fx, Fy, Fz);
+            const T dp = op * ((T)0.5 * (h[i] + h[o]) - ep) - gp * w * ((T)9 * cu * cF - (T)3 * uF);
+            const T dm = om * ((T)0.5 * (h[i] - h[o]) - em) - gm * (T)3 * w * cF;
             out[i] = h[i] - dp - dm;
             out[o] = h[o] - dp + dm;
         }
