@@ -139,7 +139,6 @@ template <class L> struct Io<L, 0> {
     __device__ static raw pk(T v, int) { return v; }
     __device__ static T ld(const raw* p, uint32_t k, int) { return ldo(p, k); }
     __device__ static void st(raw* p, uint32_t k, T v, int) { sto(p, k, v); }
-    __device__ static void st_sr(raw* p, uint32_t k, T v, int, uint32_t) { sto(p, k, v); }
 };
 template <class L> struct Io<L, 1> {
     using raw = float; using T = float;
@@ -147,7 +146,6 @@ template <class L> struct Io<L, 1> {
     __device__ static raw pk(T v, int) { return v; }
     __device__ static T ld(const raw* p, uint32_t k, int) { return ldo(p, k); }
     __device__ static void st(raw* p, uint32_t k, T v, int) { sto(p, k, v); }
-    __device__ static void st_sr(raw* p, uint32_t k, T v, int, uint32_t) { sto(p, k, v); }
 };
 template <class L> struct Io<L, 2> {
     using raw = uint16_t; using T = float;
@@ -155,11 +153,11 @@ template <class L> struct Io<L, 2> {
     __device__ static raw pk(T v, int i) { return __builtin_bit_cast(uint16_t, (_Float16)(v * f16_ainv<L>(i))); }
     __device__ static T ld(const raw* p, uint32_t k, int i) { return cv(ldo(p, k), i); }
     __device__ static void st(raw* p, uint32_t k, T v, int i) { sto(p, k, pk(v, i)); }
-    // unbiased (stochastic) rounding with the random bits rnd
-    __device__ static void st_sr(raw* p, uint32_t k, T v, int i, uint32_t rnd) {
-        typedef short s2 __attribute__((ext_vector_type(2)));
-        const s2 r = __builtin_amdgcn_cvt_sr_f16_f32(s2{0, 0}, v * f16_ainv<L>(i), rnd, false);
-        sto(p, k, (uint16_t)r[0]);
+    // unbiased (stochastic) rounding with the random bits rnd into the low
+    // half of acc (the high half is kept)
+    typedef short pair __attribute__((ext_vector_type(2)));
+    __device__ static pair sr(pair acc, T v, int i, uint32_t rnd) {
+        return __builtin_amdgcn_cvt_sr_f16_f32(acc, v * f16_ainv<L>(i), rnd, false);
     }
 };
 template <class L> struct Io<L, 3> {
@@ -168,10 +166,9 @@ template <class L> struct Io<L, 3> {
     __device__ static raw pk(T v, int i) { return f32_to_bf16(v * bf16_ainv<L>(i)); }
     __device__ static T ld(const raw* p, uint32_t k, int i) { return cv(ldo(p, k), i); }
     __device__ static void st(raw* p, uint32_t k, T v, int i) { sto(p, k, pk(v, i)); }
-    __device__ static void st_sr(raw* p, uint32_t k, T v, int i, uint32_t rnd) {
-        typedef short s2 __attribute__((ext_vector_type(2)));
-        const s2 r = __builtin_amdgcn_cvt_sr_bf16_f32(s2{0, 0}, v * bf16_ainv<L>(i), rnd, false);
-        sto(p, k, (uint16_t)r[0]);
+    typedef short pair __attribute__((ext_vector_type(2)));
+    __device__ static pair sr(pair acc, T v, int i, uint32_t rnd) {
+        return __builtin_amdgcn_cvt_sr_bf16_f32(acc, v * bf16_ainv<L>(i), rnd, false);
     }
 };
 

@@ -30,19 +30,32 @@ namespace lbm {
 template <class L, int S, bool TILE = false>
 __device__ inline void store_node(typename Io<L, S>::raw* __restrict__ dst, const KParams& kp, uint32_t own,
                                   const typename Io<L, S>::T* out) {
-    auto slot = [&](int i) { return TILE ? tslot<L>(i, own) : paddr<L>(kp, i, own); };
-    if (S >= 2 && kp.sr) {
-        uint32_t h = own * 0x9E3779B1u + kp.seed;
-        h ^= h >> 15;
-        h *= 0x2C1B3C6Du;
-        h ^= h >> 12;
+    using raw = typename Io<L, S>::raw;
+    // With the tile layout a node's populations are 64 elements apart: one
+    // 64-bit base, and each store's offset is a constant (an immediate of the
+    // store instruction rather than an add per population).
+    raw* const tb = TILE ? (raw*)((char*)dst + (size_t)tslot<L>(0, own) * sizeof(raw)) : dst;
+    auto base = [&](int i) { return TILE ? tb : dst; };
+    auto slot = [&](int i) { return TILE ? 64u * (uint32_t)i : paddr<L>(kp, i, own); };
+    if constexpr (S >= 2) {
+        if (kp.sr) {
+            uint32_t h = own * 0x9E3779B1u + kp.seed;
+            h ^= h >> 15;
+            h *= 0x2C1B3C6Du;
+            h ^= h >> 12;
+            // the conversions write the low half of one register in turn (no
+            // register to clear per population)
+            typename Io<L, S>::pair acc = {0, 0};
 #pragma unroll
-        for (int i = 0; i < L::Q; ++i)
-            Io<L, S>::st_sr(dst, slot(i), out[i], i, (h << (i & 31)) | (h >> ((32 - i) & 31)));
-    } else {
-#pragma unroll
-        for (int i = 0; i < L::Q; ++i) Io<L, S>::st(dst, slot(i), out[i], i);
+            for (int i = 0; i < L::Q; ++i) {
+                acc = Io<L, S>::sr(acc, out[i], i, (h << (i & 31)) | (h >> ((32 - i) & 31)));
+                sto(base(i), slot(i), (raw)acc[0]);
+            }
+            return;
+        }
     }
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) Io<L, S>::st(base(i), slot(i), out[i], i);
 }
 
 // ---------------------------------------------------------------------------
