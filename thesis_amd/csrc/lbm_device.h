@@ -159,6 +159,12 @@ template <class L> struct Io<L, 2> {
     __device__ static pair sr(pair acc, T v, int i, uint32_t rnd) {
         return __builtin_amdgcn_cvt_sr_f16_f32(acc, v * f16_ainv<L>(i), rnd, false);
     }
+    // the stored value x_i = h_i / a_i itself (a_i = f16_a, w_i / a_i = kw)
+    static constexpr float kw = 1024.0f;
+    __device__ static float a(int i) { return f16_a<L>(i); }
+    __device__ static T x(raw r) { return (float)__builtin_bit_cast(_Float16, r); }
+    __device__ static raw px(T v) { return __builtin_bit_cast(uint16_t, (_Float16)v); }
+    __device__ static pair srx(pair acc, T v, uint32_t rnd) { return __builtin_amdgcn_cvt_sr_f16_f32(acc, v, rnd, false); }
 };
 template <class L> struct Io<L, 3> {
     using raw = uint16_t; using T = float;
@@ -170,6 +176,11 @@ template <class L> struct Io<L, 3> {
     __device__ static pair sr(pair acc, T v, int i, uint32_t rnd) {
         return __builtin_amdgcn_cvt_sr_bf16_f32(acc, v * bf16_ainv<L>(i), rnd, false);
     }
+    static constexpr float kw = 1.0f;
+    __device__ static float a(int i) { return bf16_a<L>(i); }
+    __device__ static T x(raw r) { return bf16_to_f32(r); }
+    __device__ static raw px(T v) { return f32_to_bf16(v); }
+    __device__ static pair srx(pair acc, T v, uint32_t rnd) { return __builtin_amdgcn_cvt_sr_bf16_f32(acc, v, rnd, false); }
 };
 
 // ---------------------------------------------------------------------------
@@ -741,15 +752,18 @@ __device__ inline float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // Moments of h and the stored-velocity semantics of the force model
 // (S/GpuUpdateMacro.cu:6-41, :137-213).
-template <class L, class T>
-__device__ inline void moments(const KParams& kp, uint32_t cls, const T* h, T& drho, T& rho, T* u) {
+// Moments of the deviations h_i = sc(i) x_i (sc = 1: x holds h itself; the
+// 16-bit storages pass their stored values x_i = h_i / a_i with sc = a_i).
+template <class L, class T, class Sc>
+__device__ inline void moments(const KParams& kp, uint32_t cls, const T* x, T& drho, T& rho, T* u, Sc sc) {
     T d = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
     for (int i = 0; i < L::Q; ++i) {
-        d += h[i];
-        if (L::cx(i)) jx += L::cx(i) > 0 ? h[i] : -h[i];
-        if (L::cy(i)) jy += L::cy(i) > 0 ? h[i] : -h[i];
-        if (L::cz(i)) jz += L::cz(i) > 0 ? h[i] : -h[i];
+        const T h = sc(i) * x[i];
+        d += h;
+        if (L::cx(i)) jx += L::cx(i) > 0 ? h : -h;
+        if (L::cy(i)) jy += L::cy(i) > 0 ? h : -h;
+        if (L::cz(i)) jz += L::cz(i) > 0 ? h : -h;
     }
     drho = d;
     rho = (T)1 + d;
@@ -762,6 +776,10 @@ __device__ inline void moments(const KParams& kp, uint32_t cls, const T* h, T& d
         u[0] += (T)(0.5 * kp.g[0]); u[1] += (T)(0.5 * kp.g[1]); u[2] += (T)(0.5 * kp.g[2]);
     }
     if ((kp.parity & 0x8u) && cls == PRES_E) u[1] = 0;
+}
+template <class L, class T>
+__device__ inline void moments(const KParams& kp, uint32_t cls, const T* h, T& drho, T& rho, T* u) {
+    moments<L, T>(kp, cls, h, drho, rho, u, [](int) { return (T)1; });
 }
 
 // c_i . (a, b, c) with the zero components of c_i left out: a product with
@@ -829,6 +847,9 @@ __device__ inline void mrt_dmeq(const KParams& kp, T drho, T rho, const T* u, T*
 
 // Collision in deviation form.  BGK: S/GpuCollision.cu:528-550, :882-916;
 // TRT: :917-969 (general Lambda); MRT: :971-1037, :1039-1170.
+template <class L, class T, int C, class Wt>
+__device__ inline void collide_pairs(const KParams& kp, const T* h, T drho, T rho, const T* u, T* out, Wt wt);
+
 template <class L, class T, int C>
 __device__ inline void collide(const KParams& kp, const T* h, T drho, T rho, const T* u, T* out) {
     const bool guo = kp.force == 2;
@@ -852,11 +873,19 @@ __device__ inline void collide(const KParams& kp, const T* h, T drho, T rho, con
             out[L::Q - 1] = -(T)L::w(L::Q - 1);   // fColl_18 stays at its zero initial value
         return;
     }
-    // BGK and TRT in symmetric/antisymmetric pair form: for a link pair
-    // (i, opp i) the parts of f_eq and of the Guo source are
-    //   e+ = w (drho + rho (4.5 cu^2 - 1.5 u.u)),  e- = 3 w rho cu,
-    //   S+ = w (9 cu cF - 3 u.F),                  S- = 3 w cF,
-    // relaxed with omega+ and omega- (equal for BGK) and prefactors 1 - omega/2.
+    collide_pairs<L, T, C>(kp, h, drho, rho, u, out, [](int i) { return (T)L::w(i); });
+}
+
+// BGK and TRT in symmetric/antisymmetric pair form: for a link pair
+// (i, opp i) the parts of f_eq and of the Guo source are
+//   e+ = w (drho + rho (4.5 cu^2 - 1.5 u.u)),  e- = 3 w rho cu,
+//   S+ = w (9 cu cF - 3 u.F),                  S- = 3 w cF,
+// relaxed with omega+ and omega- (equal for BGK) and prefactors 1 - omega/2.
+// wt(i) is w_i for populations h; for stored values x_i = h_i / a_i with
+// a_i = a_opp(i) the same update divided by a_i is the one with wt = w_i / a_i.
+template <class L, class T, int C, class Wt>
+__device__ inline void collide_pairs(const KParams& kp, const T* h, T drho, T rho, const T* u, T* out, Wt wt) {
+    const bool guo = kp.force == 2;
     const T op = (T)kp.omega;
     const T om = C == 0 ? op : (T)kp.omega_m;
     const T gp = (T)1 - (T)0.5 * op, gm = (T)1 - (T)0.5 * om;
@@ -867,7 +896,7 @@ __device__ inline void collide(const KParams& kp, const T* h, T drho, T rho, con
         uF = u[0] * Fx + u[1] * Fy + u[2] * Fz;
     }
     {
-        const T w0 = (T)L::w(0);
+        const T w0 = wt(0);
         out[0] = h[0] - op * (h[0] - w0 * (drho - (T)1.5 * rho * uu));
         if (guo) out[0] -= gp * (T)3 * w0 * uF;
     }
@@ -875,7 +904,7 @@ __device__ inline void collide(const KParams& kp, const T* h, T drho, T rho, con
     for (int i = 1; i < L::Q; ++i) {
         const int o = L::opp(i);
         if (i < o) {
-            const T w = (T)L::w(i);
+            const T w = wt(i);
             const T cu = cdot<L, T>(i, u[0], u[1], u[2]);
             const T ep = w * (drho + rho * ((T)4.5 * cu * cu - (T)1.5 * uu));
             const T em = (T)3 * w * rho * cu;

@@ -27,7 +27,9 @@ namespace lbm {
 // compact node index and a per-launch seed, rotated per direction; results are
 // deterministic for a given seed sequence.
 // TILE: the buffer is known to use the tile layout (no runtime layout test).
-template <class L, int S, bool TILE = false>
+// X: out holds the stored values x_i = h_i / a_i of a 16-bit storage (no
+// scaling before the conversion).
+template <class L, int S, bool TILE = false, bool X = false>
 __device__ inline void store_node(typename Io<L, S>::raw* __restrict__ dst, const KParams& kp, uint32_t own,
                                   const typename Io<L, S>::T* out) {
     using raw = typename Io<L, S>::raw;
@@ -48,14 +50,19 @@ __device__ inline void store_node(typename Io<L, S>::raw* __restrict__ dst, cons
             typename Io<L, S>::pair acc = {0, 0};
 #pragma unroll
             for (int i = 0; i < L::Q; ++i) {
-                acc = Io<L, S>::sr(acc, out[i], i, (h << (i & 31)) | (h >> ((32 - i) & 31)));
+                const uint32_t rnd = (h << (i & 31)) | (h >> ((32 - i) & 31));
+                if constexpr (X) acc = Io<L, S>::srx(acc, out[i], rnd);
+                else acc = Io<L, S>::sr(acc, out[i], i, rnd);
                 sto(base(i), slot(i), (raw)acc[0]);
             }
             return;
         }
     }
 #pragma unroll
-    for (int i = 0; i < L::Q; ++i) Io<L, S>::st(base(i), slot(i), out[i], i);
+    for (int i = 0; i < L::Q; ++i) {
+        if constexpr (X) sto(base(i), slot(i), Io<L, S>::px(out[i]));
+        else Io<L, S>::st(base(i), slot(i), out[i], i);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -197,9 +204,13 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
         zm_o = !kp.per[2] && z == 0;  zp_o = !kp.per[2] && z == nz - 1;
     }
     const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+    using raw = typename Io<L, S>::raw;
+    constexpr bool kNarrow = sizeof(raw) == 2;
     T h[L::Q];
+    raw hr[L::Q];   // 16-bit storage: the stored values as loaded
     const uint32_t oslot = tslot<L>(0, own);
-    h[0] = Io<L, S>::ld(src, oslot, 0);
+    if constexpr (kNarrow) hr[0] = ldo(src, oslot);
+    else h[0] = Io<L, S>::ld(src, oslot, 0);
     // per source column (x+1, x, x-1 for cx = -1, 0, 1): the LDS base of its
     // word's nine neighbour words, its bit, and the mask of the bits below it
     uint32_t cb[3], cbitm[3], clow[3];
@@ -217,7 +228,7 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
         return OPEN && ((cx > 0 && xm_o) || (cx < 0 && xp_o) || (cy > 0 && ym_o) || (cy < 0 && yp_o) ||
                         (cz > 0 && zm_o) || (cz < 0 && zp_o));
     };
-    if constexpr (sizeof(typename Io<L, S>::raw) == 2) {
+    if constexpr (kNarrow) {
         // 16-bit storage is bound by instruction issue, not by HBM: first every
         // direction's LDS word and count, with no branch around the reads so
         // they are all in flight together (the count is the source's compact
@@ -253,7 +264,7 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
             // select rather than a branch per direction
             const uint32_t there = tslot<L>(i, cnt[i]);
             const bool mine = wall | out;
-            h[i] = Io<L, S>::ld(src, mine ? self : there, i);
+            hr[i] = ldo(src, mine ? self : there);
         }
     } else {
 #pragma unroll
@@ -276,6 +287,28 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
             }
             h[i] = Io<L, S>::ld(src, tslot<L>(pl, j), i);
         }
+    }
+    if constexpr (kNarrow && SIMPLE && !OPEN && C != 2) {
+        // Plain fluid, half-way walls, BGK/TRT: the update runs on the stored
+        // values x_i = h_i / a_i, so neither the loads nor the stores scale:
+        // the moments weigh x_i by a_i, and the pair update (a_i = a_opp(i))
+        // divided by a_i is the same update with w_i / a_i in place of w_i.
+        T xs[L::Q];
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i) xs[i] = Io<L, S>::x(hr[i]);
+        T drho, rho, u[3];
+        moments<L, T>(kp, cls, xs, drho, rho, u, [](int i) { return Io<L, S>::a(i); });
+        if (write_macros) {
+            fl.rho[idx] = (double)rho; fl.u[idx] = (double)u[0];
+            fl.v[idx] = (double)u[1]; fl.w[idx] = (double)u[2];
+        }
+        T out[L::Q];
+        collide_pairs<L, T, C>(kp, xs, drho, rho, u, out, [](int) { return Io<L, S>::kw; });
+        store_node<L, S, true, true>(dst, kp, own, out);
+        return;
+    } else if constexpr (kNarrow) {
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i) h[i] = Io<L, S>::cv(hr[i], i);
     }
     // 2D only when open: the host keeps extrapolating outlets and every 3D
     // face or wet-node wall on k_step
