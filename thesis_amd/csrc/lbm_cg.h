@@ -19,6 +19,25 @@ struct CgParams {
 
 constexpr int NRED = 12;     // reductions of k_cg_reduce
 
+// (kx, ky, kz) . (a, b, c) for components in {-1, 0, 1} (constants once the
+// direction loop is unrolled) with the zero components left out: the compiler
+// does not fold 0 * x, and the result is the same for finite operands.
+template <class T>
+__device__ inline T cdot(int kx, int ky, int kz, T a, T b, T c) {
+    T r = 0;
+    bool any = false;
+    const int k[3] = {kx, ky, kz};
+    const T v[3] = {a, b, c};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (k[d] == 0) continue;
+        const T t = k[d] > 0 ? v[d] : -v[d];
+        r = any ? r + t : t;
+        any = true;
+    }
+    return r;
+}
+
 }  // namespace cg
 
 // D2Q9 (lbm_cg.hip)

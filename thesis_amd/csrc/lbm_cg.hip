@@ -103,7 +103,10 @@ __device__ inline void cg_collide(const CgParams& cp, const T* h, const T* r, T 
     const T rb = rho - rr;
     T jx = 0, jy = 0;
 #pragma unroll
-    for (int i = 1; i < 9; ++i) { jx += (T)CX[i] * h[i]; jy += (T)CY[i] * h[i]; }
+    for (int i = 1; i < 9; ++i) {   // (zero components left out: 0 * x is not folded)
+        if (CX[i]) jx += CX[i] > 0 ? h[i] : -h[i];
+        if (CY[i]) jy += CY[i] > 0 ? h[i] : -h[i];
+    }
     const T ir = crecip(rho);
     const T u = jx * ir, v = jy * ir;
     const T mean_nu = (T)1 / (rr / (rho * (T)cp.r_nu) + rb / (rho * (T)cp.b_nu));
@@ -117,16 +120,16 @@ __device__ inline void cg_collide(const CgParams& cp, const T* h, const T* r, T 
     const T ugr = u * dx + v * dy;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-        const T cg = (T)CX[k] * gx + (T)CY[k] * gy;
+        const T cg = cdot<T>(CX[k], CY[k], 0, gx, gy, (T)0);
         const T cs = (on && k) ? cg * ign * (T)(1.0 / CNORM(k)) : (T)0;
         const T pert = on ? (T)cp.A * gn * ((T)W9(k) * cg * cg * ign * ign - (T)BPERT(k)) : (T)0;
         const T rest = (T)PHI(k) + (T)TETA(k) * alpha;
-        const T cu = (T)CX[k] * u + (T)CY[k] * v;
+        const T cu = cdot<T>(CX[k], CY[k], 0, u, v, (T)0);
         // f_eq - w = (rho - 1) rest + (rest - w) + rho w (3cu + 4.5cu^2 - 1.5uu)
         T feqd = (rho - (T)1) * rest + (rest - (T)W9(k)) +
                  rho * (T)W9(k) * ((T)3 * cu + (T)4.5 * cu * cu - (T)1.5 * uu);
         if (E) {   // nu (chi u.grad rho + psi G:cc), G:cc = 2 (c.u)(c.grad rho)
-            const T cgr = (T)CX[k] * dx + (T)CY[k] * dy;
+            const T cgr = cdot<T>(CX[k], CY[k], 0, dx, dy, (T)0);
             feqd += mean_nu * ((T)CHI(k) * ugr + (T)PSI(k) * (T)2 * cu * cgr);
         }
         const T fsd = om * feqd + ((T)1 - om) * h[k] + pert;       // f* - w
@@ -150,8 +153,8 @@ __device__ inline void gradient(const float* __restrict__ psi, int lx, int ly, f
             if (c2 == 0) continue;
             const float wk = H == 1 ? (c2 == 1 ? 4.0f / 12.0f : 1.0f / 12.0f) : (float)HOW(c2);
             const float p = psi[(ly + H + dy) * RW + (lx + H + dx)];
-            ax += wk * (float)dx * p;
-            ay += wk * (float)dy * p;
+            if (dx) ax += wk * (float)dx * p;
+            if (dy) ay += wk * (float)dy * p;
         }
     gx = ax; gy = ay;
 }
@@ -167,8 +170,8 @@ __device__ inline void gradient(const double* __restrict__ psi, int lx, int ly, 
             if (c2 == 0) continue;
             const double wk = H == 1 ? (c2 == 1 ? 4.0 / 12.0 : 1.0 / 12.0) : HOW(c2);
             const double p = psi[(ly + H + dy) * RW + (lx + H + dx)];
-            ax += wk * (double)dx * p;
-            ay += wk * (double)dy * p;
+            if (dx) ax += wk * (double)dx * p;
+            if (dy) ay += wk * (double)dy * p;
         }
     gx = ax; gy = ay;
 }

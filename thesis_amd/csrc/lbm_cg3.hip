@@ -111,7 +111,11 @@ __device__ inline void collide(const CgParams& cp, const T* h, const T* r, T gx,
     const T rb = rho - rr;
     T jx = 0, jy = 0, jz = 0;
 #pragma unroll
-    for (int i = 1; i < 19; ++i) { jx += (T)CX[i] * h[i]; jy += (T)CY[i] * h[i]; jz += (T)CZ[i] * h[i]; }
+    for (int i = 1; i < 19; ++i) {   // (zero components left out: 0 * x is not folded)
+        if (CX[i]) jx += CX[i] > 0 ? h[i] : -h[i];
+        if (CY[i]) jy += CY[i] > 0 ? h[i] : -h[i];
+        if (CZ[i]) jz += CZ[i] > 0 ? h[i] : -h[i];
+    }
     const T ir = (T)1 / rho;
     const T u = jx * ir, v = jy * ir, w = jz * ir;
     const T mean_nu = (T)1 / (rr / (rho * (T)cp.r_nu) + rb / (rho * (T)cp.b_nu));
@@ -125,16 +129,16 @@ __device__ inline void collide(const CgParams& cp, const T* h, const T* r, T gx,
     const T ugr = E ? u * d[0] + v * d[1] + w * d[2] : (T)0;
 #pragma unroll
     for (int k = 0; k < 19; ++k) {
-        const T cg = (T)CX[k] * gx + (T)CY[k] * gy + (T)CZ[k] * gz;
+        const T cg = cg::cdot<T>(CX[k], CY[k], CZ[k], gx, gy, gz);
         const T cs = (on && k) ? cg * ign * (T)(1.0 / CNORM(k)) : (T)0;
         const T pert = on ? (T)cp.A * gn * ((T)W19(k) * cg * cg * ign * ign - (T)BPERT(k)) : (T)0;
         const T rest = (T)PHI(k) + (T)TETA(k) * alpha;
-        const T cu = (T)CX[k] * u + (T)CY[k] * v + (T)CZ[k] * w;
+        const T cu = cg::cdot<T>(CX[k], CY[k], CZ[k], u, v, w);
         // f_eq - w = (rho - 1) rest + (rest - w) + rho w (3cu + 4.5cu^2 - 1.5uu)
         T feqd = (rho - (T)1) * rest + (rest - (T)W19(k)) +
                  rho * (T)W19(k) * ((T)3 * cu + (T)4.5 * cu * cu - (T)1.5 * uu);
         if (E) {   // nu (chi u.grad rho + psi G:cc), G:cc = 2 (c.u)(c.grad rho)
-            const T cgr = (T)CX[k] * d[0] + (T)CY[k] * d[1] + (T)CZ[k] * d[2];
+            const T cgr = cg::cdot<T>(CX[k], CY[k], CZ[k], d[0], d[1], d[2]);
             feqd += mean_nu * ((T)CHI(k) * ugr + (T)PSI(k) * (T)2 * cu * cgr);
         }
         const T fsd = om * feqd + ((T)1 - om) * h[k] + pert;       // f* - w
@@ -146,7 +150,10 @@ __device__ inline void collide(const CgParams& cp, const T* h, const T* r, T gx,
 
 // Gradient of the phase field at (x, y, z): psi(i) returns the phase field of
 // linear node index i.
-template <int H, class T, class Psi>
+// SKIP leaves out the products with a zero offset component (the compiler does
+// not fold 0 * x; same result for finite values).  Measured at 256^3 with the
+// 104-point stencil: 7 % faster with 16-bit storage, 5 % slower with FP32.
+template <int H, class T, bool SKIP = false, class Psi>
 __device__ inline void gradient(const CgParams& cp, int x, int y, int z, Psi psi, T& gx, T& gy, T& gz) {
     int xs[2 * H + 1], ys[2 * H + 1], zs[2 * H + 1];
 #pragma unroll
@@ -168,9 +175,9 @@ __device__ inline void gradient(const CgParams& cp, int x, int y, int z, Psi psi
                 const uint32_t j = ((uint32_t)zs[dz + H] * (uint32_t)cp.ny + (uint32_t)ys[dy + H]) * (uint32_t)cp.nx +
                                    (uint32_t)xs[dx + H];
                 const T p = (T)wk * psi(j);
-                ax += (T)dx * p;
-                ay += (T)dy * p;
-                az += (T)dz * p;
+                if (!SKIP || dx) ax += (T)dx * p;
+                if (!SKIP || dy) ay += (T)dy * p;
+                if (!SKIP || dz) az += (T)dz * p;
             }
     gx = ax; gy = ay; gz = az;
 }
@@ -209,10 +216,10 @@ __global__ __launch_bounds__(BX * BY) void k_cg3_step(const typename CIo<S>::raw
     T h[19], r[19];
     pull<S>(src, cp, x, y, z, h, r);
     T gx, gy, gz, d[3] = {0, 0, 0};
-    gradient<H, T>(cp, x, y, z, [psi](uint32_t j) { return psi[j]; }, gx, gy, gz);
+    gradient<H, T, (S >= 2)>(cp, x, y, z, [psi](uint32_t j) { return psi[j]; }, gx, gy, gz);
     if (E) {
         const T* rh = psi + (size_t)cp.nx * cp.ny * cp.nz;
-        gradient<H, T>(cp, x, y, z, [rh](uint32_t j) { return rh[j]; }, d[0], d[1], d[2]);
+        gradient<H, T, (S >= 2)>(cp, x, y, z, [rh](uint32_t j) { return rh[j]; }, d[0], d[1], d[2]);
     }
     T ho[19], ro[19], m[6];
     collide<T, E>(cp, h, r, gx, gy, gz, d, ho, ro, m);
