@@ -366,15 +366,16 @@ def dataset(count, seed, log=print):
     return X, Y, Mk
 
 
-def train(X, Y, Mk, epochs=30, batch=16, lr=2e-3, seed=0, log=print, net=None):
-    """Adam with a cosine learning-rate decay over the epochs."""
+def train(X, Y, Mk, epochs=30, batch=16, lr=2e-3, seed=0, log=print, net=None, rho_weight=1.0):
+    """Adam with a cosine learning-rate decay over the epochs; rho_weight
+    weighs the density channel of the loss against the two velocity channels."""
     dev = torch.device("cuda")
     net = net or Net()
     n, H, W, _ = X.shape
     Xd = torch.from_numpy(X).to(dev)
     Yd = torch.from_numpy(Y).to(dev)
     Md = torch.from_numpy(Mk).to(dev)
-    wch = torch.tensor([1.0, 1.0, 1.0], dtype=torch.float32, device=dev)
+    wch = torch.tensor([1.0, 1.0, float(rho_weight)], dtype=torch.float32, device=dev)
     g = torch.Generator(device="cpu")
     g.manual_seed(seed)
     steps_per_epoch = n // batch
@@ -510,6 +511,7 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=80)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--rho-weight", type=float, default=1.0, help="loss weight of the density channel")
     ap.add_argument("--out", default="gpurun_out/surrogate")
     ap.add_argument("--weights", default=None, help="skip training, evaluate these weights")
     a = ap.parse_args(argv)
@@ -526,11 +528,12 @@ def main(argv=None):
         X, Y, Mk = dataset(a.train, a.train_seed, log=log)
         report["data_s"] = time.time() - t0
         t1 = time.time()
-        net, hist = train(X, Y, Mk, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
+        net, hist = train(X, Y, Mk, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log, rho_weight=a.rho_weight)
         report["train_s"] = time.time() - t1
         report["loss_history"] = hist
         del X, Y, Mk
-        net.save(os.path.join(a.out, "weights.npz"), meta=dict(train=a.train, seed=a.train_seed, epochs=a.epochs))
+        net.save(os.path.join(a.out, "weights.npz"),
+                 meta=dict(train=a.train, seed=a.train_seed, epochs=a.epochs, rho_weight=a.rho_weight))
     test = LT.surrogate_channels(a.test, a.test_seed)
     report["eval"] = evaluate(net, test, log=log)
     with open(os.path.join(a.out, "report.json"), "w") as f:
