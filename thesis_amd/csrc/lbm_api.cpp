@@ -409,6 +409,7 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
         for (int i = 0; i < 19; ++i) k.s[i] = s19[i];
     }
     for (int a = 0; a < 3; ++a) k.g[a] = p->force[a];
+    set_float_params(k);
     k.force = p->force_model;
     k.wall = p->wall_model;
     k.inlet = p->inlet_model;

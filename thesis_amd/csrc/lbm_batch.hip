@@ -37,6 +37,7 @@ __device__ inline KParams inst_params(const BInst& I) {
     kp.per[2] = 1;
     kp.omega = I.omega; kp.omega_m = I.omega_m; kp.rho_out = I.rho_out;
     kp.tile = 1;
+    set_float_params(kp);
     return kp;
 }
 

@@ -752,6 +752,13 @@ __device__ inline float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // Moments of h and the stored-velocity semantics of the force model
 // (S/GpuUpdateMacro.cu:6-41, :137-213).
+// A kernel parameter in the compute type: the FP32 copy for float kernels.
+template <class T>
+__device__ inline T kv(double d, float f) {
+    if constexpr (sizeof(T) == sizeof(float)) return f;
+    else return (T)d;
+}
+
 // Moments of the deviations h_i = sc(i) x_i (sc = 1: x holds h itself; the
 // 16-bit storages pass their stored values x_i = h_i / a_i with sc = a_i).
 template <class L, class T, class Sc>
@@ -773,7 +780,8 @@ __device__ inline void moments(const KParams& kp, uint32_t cls, const T* x, T& d
         const T io = (T)(1.0 / kp.omega);
         u[0] += (T)kp.g[0] * io; u[1] += (T)kp.g[1] * io; u[2] += (T)kp.g[2] * io;
     } else if (kp.force == 2) {
-        u[0] += (T)(0.5 * kp.g[0]); u[1] += (T)(0.5 * kp.g[1]); u[2] += (T)(0.5 * kp.g[2]);
+        u[0] += kv<T>(0.5 * kp.g[0], kp.hgf[0]); u[1] += kv<T>(0.5 * kp.g[1], kp.hgf[1]);
+        u[2] += kv<T>(0.5 * kp.g[2], kp.hgf[2]);
     }
     if ((kp.parity & 0x8u) && cls == PRES_E) u[1] = 0;
 }
@@ -803,7 +811,7 @@ __device__ inline T cdot(int i, T a, T b, T c) {
 
 template <class L, class T>
 __device__ inline void guo_terms(const KParams& kp, T rho, const T* u, T* S) {
-    const T Fx = rho * (T)kp.g[0], Fy = rho * (T)kp.g[1], Fz = rho * (T)kp.g[2];
+    const T Fx = rho * kv<T>(kp.g[0], kp.gf[0]), Fy = rho * kv<T>(kp.g[1], kp.gf[1]), Fz = rho * kv<T>(kp.g[2], kp.gf[2]);
     const T uF = u[0] * Fx + u[1] * Fy + u[2] * Fz;
 #pragma unroll
     for (int i = 0; i < L::Q; ++i) {
@@ -886,13 +894,13 @@ __device__ inline void collide(const KParams& kp, const T* h, T drho, T rho, con
 template <class L, class T, int C, class Wt>
 __device__ inline void collide_pairs(const KParams& kp, const T* h, T drho, T rho, const T* u, T* out, Wt wt) {
     const bool guo = kp.force == 2;
-    const T op = (T)kp.omega;
-    const T om = C == 0 ? op : (T)kp.omega_m;
+    const T op = kv<T>(kp.omega, kp.omegaf);
+    const T om = C == 0 ? op : kv<T>(kp.omega_m, kp.omega_mf);
     const T gp = (T)1 - (T)0.5 * op, gm = (T)1 - (T)0.5 * om;
     const T uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
     T Fx = 0, Fy = 0, Fz = 0, uF = 0;
     if (guo) {
-        Fx = rho * (T)kp.g[0]; Fy = rho * (T)kp.g[1]; Fz = rho * (T)kp.g[2];
+        Fx = rho * kv<T>(kp.g[0], kp.gf[0]); Fy = rho * kv<T>(kp.g[1], kp.gf[1]); Fz = rho * kv<T>(kp.g[2], kp.gf[2]);
         uF = u[0] * Fx + u[1] * Fy + u[2] * Fz;
     }
     {

@@ -54,7 +54,19 @@ struct KParams {
     const uint2* blk;        // per rblock-node workgroup: (first row, rows spanned)
     const uint32_t* qslot;   // curved walls: per compact node, its row of qv (0xFFFFFFFF: none)
     const double* qv;        // curved walls: 19 wall fractions per row
+    // FP32 copies of omega, omega_m, g and g / 2 (set_float_params): float
+    // kernels read these rather than converting the doubles in every wave
+    float omegaf, omega_mf, gf[3], hgf[3];
 };
+
+__host__ __device__ inline void set_float_params(KParams& k) {
+    k.omegaf = (float)k.omega;
+    k.omega_mf = (float)k.omega_m;
+    for (int a = 0; a < 3; ++a) {
+        k.gf[a] = (float)k.g[a];
+        k.hgf[a] = (float)(0.5 * k.g[a]);
+    }
+}
 
 struct Fields {              // FP64 side arrays (N each)
     double *rho, *u, *v, *w; // macroscopic output / input
