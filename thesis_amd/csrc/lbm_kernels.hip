@@ -169,22 +169,20 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
         smeta[((k << lg) + w) * 9 + nb] = val;
     }
     __syncthreads();
-    // Position of every node of this workgroup: the threads owning a staged own
-    // row word walk its fluid bits and write (row, x) at the node's slot, so no
+    // Position of every node of this workgroup: one thread per bit of the
+    // staged own-row words; a fluid bit writes (row, x) at its node's slot
+    // (its rank in the word plus the word's first compact index), so no
     // thread searches for its own node.
     const uint32_t own0 = bid * blockDim.x;
     uint32_t* pos = (uint32_t*)(smeta + ((kp.rmax * 9) << lg));
-    for (int t = threadIdx.x; t < (R << lg); t += blockDim.x) {
-        const int k = t >> lg, w = t & ((1 << lg) - 1);
+    for (int t = threadIdx.x; t < ((R << lg) << 5); t += blockDim.x) {
+        const int b = t & 31, wl = t >> 5;
+        const int k = wl >> lg, w = wl & ((1 << lg) - 1);
         if (w >= W) continue;
-        const uint2 me = smeta[((k << lg) + w) * 9 + 4];
-        uint32_t m = me.x, rank = me.y;
-        while (m) {
-            const uint32_t slot = rank - own0;
-            if (slot < blockDim.x) pos[slot] = ((uint32_t)k << 16) | (uint32_t)(32 * w + __builtin_ctz(m));
-            m &= m - 1u;
-            ++rank;
-        }
+        const uint2 me = smeta[wl * 9 + 4];
+        if (!((me.x >> b) & 1u)) continue;
+        const uint32_t slot = me.y + __popc(me.x & ((1u << b) - 1u)) - own0;
+        if (slot < blockDim.x) pos[slot] = ((uint32_t)k << 16) | (uint32_t)(32 * w + b);
     }
     __syncthreads();
     const uint32_t own = own0 + threadIdx.x;
