@@ -31,6 +31,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <new>
 #include <vector>
 
 namespace {
@@ -90,7 +91,8 @@ bool is_binary(const std::string& t) { return t.size() >= 8 && std::memcmp(t.dat
 
 template <class T, class U>
 bool take(const std::string& t, size_t& off, std::vector<U>& out, size_t cnt) {
-    if (off + cnt * sizeof(T) > t.size()) return false;
+    // division form: a huge count from a malformed header cannot wrap around
+    if (off > t.size() || cnt > (t.size() - off) / sizeof(T)) return false;
     out.resize(cnt);
     for (size_t a = 0; a < cnt; ++a) {
         T v;
@@ -118,11 +120,31 @@ lbm_status load_binary(const std::string& t, Lat& L) {
               take<int32_t>(t, off, L.lj, nl) && take<int32_t>(t, off, L.lk, nl) && take<int8_t>(t, off, L.dir, nl) &&
               take<int8_t>(t, off, L.type, nl) && take<double>(t, off, L.bx, nl) && take<double>(t, off, L.by, nl) &&
               take<double>(t, off, L.bz, nl) && take<int32_t>(t, off, L.id, nl);
-    return ok && off == t.size() ? LBM_OK : LBM_EIO;
+    if (!ok || off != t.size()) return LBM_EIO;
+    // a 2D lattice (5 node columns) has no z index
+    if (L.cols == 5) {
+        for (int64_t v : L.k) if (v != 0) return LBM_EIO;
+        for (int64_t v : L.lk) if (v != 0) return LBM_EIO;
+    }
+    return LBM_OK;
 }
 
+lbm_status load_impl(const char* node_path, const char* bc_path, Lat& L);
+
 // Node file (text or binary) and, for a text node file, the BC text file.
+// Allocation failures come back as a status, never as an exception through
+// the C entry points.
 lbm_status load(const char* node_path, const char* bc_path, Lat& L) {
+    try {
+        return load_impl(node_path, bc_path, L);
+    } catch (const std::bad_alloc&) {
+        return LBM_ENOMEM;
+    } catch (...) {
+        return LBM_EIO;
+    }
+}
+
+lbm_status load_impl(const char* node_path, const char* bc_path, Lat& L) {
     std::string txt;
     if (!slurp(node_path, txt)) return LBM_EIO;
     if (is_binary(txt)) return load_binary(txt, L);
