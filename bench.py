@@ -4,22 +4,30 @@ periodic porous medium (overlapping spheres, radii U[6,12], 30 % solid,
 seed 2024), tau = 0.9, g = 1e-6 along x, FP32 storage and compute.
 
 One step = one fused stream-collide-boundary kernel over the whole lattice,
-populations resident in HBM.  Prints one JSON line:
+populations resident in HBM.  Every timed configuration first runs an untimed
+settle phase (--settle device seconds, at least --warmup steps), then the
+timed window of --steps launches, which write no FP64 macro fields.  Prints
+one JSON line:
   value      MLUPS = all nodes x steps / time / 1e6 (the reference's convention,
              T/Resources/CGPerformance.xlsx sheet3 B6-B7)
   roofline   algorithmic bytes per launch / average launch time (HIP events on
              the solver's stream) against 8 TB/s
-  cpu_baseline  the FP64 CPU oracle on the same lattice, a few steps
+  median_launch_ms  median of a second window timed per launch
+  f16_storage, bf16_storage  the same step with 16-bit deviation storage (FP32
+             compute) and its ratio to the FP32 line
+  mrt_f32, mrt_f16_storage   the workload's MRT collision
+  cpu_baseline  the FP64 CPU oracle on the same lattice, a few steps on the
+             host threads of one GPU and on one thread
   warm_start    wall time to a 1e-6 residual from rest vs from the learned start
                 on held-out 512 x 128 channels (inference included)
-  f16_storage   the same step with FP16 deviation storage (FP32 compute), its
-                MLUPS and ratio to the FP32 line
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -44,27 +52,47 @@ def traffic_from_profiles(name):
         return None
 
 
-def cpu_baseline(cls, bb, n, seconds_target=15.0):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cls, bb, n, seconds_target=12.0, seconds_one=8.0):
     """FP64 oracle (a C restatement of the reference step, OpenMP over the
-    host cores) on the same lattice; steps until ~seconds_target elapse."""
+    host cores) on the same lattice: steps until ~seconds_target elapse on the
+    host threads the GPU box gives one GPU (at most 16), then on one thread."""
     from oracle import oracle as O
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = min(threads, 16)
-    s = O.Sim(19, (n, n, n), cls, bb, collision=O.TRT, omega=1 / 0.9, trt_magic=0.25, force_model=O.FORCE_GUO,
-              g=(1e-6, 0, 0), periodic=(1, 1, 1), nthreads=threads)
-    s.init(0)
-    s.step(1)   # warm-up (first touch)
-    steps, t0 = 0, time.perf_counter()
-    while True:
-        s.step(1)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el > seconds_target or steps >= 50:
-            break
-    mlups = n ** 3 * steps / el / 1e6
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = min(avail, 16)
+
+    def rate(nthreads, target, max_steps):
+        s = O.Sim(19, (n, n, n), cls, bb, collision=O.TRT, omega=1 / 0.9, trt_magic=0.25,
+                  force_model=O.FORCE_GUO, g=(1e-6, 0, 0), periodic=(1, 1, 1), nthreads=nthreads)
+        s.init(0)
+        s.step(1)   # warm-up (first touch)
+        steps, t0 = 0, time.perf_counter()
+        while True:
+            s.step(1)
+            steps += 1
+            el = time.perf_counter() - t0
+            if el > target or steps >= max_steps:
+                break
+        return n ** 3 * steps / el / 1e6, steps, el
+
+    mlups, steps, el = rate(threads, seconds_target, 50)
+    one, steps1, el1 = rate(1, seconds_one, 10)
     return {"value": round(mlups, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle, D3Q19 TRT on the same {n}^3 lattice, {steps} steps after 1 warm-up step "
-                      f"({el:.1f} s, OpenMP {threads} threads)"}
+            "one_core_mlups": round(one, 3), "nproc": os.cpu_count(), "affinity_cpus": avail,
+            "cpu_model": cpu_model(),
+            "sample": f"FP64 oracle, D3Q19 TRT on the same {n}^3 lattice: {steps} steps after 1 warm-up step "
+                      f"({el:.1f} s) on {threads} OpenMP threads (the host share of one GPU, of {avail} CPUs "
+                      f"visible), and {steps1} steps ({el1:.1f} s) on 1 thread"}
 
 
 def warm_start(n_channels=8, seed=9001):
@@ -114,14 +142,17 @@ def batch64():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--storage", default="f32", choices=["f64", "f32", "f16", "bf16"])
     ap.add_argument("--collision", default="trt", choices=["bgk", "trt", "mrt"])
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--variants", action="store_true", help="also time f16/bf16 storage, MRT, BGK and the batch")
+    ap.add_argument("--variants", action="store_true", help="also time BGK and the 64-channel batch")
+    ap.add_argument("--no-variants", action="store_true", help="skip the BF16 and MRT lines")
+    ap.add_argument("--settle", type=float, default=0.5,
+                    help="untimed device seconds before each timed window (steady clocks)")
     ap.add_argument("--no-warm-start", action="store_true")
     ap.add_argument("--no-f16", action="store_true", help="skip the FP16-storage line of the f32 run")
     ap.add_argument("--batch-only", action="store_true", help="only the 64-channel batch line (for profiling)")
@@ -153,29 +184,60 @@ def main():
         s.init(0)
         return s
 
-    solver = make(args.storage, coll)
-    solver.step(args.warmup)
-    t0 = time.perf_counter()
-    ms = solver.time_steps(args.steps)         # HIP events on the solver's stream
-    wall = time.perf_counter() - t0
-    avg_ms = ms / args.steps
-    mlups = n_nodes * args.steps / (ms / 1e3) / 1e6
+    def settle(s):
+        """Untimed steps until args.settle seconds of device time have run, so
+        the timed window starts at steady clocks; at least args.warmup steps."""
+        steps, dev = args.warmup, s.time_steps(args.warmup) / 1e3 if args.warmup else 0.0
+        while dev < args.settle:
+            dev += s.time_steps(100) / 1e3
+            steps += 100
+        return {"steps": steps, "device_s": round(dev, 3)}
+
+    def line(storage, collision):
+        """One timed configuration: settle, the timed window (mean per step),
+        then a window of per-launch events (median per launch)."""
+        s = make(storage, collision)
+        st = settle(s)
+        t0 = time.perf_counter()
+        ms = s.time_steps(args.steps)          # HIP events on the solver's stream
+        wall = time.perf_counter() - t0
+        each = s.time_launches(min(max(args.steps, 20), 200))
+        red = s.reduce()
+        s.close()
+        sb = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2}[storage]
+        avg = ms / args.steps
+        return {"ms": ms, "avg_ms": avg, "wall": wall, "settle": st,
+                "median_launch_ms": float(np.median(each)), "mean_launch_ms": float(np.mean(each)),
+                "mlups": n_nodes / (avg / 1e3) / 1e6,
+                "gbs": algorithmic_bytes(n_nodes, n_fluid, 19, sb) / (avg / 1e3) / 1e9, "mass": red["mass"]}
+
+    main_line = line(args.storage, coll)
+    ms, avg_ms, wall = main_line["ms"], main_line["avg_ms"], main_line["wall"]
+    mlups = main_line["mlups"]
     bytes_launch = algorithmic_bytes(n_nodes, n_fluid, 19, sbytes)
     achieved = bytes_launch / (avg_ms / 1e3) / 1e9
-    red = solver.reduce()
-    solver.close()
+
+    def brief(d, ref=None):
+        o = {"mlups": round(d["mlups"], 1), "ms_per_step": round(d["avg_ms"], 5),
+             "median_launch_ms": round(d["median_launch_ms"], 5), "gbs": round(d["gbs"], 1),
+             "frac": round(d["gbs"] / PEAK_GBS, 4)}
+        if ref is not None:
+            o["vs_f32"] = round(d["mlups"] / ref, 3)
+        return o
 
     variants = {}
+    if args.storage == "f32" and args.collision == "trt" and not args.no_f16:
+        # the 16-bit targets are stated against the FP32 line; MRT is the
+        # second collision of the workload
+        f16 = line("f16", L.TRT)
+        variants["f16_storage"] = brief(f16, mlups)
+        if not args.no_variants:
+            variants["bf16_storage"] = brief(line("bf16", L.TRT), mlups)
+            mrt32 = line("f32", L.MRT)
+            variants["mrt_f32"] = brief(mrt32)
+            variants["mrt_f16_storage"] = brief(line("f16", L.MRT), mrt32["mlups"])
     if args.variants:
-        for st, co in (("f16", L.TRT), ("bf16", L.TRT), ("f32", L.MRT), ("f32", L.BGK)):
-            s = make(st, co)
-            s.step(args.warmup)
-            vms = s.time_steps(args.steps)
-            vb = algorithmic_bytes(n_nodes, n_fluid, 19, {"f32": 4, "f16": 2, "bf16": 2}[st])
-            key = f"{st}_{['bgk', 'trt', 'mrt'][co]}"
-            variants[key] = {"mlups": round(n_nodes * args.steps / (vms / 1e3) / 1e6, 1),
-                             "gbs": round(vb / (vms / args.steps / 1e3) / 1e9, 1)}
-            s.close()
+        variants["bgk_f32"] = brief(line("f32", L.BGK))
         variants["batch64"] = batch64()
 
     out = {
@@ -200,21 +262,16 @@ def main():
                      "frac": round(achieved / PEAK_GBS, 4),
                      "traffic": traffic_from_profiles(f"traffic_d3q19_{args.collision}_{args.storage}_{n}.json")},
         "mflups": round(n_fluid * args.steps / (ms / 1e3) / 1e6, 1),
-        "mass": red["mass"],
+        "mass": main_line["mass"],
+        "settle": main_line["settle"],
+        "median_launch_ms": round(main_line["median_launch_ms"], 5),
+        "mean_launch_ms": round(main_line["mean_launch_ms"], 5),
+        "timing_note": "value/ms_per_step: HIP events around the timed steps on the solver's stream after "
+                       "an untimed settle phase; the timed launches write no FP64 macro fields (an untimed "
+                       "moments pass follows); median/mean_launch_ms: a second window with one event pair "
+                       "per launch",
     }
-    if variants:
-        out["variants"] = variants
-    if args.storage == "f32" and not args.no_f16:
-        # the same step with FP16 deviation storage (the 16-bit target is stated against FP32)
-        s16 = make("f16", coll)
-        s16.step(args.warmup)
-        ms16 = s16.time_steps(args.steps)
-        s16.close()
-        b16 = algorithmic_bytes(n_nodes, n_fluid, 19, 2)
-        mlups16 = n_nodes * args.steps / (ms16 / 1e3) / 1e6
-        out["f16_storage"] = {"mlups": round(mlups16, 1), "ms_per_step": round(ms16 / args.steps, 5),
-                              "vs_f32": round(mlups16 / mlups, 3),
-                              "gbs": round(b16 / (ms16 / args.steps / 1e3) / 1e9, 1)}
+    out.update(variants)
     if not args.no_warm_start:
         out["warm_start"] = warm_start()
     if not args.no_cpu_baseline:

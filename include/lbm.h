@@ -189,9 +189,15 @@ lbm_status lbm_get_f(lbm_ctx* ctx, double* f);
 /* Replace the state by f (q*N doubles); moments are recomputed from f. */
 lbm_status lbm_set_f(lbm_ctx* ctx, const double* f);
 
-/* Timing for benchmarks: runs nsteps steps bracketed by HIP events on the
- * context stream; *ms is the elapsed device time of those steps. */
+/* Timing for benchmarks (no counterpart in the reference, whose loop times
+ * phases with host clocks, S/Iterate3D.cu:493-653): runs nsteps steps
+ * bracketed by HIP events on the context stream; *ms is the elapsed device
+ * time of those steps.  The timed launches write no macroscopic fields; one
+ * untimed moments pass after the window brings rho, u, v, w up to date. */
 lbm_status lbm_time_steps(lbm_ctx* ctx, int64_t nsteps, double* ms);
+/* As lbm_time_steps with one event pair per launch: ms_each[k] is the device
+ * time of step k (1 <= nsteps <= 100000). */
+lbm_status lbm_time_launches(lbm_ctx* ctx, int64_t nsteps, double* ms_each);
 
 /* Bytes of device memory held by the context. */
 int64_t lbm_device_bytes(const lbm_ctx* ctx);

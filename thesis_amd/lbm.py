@@ -33,7 +33,7 @@ STORAGE_NAMES = {"f64": STORE_F64, "f32": STORE_F32, "f16": STORE_F16, "bf16": S
 EXPORTED = [
     "lbm_abi_version", "lbm_device_count", "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_init",
     "lbm_step", "lbm_run_to_residual", "lbm_reduce", "lbm_reduce_all", "lbm_residual_l2", "lbm_residual_fdrel", "lbm_get_macros", "lbm_get_f", "lbm_set_f",
-    "lbm_time_steps", "lbm_device_bytes", "lbm_read_lattice", "lbm_read_wall_q", "lbm_latticegen",
+    "lbm_time_steps", "lbm_time_launches", "lbm_device_bytes", "lbm_read_lattice", "lbm_read_wall_q", "lbm_latticegen",
     "lbm_latticegen_last_error", "lbm_write_lattice_binary",
     "lbm_cg_create", "lbm_cg_destroy", "lbm_cg_last_error", "lbm_cg_init", "lbm_cg_step", "lbm_cg_get_macros",
     "lbm_cg_get_f", "lbm_cg_reduce", "lbm_cg_time_steps", "lbm_cg_init_f", "lbm_cg_get_macros3d",
@@ -124,6 +124,7 @@ def load():
     L.lbm_get_f.argtypes = [vp, dp]
     L.lbm_set_f.argtypes = [vp, dp]
     L.lbm_time_steps.argtypes = [vp, C.c_int64, dp]
+    L.lbm_time_launches.argtypes = [vp, C.c_int64, dp]
     L.lbm_device_bytes.argtypes = [vp]
     L.lbm_device_bytes.restype = C.c_int64
     L.lbm_read_lattice.argtypes = [C.c_char_p, C.c_char_p, P(C.c_int32), P(C.c_int64), P(C.c_uint8),
@@ -281,6 +282,8 @@ class Solver:
         if cls.size != self.n:
             raise ValueError("node_class size does not match the lattice shape")
         bbv = None if bb is None else np.ascontiguousarray(bb, dtype=np.uint32).ravel()
+        if bbv is not None and bbv.size != self.n:
+            raise ValueError("bb size does not match the lattice shape")
         self._keep = [cls, bbv, _f64(ubc, self.n), _f64(vbc, self.n), _f64(wbc, self.n), _f64(wall_q, 19 * self.n)]
         lat = Lattice()
         lat.node_class = cls.ctypes.data_as(C.POINTER(C.c_uint8))
@@ -363,6 +366,12 @@ class Solver:
         ms = C.c_double()
         self._check(self._L.lbm_time_steps(self.h, int(n), C.byref(ms)))
         return ms.value
+
+    def time_launches(self, n):
+        """Device time of each of n steps (ms), one event pair per launch."""
+        out = np.zeros(int(n))
+        self._check(self._L.lbm_time_launches(self.h, int(n), _dp(out)))
+        return out
 
     def device_bytes(self):
         return self._L.lbm_device_bytes(self.h)
