@@ -173,6 +173,17 @@ lbm_status lbm_reduce_all(lbm_ctx* ctx, double out[8]);
  * f the post-boundary populations of the last step and fColl the
  * post-collision populations that step streamed.  Needs at least one step. */
 lbm_status lbm_residual_l2(lbm_ctx* ctx, double* out);
+/* lbm_residual_l2 without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc); it has no counterpart in the reference.  lbm_residual_l2
+ * stages the post-boundary populations in a Q*N FP64 array the context
+ * allocates on first use (2.55 GB at 256^3 D3Q19); this variant rebuilds them
+ * in registers and keeps its 1024 partial sums and the result in a static
+ * device array of 8.3 KB per device (held until the process ends).  The
+ * scratch does not grow with the lattice, so no lattice size is refused; it
+ * must be called with the context's device current.  Calls that overlap on
+ * one device race on the shared scratch. */
+lbm_status lbm_residual_l2_noalloc(lbm_ctx* ctx, double* out);
 /* The reference's FdRelDiff residual (computeNewResidual3D + gpu_NewResidual,
  * S/ComputeResiduals.cu:28-36, :55-63; driven by S/Iterate3D.cu:677-693):
  * max over non-solid links of |f - f_prev| / |f1|, f the populations after the

@@ -52,6 +52,7 @@ struct lbm_ctx {
     double* partials = nullptr;
     double* out4 = nullptr;       // 8 doubles: the last reduction
     int64_t bytes = 0;
+    int dev = 0;                  // device current at lbm_create
     std::string err;
 };
 
@@ -376,6 +377,10 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return reject("no HIP device");
 
     lbm_ctx* c = new lbm_ctx();
+    if (hipGetDevice(&c->dev) != hipSuccess) {
+        delete c;
+        return reject("no current HIP device");
+    }
     c->p = *p;
     c->p.nz = nz;
     c->q = p->q;
@@ -596,6 +601,34 @@ lbm_status lbm_residual_l2(lbm_ctx* c, double* out) {
                             reduce_blocks(), c->out4, c->stream));
     double sum = 0;
     HIPCHK(c, hipMemcpyAsync(&sum, c->out4, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = std::sqrt(sum);
+    return LBM_OK;
+}
+
+lbm_status lbm_residual_l2_noalloc(lbm_ctx* c, double* out) {
+    if (!c || !out) return LBM_EINVAL;
+    if (c->steps < 1) return fail(c, LBM_EINVAL, "the L2 residual needs one step");
+    int dev = -1;
+    HIPCHK(c, hipGetDevice(&dev));
+    if (dev != c->dev)
+        return fail(c, LBM_EINVAL, "lbm_residual_l2_noalloc: current device %d is not the context's device %d", dev,
+                    c->dev);
+    const int nb = reduce_blocks();
+    if (nb + 1 > kL2ScratchDoubles)
+        return fail(c, LBM_EINVAL, "lbm_residual_l2_noalloc: %d reduction blocks exceed the %d-double scratch", nb,
+                    kL2ScratchDoubles);
+    double* scr = nullptr;
+    HIPCHK(c, l2_scratch(&scr));
+    Fields fl = fields(c);
+    fl.fout = nullptr;
+    if (c->om[0]) fl.om_in = c->om[(c->steps - 1) & 1];
+    // partials in scr[0, nb), the sum in scr[nb]; every slot read is written
+    // first by this call, so nothing needs clearing
+    HIPCHK(c, launch_l2_fused(c->q, c->p.storage, c->open, c->kp, c->P[c->cur ^ 1], c->node, c->cidx, fl, scr, nb,
+                              scr + nb, c->stream));
+    double sum = 0;
+    HIPCHK(c, hipMemcpyAsync(&sum, scr + nb, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     *out = std::sqrt(sum);
     return LBM_OK;

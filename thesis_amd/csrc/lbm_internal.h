@@ -99,5 +99,15 @@ hipError_t launch_fdiff2(int q, int storage, const KParams& kp, const void* pc, 
 hipError_t launch_fdrel(int q, const double* f, const double* f1, const double* fprev, const uint32_t* cidx,
                         size_t n, double* partials, int nblocks, double* out, hipStream_t st);
 int reduce_blocks();
+// Allocation-free L2 residual: the sum of k_fdiff2 with the post-boundary
+// populations rebuilt in registers from pc; partials (nblocks doubles) and out
+// may be the device-resident scratch of l2_scratch.
+hipError_t launch_l2_fused(int q, int storage, bool open, const KParams& kp, const void* pc, const uint32_t* node,
+                           const uint32_t* cidx, Fields fl, double* partials, int nblocks, double* out,
+                           hipStream_t st);
+// Device address of the calling thread's current device's copy of the static
+// scratch (kL2ScratchDoubles doubles); looked up once per device.
+constexpr int kL2ScratchDoubles = 1024 + 8;
+hipError_t l2_scratch(double** out);
 
 }  // namespace lbm

@@ -19,6 +19,8 @@
 // small flow signal instead of rounding it against w_i.
 #include "lbm_device.h"
 
+#include <mutex>
+
 namespace lbm {
 
 // Store of one node's post-collision populations.  The 16-bit storages use
@@ -528,6 +530,43 @@ __global__ __launch_bounds__(kRedThreads) void k_fdiff2(const typename Io<L, S>:
     }
 }
 
+// The same sum with f rebuilt in registers (post_boundary) instead of read
+// from a dense FP64 copy: no Q*N staging array.  Same blocks, threads and
+// summation order as k_fdiff2, so the two agree to the last bit when the
+// rebuilt populations do.
+template <class L, int S, bool OPEN>
+__global__ __launch_bounds__(kRedThreads) void k_l2_fused(const typename Io<L, S>::raw* __restrict__ pc,
+                                                          const uint32_t* __restrict__ node,
+                                                          const uint32_t* __restrict__ cidx, KParams kp,
+                                                          Fields fl, size_t n, double* partials) {
+    using T = typename Io<L, S>::T;
+    double acc = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t own = cidx[i];
+        if (own == kNone) continue;
+        const size_t row = i / (size_t)kp.nx;
+        const int x = (int)(i - row * (size_t)kp.nx);
+        const int y = (int)(row % (size_t)kp.ny), z = (int)(row / (size_t)kp.ny);
+        T h[L::Q];
+        post_boundary<L, S, OPEN>(pc, kp, fl, node, cidx, own, node[i], x, y, z, i, h);
+#pragma unroll
+        for (int q = 0; q < L::Q; ++q) {
+            const T hc = Io<L, S>::ld(pc, paddr<L>(kp, q, own), q);
+            const double d = ((double)h[q] + L::w(q)) - ((double)hc + L::w(q));
+            acc += d * d;
+        }
+    }
+    acc = wave_sum(acc);
+    __shared__ double sh[kRedThreads / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int k = 0; k < kRedThreads / 64; ++k) s += sh[k];
+        partials[blockIdx.x] = s;
+    }
+}
+
 // FdRelDiff residual: max over non-solid links of |f - f_prev| / |f1|
 // (gpu_NewResidual + gpu_max_h, S/ComputeResiduals.cu:28-36, :55-63), f, f1,
 // f_prev dense FP64 (q planes of n); a NaN ratio wins the maximum.
@@ -736,6 +775,66 @@ hipError_t launch_fdiff2(int q, int storage, const KParams& kp, const void* pc, 
                          const double* fout, double* partials, int nblocks, double* out, hipStream_t st) {
     return q == 9 ? fdiff2_s<D2Q9>(storage, kp, pc, cidx, fout, partials, nblocks, out, st)
                   : fdiff2_s<D3Q19>(storage, kp, pc, cidx, fout, partials, nblocks, out, st);
+}
+
+// Scratch of the allocation-free L2 residual: one partial sum per reduction
+// block and the final sum.  Its size does not depend on the lattice (the
+// reduction strides over the nodes with a fixed grid).  Every device that runs
+// the library holds its own copy until the process ends; calls that overlap
+// on one device race on it.
+static_assert(kRedBlocks + 1 <= kL2ScratchDoubles, "scratch holds the partials and the sum");
+static __device__ __align__(256) double scratch[kL2ScratchDoubles];
+
+hipError_t l2_scratch(double** out) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static double* ptr[kMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        // the host-side name of the array is not a device address: take the
+        // current device's copy once and use that pointer everywhere
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        ptr[dev] = (double*)p;
+    }
+    *out = ptr[dev];
+    return hipSuccess;
+}
+
+template <class L, int S>
+static hipError_t l2f_t(bool open, const KParams& kp, const void* pc, const uint32_t* node, const uint32_t* cidx,
+                        Fields fl, double* partials, int nblocks, double* out, hipStream_t st) {
+    using raw = typename Io<L, S>::raw;
+    const size_t n = (size_t)kp.nx * kp.ny * kp.nz;
+    if (open)
+        hipLaunchKernelGGL((k_l2_fused<L, S, true>), dim3(nblocks), dim3(kRedThreads), 0, st, (const raw*)pc, node,
+                           cidx, kp, fl, n, partials);
+    else
+        hipLaunchKernelGGL((k_l2_fused<L, S, false>), dim3(nblocks), dim3(kRedThreads), 0, st, (const raw*)pc, node,
+                           cidx, kp, fl, n, partials);
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, st, (const double*)partials, nblocks, out);
+    return hipGetLastError();
+}
+template <class L>
+static hipError_t l2f_s(int s, bool open, const KParams& kp, const void* pc, const uint32_t* node,
+                        const uint32_t* cidx, Fields fl, double* partials, int nblocks, double* out, hipStream_t st) {
+    switch (s) {
+    case 0: return l2f_t<L, 0>(open, kp, pc, node, cidx, fl, partials, nblocks, out, st);
+    case 1: return l2f_t<L, 1>(open, kp, pc, node, cidx, fl, partials, nblocks, out, st);
+    case 2: return l2f_t<L, 2>(open, kp, pc, node, cidx, fl, partials, nblocks, out, st);
+    default: return l2f_t<L, 3>(open, kp, pc, node, cidx, fl, partials, nblocks, out, st);
+    }
+}
+hipError_t launch_l2_fused(int q, int storage, bool open, const KParams& kp, const void* pc, const uint32_t* node,
+                           const uint32_t* cidx, Fields fl, double* partials, int nblocks, double* out,
+                           hipStream_t st) {
+    return q == 9 ? l2f_s<D2Q9>(storage, open, kp, pc, node, cidx, fl, partials, nblocks, out, st)
+                  : l2f_s<D3Q19>(storage, open, kp, pc, node, cidx, fl, partials, nblocks, out, st);
 }
 
 }  // namespace lbm

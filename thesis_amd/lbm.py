@@ -32,7 +32,7 @@ STORAGE_NAMES = {"f64": STORE_F64, "f32": STORE_F32, "f16": STORE_F16, "bf16": S
 
 EXPORTED = [
     "lbm_abi_version", "lbm_device_count", "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_init",
-    "lbm_step", "lbm_run_to_residual", "lbm_reduce", "lbm_reduce_all", "lbm_residual_l2", "lbm_residual_fdrel", "lbm_get_macros", "lbm_get_f", "lbm_set_f",
+    "lbm_step", "lbm_run_to_residual", "lbm_reduce", "lbm_reduce_all", "lbm_residual_l2", "lbm_residual_l2_noalloc", "lbm_residual_fdrel", "lbm_get_macros", "lbm_get_f", "lbm_set_f",
     "lbm_time_steps", "lbm_time_launches", "lbm_device_bytes", "lbm_read_lattice", "lbm_read_wall_q", "lbm_latticegen",
     "lbm_latticegen_last_error", "lbm_write_lattice_binary",
     "lbm_cg_create", "lbm_cg_destroy", "lbm_cg_last_error", "lbm_cg_init", "lbm_cg_step", "lbm_cg_get_macros",
@@ -119,6 +119,7 @@ def load():
     L.lbm_reduce.argtypes = [vp, dp]
     L.lbm_reduce_all.argtypes = [vp, dp]
     L.lbm_residual_l2.argtypes = [vp, dp]
+    L.lbm_residual_l2_noalloc.argtypes = [vp, dp]
     L.lbm_residual_fdrel.argtypes = [vp, dp]
     L.lbm_get_macros.argtypes = [vp, dp, dp, dp, dp]
     L.lbm_get_f.argtypes = [vp, dp]
@@ -337,6 +338,13 @@ class Solver:
         """The reference's L2 residual sqrt(sum (f - fColl)^2) of the last step."""
         out = C.c_double()
         self._check(self._L.lbm_residual_l2(self.h, C.byref(out)))
+        return out.value
+
+    def residual_l2_noalloc(self):
+        """The same L2 residual with no device allocation (populations rebuilt
+        in registers, partial sums in the library's static scratch)."""
+        out = C.c_double()
+        self._check(self._L.lbm_residual_l2_noalloc(self.h, C.byref(out)))
         return out.value
 
     def residual_fdrel(self):
