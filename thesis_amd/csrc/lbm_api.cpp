@@ -127,7 +127,8 @@ lbm_status ensure_fout(lbm_ctx* c) {
 
 // macros_last: the last launch also writes the FP64 macroscopic fields
 // (timed runs leave them to an untimed moments pass, refresh_macros).
-lbm_status run_steps(lbm_ctx* c, int64_t nsteps, bool macros_last = true) {
+// ev_after (optional, nsteps events): recorded after each launch.
+lbm_status run_steps(lbm_ctx* c, int64_t nsteps, bool macros_last = true, hipEvent_t* ev_after = nullptr) {
     for (int64_t k = 0; k < nsteps; ++k) {
         const bool wm = macros_last && (k == nsteps - 1);
         // per-launch seed of the stochastic rounding: a hash of the step count
@@ -144,6 +145,10 @@ lbm_status run_steps(lbm_ctx* c, int64_t nsteps, bool macros_last = true) {
                                    c->P[c->cur ^ 1], c->node, c->cidx, fl, wm, c->stream);
         if (e != hipSuccess) return fail(c, LBM_EDEVICE, "step launch: %s", hipGetErrorString(e));
         c->cur ^= 1;
+        if (ev_after) {
+            e = hipEventRecord(ev_after[k], c->stream);
+            if (e != hipSuccess) return fail(c, LBM_EDEVICE, "event record: %s", hipGetErrorString(e));
+        }
     }
     c->steps += nsteps;
     return LBM_OK;
@@ -362,7 +367,7 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     while ((1 << lgw) < words) ++lgw;
     // LDS: staged row words at a power-of-two pitch plus one node position per
     // thread; positions pack x in 16 bits
-    const bool rank_ok = (((size_t)rmax * 9) << lgw) * 8 + 4 * rblock <= 48 * 1024 && p->nx < 65536;
+    const bool rank_ok = (((size_t)rmax * 9) << lgw) * 8 + 4 * rblock + 36 * (size_t)rmax <= 48 * 1024 && p->nx < 65536;
     if (any_open_cls && !lat->u_bc) {
         bool need = false;
         for (size_t i = 0; i < n; ++i) {
@@ -444,10 +449,14 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     }
     // 256 B guard before and after the planes: the shifted vector loads of the
     // first and last rows may touch one element outside the lattice (discarded).
-    const size_t pbytes = c->raw_bytes * k.qstride * c->q + 512;
+    // In front of that, kRankShift nodes' worth of tiles, so that the 16-bit
+    // rank step's base pointer (kRankShift nodes before the buffer) still lies
+    // inside the allocation.
+    const size_t front = 256 + (size_t)kRankShift * c->q * c->raw_bytes;
+    const size_t pbytes = c->raw_bytes * k.qstride * c->q + front + 256;
     for (int b = 0; b < 2 && st == LBM_OK; ++b) {
         st = dalloc(c, &c->Palloc[b], pbytes);
-        if (st == LBM_OK) c->P[b] = (char*)c->Palloc[b] + 256;
+        if (st == LBM_OK) c->P[b] = (char*)c->Palloc[b] + front;
     }
     if (st == LBM_OK) st = dalloc(c, (void**)&c->node, sizeof(uint32_t) * n);
     if (st == LBM_OK) st = dalloc(c, (void**)&c->cidx, sizeof(uint32_t) * n);
@@ -757,11 +766,7 @@ lbm_status lbm_time_launches(lbm_ctx* c, int64_t nsteps, double* ms_each) {
         if (hipEventCreate(&e) != hipSuccess) { s = fail(c, LBM_EDEVICE, "event create failed"); break; }
     if (s == LBM_OK && hipStreamSynchronize(c->stream) != hipSuccess) s = fail(c, LBM_EDEVICE, "sync failed");
     if (s == LBM_OK && hipEventRecord(ev[0], c->stream) != hipSuccess) s = fail(c, LBM_EDEVICE, "event record failed");
-    for (int64_t k = 0; s == LBM_OK && k < nsteps; ++k) {
-        s = run_steps(c, 1, false);
-        if (s == LBM_OK && hipEventRecord(ev[(size_t)k + 1], c->stream) != hipSuccess)
-            s = fail(c, LBM_EDEVICE, "event record failed");
-    }
+    if (s == LBM_OK) s = run_steps(c, nsteps, false, ev.data() + 1);
     if (s == LBM_OK) s = refresh_macros(c);
     if (s == LBM_OK && hipStreamSynchronize(c->stream) != hipSuccess) s = fail(c, LBM_EDEVICE, "sync failed");
     for (int64_t k = 0; s == LBM_OK && k < nsteps; ++k) {

@@ -88,6 +88,7 @@ template <class L, int R, int I> struct Mc {
 };
 template <class L, class T, int R, int... I>
 __device__ inline T m_row_dot(const T* h, std::integer_sequence<int, I...>) {
+#pragma clang fp contract(on)
     T acc = 0;
     ((Mc<L, R, I>::m != 0.0 ? (void)(acc += (T)Mc<L, R, I>::m * h[I]) : (void)0), ...);
     return acc;
@@ -98,6 +99,7 @@ __device__ inline void m_apply(const T* h, T* m, std::integer_sequence<int, R...
 }
 template <class L, class T, int I, int... R>
 __device__ inline T mi_col_dot(const T* m, std::integer_sequence<int, R...>) {
+#pragma clang fp contract(on)
     T acc = 0;
     ((Mc<L, R, I>::mi != 0.0 ? (void)(acc += (T)Mc<L, R, I>::mi * m[R]) : (void)0), ...);
     return acc;
@@ -761,28 +763,32 @@ __device__ inline T kv(double d, float f) {
 
 // Moments of the deviations h_i = sc(i) x_i (sc = 1: x holds h itself; the
 // 16-bit storages pass their stored values x_i = h_i / a_i with sc = a_i).
+// The moments and collisions contract multiply-adds only within one statement
+// (each written as one), so every kernel that inlines them computes the same
+// bits whatever code surrounds them.
 template <class L, class T, class Sc>
 __device__ inline void moments(const KParams& kp, uint32_t cls, const T* x, T& drho, T& rho, T* u, Sc sc) {
+#pragma clang fp contract(on)
     T d = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
     for (int i = 0; i < L::Q; ++i) {
-        const T h = sc(i) * x[i];
-        d += h;
-        if (L::cx(i)) jx += L::cx(i) > 0 ? h : -h;
-        if (L::cy(i)) jy += L::cy(i) > 0 ? h : -h;
-        if (L::cz(i)) jz += L::cz(i) > 0 ? h : -h;
+        d += sc(i) * x[i];
+        if (L::cx(i)) jx += (L::cx(i) > 0 ? sc(i) : -sc(i)) * x[i];
+        if (L::cy(i)) jy += (L::cy(i) > 0 ? sc(i) : -sc(i)) * x[i];
+        if (L::cz(i)) jz += (L::cz(i) > 0 ? sc(i) : -sc(i)) * x[i];
     }
     drho = d;
     rho = (T)1 + d;
     const T ir = recip(rho);
-    u[0] = jx * ir; u[1] = jy * ir; u[2] = jz * ir;
+    T s0 = 0, s1 = 0, s2 = 0;
     if (kp.force == 1) {
         const T io = (T)(1.0 / kp.omega);
-        u[0] += (T)kp.g[0] * io; u[1] += (T)kp.g[1] * io; u[2] += (T)kp.g[2] * io;
+        s0 = (T)kp.g[0] * io; s1 = (T)kp.g[1] * io; s2 = (T)kp.g[2] * io;
     } else if (kp.force == 2) {
-        u[0] += kv<T>(0.5 * kp.g[0], kp.hgf[0]); u[1] += kv<T>(0.5 * kp.g[1], kp.hgf[1]);
-        u[2] += kv<T>(0.5 * kp.g[2], kp.hgf[2]);
+        s0 = kv<T>(0.5 * kp.g[0], kp.hgf[0]); s1 = kv<T>(0.5 * kp.g[1], kp.hgf[1]);
+        s2 = kv<T>(0.5 * kp.g[2], kp.hgf[2]);
     }
+    u[0] = jx * ir + s0; u[1] = jy * ir + s1; u[2] = jz * ir + s2;
     if ((kp.parity & 0x8u) && cls == PRES_E) u[1] = 0;
 }
 template <class L, class T>
@@ -795,6 +801,7 @@ __device__ inline void moments(const KParams& kp, uint32_t cls, const T* h, T& d
 // and the result is the same for finite operands.
 template <class L, class T>
 __device__ inline T cdot(int i, T a, T b, T c) {
+#pragma clang fp contract(on)
     T r = 0;
     bool any = false;
     const int k[3] = {L::cx(i), L::cy(i), L::cz(i)};
@@ -811,6 +818,7 @@ __device__ inline T cdot(int i, T a, T b, T c) {
 
 template <class L, class T>
 __device__ inline void guo_terms(const KParams& kp, T rho, const T* u, T* S) {
+#pragma clang fp contract(on)
     const T Fx = rho * kv<T>(kp.g[0], kp.gf[0]), Fy = rho * kv<T>(kp.g[1], kp.gf[1]), Fz = rho * kv<T>(kp.g[2], kp.gf[2]);
     const T uF = u[0] * Fx + u[1] * Fy + u[2] * Fz;
 #pragma unroll
@@ -825,6 +833,7 @@ __device__ inline void guo_terms(const KParams& kp, T rho, const T* u, T* S) {
 // D2Q9: S/GpuCollision.cu:986-994; D3Q19: :1065-1083.
 template <class L, class T>
 __device__ inline void mrt_dmeq(const KParams& kp, T drho, T rho, const T* u, T* dm) {
+#pragma clang fp contract(on)
     if (L::Q == 9) {
         const T uu = u[0] * u[0] + u[1] * u[1];
         const T k = (kp.parity & 0x1u) ? rho : (T)1;
@@ -860,6 +869,7 @@ __device__ inline void collide_pairs(const KParams& kp, const T* h, T drho, T rh
 
 template <class L, class T, int C>
 __device__ inline void collide(const KParams& kp, const T* h, T drho, T rho, const T* u, T* out) {
+#pragma clang fp contract(on)
     const bool guo = kp.force == 2;
     if (C == 2) {
         T S[L::Q];
@@ -888,11 +898,21 @@ __device__ inline void collide(const KParams& kp, const T* h, T drho, T rho, con
 // (i, opp i) the parts of f_eq and of the Guo source are
 //   e+ = w (drho + rho (4.5 cu^2 - 1.5 u.u)),  e- = 3 w rho cu,
 //   S+ = w (9 cu cF - 3 u.F),                  S- = 3 w cF,
-// relaxed with omega+ and omega- (equal for BGK) and prefactors 1 - omega/2.
-// wt(i) is w_i for populations h; for stored values x_i = h_i / a_i with
-// a_i = a_opp(i) the same update divided by a_i is the one with wt = w_i / a_i.
+// relaxed with omega+ and omega- (equal for BGK) and prefactors 1 - omega/2:
+//   out_i = h_i - op (h+ - e+) - om (h- - e-) + (1 - op/2) S+ + (1 - om/2) S-.
+// Collected per pair this is
+//   out_i   = a h_i + b h_opp + w (E + G),   out_opp = a h_opp + b h_i + w (E - G),
+//   a = 1 - (op + om)/2,  b = (om - op)/2,
+//   E = K0 + cu (P cu + Qf cF),  G = R cu + Sg cF,
+// with the node constants K0 = op (drho - 1.5 rho u.u) - 3 (1 - op/2) u.F,
+// P = 4.5 op rho, Qf = 9 (1 - op/2), R = 3 om rho, Sg = 3 (1 - om/2), scaled
+// by w once per link class: about a dozen operations per pair instead of two
+// dozen.  wt(i) is w_i for
+// populations h; for stored values x_i = h_i / a_i with a_i = a_opp(i) the same
+// update divided by a_i is the one with wt = w_i / a_i.
 template <class L, class T, int C, class Wt>
 __device__ inline void collide_pairs(const KParams& kp, const T* h, T drho, T rho, const T* u, T* out, Wt wt) {
+#pragma clang fp contract(on)
     const bool guo = kp.force == 2;
     const T op = kv<T>(kp.omega, kp.omegaf);
     const T om = C == 0 ? op : kv<T>(kp.omega_m, kp.omega_mf);
@@ -903,27 +923,25 @@ __device__ inline void collide_pairs(const KParams& kp, const T* h, T drho, T rh
         Fx = rho * kv<T>(kp.g[0], kp.gf[0]); Fy = rho * kv<T>(kp.g[1], kp.gf[1]); Fz = rho * kv<T>(kp.g[2], kp.gf[2]);
         uF = u[0] * Fx + u[1] * Fy + u[2] * Fz;
     }
-    {
-        const T w0 = wt(0);
-        out[0] = h[0] - op * (h[0] - w0 * (drho - (T)1.5 * rho * uu));
-        if (guo) out[0] -= gp * (T)3 * w0 * uF;
-    }
+    const T a = (T)1 - (T)0.5 * (op + om), b = (T)0.5 * (om - op);
+    const T K0 = op * (drho - (T)1.5 * rho * uu) - (T)3 * gp * uF;
+    const T P = (T)4.5 * op * rho, Qf = (T)9 * gp, R = (T)3 * om * rho, Sg = (T)3 * gm;
+    out[0] = ((T)1 - op) * h[0] + wt(0) * K0;
 #pragma unroll
     for (int i = 1; i < L::Q; ++i) {
         const int o = L::opp(i);
         if (i < o) {
             const T w = wt(i);
             const T cu = cdot<L, T>(i, u[0], u[1], u[2]);
-            const T ep = w * (drho + rho * ((T)4.5 * cu * cu - (T)1.5 * uu));
-            const T em = (T)3 * w * rho * cu;
             // the source terms are applied unconditionally (zero without
-            // forcing, which leaves dp and dm unchanged): a test per link
-            // becomes a select per link
+            // forcing): a test per link would become a select per link
             const T cF = cdot<L, T>(i, Fx, Fy, Fz);
-            const T dp = op * ((T)0.5 * (h[i] + h[o]) - ep) - gp * w * ((T)9 * cu * cF - (T)3 * uF);
-            const T dm = om * ((T)0.5 * (h[i] - h[o]) - em) - gm * (T)3 * w * cF;
-            out[i] = h[i] - dp - dm;
-            out[o] = h[o] - dp + dm;
+            // w is a literal per link class: the scaled constants are shared
+            // by every pair of the class
+            const T E = w * K0 + cu * ((w * P) * cu + (w * Qf) * cF);
+            const T G = (w * R) * cu + (w * Sg) * cF;
+            out[i] = a * h[i] + (b * h[o] + (E + G));
+            out[o] = a * h[o] + (b * h[i] + (E - G));
         }
     }
 }

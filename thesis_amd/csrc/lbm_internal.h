@@ -20,6 +20,12 @@ constexpr uint32_t kBBMask = 0x7FFFFu;
 constexpr int kHHShift = 19;
 constexpr int kClsShift = 24;
 
+// The 16-bit rank step on periodic lattices addresses population slots from a
+// base kRankShift compact nodes (whole 64-node tiles) before the buffer, so
+// that every per-lane offset it selects is non-negative; the buffers keep that
+// much room in front (lbm_create).
+constexpr uint32_t kRankShift = 256;
+
 // Everything a kernel needs, passed by value as a kernel argument.
 struct KParams {
     int nx, ny, nz;

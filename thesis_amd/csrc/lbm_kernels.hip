@@ -63,7 +63,7 @@ __device__ inline void store_node(typename Io<L, S>::raw* __restrict__ dst, cons
 #pragma unroll
     for (int i = 0; i < L::Q; ++i) {
         if constexpr (X) sto(base(i), slot(i), Io<L, S>::px(out[i]));
-        else Io<L, S>::st(base(i), slot(i), out[i], i);
+        else sto(base(i), slot(i), Io<L, S>::pk(out[i], i));
     }
 }
 
@@ -154,44 +154,91 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
     // node reaches all of them from three base addresses with constant
     // offsets.  Rows outside a non-periodic face read as all-solid (outside
     // sources are resolved by coordinates).
+    // 16-bit periodic lattices count compact indices from kRankShift (the
+    // base pointer of their loads sits that many nodes before the buffer)
+    constexpr bool kShifted = sizeof(typename Io<L, S>::raw) == 2 && SIMPLE && !OPEN;
+    constexpr uint32_t kSh = kShifted ? kRankShift : 0u;
     const int tot = (R * 9) << lg;
-    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-        const int w = e & ((1 << lg) - 1), slot = e >> lg;
-        const int k = slot / 9, nb = slot - 9 * k;
-        uint2 val = make_uint2(0u, 0u);
-        if (w < W) {
-            const uint32_t r = r0 + (uint32_t)k;
-            const uint32_t zq = fdiv(r, kp.my, kp.sy);
-            int y = (int)(r - zq * (uint32_t)ny) + nb % 3 - 1, z = (int)zq + nb / 3 - 1;
-            bool out = false;
-            if (y < 0) { y += ny; out = !kp.per[1]; } else if (y >= ny) { y -= ny; out = !kp.per[1]; }
-            if (z < 0) { z += nz; out |= !kp.per[2]; } else if (z >= nz) { z -= nz; out |= !kp.per[2]; }
-            if (!out) val = rowmeta[((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)W + (uint32_t)w];
-        }
-        smeta[((k << lg) + w) * 9 + nb] = val;
-    }
-    __syncthreads();
-    // Position of every node of this workgroup: one thread per bit of the
-    // staged own-row words; a fluid bit writes (row, x) at its node's slot
-    // (its rank in the word plus the word's first compact index), so no
-    // thread searches for its own node.
     const uint32_t own0 = bid * blockDim.x;
-    uint32_t* pos = (uint32_t*)(smeta + ((kp.rmax * 9) << lg));
-    for (int t = threadIdx.x; t < ((R << lg) << 5); t += blockDim.x) {
-        const int b = t & 31, wl = t >> 5;
-        const int k = wl >> lg, w = wl & ((1 << lg) - 1);
-        if (w >= W) continue;
-        const uint2 me = smeta[wl * 9 + 4];
-        if (!((me.x >> b) & 1u)) continue;
-        const uint32_t slot = me.y + __popc(me.x & ((1u << b) - 1u)) - own0;
-        if (slot < blockDim.x) pos[slot] = ((uint32_t)k << 16) | (uint32_t)(32 * w + b);
+    uint32_t own, r;
+    int k, x;
+    // kShifted: the node's dense index, loaded before the staging so that its
+    // latency hides behind it
+    uint32_t dj = 0;
+    if constexpr (kShifted) {
+        if (own0 + threadIdx.x < kp.nf) dj = kp.didx[own0 + threadIdx.x];
     }
-    __syncthreads();
-    const uint32_t own = own0 + threadIdx.x;
-    if (own >= kp.nf) return;
-    const uint32_t p = pos[threadIdx.x];
-    const int k = (int)(p >> 16), x = (int)(p & 0xFFFFu);
-    const uint32_t r = r0 + (uint32_t)k;
+    if constexpr (kShifted) {
+        // Periodic lattice: the R * 9 neighbour-row offsets first (one thread
+        // each), then a plain copy of the staged words; every node's position
+        // comes from its dense index (4 B per node) rather than a scatter of
+        // the staged bits, which costs more instructions than it saves bytes
+        // in this instruction-bound variant.
+        uint32_t* rowoff = (uint32_t*)(smeta + ((kp.rmax * 9) << lg)) + kp.rblock;
+        for (int t = threadIdx.x; t < R * 9; t += blockDim.x) {
+            const int kk = t / 9, nb = t - 9 * kk;
+            const uint32_t rr = r0 + (uint32_t)kk;
+            const uint32_t zq = fdiv(rr, kp.my, kp.sy);
+            int y = (int)(rr - zq * (uint32_t)ny) + nb % 3 - 1, z = (int)zq + nb / 3 - 1;
+            y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
+            z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
+            rowoff[t] = ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)W;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+            const int w = e & ((1 << lg) - 1), slot = e >> lg;
+            const int kk = slot / 9, nb = slot - 9 * kk;
+            uint2 val = make_uint2(0u, 0u);
+            if (w < W) val = rowmeta[rowoff[slot] + (uint32_t)w];
+            val.y += kSh;
+            smeta[((kk << lg) + w) * 9 + nb] = val;
+        }
+        __syncthreads();
+        own = own0 + threadIdx.x;
+        if (own >= kp.nf) return;
+        r = fdiv(dj, kp.mx, kp.sx);
+        x = (int)(dj - r * (uint32_t)nx);
+        k = (int)(r - r0);
+    } else {
+        for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+            const int w = e & ((1 << lg) - 1), slot = e >> lg;
+            const int k = slot / 9, nb = slot - 9 * k;
+            uint2 val = make_uint2(0u, 0u);
+            if (w < W) {
+                const uint32_t r = r0 + (uint32_t)k;
+                const uint32_t zq = fdiv(r, kp.my, kp.sy);
+                int y = (int)(r - zq * (uint32_t)ny) + nb % 3 - 1, z = (int)zq + nb / 3 - 1;
+                bool out = false;
+                if (y < 0) { y += ny; out = !kp.per[1]; } else if (y >= ny) { y -= ny; out = !kp.per[1]; }
+                if (z < 0) { z += nz; out |= !kp.per[2]; } else if (z >= nz) { z -= nz; out |= !kp.per[2]; }
+                if (!out) val = rowmeta[((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)W + (uint32_t)w];
+            }
+            val.y += kSh;
+            smeta[((k << lg) + w) * 9 + nb] = val;
+        }
+        __syncthreads();
+        // Position of every node of this workgroup: one thread per bit of the
+        // staged own-row words; a fluid bit writes (row, x) at its node's slot
+        // (its rank in the word plus the word's first compact index), so no
+        // thread searches for its own node.
+        uint32_t* pos = (uint32_t*)(smeta + ((kp.rmax * 9) << lg));
+        for (int t = threadIdx.x; t < ((R << lg) << 5); t += blockDim.x) {
+            const int b = t & 31, wl = t >> 5;
+            const int k = wl >> lg, w = wl & ((1 << lg) - 1);
+            if (w >= W) continue;
+            const uint2 me = smeta[wl * 9 + 4];
+            if (!((me.x >> b) & 1u)) continue;
+            const uint32_t slot = me.y + __popc(me.x & ((1u << b) - 1u)) - (own0 + kSh);
+            if (slot < blockDim.x) pos[slot] = ((uint32_t)k << 16) | (uint32_t)(32 * w + b);
+        }
+        __syncthreads();
+        own = own0 + threadIdx.x;
+        if (own >= kp.nf) return;
+        const uint32_t p = pos[threadIdx.x];
+        k = (int)(p >> 16);
+        x = (int)(p & 0xFFFFu);
+        r = r0 + (uint32_t)k;
+    }
     const uint32_t zq = fdiv(r, kp.my, kp.sy);
     const int y = (int)(r - zq * (uint32_t)ny), z = (int)zq;
     const uint32_t idx = r * (uint32_t)nx + (uint32_t)x;
@@ -213,13 +260,14 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
     else h[0] = Io<L, S>::ld(src, oslot, 0);
     // per source column (x+1, x, x-1 for cx = -1, 0, 1): the LDS base of its
     // word's nine neighbour words, its bit, and the mask of the bits below it
-    uint32_t cb[3], cbitm[3], clow[3];
+    uint32_t cb[3], cbitm[3], clow[3], cpos[3];
     {
         const int sxs[3] = {xm, x, xp};
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             cb[c] = (((uint32_t)k << lg) + ((uint32_t)sxs[c] >> 5)) * 9u;
-            cbitm[c] = 1u << ((uint32_t)sxs[c] & 31u);
+            cpos[c] = (uint32_t)sxs[c] & 31u;
+            cbitm[c] = 1u << cpos[c];
             clow[c] = cbitm[c] - 1u;
         }
     }
@@ -228,7 +276,44 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
         return OPEN && ((cx > 0 && xm_o) || (cx < 0 && xp_o) || (cy > 0 && ym_o) || (cy < 0 && yp_o) ||
                         (cz > 0 && zm_o) || (cz < 0 && zp_o));
     };
-    if constexpr (kNarrow) {
+    if constexpr (kShifted) {
+        // Periodic lattice whose walls are exactly the links into solid nodes,
+        // 16-bit storage (bound by instruction issue): all LDS words and counts
+        // first, then per direction the byte offset of the source slot, the
+        // fluid bit of the source as an all-ones / zero mask (one bit-field
+        // extract) and a bitwise select against the node's own opposite slot
+        // (half-way bounce-back).  The tile slot of compact index j is
+        // (j >> 6) 64 (Q - 1) + j + 64 i: one shift and one 24-bit multiply-add.
+        uint32_t cnt[L::Q], word[L::Q];
+#pragma unroll
+        for (int i = 1; i < L::Q; ++i) {
+            const int c = 1 - L::cx(i);
+            const uint2 e = smeta[cb[c] + (uint32_t)((1 - L::cz(i)) * 3 + (1 - L::cy(i)))];
+            cnt[i] = e.y + __popc(e.x & clow[c]);
+            word[i] = e.x;
+        }
+#pragma unroll
+        for (int i = 1; i < L::Q; ++i) asm volatile("" : "+v"(cnt[i]));
+        // counts and the own slot are taken kRankShift nodes (kRankShift / 64
+        // whole tiles) further on, against a base that many tiles before the
+        // buffer: the own-slot offsets of the opposite planes, down to 3 planes
+        // below plane i, then stay non-negative, and the plane term 64 i of
+        // every load is an immediate of the instruction
+        constexpr uint32_t rb = (uint32_t)sizeof(raw);
+        constexpr uint32_t kShBytes = (kSh / 64u) * 64u * L::Q * rb;
+        static_assert(kSh % 64u == 0 && kSh * L::Q * rb >= 64u * 3u * rb, "shift covers the plane offsets");
+        const char* const srcb = (const char*)src - kShBytes;
+        const uint32_t obyte = oslot * rb + kShBytes;
+#pragma unroll
+        for (int i = 1; i < L::Q; ++i) {
+            const int c = 1 - L::cx(i), o = L::opp(i);
+            const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int)word[i], cpos[c], 1);
+            const uint32_t there = (__umul24(cnt[i] >> 6, 64u * (L::Q - 1)) + cnt[i]) * rb;
+            const uint32_t self = obyte + (uint32_t)(64 * (int)rb * (o - i));
+            const uint32_t off = (m & there) | (~m & self);
+            hr[i] = ((const raw*)(srcb + off))[64 * i];
+        }
+    } else if constexpr (kNarrow) {
         // 16-bit storage is bound by instruction issue, not by HBM: first every
         // direction's LDS word and count, with no branch around the reads so
         // they are all in flight together (the count is the source's compact
@@ -635,7 +720,9 @@ static hipError_t step_t(const KParams& kp, const void* src, void* dst, const ui
     if (kp.nf == 0) return hipSuccess;
     const unsigned blocks = (unsigned)((kp.nf + 255) / 256);
     if (kp.rank) {
-        const size_t lds = (((size_t)kp.rmax * 9) << kp.lgw) * sizeof(uint2) + (size_t)kp.rblock * sizeof(uint32_t);
+        // staged row words, node positions, neighbour-row offsets
+        const size_t lds = (((size_t)kp.rmax * 9) << kp.lgw) * sizeof(uint2) + (size_t)kp.rblock * sizeof(uint32_t) +
+                           (size_t)kp.rmax * 9 * sizeof(uint32_t);
         const unsigned rb = (unsigned)((kp.nf + kp.rblock - 1) / kp.rblock);
         if (kp.simple)
             hipLaunchKernelGGL((k_stepr<L, S, C, OPEN, true>), dim3(rb), dim3(kp.rblock), lds, st, (const raw*)src,
