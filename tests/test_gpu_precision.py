@@ -48,3 +48,31 @@ def test_reduced_storage_error_vs_fp64_256():
     assert out["f32"][0] < 1e-4
     assert out["f16"][0] < 1e-3          # the FP16 target: <= 1e-3 relative velocity error
     assert out["bf16"][0] < 1e-2         # BF16 (8-bit mantissa) is reported, not held to 1e-3
+
+
+def test_reduced_storage_error_vs_fp64_256_mrt():
+    """The same configuration with the reference's MRT collision (d'Humieres
+    D3Q19 moments and rates, S/CellFunctions.cu:65-154): reduced storages
+    against the FP64 MRT run."""
+    print("case: precision 256^3 MRT 5000 steps", flush=True)
+    cls, bb, info = LT.porous3d(256, seed=2024)
+    fl = cls.astype(bool)
+    r64, u64 = velocity(cls, bb, L.STORE_F64, L.MRT)
+    ref = u64[:, fl]
+    umax = np.sqrt((ref ** 2).sum(0)).max()
+    l2 = np.sqrt((ref ** 2).sum())
+    out = {}
+    for name, st in (("f32", L.STORE_F32), ("f16", L.STORE_F16), ("bf16", L.STORE_BF16)):
+        r, u = velocity(cls, bb, st, L.MRT)
+        d = u[:, fl] - ref
+        out[name] = (np.sqrt((d ** 2).sum(0)).max() / umax, np.sqrt((d ** 2).sum()) / l2,
+                     np.abs(r[fl] - r64[fl]).max())
+    print(f"max |u| (FP64 MRT) = {umax:.4e} after {STEPS} steps")
+    print("storage  max-rel-u-err  L2-rel-u-err  max|drho|")
+    for k, (a, b, c) in out.items():
+        print(f"{k:6s}   {a:.3e}      {b:.3e}     {c:.3e}", flush=True)
+    # MRT keeps its cubic ghost moments barely damped (rate 1.98): rounding of
+    # 16-bit stores stays visible at a few wall nodes (max), not in the bulk (L2)
+    assert out["f32"][0] < 1e-4
+    assert out["f16"][0] < 5e-2 and out["f16"][1] < 1e-3
+    assert out["bf16"][0] < 0.5 and out["bf16"][1] < 1e-2

@@ -438,6 +438,9 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     k.inlet = p->inlet_model;
     k.outlet = p->outlet_model;
     k.parity = p->parity_flags;
+    // D3Q19 MRT stores deviations from its own rest state (w_eps = 0 moves it
+    // off w), so reduced-precision copies keep their digits for the flow
+    k.mref = (p->q == 19 && p->collision == LBM_MRT) ? 1 : 0;
     k.rho_out = p->rho_out > 0 ? p->rho_out : 1.0;
 
     lbm_status st = LBM_OK;

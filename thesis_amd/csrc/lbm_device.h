@@ -80,6 +80,20 @@ static_assert(mrest<D2Q9>(2) > 0.9999999 && mrest<D2Q9>(2) < 1.0000001, "D2Q9 re
 static_assert(mrest<D3Q19>(1) > -11.0000001 && mrest<D3Q19>(1) < -10.9999999, "D3Q19 rest e");
 static_assert(mrest<D3Q19>(2) > 2.9999999 && mrest<D3Q19>(2) < 3.0000001, "D3Q19 rest eps");
 
+// The reference's D3Q19 MRT equilibrium has no density term in epsilon
+// (w_eps = 0, S/GpuCollision.cu:1067), while the rest weights carry
+// epsilon = 3: its rest state is f_ref = w - r with r = 3 M^-1 e_eps, 14-86 %
+// away from w.  D3Q19 MRT runs therefore store h' = f - f_ref = (f - w) + r,
+// which is small near rest, instead of f - w (whose reduced-precision copies
+// would spend their digits on r).  r_i depends on |c_i| only, so bounce-back,
+// on-node and curved walls and symmetry planes act on h' unchanged; boundary
+// closures that mix speeds convert to f - w and back.  The MRT collision in h'
+// drops the rest moment from epsilon_eq.
+template <class L>
+__host__ __device__ constexpr double mref_off(int i) {
+    return L::Q == 19 ? 3.0 * mrow<L>(2, i) / mnorm2<L>(2) : 0.0;
+}
+
 // Compile-time coefficients of M and Minv; the products below are expanded by
 // fold expressions so every zero drops out and every +-1 becomes an add.
 template <class L, int R, int I> struct Mc {
@@ -602,6 +616,13 @@ __device__ inline void bouzidi(const typename Io<L, S>::raw* __restrict__ src, c
 // Populations of a node after streaming and its inlet and wall conditions,
 // i.e. everything before the outlet (the reference orders inlet, wall,
 // outlet: S/Iterate.cu:468-473, S/Iterate3D.cu:602-621).
+// h' <-> h = f - w of a D3Q19 MRT run (sign +1: add r, -1: subtract it).
+template <class L, class T>
+__device__ inline void mref_shift(T* h, int sign) {
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) h[i] += (T)sign * (T)mref_off<L>(i);
+}
+
 template <class L, int S, bool OPEN>
 __device__ inline void inflow_state(const typename Io<L, S>::raw* __restrict__ src, const KParams& kp,
                                     const Fields& fl, const uint32_t* __restrict__ cidx, uint32_t own,
@@ -614,9 +635,13 @@ __device__ inline void inflow_state(const typename Io<L, S>::raw* __restrict__ s
     const uint32_t cls = nd >> kClsShift;
     if (OPEN && L::Q == 9 && cls == VEL_W) zou_inlet<L, T>(kp, fl, idx, nd, h);
     if (OPEN && L::Q == 9 && cls == VEL_N) zou_lid<L, T>(kp, fl, idx, nd, h);
+    // velocity faces and wet-node walls mix link speeds: they act on f - w
+    const bool conv = L::Q == 19 && kp.mref && ((OPEN && cls != FLUID) || kp.wall == 2);
+    if (conv) mref_shift<L, T>(h, -1);
     if (OPEN && L::Q == 19) vel3d<L, T>(kp, fl, idx, cls, h);
     if (kp.wall == 1 && (nd & kBBMask)) onnode_walls<L, T>(nd, h);
     if (L::Q == 19 && kp.wall == 2) hh_walls<L, T>(nd, h);
+    if (conv) mref_shift<L, T>(h, 1);
 }
 
 // Extrapolating east outlet (S/GpuBoundaries.cu:1609-1640): f1, f5, f8 of an
@@ -736,6 +761,9 @@ __device__ inline void outlet3d(const typename Io<L, S>::raw* __restrict__ src, 
         default: extrap_mix<L, T, 2, -1>(f, ha, hb, second); break;
         }
     } else {
+        // the pressure and f_eq closures act on f - w (extrapolations, whose
+        // coefficients sum to one, do not need it)
+        if (kp.mref) mref_shift<L, T>(f, -1);
         switch (cls) {
         case PRES_E: outlet3_face<L, S, OPEN, 0, -1>(kp, fl, idx, f); break;
         case OUT_W: outlet3_face<L, S, OPEN, 0, 1>(kp, fl, idx, f); break;
@@ -744,6 +772,7 @@ __device__ inline void outlet3d(const typename Io<L, S>::raw* __restrict__ src, 
         case OUT_B: outlet3_face<L, S, OPEN, 2, 1>(kp, fl, idx, f); break;
         default: outlet3_face<L, S, OPEN, 2, -1>(kp, fl, idx, f); break;
         }
+        if (kp.mref) mref_shift<L, T>(f, 1);
     }
 #pragma unroll
     for (int i = 0; i < L::Q; ++i) h[i] = f[i];
@@ -849,7 +878,9 @@ __device__ inline void mrt_dmeq(const KParams& kp, T drho, T rho, const T* u, T*
         const T j2 = jx * jx + jy * jy + jz * jz;
         dm[0] = drho;
         dm[1] = (T)-11 * drho + (T)19 * j2;
-        dm[2] = (T)(-475.0 / 63.0) * j2 - (T)mrest<D3Q19>(2);
+        // in h' = h + r (kp.mref) the equilibrium deviation of epsilon loses
+        // the rest moment: M r = 3 e_eps = mrest(eps) e_eps
+        dm[2] = kp.mref ? (T)(-475.0 / 63.0) * j2 : (T)(-475.0 / 63.0) * j2 - (T)mrest<D3Q19>(2);
         dm[3] = jx; dm[4] = (T)(-2.0 / 3.0) * jx;
         dm[5] = jy; dm[6] = (T)(-2.0 / 3.0) * jy;
         dm[7] = jz; dm[8] = (T)(-2.0 / 3.0) * jz;
@@ -887,8 +918,8 @@ __device__ inline void collide(const KParams& kp, const T* h, T drho, T rho, con
         mi_apply<L, T>(m, dd, std::make_integer_sequence<int, L::Q>{});
 #pragma unroll
         for (int i = 0; i < L::Q; ++i) out[i] = h[i] - dd[i];
-        if (L::Q == 19 && (kp.parity & 0x2u))
-            out[L::Q - 1] = -(T)L::w(L::Q - 1);   // fColl_18 stays at its zero initial value
+        if (L::Q == 19 && (kp.parity & 0x2u))   // fColl_18 stays at its zero initial value
+            out[L::Q - 1] = -(T)L::w(L::Q - 1) + (kp.mref ? (T)mref_off<L>(L::Q - 1) : (T)0);
         return;
     }
     collide_pairs<L, T, C>(kp, h, drho, rho, u, out, [](int i) { return (T)L::w(i); });

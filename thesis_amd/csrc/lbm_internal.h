@@ -40,6 +40,7 @@ struct KParams {
     int inlet;               // 0 zou-he, 1 reference 2D (west inlet and north lid)
     int outlet;              // 0 pressure, 1 reference velocity (2D), 2/3 extrapolation, 4 f_eq (3D)
     uint32_t parity;
+    int mref;                // D3Q19 MRT: populations stored as f - (w - r), r = 3 M^-1 e_eps
     double rho_out;
     uint32_t nf;             // non-solid nodes (compact population count)
     uint64_t mx, my;         // invariant-divisor multipliers for nx, ny

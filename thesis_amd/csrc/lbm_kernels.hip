@@ -462,7 +462,7 @@ __global__ __launch_bounds__(256) void k_recon(const typename Io<L, S>::raw* __r
     fl.v[idx] = (double)u[1]; fl.w[idx] = (double)u[2];
     if (!fl.fout) return;   // moments only
 #pragma unroll
-    for (int i = 0; i < L::Q; ++i) fl.fout[i * n + idx] = (double)h[i] + L::w(i);
+    for (int i = 0; i < L::Q; ++i) fl.fout[i * n + idx] = (double)h[i] + (L::w(i) - (kp.mref ? mref_off<L>(i) : 0.0));
 }
 
 // First post-collision state from an initial f and macroscopic fields.
@@ -504,6 +504,7 @@ __global__ __launch_bounds__(256) void k_init(typename Io<L, S>::raw* __restrict
             }
         }
     }
+    if (kp.mref) mref_shift<L, T>(h, 1);   // D3Q19 MRT stores f - (w - r)
     T out[L::Q];
     collide<L, T, C>(kp, h, drho, rho, u, out);
 #pragma unroll

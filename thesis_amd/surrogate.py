@@ -534,6 +534,11 @@ def main(argv=None):
         del X, Y, Mk
         net.save(os.path.join(a.out, "weights.npz"),
                  meta=dict(train=a.train, seed=a.train_seed, epochs=a.epochs, rho_weight=a.rho_weight))
+    if a.test == 0:
+        # training only (e.g. to profile the kernels of the training steps)
+        with open(os.path.join(a.out, "report.json"), "w") as f:
+            json.dump(report, f, indent=1, default=float)
+        return
     test = LT.surrogate_channels(a.test, a.test_seed)
     report["eval"] = evaluate(net, test, log=log)
     with open(os.path.join(a.out, "report.json"), "w") as f:
