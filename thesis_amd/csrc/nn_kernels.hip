@@ -5,6 +5,7 @@
 // is new work for the warm start of S/Iterate.cu:211-275 (initial fields).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "lbm_nn.h"
@@ -161,21 +162,40 @@ hipError_t gemm_shape(int M, int N, int R, const u16* A, int lda, const u16* B, 
 // ---------------------------------------------------------------------------
 // elementwise and data-movement kernels (NHWC, FP32 unless noted)
 
+// Quotient by an invariant divisor d (1 <= d < 2^31) for n < 2^32:
+// ((mulhi(m, n) + n) >> s) with s = ceil(log2 d), m = floor(2^32 (2^s - d) / d) + 1.
+struct Div {
+    uint32_t m, s;
+};
+inline Div make_div(uint32_t d) {
+    uint32_t s = 0;
+    while ((1ull << s) < d) ++s;
+    return Div{(uint32_t)(((1ull << 32) * ((1ull << s) - d)) / d + 1), s};
+}
+__device__ inline uint32_t qdiv(uint32_t n, Div d) {
+    return (uint32_t)(((uint64_t)__umulhi(d.m, n) + n) >> d.s);
+}
+
 // col[p][k], k = (ky*3 + kx)*C + c for k < 9C, zero for 9C <= k < Kp; zero padding
-// outside the image.  One thread per (pixel, k).
-__global__ void k_im2col(const float* __restrict__ x, int Bn, int H, int W, int C, int Kp, u16* __restrict__ col) {
-    const size_t total = (size_t)Bn * H * W * Kp;
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const int k = (int)(e % Kp);
-        const size_t p = e / Kp;
+// outside the image.  One thread per (pixel, k), consecutive threads along k
+// (coalesced stores); 32-bit indices with invariant-divisor quotients (the
+// launcher checks that every index fits).
+__global__ void k_im2col(const float* __restrict__ x, int H, int W, int C, int Kp, Div dKp, Div dC, Div dW, Div dH,
+                         uint32_t total, u16* __restrict__ col) {
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+        const uint32_t p = qdiv(e, dKp);
+        const int k = (int)(e - p * (uint32_t)Kp);
         float v = 0.f;
         if (k < 9 * C) {
-            const int tap = k / C, c = k - tap * C;
-            const int ky = tap / 3, kx = tap - ky * 3;
-            const int xw = (int)(p % W), yh = (int)((p / W) % H);
-            const size_t b = p / ((size_t)W * H);
+            const uint32_t tap = qdiv((uint32_t)k, dC);
+            const int c = k - (int)tap * C;
+            const int ky = (int)tap / 3, kx = (int)tap - ky * 3;
+            const uint32_t row = qdiv(p, dW);
+            const int xw = (int)(p - row * (uint32_t)W);
+            const uint32_t b = qdiv(row, dH);
+            const int yh = (int)(row - b * (uint32_t)H);
             const int ys = yh + ky - 1, xs = xw + kx - 1;
-            if (ys >= 0 && ys < H && xs >= 0 && xs < W) v = x[((b * H + ys) * W + xs) * C + c];
+            if (ys >= 0 && ys < H && xs >= 0 && xs < W) v = x[((size_t)(b * H + ys) * W + xs) * C + c];
         }
         col[e] = f2bf(v);
     }
@@ -183,12 +203,14 @@ __global__ void k_im2col(const float* __restrict__ x, int Bn, int H, int W, int 
 
 // dx[b][y][x][c] = sum over taps of dcol at the output pixel that read it.
 __global__ void k_col2im(const float* __restrict__ dcol, int Bn, int H, int W, int C, int Kp, float* __restrict__ dx) {
-    const size_t total = (size_t)Bn * H * W * C;
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const int c = (int)(e % C);
-        const size_t p = e / C;
-        const int xw = (int)(p % W), yh = (int)((p / W) % H);
-        const size_t b = p / ((size_t)W * H);
+    const uint32_t total = (uint32_t)Bn * H * W * C;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+        const uint32_t p = e / (uint32_t)C;
+        const int c = (int)(e - p * (uint32_t)C);
+        const uint32_t row = p / (uint32_t)W;
+        const int xw = (int)(p - row * (uint32_t)W);
+        const uint32_t b = row / (uint32_t)H;
+        const int yh = (int)(row - b * (uint32_t)H);
         float s = 0.f;
         for (int ky = 0; ky < 3; ++ky) {
             const int yo = yh - ky + 1;
@@ -196,7 +218,7 @@ __global__ void k_col2im(const float* __restrict__ dcol, int Bn, int H, int W, i
             for (int kx = 0; kx < 3; ++kx) {
                 const int xo = xw - kx + 1;
                 if (xo < 0 || xo >= W) continue;
-                s += dcol[((b * H + yo) * W + xo) * (size_t)Kp + (ky * 3 + kx) * C + c];
+                s += dcol[(size_t)((b * H + yo) * W + xo) * Kp + (ky * 3 + kx) * C + c];
             }
         }
         dx[e] = s;
@@ -215,23 +237,30 @@ __global__ void k_to_bf16(const float* __restrict__ x, size_t n, u16* __restrict
         y[e] = f2bf(x[e]);
 }
 
-// db[n] = sum_m dz[m][n] (dz FP32 with the relu mask applied through y)
+// db[n] += sum_m dz[m][n] (dz FP32 with the relu mask applied through y).
+// Each workgroup strides over rows with its 256 threads spread over the N
+// columns (consecutive threads read consecutive columns of a row), sums per
+// column in LDS and adds one FP32 atomic per column.
 __global__ void k_colsum(const float* __restrict__ dy, const float* __restrict__ y, int relu, int M, int N,
                          float* __restrict__ db) {
-    const int n = blockIdx.x;
+    const int rows = blockDim.x / N;          // rows per pass (N <= 256)
+    const int n = threadIdx.x % N, r = threadIdx.x / N;
     float s = 0.f;
-    for (int m = threadIdx.x; m < M; m += blockDim.x) {
-        const float g = dy[(size_t)m * N + n];
-        s += (relu && !(y[(size_t)m * N + n] > 0.f)) ? 0.f : g;
+    if (r < rows) {
+        for (int m = blockIdx.x * rows + r; m < M; m += gridDim.x * rows) {
+            const size_t i = (size_t)m * N + n;
+            const float g = dy[i];
+            s += (relu && !(y[i] > 0.f)) ? 0.f : g;
+        }
     }
     __shared__ float red[256];
     red[threadIdx.x] = s;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
+    if ((int)threadIdx.x < N) {
+        float t = 0.f;
+        for (int k = 0; k < rows; ++k) t += red[k * N + threadIdx.x];
+        atomicAdd(db + threadIdx.x, t);
     }
-    if (threadIdx.x == 0) db[n] += red[0];
 }
 
 __global__ void k_avgpool2(const float* __restrict__ x, int Bn, int H, int W, int C, float* __restrict__ y) {
@@ -417,11 +446,16 @@ int nn_gemm_bf16(int M, int N, int R, const void* A, int lda, int ta, const void
     } while (0)
 
 int nn_im2col3x3(const float* x, int Bn, int H, int W, int C, int Kp, void* col, void* stream) {
-    if (Kp < 9 * C) return 3;
-    NN_LAUNCH(k_im2col, (size_t)Bn * H * W * Kp, x, Bn, H, W, C, Kp, (u16*)col);
+    if (Kp < 9 * C || C < 1 || H < 1 || W < 1 || Bn < 0) return 3;
+    const uint64_t total = (uint64_t)Bn * H * W * Kp;
+    if (total >= (1ull << 31)) return 3;
+    if (total == 0) return 0;
+    NN_LAUNCH(k_im2col, (size_t)total, x, H, W, C, Kp, make_div((uint32_t)Kp), make_div((uint32_t)C),
+              make_div((uint32_t)W), make_div((uint32_t)H), (uint32_t)total, (u16*)col);
 }
 int nn_col2im3x3(const float* dcol, int Bn, int H, int W, int C, int Kp, float* dx, void* stream) {
     if (Kp < 9 * C) return 3;
+    if ((uint64_t)Bn * H * W * C >= (1ull << 32)) return 3;
     NN_LAUNCH(k_col2im, (size_t)Bn * H * W * C, dcol, Bn, H, W, C, Kp, dx);
 }
 int nn_grad_bf16(const float* dy, const float* y, int64_t n, int relu, void* dz, void* stream) {
@@ -431,8 +465,11 @@ int nn_to_bf16(const float* x, int64_t n, void* y, void* stream) {
     NN_LAUNCH(k_to_bf16, (size_t)n, x, (size_t)n, (u16*)y);
 }
 int nn_colsum(const float* dy, const float* y, int relu, int M, int N, float* db, void* stream) {
-    if (N <= 0) return 0;
-    hipLaunchKernelGGL(k_colsum, dim3(N), dim3(256), 0, (hipStream_t)stream, dy, y, relu, M, N, db);
+    if (N <= 0 || M <= 0) return 0;
+    if (N > 256) return 3;
+    const int rows = 256 / N;
+    const int blocks = (int)std::min<int64_t>(1024, ((int64_t)M + rows - 1) / rows);
+    hipLaunchKernelGGL(k_colsum, dim3(blocks), dim3(256), 0, (hipStream_t)stream, dy, y, relu, M, N, db);
     return hipGetLastError() == hipSuccess ? 0 : 4;
 }
 int nn_avgpool2(const float* x, int Bn, int H, int W, int C, float* y, void* stream) {
