@@ -90,6 +90,8 @@ enum { LBM_OUTLET_PRESSURE = 0, LBM_OUTLET_REF_VELOCITY = 1, LBM_OUTLET_EXTRAP2 
 /* reference quirks reproduced on request */
 #define LBM_PARITY_MRT_RHO2   0x1u /* 2D MRT m_eq with rho^2 (S/GpuCollision.cu:987-988) */
 #define LBM_PARITY_MRT_NO_F18 0x2u /* 3D MRT leaves f18 unwritten (S/GpuCollision.cu:1161) */
+#define LBM_PARITY_INLET_FROZEN 0x4u /* macros of velocity-face nodes keep the values lbm_init set:
+                                       the reference's UpdateInltOutl = no (S/GpuUpdateMacro.cu:150-151) */
 #define LBM_PARITY_OUTLET_V0  0x8u /* 2D v := 0 on outlet nodes (S/GpuUpdateMacro.cu:30) */
 #define LBM_PARITY_INLET3D_FLIP 0x10u /* 3D W/E velocity faces divide by 1 + G u (S/GpuBoundaries.cu:102,148) */
 

@@ -73,6 +73,39 @@ def test_reference_west_east_flip_f64(oracle, k):
     compare(gg, o, 60)
 
 
+@pytest.mark.parametrize("k", [0, 3, 5])
+def test_frozen_inlet_macros_f64(oracle, k):
+    """UpdateInltOutl = no (LBM_PARITY_INLET_FROZEN): velocity-face nodes keep
+    the macros lbm_init set and collide with them; also through lbm_set_f,
+    which leaves them frozen."""
+    vel, out, axis, g = PAIRS[k]
+    cls, U, per = open_box(vel, out, axis, g)
+    shape = cls.shape
+    rng = np.random.default_rng(20 + k)
+    r0 = 1 + 0.01 * rng.standard_normal(shape)
+    m0 = [0.02 * rng.standard_normal(shape) for _ in range(3)]
+    kw = dict(collision=L.TRT, omega=1 / 0.8, trt_magic=0.25, periodic=per, ubc=U[0], vbc=U[1], wbc=U[2],
+              outlet_model=L.OUTLET_EXTRAP1)
+    gg, o = pair(oracle, 19, shape, cls, None, parity=L.PARITY_INLET_FROZEN, **kw)
+    for s in (gg, o):
+        s.init(0, rho=r0, u=m0[0], v=m0[1], w=m0[2])
+    compare(gg, o, 40)
+    face = cls == vel
+    got = gg.macros()
+    for a, b in zip(got, [r0] + m0):
+        assert np.array_equal(a[face], b[face])
+    f = o.get_f()
+    gg.set_f(f)
+    o.set_f(f)
+    compare(gg, o, 20)
+    # without the flag the face's macros are its moments (u = u_bc)
+    free = L.Solver(19, shape, cls, None, **kw)
+    free.init(0, rho=r0, u=m0[0], v=m0[1], w=m0[2])
+    free.step(40)
+    assert np.abs(free.macros()[1][face] - U[0][face]).max() < 1e-13
+    assert np.abs(got[1][face] - U[0][face]).max() > 1e-3
+
+
 def lid_cavity():
     paths = []
     import tempfile
@@ -228,3 +261,28 @@ def test_lbmsolver_reference_lid_cavity(tmp_path):
     assert np.abs(ml[:, 3] - w3[15, 15, :] / 0.05).max() < 1e-7
     plane = np.loadtxt(tmp_path / "Results" / "channelPlane.txt", skiprows=2)
     assert plane.shape == (900, 6) and np.abs(plane[:, 3] - u3[:, :, 22].ravel()).max() < 1e-9
+
+
+def test_lbmsolver_update_inlet_outlet_no(tmp_path):
+    """UpdateInltOutl = no freezes the lid's macros (S/GpuUpdateMacro.cu:150-151):
+    lbmsolver equals the ABI run with LBM_PARITY_INLET_FROZEN and differs
+    from the run without it."""
+    print("case: lbmsolver lidvalidation UpdateInltOutl = no", flush=True)
+    cls, bb, U, paths = lid_cavity()
+    os.makedirs(tmp_path / "Mesh")
+    for p in paths:
+        os.replace(p, tmp_path / "Mesh" / os.path.basename(p))
+    (tmp_path / "SetUpData.ini").write_text(INI3.replace("UpdateInltOutl = yes", "UpdateInltOutl = no"))
+    r = subprocess.run([CLI, "-f", "SetUpData.ini"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-300:], r.stderr[-300:])
+    assert r.returncode == 0
+    data = np.loadtxt(tmp_path / "Results" / "FinalData.csv", delimiter=",", skiprows=1)
+    us = {}
+    for parity in (L.PARITY_INLET_FROZEN, 0):
+        s = L.Solver(19, cls.shape, cls, bb, collision=L.BGK, omega=1 / (3 * 0.1 + 0.5), wall_model=L.WALL_HH,
+                     ubc=U[0], vbc=U[1], wbc=U[2], parity=parity)
+        s.init(1, rho=np.ones(cls.shape), u=np.full(cls.shape, 0.05), v=np.zeros(cls.shape), w=np.zeros(cls.shape))
+        s.step(300)
+        us[parity] = s.macros()[1].ravel()
+    assert np.abs(data[:, 3] - us[L.PARITY_INLET_FROZEN]).max() < 1e-9
+    assert np.abs(data[:, 3] - us[0]).max() > 1e-6

@@ -66,6 +66,35 @@ def test_velocity_face_imposes_u_and_normal_bounce_back(oracle, face):
         assert abs((h[nin] - e[nin]) - (h[nout] - e[nout])) < 1e-15
 
 
+@pytest.mark.parametrize("face", ["VEL_W", "VEL_N", "VEL_T"])
+def test_frozen_inlet_collides_with_initial_macros(oracle, face):
+    """UpdateInltOutl = no (S/GpuUpdateMacro.cu:150-151): the 3D macro update
+    skips velocity-face nodes, which keep their initial macros and collide
+    with them.  BGK at omega = 1 makes the post-collision populations
+    f_eq(rho, u) of the macros the collision reads, and streaming leaves the
+    rest population in place, so after the second step f_0 = w_0 rho
+    (1 - 1.5 |u|^2) with (rho, u) the initial macros under the flag and the
+    face's own moments of the first step (rho_1, u_bc) without it."""
+    rho0, u0, ubc = 1.02, np.array([0.01, -0.02, 0.015]), np.array([0.04, 0.01, -0.02])
+    init = dict(rho=np.full(27, rho0), u=np.full(27, u0[0]), v=np.full(27, u0[1]), w=np.full(27, u0[2]))
+    res = {}
+    for frozen in (True, False):
+        s, ind = node(oracle, getattr(oracle, face), u=ubc, collision=0, omega=1.0,
+                      parity=oracle.PARITY_INLET_FROZEN if frozen else 0)
+        s.init(0, **init)
+        s.step(1)
+        m1 = [a.ravel()[ind] for a in s.macros()]
+        s.step(1)
+        m2 = [a.ravel()[ind] for a in s.macros()]
+        res[frozen] = (m1, m2, s.get_f()[0].ravel()[ind])
+    m1, m2, f0 = res[True]
+    assert m1 == m2 == [rho0, *u0]
+    assert abs(f0 - W[0] * rho0 * (1 - 1.5 * u0 @ u0)) < 1e-15
+    m1, m2, f0 = res[False]
+    assert np.abs(np.array(m1[1:]) - ubc).max() < 1e-14 and abs(m1[0] - rho0) > 1e-5
+    assert abs(f0 - W[0] * m1[0] * (1 - 1.5 * ubc @ ubc)) < 1e-15
+
+
 @pytest.mark.parametrize("face", list(FACES))
 def test_reference_west_east_density_flip(oracle, face):
     """With the parity flag the W and E faces divide by 1 + g u_a, so the node

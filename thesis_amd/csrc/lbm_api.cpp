@@ -49,6 +49,7 @@ struct lbm_ctx {
     double *f1 = nullptr, *fprev = nullptr;   // FdRelDiff: f at the first check and at the last one
     bool fdrel_started = false;
     double* om[2] = {nullptr, nullptr};   // f_eq outlet: macros of even / odd steps (4 N each)
+    double* frz = nullptr;        // LBM_PARITY_INLET_FROZEN: macros at lbm_init (4 N)
     double* partials = nullptr;
     double* out4 = nullptr;       // 8 doubles: the last reduction
     int64_t bytes = 0;
@@ -92,6 +93,7 @@ Fields fields(lbm_ctx* c) {
     f.fout = c->fout;
     f.om_in = nullptr;
     f.om_out = nullptr;
+    f.frz = c->frz;
     return f;
 }
 
@@ -481,6 +483,16 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     }
     if (st == LBM_OK && any_outlet && p->q == 19 && p->outlet_model == LBM_OUTLET_FEQ)
         for (int b = 0; b < 2 && st == LBM_OK; ++b) st = dalloc(c, (void**)&c->om[b], sizeof(double) * 4 * n);
+    if (st == LBM_OK && any_open_cls && (p->parity_flags & LBM_PARITY_INLET_FROZEN)) {
+        // until lbm_init sets them, the frozen macros are the rest state
+        st = dalloc(c, (void**)&c->frz, sizeof(double) * 4 * n);
+        if (st == LBM_OK) {
+            std::vector<double> rest(4 * n, 0.0);
+            std::fill(rest.begin(), rest.begin() + n, 1.0);
+            if (hipMemcpy(c->frz, rest.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice) != hipSuccess)
+                st = fail(c, LBM_EDEVICE, "cannot seed the frozen inlet macros");
+        }
+    }
     if (st == LBM_OK) st = dalloc(c, (void**)&c->partials, sizeof(double) * 8 * reduce_blocks());
     if (st == LBM_OK) st = dalloc(c, (void**)&c->out4, sizeof(double) * 8);
     if (st != LBM_OK) {
@@ -537,7 +549,7 @@ void lbm_destroy(lbm_ctx* c) {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
     void* bufs[] = {c->Palloc[0], c->Palloc[1], c->node, c->cidx, c->didx, c->nodew, c->rowmeta, c->blk, c->rho, c->u, c->v, c->w, c->up, c->vp, c->wp,
-                    c->rp, c->qslot, c->qv, c->ubc, c->vbc, c->wbc, c->fout, c->f1, c->fprev, c->om[0], c->om[1], c->partials, c->out4};
+                    c->rp, c->qslot, c->qv, c->ubc, c->vbc, c->wbc, c->fout, c->f1, c->fprev, c->om[0], c->om[1], c->frz, c->partials, c->out4};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -558,6 +570,11 @@ lbm_status lbm_init(lbm_ctx* c, int mode, const double* rho, const double* u, co
     for (int a = 0; a < 3; ++a) {
         if (src[a]) HIPCHK(c, hipMemcpyAsync(dst[a], src[a], b, hipMemcpyHostToDevice, c->stream));
         else HIPCHK(c, hipMemsetAsync(dst[a], 0, b, c->stream));
+    }
+    if (c->frz) {
+        const double* mac[4] = {c->rho, c->u, c->v, c->w};
+        for (int a = 0; a < 4; ++a)
+            HIPCHK(c, hipMemcpyAsync(c->frz + a * n, mac[a], b, hipMemcpyDeviceToDevice, c->stream));
     }
     HIPCHK(c, launch_init(c->q, c->p.storage, c->p.collision, mode == 0 ? INIT_EQ : INIT_ZERO, c->kp,
                           c->P[0], c->node, c->cidx, fields(c), c->stream));

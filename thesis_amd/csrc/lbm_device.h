@@ -94,33 +94,143 @@ __host__ __device__ constexpr double mref_off(int i) {
     return L::Q == 19 ? 3.0 * mrow<L>(2, i) / mnorm2<L>(2) : 0.0;
 }
 
-// Compile-time coefficients of M and Minv; the products below are expanded by
-// fold expressions so every zero drops out and every +-1 becomes an add.
-template <class L, int R, int I> struct Mc {
-    static constexpr double m = mrow<L>(R, I);
-    static constexpr double mi = mrow<L>(R, I) / mnorm2<L>(R);
-};
-template <class L, class T, int R, int... I>
-__device__ inline T m_row_dot(const T* h, std::integer_sequence<int, I...>) {
-#pragma clang fp contract(on)
+// The MRT transforms in link-pair form.  Every moment polynomial is even or
+// odd under c -> -c, so an even row of M weighs h_i and h_opp(i) alike and an
+// odd row with opposite signs: with s = h_i + h_opp and a = h_i - h_opp per
+// link pair, even moments are rows over (h_0, s) and odd ones rows over a, and
+// the way back gives each pair E +- O from the even and odd rows of M^-1.
+// Coefficients are compile-time constants; zero ones drop out and +-1 ones
+// become adds, about half the operations of the full 19 x 19 products.
+template <class L>
+__host__ __device__ constexpr int pair_rep(int k) {   // k-th link i with i < opp(i)
+    for (int i = 1; i < L::Q; ++i)
+        if (i < L::opp(i) && k-- == 0) return i;
+    return -1;
+}
+template <class L>
+__host__ __device__ constexpr int mparity(int r) {    // +1 even row, -1 odd row, 0 neither
+    bool even = true, odd = true;
+    for (int i = 0; i < L::Q; ++i) {
+        even = even && mrow<L>(r, L::opp(i)) == mrow<L>(r, i);
+        odd = odd && mrow<L>(r, L::opp(i)) == -mrow<L>(r, i);
+    }
+    return even ? 1 : odd ? -1 : 0;
+}
+template <class L>
+__host__ __device__ constexpr bool mrows_have_parity() {
+    for (int r = 0; r < L::Q; ++r)
+        if (mparity<L>(r) == 0) return false;
+    return pair_rep<L>((L::Q - 1) / 2 - 1) > 0 && pair_rep<L>((L::Q - 1) / 2) < 0;
+}
+// Moments of the Guo source S_i = w_i (3 (c_i.F - u.F) + 9 (c_i.u)(c_i.F)):
+// (M S)_r = 3 sum_d A_rd F_d - 3 (u.F) B_r + 9 sum_de C_rde u_d F_e.
+template <class L>
+__host__ __device__ constexpr double cc(int i, int d) { return d == 0 ? L::cx(i) : d == 1 ? L::cy(i) : L::cz(i); }
+template <class L>
+__host__ __device__ constexpr double guoA(int r, int d) {
+    double t = 0;
+    for (int i = 0; i < L::Q; ++i) t += mrow<L>(r, i) * L::w(i) * cc<L>(i, d);
+    return t;
+}
+template <class L>
+__host__ __device__ constexpr double guoC(int r, int d, int e) {
+    double t = 0;
+    for (int i = 0; i < L::Q; ++i) t += mrow<L>(r, i) * L::w(i) * cc<L>(i, d) * cc<L>(i, e);
+    return t;
+}
+
+// The coefficients as template constants, so that each is evaluated at
+// compile time (a constexpr call in an ordinary expression need not be).
+template <class L, int K> inline constexpr int kPair = pair_rep<L>(K);
+template <class L, int R> inline constexpr int kPar = mparity<L>(R);
+template <class L, int R, int I> inline constexpr double kM = mrow<L>(R, I);
+template <class L, int R, int I> inline constexpr double kMi = mrow<L>(R, I) / mnorm2<L>(R);
+template <class L, int R, int D> inline constexpr double kGA = 3.0 * guoA<L>(R, D);
+template <class L, int R> inline constexpr double kGB = -3.0 * mrest<L>(R);
+template <class L, int R, int D, int E> inline constexpr double kGC = 9.0 * guoC<L>(R, D, E);
+template <class L, int R>
+inline constexpr bool kGuoRow = kGB<L, R> != 0.0 || kGA<L, R, 0> != 0.0 || kGA<L, R, 1> != 0.0 ||
+                                kGA<L, R, 2> != 0.0 || kGC<L, R, 0, 0> != 0.0 || kGC<L, R, 0, 1> != 0.0 ||
+                                kGC<L, R, 0, 2> != 0.0 || kGC<L, R, 1, 0> != 0.0 || kGC<L, R, 1, 1> != 0.0 ||
+                                kGC<L, R, 1, 2> != 0.0 || kGC<L, R, 2, 0> != 0.0 || kGC<L, R, 2, 1> != 0.0 ||
+                                kGC<L, R, 2, 2> != 0.0;
+
+// acc (+)= c x for compile-time c, skipping zero coefficients; the first
+// term starts the sum (no 0 + ...), so the operation count is the nonzero count
+template <class T>
+struct CSum {
     T acc = 0;
-    ((Mc<L, R, I>::m != 0.0 ? (void)(acc += (T)Mc<L, R, I>::m * h[I]) : (void)0), ...);
-    return acc;
+    bool any = false;
+    __device__ __forceinline__ void add(double c, T x) {
+#pragma clang fp contract(on)
+        if (c == 0.0) return;
+        acc = any ? acc + (T)c * x : (T)c * x;
+        any = true;
+    }
+};
+
+template <class L, class T, int... K>
+__device__ __forceinline__ void pair_split(const T* h, T* s, T* a, std::integer_sequence<int, K...>) {
+    ((s[K] = h[kPair<L, K>] + h[L::opp(kPair<L, K>)]), ...);
+    ((a[K] = h[kPair<L, K>] - h[L::opp(kPair<L, K>)]), ...);
+}
+template <class L, class T, int R, int... K>
+__device__ __forceinline__ T m_row_pairs(T h0, const T* s, const T* a, std::integer_sequence<int, K...>) {
+    CSum<T> m;
+    if constexpr (kPar<L, R> > 0) {
+        m.add(kM<L, R, 0>, h0);
+        (m.add(kM<L, R, kPair<L, K>>, s[K]), ...);
+    } else {
+        (m.add(kM<L, R, kPair<L, K>>, a[K]), ...);
+    }
+    return m.acc;
+}
+template <class L, class T, int R>
+__device__ __forceinline__ T guo_row(const T* u, const T* F, T uF) {
+    CSum<T> g;
+    g.add(kGA<L, R, 0>, F[0]); g.add(kGA<L, R, 1>, F[1]); g.add(kGA<L, R, 2>, F[2]);
+    g.add(kGB<L, R>, uF);
+    g.add(kGC<L, R, 0, 0>, u[0] * F[0]); g.add(kGC<L, R, 0, 1>, u[0] * F[1]); g.add(kGC<L, R, 0, 2>, u[0] * F[2]);
+    g.add(kGC<L, R, 1, 0>, u[1] * F[0]); g.add(kGC<L, R, 1, 1>, u[1] * F[1]); g.add(kGC<L, R, 1, 2>, u[1] * F[2]);
+    g.add(kGC<L, R, 2, 0>, u[2] * F[0]); g.add(kGC<L, R, 2, 1>, u[2] * F[1]); g.add(kGC<L, R, 2, 2>, u[2] * F[2]);
+    return g.acc;
+}
+// relaxed moment deviations y_r = s_r (m_r - m_eq,r) - (1 - s_r / 2) (M S)_r
+template <class L, class T, int R>
+__device__ __forceinline__ T mrt_relax_row(const KParams& kp, T h0, const T* s, const T* a, const T* dm, bool guo,
+                                           const T* u, const T* F, T uF) {
+#pragma clang fp contract(on)
+    const T sr = (T)kp.s[R];
+    T y = sr * (m_row_pairs<L, T, R>(h0, s, a, std::make_integer_sequence<int, (L::Q - 1) / 2>{}) - dm[R]);
+    if constexpr (kGuoRow<L, R>) {
+        if (guo) y -= ((T)1 - (T)0.5 * sr) * guo_row<L, T, R>(u, F, uF);
+    }
+    return y;
 }
 template <class L, class T, int... R>
-__device__ inline void m_apply(const T* h, T* m, std::integer_sequence<int, R...>) {
-    ((m[R] = m_row_dot<L, T, R>(h, std::make_integer_sequence<int, L::Q>{})), ...);
+__device__ __forceinline__ void mrt_relax(const KParams& kp, T h0, const T* s, const T* a, const T* dm, bool guo,
+                                          const T* u, const T* F, T uF, T* y, std::integer_sequence<int, R...>) {
+    ((y[R] = mrt_relax_row<L, T, R>(kp, h0, s, a, dm, guo, u, F, uF)), ...);
 }
-template <class L, class T, int I, int... R>
-__device__ inline T mi_col_dot(const T* m, std::integer_sequence<int, R...>) {
+template <class L, class T, int I, int P, int... R>
+__device__ __forceinline__ T mi_part(const T* y, std::integer_sequence<int, R...>) {
+    CSum<T> d;
+    (d.add(kPar<L, R> == P ? kMi<L, R, I> : 0.0, y[R]), ...);
+    return d.acc;
+}
+template <class L, class T, int K>
+__device__ __forceinline__ void mrt_back_pair(const T* h, const T* y, T* out) {
 #pragma clang fp contract(on)
-    T acc = 0;
-    ((Mc<L, R, I>::mi != 0.0 ? (void)(acc += (T)Mc<L, R, I>::mi * m[R]) : (void)0), ...);
-    return acc;
+    constexpr int i = kPair<L, K>, o = L::opp(i);
+    constexpr auto rows = std::make_integer_sequence<int, L::Q>{};
+    const T E = mi_part<L, T, i, 1>(y, rows), O = mi_part<L, T, i, -1>(y, rows);
+    out[i] = h[i] - (E + O);
+    out[o] = h[o] - (E - O);
 }
-template <class L, class T, int... I>
-__device__ inline void mi_apply(const T* m, T* out, std::integer_sequence<int, I...>) {
-    ((out[I] = mi_col_dot<L, T, I>(m, std::make_integer_sequence<int, L::Q>{})), ...);
+template <class L, class T, int... K>
+__device__ __forceinline__ void mrt_back(const T* h, const T* y, T* out, std::integer_sequence<int, K...>) {
+    out[0] = h[0] - mi_part<L, T, 0, 1>(y, std::make_integer_sequence<int, L::Q>{});
+    (mrt_back_pair<L, T, K>(h, y, out), ...);
 }
 
 // ---------------------------------------------------------------------------
@@ -825,6 +935,21 @@ __device__ inline void moments(const KParams& kp, uint32_t cls, const T* h, T& d
     moments<L, T>(kp, cls, h, drho, rho, u, [](int) { return (T)1; });
 }
 
+// Velocity-face nodes with frozen macros (fl.frz set): the collision and the
+// macros written use the stored values in place of the node's moments, as
+// the reference's 3D macro update skips inlet nodes when UpdateInltOutl = no
+// (S/GpuUpdateMacro.cu:150-151).
+template <class T>
+__device__ inline void frozen_macros(const KParams& kp, const Fields& fl, uint32_t cls, size_t idx, T& drho, T& rho,
+                                     T* u) {
+    if (!fl.frz || !(cls == VEL_W || cls == VEL_N || (cls >= VEL_E && cls <= VEL_T))) return;
+    const size_t n = (size_t)kp.nx * kp.ny * kp.nz;
+    const double r = fl.frz[idx];
+    drho = (T)(r - 1.0);
+    rho = (T)r;
+    u[0] = (T)fl.frz[n + idx]; u[1] = (T)fl.frz[2 * n + idx]; u[2] = (T)fl.frz[3 * n + idx];
+}
+
 // c_i . (a, b, c) with the zero components of c_i left out: a product with
 // zero is not folded away by the compiler (0 * x is not 0 for every float x),
 // and the result is the same for finite operands.
@@ -845,23 +970,10 @@ __device__ inline T cdot(int i, T a, T b, T c) {
     return r;
 }
 
-template <class L, class T>
-__device__ inline void guo_terms(const KParams& kp, T rho, const T* u, T* S) {
-#pragma clang fp contract(on)
-    const T Fx = rho * kv<T>(kp.g[0], kp.gf[0]), Fy = rho * kv<T>(kp.g[1], kp.gf[1]), Fz = rho * kv<T>(kp.g[2], kp.gf[2]);
-    const T uF = u[0] * Fx + u[1] * Fy + u[2] * Fz;
-#pragma unroll
-    for (int i = 0; i < L::Q; ++i) {
-        const T cF = cdot<L, T>(i, Fx, Fy, Fz);
-        const T cu = cdot<L, T>(i, u[0], u[1], u[2]);
-        S[i] = (T)L::w(i) * ((T)3 * (cF - uF) + (T)9 * cu * cF);
-    }
-}
-
 // Deviations of the MRT equilibrium moments from the rest moments (M w).
 // D2Q9: S/GpuCollision.cu:986-994; D3Q19: :1065-1083.
 template <class L, class T>
-__device__ inline void mrt_dmeq(const KParams& kp, T drho, T rho, const T* u, T* dm) {
+__device__ __forceinline__ void mrt_dmeq(const KParams& kp, T drho, T rho, const T* u, T* dm) {
 #pragma clang fp contract(on)
     if (L::Q == 9) {
         const T uu = u[0] * u[0] + u[1] * u[1];
@@ -899,25 +1011,23 @@ template <class L, class T, int C, class Wt>
 __device__ inline void collide_pairs(const KParams& kp, const T* h, T drho, T rho, const T* u, T* out, Wt wt);
 
 template <class L, class T, int C>
-__device__ inline void collide(const KParams& kp, const T* h, T drho, T rho, const T* u, T* out) {
+__device__ __forceinline__ void collide(const KParams& kp, const T* h, T drho, T rho, const T* u, T* out) {
 #pragma clang fp contract(on)
     const bool guo = kp.force == 2;
     if (C == 2) {
-        T S[L::Q];
-        if (guo) guo_terms<L, T>(kp, rho, u, S);
-        T m[L::Q], dm[L::Q], gs[L::Q], dd[L::Q];
+        static_assert(mrows_have_parity<L>(), "MRT rows are even or odd, links come in pairs");
+        constexpr int NP = (L::Q - 1) / 2;
+        T s[NP], a[NP];
+        pair_split<L, T>(h, s, a, std::make_integer_sequence<int, NP>{});
+        T dm[L::Q], y[L::Q], F[3] = {0, 0, 0}, uF = 0;
         mrt_dmeq<L, T>(kp, drho, rho, u, dm);
-        m_apply<L, T>(h, m, std::make_integer_sequence<int, L::Q>{});
-        if (guo) m_apply<L, T>(S, gs, std::make_integer_sequence<int, L::Q>{});
-#pragma unroll
-        for (int r = 0; r < L::Q; ++r) {
-            const T s = (T)kp.s[r];
-            m[r] = s * (m[r] - dm[r]);
-            if (guo) m[r] -= ((T)1 - (T)0.5 * s) * gs[r];
+        if (guo) {
+            F[0] = rho * kv<T>(kp.g[0], kp.gf[0]); F[1] = rho * kv<T>(kp.g[1], kp.gf[1]);
+            F[2] = rho * kv<T>(kp.g[2], kp.gf[2]);
+            uF = u[0] * F[0] + u[1] * F[1] + u[2] * F[2];
         }
-        mi_apply<L, T>(m, dd, std::make_integer_sequence<int, L::Q>{});
-#pragma unroll
-        for (int i = 0; i < L::Q; ++i) out[i] = h[i] - dd[i];
+        mrt_relax<L, T>(kp, h[0], s, a, dm, guo, u, F, uF, y, std::make_integer_sequence<int, L::Q>{});
+        mrt_back<L, T>(h, y, out, std::make_integer_sequence<int, NP>{});
         if (L::Q == 19 && (kp.parity & 0x2u))   // fColl_18 stays at its zero initial value
             out[L::Q - 1] = -(T)L::w(L::Q - 1) + (kp.mref ? (T)mref_off<L>(L::Q - 1) : (T)0);
         return;

@@ -83,6 +83,9 @@ struct Fields {              // FP64 side arrays (N each)
     // read by this step, and the block this step writes (null: none)
     const double* om_in;
     double* om_out;
+    // frozen macros of velocity-face nodes (rho, u, v, w blocks of N; null:
+    // none): their collision and written macros use these, not the moments
+    const double* frz;
 };
 
 // Launchers; q in {9,19}; storage 0..3; collision 0..2.

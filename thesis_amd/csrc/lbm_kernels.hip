@@ -109,6 +109,7 @@ __global__ __launch_bounds__(256) void k_step(const typename Io<L, S>::raw* __re
     }
     T drho, rho, u[3];
     moments<L, T>(kp, cls, h, drho, rho, u);
+    if (OPEN) frozen_macros<T>(kp, fl, cls, idx, drho, rho, u);
     if (write_macros) {
         fl.rho[idx] = (double)rho; fl.u[idx] = (double)u[0];
         fl.v[idx] = (double)u[1]; fl.w[idx] = (double)u[2];
@@ -403,6 +404,7 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
     if (OPEN && L::Q == 9 && cls == PRES_E) zou_outlet<L, T>(kp, fl, idx, h);
     T drho, rho, u[3];
     moments<L, T>(kp, cls, h, drho, rho, u);
+    if (OPEN) frozen_macros<T>(kp, fl, cls, idx, drho, rho, u);
     if (write_macros) {
         fl.rho[idx] = (double)rho; fl.u[idx] = (double)u[0];
         fl.v[idx] = (double)u[1]; fl.w[idx] = (double)u[2];
@@ -458,6 +460,7 @@ __global__ __launch_bounds__(256) void k_recon(const typename Io<L, S>::raw* __r
     post_boundary<L, S, OPEN>(src, kp, fl, node, cidx, own, nd, x, y, z, idx, h);
     T drho, rho, u[3];
     moments<L, T>(kp, cls, h, drho, rho, u);
+    if (OPEN) frozen_macros<T>(kp, fl, cls, idx, drho, rho, u);
     fl.rho[idx] = (double)rho; fl.u[idx] = (double)u[0];
     fl.v[idx] = (double)u[1]; fl.w[idx] = (double)u[2];
     if (!fl.fout) return;   // moments only
@@ -486,6 +489,7 @@ __global__ __launch_bounds__(256) void k_init(typename Io<L, S>::raw* __restrict
 #pragma unroll
         for (int i = 0; i < L::Q; ++i) h[i] = (T)(fl.fout[i * n + idx] - L::w(i));
         moments<L, T>(kp, cls, h, drho, rho, u);
+        frozen_macros<T>(kp, fl, cls, idx, drho, rho, u);
         fl.rho[idx] = (double)rho; fl.u[idx] = (double)u[0];
         fl.v[idx] = (double)u[1]; fl.w[idx] = (double)u[2];
     } else {

@@ -115,7 +115,8 @@ bool read_ini(const std::string& path, Args& a) {
     a.autosave_every = atol(v[13].c_str());
     a.residual_model = trim(v[18]);
     a.boundary_id = atoi(v[21].c_str());
-    a.update_inlet_outlet = has(v[22], "yes");
+    // neither "yes" nor "no" keeps the default, yes (FilesReading.cu:211-219)
+    a.update_inlet_outlet = has(v[22], "yes") || !has(v[22], "no");
     a.g = atof(v[20].c_str());
     // optional section after line 23
     std::map<std::string, std::string> kv;
@@ -794,6 +795,9 @@ int main(int argc, char** argv) {
                 }
     }
     p.inlet_model = a.inlet_model;
+    // UpdateInltOutl = no: the 3D macro update leaves inlet nodes alone, so
+    // their collision keeps the initial macros (S/GpuUpdateMacro.cu:150-151)
+    if (three_d && !a.update_inlet_outlet) p.parity_flags |= LBM_PARITY_INLET_FROZEN;
     if (!three_d) {
         p.outlet_model = a.outlet == 1 ? LBM_OUTLET_REF_VELOCITY : a.outlet == 2 ? LBM_OUTLET_EXTRAP2
                        : a.outlet == 3 ? LBM_OUTLET_EXTRAP1 : LBM_OUTLET_PRESSURE;

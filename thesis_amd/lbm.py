@@ -26,6 +26,7 @@ WALL_HALFWAY, WALL_ONNODE, WALL_HH, WALL_CURVED = 0, 1, 2, 3
 INLET_ZOUHE, INLET_REF2D = 0, 1
 OUTLET_PRESSURE, OUTLET_REF_VELOCITY, OUTLET_EXTRAP2, OUTLET_EXTRAP1, OUTLET_FEQ = 0, 1, 2, 3, 4
 PARITY_MRT_RHO2, PARITY_MRT_NO_F18, PARITY_OUTLET_V0, PARITY_INLET3D_FLIP = 0x1, 0x2, 0x8, 0x10
+PARITY_INLET_FROZEN = 0x4
 
 STATUS = {0: "OK", 1: "DIVERGED", 2: "EIO", 3: "EINVAL", 4: "EDEVICE", 5: "ENOMEM", 6: "EUNSUPPORTED"}
 STORAGE_NAMES = {"f64": STORE_F64, "f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16}
