@@ -510,9 +510,9 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     k.simple = simple ? 1 : 0;
     k.qslot = c->qslot;
     k.qv = c->qv;
-    // the rank step addresses the tile layout only, and has no extrapolating
-    // outlet, 3D face or wet-node wall
-    const bool rank_fits = rank_ok && k.tile && !(any_outlet && p->outlet_model >= LBM_OUTLET_EXTRAP2) &&
+    // the rank step addresses the tile layout (SoA planes only in its 16-bit
+    // periodic variant), and has no extrapolating outlet, 3D face or wet-node wall
+    const bool rank_fits = rank_ok && (k.tile || (c->raw_bytes == 2 && simple && !c->open)) && !(any_outlet && p->outlet_model >= LBM_OUTLET_EXTRAP2) &&
                            !any_face3 && p->wall_model != LBM_WALL_HH && p->wall_model != LBM_WALL_CURVED;
     k.rank = rank_fits ? 1 : 0;
     if (const char* e = std::getenv("LBM_RANK")) k.rank = rank_fits && e[0] == '1';

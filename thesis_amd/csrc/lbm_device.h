@@ -369,6 +369,14 @@ __device__ inline uint32_t paddr(const KParams& kp, int i, uint32_t j) {
     return (uint32_t)i * kp.qstride32 + j;
 }
 
+// Largest |opp(i) - i|: how far a link's opposite plane lies from its own.
+template <class L>
+__host__ __device__ constexpr int max_opp_gap() {
+    int g = 0;
+    for (int i = 0; i < L::Q; ++i) g = L::opp(i) - i > g ? L::opp(i) - i : (i - L::opp(i) > g ? i - L::opp(i) : g);
+    return g;
+}
+
 // Tile-layout slot of plane i of compact node j.  j >> 6 < 2^24 whenever the
 // buffer is below 4 GiB, so the tile stride is a full-rate 24-bit multiply.
 template <class L>

@@ -28,19 +28,22 @@ namespace lbm {
 // stored deviation still accumulates), with random bits from a hash of the
 // compact node index and a per-launch seed, rotated per direction; results are
 // deterministic for a given seed sequence.
-// TILE: the buffer is known to use the tile layout (no runtime layout test).
+// LAY: the buffer's layout, 0 tested at run time (kp.tile), 1 known to be
+// tiles, 2 known to be SoA planes.
 // X: out holds the stored values x_i = h_i / a_i of a 16-bit storage (no
 // scaling before the conversion).
-template <class L, int S, bool TILE = false, bool X = false>
+template <class L, int S, int LAY = 0, bool X = false>
 __device__ inline void store_node(typename Io<L, S>::raw* __restrict__ dst, const KParams& kp, uint32_t own,
                                   const typename Io<L, S>::T* out) {
     using raw = typename Io<L, S>::raw;
     // With the tile layout a node's populations are 64 elements apart: one
     // 64-bit base, and each store's offset is a constant (an immediate of the
-    // store instruction rather than an add per population).
-    raw* const tb = TILE ? (raw*)((char*)dst + (size_t)tslot<L>(0, own) * sizeof(raw)) : dst;
-    auto base = [&](int i) { return TILE ? tb : dst; };
-    auto slot = [&](int i) { return TILE ? 64u * (uint32_t)i : paddr<L>(kp, i, own); };
+    // store instruction rather than an add per population).  With known SoA
+    // planes each plane has its own (uniform) base and every store the same
+    // offset, the node's index.
+    raw* const tb = LAY == 1 ? (raw*)((char*)dst + (size_t)tslot<L>(0, own) * sizeof(raw)) : dst;
+    auto base = [&](int i) { return LAY == 1 ? tb : LAY == 2 ? dst + (size_t)i * kp.qstride : dst; };
+    auto slot = [&](int i) { return LAY == 1 ? 64u * (uint32_t)i : LAY == 2 ? own : paddr<L>(kp, i, own); };
     if constexpr (S >= 2) {
         if (kp.sr) {
             uint32_t h = own * 0x9E3779B1u + kp.seed;
@@ -135,7 +138,7 @@ __global__ __launch_bounds__(256) void k_step(const typename Io<L, S>::raw* __re
 
 constexpr uint32_t kSolidSrc = 0xFFFFFFFEu;
 
-template <class L, int S, int C, bool OPEN, bool SIMPLE>
+template <class L, int S, int C, bool OPEN, bool SIMPLE, bool SOA = false>
 __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __restrict__ src,
                                                typename Io<L, S>::raw* __restrict__ dst,
                                                const uint2* __restrict__ rowmeta,
@@ -159,6 +162,11 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
     // base pointer of their loads sits that many nodes before the buffer)
     constexpr bool kShifted = sizeof(typename Io<L, S>::raw) == 2 && SIMPLE && !OPEN;
     constexpr uint32_t kSh = kShifted ? kRankShift : 0u;
+    // kShifted on SoA planes: the counts run 3 planes ahead instead (every
+    // opposite plane lies within 3 planes of its link's own)
+    constexpr bool kSoA = kShifted && SOA;
+    static_assert(!SOA || kShifted, "the SoA variant is the 16-bit periodic one");
+    const uint32_t kShv = kSoA ? 3u * kp.qstride32 : kSh;
     const int tot = (R * 9) << lg;
     const uint32_t own0 = bid * blockDim.x;
     uint32_t own, r;
@@ -191,7 +199,7 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
             const int kk = slot / 9, nb = slot - 9 * kk;
             uint2 val = make_uint2(0u, 0u);
             if (w < W) val = rowmeta[rowoff[slot] + (uint32_t)w];
-            val.y += kSh;
+            val.y += kShv;
             smeta[((kk << lg) + w) * 9 + nb] = val;
         }
         __syncthreads();
@@ -256,7 +264,7 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
     constexpr bool kNarrow = sizeof(raw) == 2;
     T h[L::Q];
     raw hr[L::Q];   // 16-bit storage: the stored values as loaded
-    const uint32_t oslot = tslot<L>(0, own);
+    const uint32_t oslot = kSoA ? own : tslot<L>(0, own);
     if constexpr (kNarrow) hr[0] = ldo(src, oslot);
     else h[0] = Io<L, S>::ld(src, oslot, 0);
     // per source column (x+1, x, x-1 for cx = -1, 0, 1): the LDS base of its
@@ -301,18 +309,37 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
         // below plane i, then stay non-negative, and the plane term 64 i of
         // every load is an immediate of the instruction
         constexpr uint32_t rb = (uint32_t)sizeof(raw);
-        constexpr uint32_t kShBytes = (kSh / 64u) * 64u * L::Q * rb;
-        static_assert(kSh % 64u == 0 && kSh * L::Q * rb >= 64u * 3u * rb, "shift covers the plane offsets");
-        const char* const srcb = (const char*)src - kShBytes;
-        const uint32_t obyte = oslot * rb + kShBytes;
+        if constexpr (kSoA) {
+            // SoA planes: the slot of compact index j in plane i is i qstride + j,
+            // so a source's byte offset is its (shifted) count times 2, and plane
+            // i's base (3 planes back, uniform) is a scalar base of the load
+            static_assert(max_opp_gap<L>() <= 3, "opposite planes within 3 planes");
+            const size_t qsb = (size_t)kp.qstride * rb;
+            const char* const srcb = (const char*)src - 3 * qsb;
+            const uint32_t obyte = (own + kShv) * rb;
 #pragma unroll
-        for (int i = 1; i < L::Q; ++i) {
-            const int c = 1 - L::cx(i), o = L::opp(i);
-            const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int)word[i], cpos[c], 1);
-            const uint32_t there = (__umul24(cnt[i] >> 6, 64u * (L::Q - 1)) + cnt[i]) * rb;
-            const uint32_t self = obyte + (uint32_t)(64 * (int)rb * (o - i));
-            const uint32_t off = (m & there) | (~m & self);
-            hr[i] = ((const raw*)(srcb + off))[64 * i];
+            for (int i = 1; i < L::Q; ++i) {
+                const int c = 1 - L::cx(i), o = L::opp(i);
+                const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int)word[i], cpos[c], 1);
+                const uint32_t there = cnt[i] * rb;
+                const uint32_t self = obyte + (uint32_t)(o - i) * (uint32_t)qsb;
+                const uint32_t off = (m & there) | (~m & self);
+                hr[i] = *(const raw*)(srcb + (size_t)i * qsb + off);
+            }
+        } else {
+            constexpr uint32_t kShBytes = (kSh / 64u) * 64u * L::Q * rb;
+            static_assert(kSh % 64u == 0 && kSh * L::Q * rb >= 64u * 3u * rb, "shift covers the plane offsets");
+            const char* const srcb = (const char*)src - kShBytes;
+            const uint32_t obyte = oslot * rb + kShBytes;
+#pragma unroll
+            for (int i = 1; i < L::Q; ++i) {
+                const int c = 1 - L::cx(i), o = L::opp(i);
+                const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int)word[i], cpos[c], 1);
+                const uint32_t there = (__umul24(cnt[i] >> 6, 64u * (L::Q - 1)) + cnt[i]) * rb;
+                const uint32_t self = obyte + (uint32_t)(64 * (int)rb * (o - i));
+                const uint32_t off = (m & there) | (~m & self);
+                hr[i] = ((const raw*)(srcb + off))[64 * i];
+            }
         }
     } else if constexpr (kNarrow) {
         // 16-bit storage is bound by instruction issue, not by HBM: first every
@@ -390,7 +417,7 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
         }
         T out[L::Q];
         collide_pairs<L, T, C>(kp, xs, drho, rho, u, out, [](int) { return Io<L, S>::kw; });
-        store_node<L, S, true, true>(dst, kp, own, out);
+        store_node<L, S, kSoA ? 2 : 1, true>(dst, kp, own, out);
         return;
     } else if constexpr (kNarrow) {
 #pragma unroll
@@ -411,7 +438,7 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
     }
     T out[L::Q];
     collide<L, T, C>(kp, h, drho, rho, u, out);
-    store_node<L, S, true>(dst, kp, own, out);
+    store_node<L, S, kSoA ? 2 : 1>(dst, kp, own, out);
 }
 
 // Post-boundary populations h of non-solid node idx (compact slot own) from
@@ -729,7 +756,13 @@ static hipError_t step_t(const KParams& kp, const void* src, void* dst, const ui
         const size_t lds = (((size_t)kp.rmax * 9) << kp.lgw) * sizeof(uint2) + (size_t)kp.rblock * sizeof(uint32_t) +
                            (size_t)kp.rmax * 9 * sizeof(uint32_t);
         const unsigned rb = (unsigned)((kp.nf + kp.rblock - 1) / kp.rblock);
-        if (kp.simple)
+        bool soa = false;
+        if constexpr (sizeof(raw) == 2 && !OPEN) soa = kp.simple && !kp.tile;
+        if (soa) {
+            if constexpr (sizeof(raw) == 2 && !OPEN)
+                hipLaunchKernelGGL((k_stepr<L, S, C, OPEN, true, true>), dim3(rb), dim3(kp.rblock), lds, st,
+                                   (const raw*)src, (raw*)dst, kp.rowmeta, kp.blk, kp.nodew, kp, fl, wm ? 1 : 0);
+        } else if (kp.simple)
             hipLaunchKernelGGL((k_stepr<L, S, C, OPEN, true>), dim3(rb), dim3(kp.rblock), lds, st, (const raw*)src,
                                (raw*)dst, kp.rowmeta, kp.blk, kp.nodew, kp, fl, wm ? 1 : 0);
         else
