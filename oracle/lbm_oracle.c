@@ -977,6 +977,8 @@ void orc_cg_init(orc_cg* s, const double* r_rho, const double* b_rho)
         }
         s->rr[ind] = rr; s->br[ind] = br; s->rho[ind] = rr + br;
         s->u[ind] = 0; s->v[ind] = 0; s->w3[ind] = 0;
+        /* the moving north wall's velocity (initInletVelocity, S/Multiphase.cu:1199-1209) */
+        if (s->p.walls == 1 && s->q == 9 && ind / s->p.nx == s->p.ny - 1) s->u[ind] = s->p.u_wall;
     }
 }
 
@@ -997,6 +999,26 @@ static void cg_grad_node(const orc_cg* s, int x, int y, int z, double* g)
 {
     const int nx = s->p.nx, ny = s->p.ny, nz = s->nz;
     double ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+    if (s->p.walls && s->q == 9 && (y == 0 || y == ny - 1 || (s->p.high_order && (y == 1 || y == ny - 2)))) {
+        /* wall rows: the 8-point stencil with the rows beyond the wall
+         * mirrored onto the row inside, x components only (calculateColorGradient
+         * cases 1 and 2, S/GpuCollision.cu:135-156); next to the wall rows the
+         * 24-point stencil falls back to the 8-point one (case 11, :262-275) */
+        const int wall = y == 0 || y == ny - 1;
+        const int into = y == 0 ? 1 : -1;
+        for (int k = 1; k < 9; ++k) {
+            const int dy = wall ? into * abs(s->cy[k]) : s->cy[k];
+            long j = (long)(y + dy) * nx + cg_wrap(x + s->cx[k], nx);
+            double wk = k < 5 ? 4.0 / 12.0 : 1.0 / 12.0;
+            double psi = wk * (s->rr[j] - s->br[j]) / s->rho[j];
+            double rw = wk * s->rho[j];
+            ax += psi * s->cx[k]; bx += rw * s->cx[k];
+            if (!wall) { ay += psi * s->cy[k]; by += rw * s->cy[k]; }
+        }
+        g[0] = ax; g[1] = ay; g[2] = 0.0;
+        g[3] = bx; g[4] = by; g[5] = 0.0;
+        return;
+    }
     for (int k = 0; k < s->nst; ++k) {
         int xx = cg_wrap(x + s->hx[k], nx), yy = cg_wrap(y + s->hy[k], ny), zz = cg_wrap(z + s->hz[k], nz);
         long j = ((long)zz * ny + yy) * nx + xx;
@@ -1018,6 +1040,33 @@ void orc_cg_gradient(const orc_cg* s, double* cgx, double* cgy, double* cgz)
         cgx[ind] = g[0]; cgy[ind] = g[1];
         if (cgz) cgz[ind] = g[2];
     }
+}
+
+/* Zou-He velocity condition of one colour's populations f[9] on the north
+ * (unknown f4, f7, f8) or south (f2, f5, f6) wall row with the node's stored
+ * velocity (u, v) (S/GpuBoundaries.cu:1888-1899, :1929-1942); returns the
+ * colour density it imposes */
+static double cg_zouhe(double* f, double u, double v, int north)
+{
+    if (north) {
+        double r = (1.0 / (1.0 + v)) * (f[0] + f[1] + f[3] + 2 * (f[2] + f[6] + f[5]));
+        f[4] = f[2] - (2.0 / 3.0) * r * v;
+        f[7] = f[5] + 0.5 * (f[1] - f[3]) - (1.0 / 6.0) * (r * v) - 0.5 * (r * u);
+        f[8] = f[6] + 0.5 * (f[3] - f[1]) - (1.0 / 6.0) * (r * v) + 0.5 * (r * u);
+        return r;
+    }
+    double r = (1.0 / (1.0 - v)) * (f[0] + f[1] + f[3] + 2 * (f[4] + f[7] + f[8]));
+    f[2] = f[4] + (2.0 / 3.0) * r * v;
+    f[5] = f[7] - 0.5 * (f[1] - f[3]) + (1.0 / 6.0) * (r * v) + 0.5 * (r * u);
+    f[6] = f[8] + 0.5 * (f[1] - f[3]) + (1.0 / 6.0) * (r * v) - 0.5 * (r * u);
+    return r;
+}
+
+/* on-node bounce-back of the incoming links (S/GpuBoundaries.cu:1908-1915, :1951-1958) */
+static void cg_bounce(double* f, int north)
+{
+    if (north) { f[4] = f[2]; f[7] = f[5]; f[8] = f[6]; }
+    else { f[2] = f[4]; f[5] = f[7]; f[6] = f[8]; }
 }
 
 void orc_cg_step(orc_cg* s, int nsteps)
@@ -1062,17 +1111,43 @@ void orc_cg_step(orc_cg* s, int nsteps)
                 s->bc[k * n + ind] = kb * fs - kk * cs * rest;
             }
         }
+        const int walls = q == 9 ? P->walls : 0;
 #pragma omp parallel for
         for (long ind = 0; ind < n; ++ind) {
             int x = (int)(ind % nx), y = (int)(ind / nx % ny), z = (int)(ind / ((long)nx * ny));
-            double rr0 = s->rr[ind], br0 = s->br[ind], r0 = s->rho[ind];
-            double R = 0, B = 0, jx = 0, jy = 0, jz = 0;
             for (int k = 0; k < q; ++k) {
+                /* a wall row keeps the links from beyond the wall (gpuStreaming2DCG,
+                 * S/GpuStream.cu:62-85); the wall condition below sets them */
+                if (walls && (y - s->cy[k] < 0 || y - s->cy[k] >= ny)) continue;
                 int sx = cg_wrap(x - s->cx[k], nx), sy = cg_wrap(y - s->cy[k], ny), sz = cg_wrap(z - s->cz[k], nz);
                 long src = ((long)sz * ny + sy) * nx + sx;
-                double rv = s->rc[k * n + src], bv = s->bc[k * n + src];
-                s->r[k * n + ind] = rv;
-                s->b[k * n + ind] = bv;
+                s->r[k * n + ind] = s->rc[k * n + src];
+                s->b[k * n + ind] = s->bc[k * n + src];
+            }
+        }
+#pragma omp parallel for
+        for (long ind = 0; ind < n; ++ind) {
+            int y = (int)(ind / nx % ny);
+            double rr0 = s->rr[ind], br0 = s->br[ind], r0 = s->rho[ind];
+            if (walls && (y == 0 || y == ny - 1)) {
+                /* per-colour wall condition (gpuBcPeriodic2D, S/GpuBoundaries.cu:1885-1968) */
+                double fr[9], fb[9];
+                for (int k = 0; k < 9; ++k) { fr[k] = s->r[k * n + ind]; fb[k] = s->b[k * n + ind]; }
+                if (walls == 1) {
+                    double rt = cg_zouhe(fr, s->u[ind], s->v[ind], y == ny - 1);
+                    double bt = cg_zouhe(fb, s->u[ind], s->v[ind], y == ny - 1);
+                    /* the condition stores these colour densities, which the
+                     * macro update then reads for omega_eff */
+                    rr0 = rt; br0 = bt; r0 = rt + bt;
+                } else {
+                    cg_bounce(fr, y == ny - 1);
+                    cg_bounce(fb, y == ny - 1);
+                }
+                for (int k = 0; k < 9; ++k) { s->r[k * n + ind] = fr[k]; s->b[k * n + ind] = fb[k]; }
+            }
+            double R = 0, B = 0, jx = 0, jy = 0, jz = 0;
+            for (int k = 0; k < q; ++k) {
+                double rv = s->r[k * n + ind], bv = s->b[k * n + ind];
                 s->f[k * n + ind] = rv + bv;
                 R += rv; B += bv;
                 jx += s->cx[k] * (rv + bv);

@@ -44,7 +44,7 @@ class OrcCgParams(C.Structure):
         ("r_nu", C.c_double), ("b_nu", C.c_double), ("beta", C.c_double), ("A", C.c_double),
         ("g_limit", C.c_double), ("high_order", C.c_int), ("g", C.c_double),
         ("external_force", C.c_int), ("kappa", C.c_double), ("nthreads", C.c_int), ("nz", C.c_int),
-        ("enhanced", C.c_int),
+        ("enhanced", C.c_int), ("walls", C.c_int), ("u_wall", C.c_double),
     ]
 
 
@@ -222,12 +222,12 @@ class CgSim:
 
     def __init__(self, nx, ny, *, nz=1, r_alpha=4 / 9, b_alpha=4 / 9, r_nu=1 / 6, b_nu=1 / 6, beta=0.7,
                  A=0.01125, g_limit=0.0, high_order=0, g=0.0, external_force=0, kappa=0.9,
-                 nthreads=0, enhanced=0):
+                 nthreads=0, enhanced=0, walls=0, u_wall=0.0):
         p = OrcCgParams()
         p.nx, p.ny, p.r_alpha, p.b_alpha, p.r_nu, p.b_nu = nx, ny, r_alpha, b_alpha, r_nu, b_nu
         p.beta, p.A, p.g_limit, p.high_order, p.g = beta, A, g_limit, high_order, g
         p.external_force, p.kappa, p.nthreads, p.nz = external_force, kappa, nthreads, nz
-        p.enhanced = enhanced
+        p.enhanced, p.walls, p.u_wall = enhanced, walls, u_wall
         self.params, self.nx, self.ny, self.nz = p, nx, ny, nz
         self.n = nx * ny * nz
         self.q = 19 if nz > 1 else 9

@@ -19,6 +19,16 @@ struct CgParams {
 
 constexpr int NRED = 12;     // reductions of k_cg_reduce
 
+// D2Q9 box with walls on the y rows and a body force (lbm_cgw.hip)
+struct WallParams {
+    CgParams cp;
+    int walls;               // 0 none (periodic), 1 Zou-He velocity walls, 2 bounce-back walls
+    int ho;                  // 24-point stencil away from the walls
+    double u_wall;           // initial velocity of the north wall row (walls == 1)
+    double g;                // body force, by the velocity shift of the macroscopic update
+    int external_force;      // 1: along x, u += g / (rho omega_eff); 0: along y, v += g / omega_eff
+};
+
 // (kx, ky, kz) . (a, b, c) for components in {-1, 0, 1} (constants once the
 // direction loop is unrolled) with the zero components left out: the compiler
 // does not fold 0 * x, and the result is the same for finite operands.
@@ -53,6 +63,14 @@ hipError_t cg_launch_dens(int q, const double* rf, const double* bf, size_t n, d
 // mac: rho_r, rho_b, rho, u, v, w (n each; w ignored when nz == 1)
 hipError_t cg_launch_reduce(const double* mac, size_t n, bool three_d, double kappa, double* partials,
                             int nblocks, hipStream_t st);
+
+// D2Q9 with walls / force (lbm_cgw.hip): P holds the colour populations after
+// the wall condition (2 x 9 planes of n), Pc after collision
+hipError_t cgw_launch_step(int storage, const void* P, void* Pc, double* mac, const cg::WallParams& wp,
+                           hipStream_t st);
+hipError_t cgw_launch_init(int storage, void* P, double* mac, const double* rr, const double* rb, const double* rf,
+                           const double* bf, const cg::WallParams& wp, hipStream_t st);
+hipError_t cgw_launch_out(int storage, const void* P, size_t n, double* rf, double* bf, hipStream_t st);
 
 // D3Q19 (lbm_cg3.hip): phase field of the post-streaming state, then the step
 hipError_t cg3_launch_step(int storage, int ho, const void* src, void* dst, void* psi, const cg::CgParams& cp,

@@ -31,6 +31,8 @@ struct lbm_cg_ctx {
     void* psi = nullptr;         // 2 N phase field and density (D3Q19)
     double* fout = nullptr;      // 2 q N: populations in and out
     double* partials = nullptr;
+    bool walls = false;          // D2Q9 walls or body force: the reference-order path (lbm_cgw.hip)
+    lbm::cg::WallParams wp{};
     std::string err;
 };
 
@@ -63,6 +65,12 @@ lbm_status lbm_cg_create(const lbm_cg_params* p, lbm_cg_ctx** out) {
     if (!(p->r_nu > 0 && p->b_nu > 0)) return LBM_EINVAL;
     if (p->enhanced != 0 && p->enhanced != 1) return LBM_EINVAL;
     const bool three_d = p->nz >= 3;
+    if (p->walls < 0 || p->walls > 2 || (p->external_force != 0 && p->external_force != 1) || !std::isfinite(p->g) ||
+        !std::isfinite(p->u_wall))
+        return LBM_EINVAL;
+    const bool walls = p->walls != 0 || p->g != 0.0;
+    if (walls && (three_d || p->storage == LBM_STORE_F16)) return LBM_EUNSUPPORTED;   // D2Q9, F64 / F32 only
+    if (p->walls && p->ny < 5) return LBM_EINVAL;
     const int nz = three_d ? p->nz : 1, q = three_d ? 19 : 9;
     const size_t n = (size_t)p->nx * p->ny * nz;
     const size_t plane = (n + 63) / 64 * 64;
@@ -80,6 +88,13 @@ lbm_status lbm_cg_create(const lbm_cg_params* p, lbm_cg_ctx** out) {
     k.beta = p->beta; k.A = p->A; k.g_limit = p->g_limit; k.kappa = p->kappa;
     k.plane = (uint32_t)plane;
     k.enhanced = p->enhanced;
+    c->walls = walls;
+    c->wp.cp = k;
+    c->wp.walls = p->walls;
+    c->wp.ho = p->high_order;
+    c->wp.u_wall = p->u_wall;
+    c->wp.g = p->g;
+    c->wp.external_force = p->external_force;
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
     const size_t pb = 2 * (size_t)q * plane * sbytes(p->storage);
@@ -112,6 +127,7 @@ const char* lbm_cg_last_error(const lbm_cg_ctx* c) { return c ? c->err.c_str() :
 
 static hipError_t launch_init(lbm_cg_ctx* c, const double* rf, const double* bf) {
     const double *rr = c->rr0, *rb = c->rr0 + c->n;
+    if (c->walls) return lbm::cgw_launch_init(c->p.storage, c->P[0], c->mac, rr, rb, rf, bf, c->wp, c->stream);
     if (c->three_d)
         return lbm::cg3_launch_init(c->p.storage, c->p.high_order, c->P[0], c->psi, c->cp, rr, rb, rf, bf, c->mac,
                                     c->stream);
@@ -161,6 +177,11 @@ lbm_status lbm_cg_init_f(lbm_cg_ctx* c, const double* r_f, const double* b_f) {
 static lbm_status cg_run(lbm_cg_ctx* c, int64_t nsteps) {
     for (int64_t k = 0; k < nsteps; ++k) {
         const bool wm = k == nsteps - 1;
+        if (c->walls) {
+            // P[0]: populations after the wall condition, P[1]: after collision
+            CGCHK(c, lbm::cgw_launch_step(c->p.storage, c->P[0], c->P[1], c->mac, c->wp, c->stream));
+            continue;
+        }
         if (c->three_d)
             CGCHK(c, lbm::cg3_launch_step(c->p.storage, c->p.high_order, c->P[c->cur], c->P[c->cur ^ 1], c->psi,
                                           c->cp, c->mac, wm, c->stream));
@@ -201,7 +222,9 @@ lbm_status lbm_cg_get_f(lbm_cg_ctx* c, double* rf, double* bf) {
     if (c->steps < 1) return cfail(c, LBM_EINVAL, "post-streaming populations exist after the first step");
     const size_t qn = (size_t)c->q * c->n;
     if (!c->fout && hipMalloc((void**)&c->fout, 2 * qn * 8) != hipSuccess) return LBM_ENOMEM;
-    if (c->three_d)
+    if (c->walls)
+        CGCHK(c, lbm::cgw_launch_out(c->p.storage, c->P[0], c->n, c->fout, c->fout + qn, c->stream));
+    else if (c->three_d)
         CGCHK(c, lbm::cg3_launch_recon(c->p.storage, c->P[c->cur ^ 1], c->cp, c->fout, c->fout + qn, c->stream));
     else
         CGCHK(c, lbm::cg_launch_recon(c->p.storage, c->P[c->cur ^ 1], c->cp, c->fout, c->fout + qn, c->stream));

@@ -13,7 +13,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblbm.so")
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 SOLID, FLUID, VEL_W, PRES_E, VEL_N = 0, 1, 2, 3, 4
 VEL_E, VEL_S, VEL_B, VEL_T = 5, 6, 7, 8
@@ -65,7 +65,8 @@ class CgParams(C.Structure):
         ("abi_version", C.c_uint32), ("nx", C.c_int32), ("ny", C.c_int32), ("r_alpha", C.c_double),
         ("b_alpha", C.c_double), ("r_nu", C.c_double), ("b_nu", C.c_double), ("beta", C.c_double),
         ("A", C.c_double), ("g_limit", C.c_double), ("high_order", C.c_int32), ("kappa", C.c_double),
-        ("storage", C.c_int32), ("nz", C.c_int32), ("enhanced", C.c_int32),
+        ("storage", C.c_int32), ("nz", C.c_int32), ("enhanced", C.c_int32), ("walls", C.c_int32),
+        ("u_wall", C.c_double), ("g", C.c_double), ("external_force", C.c_int32),
     ]
 
 
@@ -387,12 +388,15 @@ class Solver:
 
 
 class CgSolver:
-    """Colour-gradient two-phase step on a periodic box (GPU): D2Q9 for
-    CgSolver(nx, ny) with arrays (ny, nx), D3Q19 for CgSolver(nx, ny, nz=...)
-    with arrays (nz, ny, nx).  Same keyword arguments as the oracle's CgSim."""
+    """Colour-gradient two-phase step (GPU): D2Q9 for CgSolver(nx, ny) with
+    arrays (ny, nx), D3Q19 for CgSolver(nx, ny, nz=...) with arrays
+    (nz, ny, nx); periodic, or for D2Q9 walls on the y rows (walls=1 Zou-He
+    velocity walls, the north one at u_wall; walls=2 bounce-back) and a body
+    force g.  Same keyword arguments as the oracle's CgSim."""
 
     def __init__(self, nx, ny, *, nz=1, r_alpha=4 / 9, b_alpha=4 / 9, r_nu=1 / 6, b_nu=1 / 6, beta=0.7, A=0.01125,
-                 g_limit=0.0, high_order=0, kappa=0.9, storage=STORE_F64, enhanced=0):
+                 g_limit=0.0, high_order=0, kappa=0.9, storage=STORE_F64, enhanced=0, walls=0, u_wall=0.0,
+                 g=0.0, external_force=0):
         L = load()
         if isinstance(storage, str):
             storage = STORAGE_NAMES[storage]
@@ -400,6 +404,7 @@ class CgSolver:
         p.abi_version, p.nx, p.ny, p.nz, p.enhanced = ABI_VERSION, nx, ny, nz, enhanced
         p.r_alpha, p.b_alpha, p.r_nu, p.b_nu, p.beta, p.A = r_alpha, b_alpha, r_nu, b_nu, beta, A
         p.g_limit, p.high_order, p.kappa, p.storage = g_limit, high_order, kappa, storage
+        p.walls, p.u_wall, p.g, p.external_force = walls, u_wall, g, external_force
         self.nx, self.ny, self.nz = nx, ny, nz
         self.n = nx * ny * max(nz, 1)
         self.q = 19 if nz > 1 else 9
