@@ -163,16 +163,19 @@ def min_y_rt(r_rho, b_rho, y=None):
     return _crossing(phi[j, i], phi[j + 1, i], yy[j], yy[j + 1])
 
 
-def oscillation_frequency(extremes, sigma, r_density, b_density, radius, start=500):
-    """Case 5 (validateOscilating, S/Multiphase.cu:1271-1301): from the
-    per-step trace of max_y_oscillating, the first maximum and second minimum
-    after `start` steps give the numerical angular frequency pi / (t_min2 -
-    t_max1); the theoretical one is sqrt(6 sigma / ((rho_r + rho_b) R^3)).
-    Returns (theoretical, numerical, error %)."""
+def oscillation_frequency(extremes, sigma, r_density, b_density, radius, start=500, every=1):
+    """Case 5 (validateOscilating, S/Multiphase.cu:1271-1301): from the trace
+    of max_y_oscillating, the first maximum and second minimum after `start`
+    steps give the numerical angular frequency pi / (t_min2 - t_max1); the
+    theoretical one is sqrt(6 sigma / ((rho_r + rho_b) R^3)), the 2D mode-2
+    frequency of Lamb.  The reference samples every step; a trace sampled
+    every `every` steps is read the same way with its times scaled (sampling
+    steps apart keeps the step-to-step jitter of the interpolated interface
+    out of the extremum search).  Returns (theoretical, numerical, error %)."""
     e = np.asarray(extremes, float)
     maxima, minima = [], []
     rising = False
-    for i in range(start, len(e) - 1):
+    for i in range(max(1, -(-start // every)), len(e) - 1):
         d = (e[i + 1] - e[i - 1]) / 2.0
         if rising and d < 0.0:
             maxima.append(i)
@@ -183,5 +186,5 @@ def oscillation_frequency(extremes, sigma, r_density, b_density, radius, start=5
     theo = math.sqrt(6.0 * sigma / ((r_density + b_density) * radius ** 3))
     if not maxima or len(minima) < 2:
         return theo, float("nan"), float("nan")
-    num = math.pi / (minima[1] - maxima[0])
+    num = math.pi / ((minima[1] - maxima[0]) * every)
     return theo, num, abs(theo - num) / theo * 100.0
