@@ -368,3 +368,84 @@ def test_lbmsolver_multiphase_bubble(tmp_path, three_d):
     pin = pf * (5 / 9) * red["pin_sum"] / red["pin_n"]
     pout = pf * (5 / 9) * red["pout_sum"] / red["pout_n"]
     assert abs(err[-1] - abs(sigma - R * (pin - pout)) / sigma * 100) < 1e-6
+
+
+def _mp_case_ini(case, U, iters, extra):
+    return (INI.replace('Lattice = "chan"', 'Lattice = "box"').replace("BCoutlet = Vout", "BCoutlet = 1stE")
+            .format(U=U, nu=0.1, iters=iters) + f"multiphase = yes\ncg_test_case = {case}\n" + extra)
+
+
+@pytest.mark.parametrize("case", [2, 3, 5, 6])
+def test_lbmsolver_multiphase_test_cases(tmp_path, case):
+    """[mi355x] cg_test_case: the reference's 2D two-phase cases through
+    lbmsolver equal the ABI run from the same start, and the validators'
+    files and prints hold what thesis_amd.cg_cases computes from it:
+    Profile_Couette.txt (case 2), the radius of the relaxed square (3), the
+    interface traces Oscilating_frequency.txt (5) and RT_Interface.txt (6)."""
+    from thesis_amd import cg_cases as CC
+    print(f"case: lbmsolver two-phase test case {case}", flush=True)
+    os.makedirs(tmp_path / "Mesh")
+    n = 32 if case in (2, 6) else 64
+    m = 4 * n if case == 6 else n          # the Rayleigh-Taylor box is 1 x 4
+    LT.write_latticegen2(str(tmp_path / "Mesh" / "box"), np.ones((m + 1, n + 1), np.uint8),
+                         np.zeros((m + 1, n + 1), np.uint32), delta=1.0 / n)
+    nu_r, nu_b, U, g, gamma, A = 1 / 6, 1 / 6, 0.0, 0.0, 1.0, 1e-3
+    iters = {2: 3000, 3: 3000, 5: 2500, 6: 2000}[case]
+    if case == 2:
+        nu_b, U = 1 / 24, 0.01
+    if case == 5:
+        nu_r = nu_b = 0.02
+        A = 0.01
+    if case == 6:
+        g, gamma = -5e-5, 3.0
+    extra = (f"cg_r_viscosity = {nu_r!r}\ncg_b_viscosity = {nu_b!r}\ncg_A = {A!r}\ncg_beta = 0.7\n"
+             f"cg_gamma = {gamma!r}\ncg_high_order = 0\n")
+    (tmp_path / "SetUpData.ini").write_text(_mp_case_ini(case, U, iters, extra).replace("Force = 0", f"Force = {g!r}"))
+    r = subprocess.run([CLI, "-f", "SetUpData.ini"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-600:], r.stderr[-300:])
+    assert r.returncode == 0
+    # the same run through the C ABI: x periodic with period n, all m + 1
+    # rows for the wall cases, period m otherwise
+    ny = m + 1 if case in (2, 6) else m
+    b_alpha = 4 / 9
+    kw = dict(r_alpha=1 - (1 - b_alpha) / gamma, b_alpha=b_alpha, r_nu=nu_r, b_nu=nu_b, beta=0.7, A=A,
+              high_order=0, kappa=0.9, walls={2: 1, 6: 2}.get(case, 0), u_wall=U, g=g, external_force=0)
+    y, x = np.mgrid[0:ny, 0:n]
+    xs, ys = x / n, y / n
+    N, M = n + 1, m + 1                    # the reference's length and depth
+    red = {2: ys > 0.5,
+           3: (x < 0.75 * N) & (x > 0.25 * N) & (y < 0.75 * M) & (y > 0.25 * M),
+           5: ((x - 0.5 * N) / (0.125 * N)) ** 2 + ((y - 0.5 * M) / (0.1875 * M)) ** 2 <= 1,
+           6: ys > 2.0 + 0.1 * np.cos(2 * np.pi * xs)}[case].astype(float)
+    s = L.CgSolver(n, ny, **kw)
+    s.init(red, (1 - red) / gamma)
+    full = np.ix_(np.arange(M) % ny, np.arange(N) % n)
+    trace = []
+    for _ in range(iters if case in (5, 6) else 1):
+        s.step(1 if case in (5, 6) else iters)
+        if case in (5, 6):
+            mm = s.macros()
+            fn = CC.max_y_oscillating if case == 5 else CC.min_y_rt
+            trace.append(fn(mm["r_rho"][full], mm["b_rho"][full], np.arange(M) / n))
+    mm = s.macros()
+    s.close()
+    data = np.loadtxt(tmp_path / "Results" / "FinalData.csv", delimiter=",", skiprows=1)
+    assert np.abs(data[:, 3] - mm["u"][full].ravel()).max() < 1e-9
+    assert np.abs(data[:, 7] - mm["rho"][full].ravel()).max() < 1e-9
+    if case == 2:
+        prof = np.loadtxt(tmp_path / "Results" / "Profile_Couette.txt", skiprows=1)
+        assert prof.shape == (M, 3)
+        yy = np.arange(M) / n
+        assert np.abs(prof[:, 0] - np.round(yy, 4)).max() < 1e-9
+        assert np.abs(prof[:, 1] - mm["u"][full][:, N // 2 if N % 2 == 0 else (N + 1) // 2]).max() < 1e-9
+        assert np.abs(prof[:, 2] - CC.analytical_couette((nu_r / nu_b), yy, U)).max() < 1e-9
+    elif case == 3:
+        got, want = CC.deforming_radius(mm["r_rho"][full], mm["b_rho"][full])
+        assert f"Predicted radius: {want:.6f} Final radius: {got:.6f}" in r.stdout
+    else:
+        name = "Oscilating_frequency" if case == 5 else "RT_Interface"
+        rows = np.loadtxt(tmp_path / "Results" / f"{name}.txt")
+        assert (rows[:, 0] == np.arange(iters)).all()
+        assert np.abs(rows[:, 1] - np.array(trace)).max() < 1e-9
+        if case == 5:
+            assert "Theoretical" in r.stdout

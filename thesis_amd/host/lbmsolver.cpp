@@ -21,6 +21,11 @@
 //                                  parameters by cg_beta, cg_A, cg_b_alpha, cg_gamma, cg_r_density,
 //                                  cg_r_viscosity, cg_b_viscosity, cg_g_limit, cg_high_order,
 //                                  cg_enhanced, cg_bubble_radius, cg_control_param)
+//   cg_test_case = 1..6           (2D: the reference's test_case, S/GpuInit.cu:323-534: 1 steady
+//                                  bubble, 2 two layers between Zou-He walls, the north one at U
+//                                  (Couette; Poiseuille with Force != 0), 3 square, 4 two bubbles,
+//                                  5 ellipse, 6 Rayleigh-Taylor between bounce-back walls under
+//                                  Force; cg_external_force = 0 gravity along y, 1 along x)
 // BCoutlet maps as the reference reads it (S/FilesReading.cu:180-190 with
 // S/include/Arguments.h:22-28): in 2D Vout -> velocity outlet, 1stE ->
 // second-order extrapolation, 2ndE -> first-order extrapolation, HEmodel ->
@@ -68,7 +73,7 @@ struct Args {
     bool multiphase = false;
     double cg_r_density = 1.0, cg_gamma = 1.0, cg_b_alpha = 4.0 / 9.0, cg_r_nu = 0.5, cg_b_nu = 0.5;
     double cg_beta = 0.99, cg_A = 1e-4, cg_kappa = 0.9, cg_g_limit = 0.0, cg_radius = 18.0;
-    int cg_high_order = 1, cg_enhanced = 0;
+    int cg_high_order = 1, cg_enhanced = 0, cg_test_case = 1, cg_external_force = 0;
 };
 
 std::string trim(const std::string& s) {
@@ -153,6 +158,8 @@ bool read_ini(const std::string& path, Args& a) {
     num("cg_bubble_radius", a.cg_radius);
     if (kv.count("cg_high_order")) a.cg_high_order = atoi(kv["cg_high_order"].c_str());
     if (kv.count("cg_enhanced")) a.cg_enhanced = atoi(kv["cg_enhanced"].c_str());
+    if (kv.count("cg_test_case")) a.cg_test_case = atoi(kv["cg_test_case"].c_str());
+    if (kv.count("cg_external_force")) a.cg_external_force = atoi(kv["cg_external_force"].c_str());
     return true;
 }
 
@@ -491,6 +498,104 @@ double now_s() {
 // S/Iterate3D.cu:222-230); it goes to "surface tension.txt" in 3D, one line
 // per iteration (S/FilesWriting.cu:338-357), and "Oscilating_frequency.txt"
 // in 2D, "i value" per line (S/FilesWriting.cu:384-394).
+// End-of-run validation of the 2D test cases 2-6 (S/Iterate.cu:663-700 with
+// S/Multiphase.cu:1104-1340 and S/FilesWriting.cu:359-394).  The Couette
+// profile takes kappa = mu_r / mu_b of the run (the reference's args.kappa,
+// fixed at 1 in S/main.cu:56).
+template <class Mid, class Phase>
+void multiphase_validation(const Args& a, int tcase, const int dims[3], const int sim[3], double dx,
+                           const std::vector<double>& hr, const std::vector<double>& hb, lbm_cg_ctx* cg,
+                           const std::vector<double>& trace, double sigma, double b_density, double rad, Mid mid,
+                           Phase phase) {
+    const int n = dims[0], m = dims[1];
+    // sum of the positive phase field along the middle row
+    auto red_chord = [&]() {
+        double s2 = 0;
+        for (int i = 0; i < n; ++i) {
+            const double ph = phase(i, mid(m));
+            if (ph > 0.0) s2 += ph;
+        }
+        return s2;
+    };
+    auto write_trace = [&](const char* name) {
+        if (FILE* f = std::fopen((a.out_dir + name + ".txt").c_str(), "w")) {
+            for (size_t i = 0; i < trace.size(); ++i) std::fprintf(f, "%zu %.10f\n", i, trace[i]);
+            std::fclose(f);
+        }
+    };
+    switch (tcase) {
+    case 2: {   // analyticalCouette / analyticalPoiseuille, writeCouetteSolution
+        const size_t NS = (size_t)sim[0] * sim[1];
+        std::vector<double> u(NS);
+        lbm_cg_get_macros(cg, nullptr, nullptr, nullptr, u.data(), nullptr);
+        const int i = mid(n);
+        const double r_mu = a.cg_r_density * a.cg_r_nu, b_mu = b_density * a.cg_b_nu;
+        const bool pois = a.g != 0.0;
+        const std::string name = pois ? "Profile_Poiseuille" : "Profile_Couette";
+        if (FILE* f = std::fopen((a.out_dir + name + ".txt").c_str(), "w")) {
+            std::fprintf(f, "y numerical analytical\n");
+            for (int j = 0; j < m; ++j) {
+                const double y = j * dx, g = a.g;
+                double an;
+                if (!pois) {
+                    const double kappa = r_mu / b_mu;
+                    an = j >= mid(m) ? (2.0 * y / (kappa + 1.0) + (kappa - 1.0) / (kappa + 1.0)) * a.u
+                                     : (2.0 * kappa * y / (kappa + 1.0)) * a.u;
+                } else if (j >= mid(m)) {
+                    an = -g * y * y / (2.0 * r_mu) + g * ((3.0 * b_mu + r_mu) * y / (4.0 * r_mu * (r_mu + b_mu))) +
+                         g / (2.0 * r_mu) - g * ((3.0 * b_mu + r_mu) / (4.0 * r_mu * (r_mu + b_mu)));
+                } else {
+                    an = -g * y * y / (2.0 * b_mu) + g * ((3.0 * b_mu + r_mu) * y) / (4.0 * b_mu * (r_mu + b_mu));
+                }
+                std::fprintf(f, "%.4f %.10f %.10f\n", y, u[(size_t)(j % sim[1]) * sim[0] + (size_t)(i % sim[0])], an);
+            }
+            std::fclose(f);
+        }
+        std::printf("%s profile written to %s in Results/\n", pois ? "Poiseuille" : "Couette", name.c_str());
+        break;
+    }
+    case 3: {   // deformingBubbleValid
+        const double got = red_chord() / 2.0, want = n / (2.0 * std::sqrt(M_PI));
+        std::printf("Predicted radius: %lf Final radius: %lf\n", want, got);
+        std::printf("Radius error %% = %lf \n", std::fabs(got - want) / want * 100.0);
+        break;
+    }
+    case 4: {   // validateCoalescenceCase
+        const double got = red_chord() / (n * 2.0), want = rad * std::sqrt(2.0);
+        std::printf("Predicted radius: %lf Final radius: %lf  (physical units)\n", want, got);
+        std::printf("Radius error %% = %lf \n", std::fabs(got - want) / want * 100.0);
+        break;
+    }
+    case 5: {   // writeOscilatingSolution + validateOscilating
+        write_trace("Oscilating_frequency");
+        std::printf("Oscilation frequency written to Oscilating_frequency in Results/\n");
+        const double radius = red_chord() / 2.0;
+        std::vector<long> maxima, minima;
+        bool rising = false;
+        for (size_t i = 500; i + 1 < trace.size(); ++i) {
+            const double d = (trace[i + 1] - trace[i - 1]) / 2.0;
+            if (rising && d < 0.0) { maxima.push_back((long)i); rising = false; }
+            else if (!rising && d > 0.0) { minima.push_back((long)i); rising = true; }
+        }
+        const double theo = std::sqrt(6.0 * sigma / ((a.cg_r_density + b_density) * radius * radius * radius));
+        if (!maxima.empty() && minima.size() > 1) {
+            const double num = M_PI / (double)(minima[1] - maxima[0]);
+            std::printf("Theoretical %lf vs Numerical %lf\n", theo, num);
+            std::printf("Error %% : %lf\n", std::fabs(theo - num) / theo * 100.0);
+        } else {
+            std::printf("Theoretical %lf vs Numerical: no full oscillation in the run\n", theo);
+        }
+        break;
+    }
+    case 6:
+        write_trace("RT_Interface");
+        break;
+    default:
+        break;
+    }
+    (void)hr; (void)hb;
+}
+
 int run_multiphase(const Args& a, const int dims[3], const int sim[3], double dx, Times& tm, double t_start) {
     const bool three_d = dims[2] > 1;
     const double b_density = a.cg_r_density / a.cg_gamma;
@@ -504,6 +609,16 @@ int run_multiphase(const Args& a, const int dims[3], const int sim[3], double dx
     p.high_order = a.cg_high_order; p.kappa = a.cg_kappa;
     p.storage = a.storage == LBM_STORE_BF16 ? LBM_STORE_F16 : a.storage;
     p.enhanced = a.cg_enhanced;
+    const int tcase = three_d ? 1 : a.cg_test_case;
+    if (!three_d) {
+        // walls on the first and last rows for the Couette and Rayleigh-Taylor
+        // cases (gpuBcPeriodic2D, S/GpuBoundaries.cu:1876-1968); the force of
+        // the macroscopic update in every 2D case (S/GpuUpdateMacro.cu:118-119)
+        p.walls = tcase == 2 ? 1 : tcase == 6 ? 2 : 0;
+        p.u_wall = a.u;
+        p.g = a.g;
+        p.external_force = a.cg_external_force;
+    }
     lbm_cg_ctx* cg = nullptr;
     lbm_status st = lbm_cg_create(&p, &cg);
     if (st != LBM_OK) { std::fprintf(stderr, "colour-gradient set-up failed (%d)\n", (int)st); return 2; }
@@ -512,10 +627,27 @@ int run_multiphase(const Args& a, const int dims[3], const int sim[3], double dx
     const double rad = a.cg_radius / dims[0];
     std::vector<double> rr(NS, 0.0), rb(NS, 0.0);
     for (size_t i = 0; i < NS; ++i) {
-        const double x = (double)(i % sim[0]) * dx, y = (double)((i / sim[0]) % sim[1]) * dx;
+        const int ix = (int)(i % sim[0]), iy = (int)((i / sim[0]) % sim[1]);
+        const double x = ix * dx, y = iy * dx;
         const double z = (double)(i / ((size_t)sim[0] * sim[1])) * dx;
         const double d2 = (x - 0.5) * (x - 0.5) + (y - 0.5) * (y - 0.5) + (three_d ? (z - 0.5) * (z - 0.5) : 0.0);
-        if (std::sqrt(d2) <= rad) rr[i] = a.cg_r_density;
+        // initCGBubble's shapes (S/GpuInit.cu:331-470), in the lattice's
+        // coordinates and node indices of the full n x m lattice
+        const double n = dims[0], m = dims[1];
+        bool red = false;
+        switch (tcase) {
+        case 2: red = y > 0.5; break;
+        case 3: red = ix < 0.75 * n && ix > 0.25 * n && iy < 0.75 * m && iy > 0.25 * m; break;
+        case 4: red = std::hypot(x - 0.5, y - 0.5 + rad) <= rad || std::hypot(x - 0.5, y - 0.5 - rad) <= rad; break;
+        case 5: {
+            const double ex = (ix - 0.5 * n) / (0.125 * n), ey = (iy - 0.5 * m) / (0.1875 * m);
+            red = ex * ex + ey * ey <= 1.0;
+            break;
+        }
+        case 6: red = y > 2.0 + 0.1 * std::cos(2.0 * M_PI * x); break;
+        default: red = std::sqrt(d2) <= rad; break;
+        }
+        if (red) rr[i] = a.cg_r_density;
         else rb[i] = b_density;
     }
     if (three_d) {
@@ -538,6 +670,30 @@ int run_multiphase(const Args& a, const int dims[3], const int sim[3], double dx
     const double t_loop = now_s();
     std::vector<double> st_err((size_t)a.iterations, 0.0);
     double pin_mean = 0, pout_mean = 0;
+    // phase field at node (i, j) of the full lattice (periodic copies map back)
+    std::vector<double> hr(NS), hb(NS);
+    auto phase = [&](int i, int j) {
+        const size_t s2 = (size_t)(j % sim[1]) * sim[0] + (size_t)(i % sim[0]);
+        return (hr[s2] - hb[s2]) / (hr[s2] + hb[s2]);
+    };
+    auto mid = [](int k) { return k % 2 == 0 ? k / 2 : (k + 1) / 2; };
+    // the zero of the phase field between rows j and j2 on column i, in y
+    auto crossing = [&](int i, int j, int j2) {
+        const double p1 = phase(i, j), p2 = phase(i, j2);
+        const double slope = (p1 - p2) / ((j - j2) * dx);
+        return -(p1 - slope * j * dx) / slope;
+    };
+    // getMaxYOscilating / getMinYRT (S/Multiphase.cu:1211-1269) on the middle column
+    auto interface_y = [&](bool up) {
+        const int i = mid(dims[0]);
+        int j = mid(dims[1]);
+        if (up) {
+            while (j < dims[1] - 1 && phase(i, j) >= 0.0) ++j;
+            return crossing(i, j, j - 1);
+        }
+        while (j > 0 && phase(i, j) >= 0.0) --j;
+        return crossing(i, j, j + 1);
+    };
     int code = 0;
     for (long it = 0; it < a.iterations; ++it) {
         double t0 = now_s();
@@ -549,17 +705,25 @@ int run_multiphase(const Args& a, const int dims[3], const int sim[3], double dx
         lbm_cg_reduce(cg, red);
         tm.resi += now_s() - t0;
         if (red[11] > 0) { std::fprintf(stderr, "\nDIVERGENCE!\n"); code = 1; break; }
-        pin_mean = red[2] / red[3];
-        pout_mean = red[4] / red[5];
-        const double pin = pf * (1.0 - r_alpha) * pin_mean, pout = pf * (1.0 - a.cg_b_alpha) * pout_mean;
-        st_err[it] = std::fabs(sigma - a.cg_radius * (pin - pout)) / sigma * 100.0;
+        if (tcase == 5 || tcase == 6) {   // the interface trace (S/Iterate.cu:505-515)
+            lbm_cg_get_macros(cg, hr.data(), hb.data(), nullptr, nullptr, nullptr);
+            st_err[it] = interface_y(tcase == 5);
+        } else {
+            pin_mean = red[2] / red[3];
+            pout_mean = red[4] / red[5];
+            const double pin = pf * (1.0 - r_alpha) * pin_mean, pout = pf * (1.0 - a.cg_b_alpha) * pout_mean;
+            st_err[it] = std::fabs(sigma - a.cg_radius * (pin - pout)) / sigma * 100.0;
+        }
         if ((it + 1) % (a.residual_every > 0 ? a.residual_every : a.iterations) == 0)
             std::printf("iteration %ld surface tension error %.4f %%, max |u| %.3e\n", it + 1, st_err[it], red[10]);
     }
     tm.iter = now_s() - t_loop;
     const double t_write = now_s();
-    if (a.iterations > 0) std::printf("Suface tension error: %lf\n", st_err[(size_t)a.iterations - 1]);
-    if (three_d) {   // WriteArray (S/Iterate3D.cu:871-872)
+    if (a.iterations > 0 && tcase == 1) std::printf("Suface tension error: %lf\n", st_err[(size_t)a.iterations - 1]);
+    if (tcase != 1) {
+        lbm_cg_get_macros(cg, hr.data(), hb.data(), nullptr, nullptr, nullptr);
+        multiphase_validation(a, tcase, dims, sim, dx, hr, hb, cg, st_err, sigma, b_density, rad, mid, phase);
+    } else if (three_d) {   // WriteArray (S/Iterate3D.cu:871-872)
         if (FILE* f = std::fopen((a.out_dir + "surface tension.txt").c_str(), "w")) {
             for (double e : st_err) std::fprintf(f, "%.10f\n", e);
             std::fclose(f);
@@ -686,7 +850,11 @@ int main(int argc, char** argv) {
     }
     if (a.multiphase) {
         // 2D: periodic box (period n - 1 on both axes); 3D: CYCL faces only
-        if (!three_d) { sim[0] = dims[0] - 1; sim[1] = dims[1] - 1; }
+        if (!three_d) {
+            sim[0] = dims[0] - 1;
+            // the Couette and Rayleigh-Taylor cases keep every row: walls on the first and last
+            sim[1] = a.cg_test_case == 2 || a.cg_test_case == 6 ? dims[1] : dims[1] - 1;
+        }
         else if (tc[1] + tc[2] + tc[3] + tc[5] > 0) {
             std::fprintf(stderr, "the colour-gradient run needs a periodic lattice (CYCL links only)\n");
             return 2;
