@@ -1199,3 +1199,223 @@ void orc_cg_laplace(const orc_cg* s, double* out)
     }
     out[0] = pin; out[1] = nin; out[2] = pout; out[3] = nout; out[4] = umax;
 }
+
+/* ---- the reference's CPU colour-gradient path (D2Q9, the #define CUDA 0
+ * branch of S/Iterate.cu:178-208, :396-399, :429-431, :460-462, :486-489):
+ * mp2DColl (S/Multiphase.cu:6-138), streamMP (:851-1010),
+ * peridicBoundaries (:598-739), updateMacroMP (:451-529).  A second opinion
+ * beside the GPU formulation above, not what the GPU kernels follow: the
+ * gradient is of rho_r - rho_b (not divided by rho), each colour collides
+ * towards its own equilibrium rho_k (phi_k + w (...)) with half the
+ * perturbation, and recolouring carries beta rho_r rho_b / rho^2 times
+ * (rho_r phi_r + rho_b phi_b).  The box is the reference's full lattice:
+ * links from beyond an edge are not streamed; the edges copy them from the
+ * opposite edge (period n - 1, m - 1) or, in test case 2, impose Zou-He walls
+ * on the first and last rows; corners copy in the reference's sequential
+ * order.  Test case 2 updates no macros on the wall rows. */
+struct orc_cgcpu {
+    orc_cgcpu_params p;
+    long n;
+    double *rf, *bf, *rc, *bc, *rr, *br, *rho, *u, *v, *gx, *gy;
+    double rphi[9], bphi[9];
+};
+
+orc_cgcpu* orc_cgcpu_create(const orc_cgcpu_params* p)
+{
+    orc_cgcpu* s = (orc_cgcpu*)calloc(1, sizeof *s);
+    s->p = *p;
+    s->n = (long)p->nx * p->ny;
+    const size_t n = (size_t)s->n, nq = n * 9;
+    s->rf = calloc(nq, 8); s->bf = calloc(nq, 8); s->rc = calloc(nq, 8); s->bc = calloc(nq, 8);
+    s->rr = calloc(n, 8); s->br = calloc(n, 8); s->rho = calloc(n, 8);
+    s->u = calloc(n, 8); s->v = calloc(n, 8); s->gx = calloc(n, 8); s->gy = calloc(n, 8);
+    /* r_phi, b_phi (S/Iterate.cu:194-204) */
+    for (int k = 0; k < 9; ++k) {
+        s->rphi[k] = k == 0 ? p->r_alpha : (k < 5 ? (1.0 - p->r_alpha) / 5.0 : (1.0 - p->r_alpha) / 20.0);
+        s->bphi[k] = k == 0 ? p->b_alpha : (k < 5 ? (1.0 - p->b_alpha) / 5.0 : (1.0 - p->b_alpha) / 20.0);
+    }
+    return s;
+}
+
+void orc_cgcpu_destroy(orc_cgcpu* s)
+{
+    if (!s) return;
+    free(s->rf); free(s->bf); free(s->rc); free(s->bc); free(s->rr); free(s->br); free(s->rho);
+    free(s->u); free(s->v); free(s->gx); free(s->gy);
+    free(s);
+}
+
+/* createBubble (S/Multiphase.cu:325-413): each colour's rest populations
+ * where it has density; initInletVelocity (:1199-1209) in test case 2 */
+void orc_cgcpu_init(orc_cgcpu* s, const double* r_rho, const double* b_rho)
+{
+    const long n = s->n;
+    for (long i = 0; i < n; ++i) {
+        s->rr[i] = r_rho[i]; s->br[i] = b_rho[i]; s->rho[i] = r_rho[i] + b_rho[i];
+        s->u[i] = 0.0; s->v[i] = 0.0;
+        for (int k = 0; k < 9; ++k) {
+            s->rf[k * n + i] = r_rho[i] * s->rphi[k];
+            s->bf[k * n + i] = b_rho[i] * s->bphi[k];
+        }
+    }
+    if (s->p.test_case == 2)
+        for (int i = 0; i < s->p.nx; ++i) s->u[(long)(s->p.ny - 1) * s->p.nx + i] = s->p.u_wall;
+}
+
+static void cgcpu_collide(orc_cgcpu* s)
+{
+    const int nx = s->p.nx, ny = s->p.ny;
+    const long n = s->n;
+    static const double cgw[9] = {0.0, 4.0 / 12.0, 4.0 / 12.0, 4.0 / 12.0, 4.0 / 12.0,
+                                  1.0 / 12.0, 1.0 / 12.0, 1.0 / 12.0, 1.0 / 12.0};
+    static const double wpert[9] = {-4.0 / 27.0, 2.0 / 27.0, 2.0 / 27.0, 2.0 / 27.0, 2.0 / 27.0,
+                                    5.0 / 108.0, 5.0 / 108.0, 5.0 / 108.0, 5.0 / 108.0};
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            const long ind = (long)j * nx + i;
+            double gx = 0.0, gy = 0.0;
+            const int north = j == ny - 1, south = j == 0, east = i == nx - 1, west = i == 0;
+            for (int k = 0; k < 9; ++k) {
+                /* the 8-point gradient of rho_r - rho_b; an edge mirrors the
+                 * rows or columns beyond it and keeps the tangential component;
+                 * corners get none */
+                long t;
+                if (!north && !south && !east && !west) {
+                    t = (long)(j + CY9[k]) * nx + i + CX9[k];
+                    gx += (s->rr[t] - s->br[t]) * CX9[k] * cgw[k];
+                    gy += (s->rr[t] - s->br[t]) * CY9[k] * cgw[k];
+                } else if ((north || south) && !east && !west) {
+                    t = (long)(j + (north ? -1 : 1) * abs(CY9[k])) * nx + i + CX9[k];
+                    gx += (s->rr[t] - s->br[t]) * CX9[k] * cgw[k];
+                } else if ((east || west) && !north && !south) {
+                    t = (long)(j + CY9[k]) * nx + i + (east ? -1 : 1) * abs(CX9[k]);
+                    gy += (s->rr[t] - s->br[t]) * CY9[k] * cgw[k];
+                }
+            }
+            s->gx[ind] = gx; s->gy[ind] = gy;
+            const double aux1 = s->rr[ind] / (s->rho[ind] * s->p.r_nu) + s->br[ind] / (s->rho[ind] * s->p.b_nu);
+            const double om = 1.0 / (3.0 / aux1 + 0.5);
+            const double u = s->u[ind], v = s->v[ind], cu1 = u * u + v * v;
+            const double gn = sqrt(gx * gx + gy * gy);
+            const double kr = s->rr[ind] / s->rho[ind], kb = s->br[ind] / s->rho[ind];
+            const double kk = s->p.beta * s->rr[ind] * s->br[ind] / (s->rho[ind] * s->rho[ind]);
+            for (int k = 0; k < 9; ++k) {
+                double cosin = 0.0, rp = 0.0, bp = 0.0;
+                if (gn > s->p.g_limit) {
+                    const double cg = CX9[k] * gx + CY9[k] * gy;
+                    if (k) cosin = cg / (gn * sqrt((double)(CX9[k] * CX9[k] + CY9[k] * CY9[k])));
+                    const double t = W9[k] * (cg / gn) * (cg / gn) - wpert[k];
+                    rp = 0.5 * s->p.A * gn * t;
+                    bp = 0.5 * s->p.A * gn * t;
+                }
+                const double cu2 = u * CX9[k] + v * CY9[k];
+                const double sq = W9[k] * (3 * cu2 + 4.5 * cu2 * cu2 - 1.5 * cu1);
+                const double rfe = s->rr[ind] * (s->rphi[k] + sq), bfe = s->br[ind] * (s->bphi[k] + sq);
+                const long q = (long)k * n + ind;
+                const double fs = (om * rfe + (1 - om) * s->rf[q] + rp) + (om * bfe + (1 - om) * s->bf[q] + bp);
+                const double rec = kk * cosin * (s->rr[ind] * s->rphi[k] + s->br[ind] * s->bphi[k]);
+                s->rc[q] = kr * fs + rec;
+                s->bc[q] = kb * fs - rec;
+            }
+        }
+}
+
+static void cgcpu_copy(orc_cgcpu* s, long to, long from, const int* ks, int nk)
+{
+    for (int a = 0; a < nk; ++a) {
+        s->rf[ks[a] * s->n + to] = s->rf[ks[a] * s->n + from];
+        s->bf[ks[a] * s->n + to] = s->bf[ks[a] * s->n + from];
+    }
+}
+
+void orc_cgcpu_step(orc_cgcpu* s, int nsteps)
+{
+    const int nx = s->p.nx, ny = s->p.ny;
+    const long n = s->n;
+    for (int it = 0; it < nsteps; ++it) {
+        cgcpu_collide(s);
+        /* streamMP: pull the links whose source lies in the box */
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i)
+                for (int k = 0; k < 9; ++k) {
+                    const int si = i - CX9[k], sj = j - CY9[k];
+                    if (si < 0 || si >= nx || sj < 0 || sj >= ny) continue;
+                    const long d = (long)k * n + (long)j * nx + i, src = (long)k * n + (long)sj * nx + si;
+                    s->rf[d] = s->rc[src];
+                    s->bf[d] = s->bc[src];
+                }
+        /* peridicBoundaries, in its order */
+        for (int i = 1; i < nx - 1; ++i) {
+            const long e = (long)(ny - 1) * nx + i, st = i;
+            if (s->p.test_case == 2) {
+                double* fs[2] = {s->rf, s->bf};
+                double dens[2][2];
+                for (int c = 0; c < 2; ++c) {
+                    double f[9], g[9];
+                    for (int k = 0; k < 9; ++k) { f[k] = fs[c][k * n + e]; g[k] = fs[c][k * n + st]; }
+                    dens[c][0] = cg_zouhe(f, s->u[e], s->v[e], 1);
+                    dens[c][1] = cg_zouhe(g, s->u[st], s->v[st], 0);
+                    for (int k = 0; k < 9; ++k) { fs[c][k * n + e] = f[k]; fs[c][k * n + st] = g[k]; }
+                }
+                s->rr[e] = dens[0][0]; s->br[e] = dens[1][0]; s->rho[e] = dens[0][0] + dens[1][0];
+                s->rr[st] = dens[0][1]; s->br[st] = dens[1][1]; s->rho[st] = dens[0][1] + dens[1][1];
+            } else {
+                static const int up[3] = {4, 7, 8}, dn[3] = {2, 5, 6};
+                cgcpu_copy(s, e, st, up, 3);
+                cgcpu_copy(s, st, e, dn, 3);
+            }
+        }
+        for (int j = 1; j < ny - 1; ++j) {
+            const long e = (long)j * nx + nx - 1, w = (long)j * nx;
+            static const int we[3] = {3, 7, 6}, ea[3] = {1, 5, 8};
+            cgcpu_copy(s, e, w, we, 3);
+            cgcpu_copy(s, w, e, ea, 3);
+        }
+        {
+            const long ne = (long)(ny - 1) * nx + nx - 1, nw = (long)(ny - 1) * nx, se = nx - 1, sw = 0;
+            static const int kne[3] = {3, 4, 7}, knw[3] = {1, 4, 8}, kse[3] = {2, 3, 6}, ksw[3] = {2, 1, 5};
+            cgcpu_copy(s, ne, nw, kne, 3);
+            cgcpu_copy(s, nw, ne, knw, 3);
+            cgcpu_copy(s, se, sw, kse, 3);
+            cgcpu_copy(s, sw, se, ksw, 3);
+        }
+        /* updateMacroMP: every row, or the rows between the walls */
+        const int j0 = s->p.test_case == 2 ? 1 : 0, j1 = s->p.test_case == 2 ? ny - 1 : ny;
+        for (int j = j0; j < j1; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const long ind = (long)j * nx + i;
+                double R = 0, B = 0, jx = 0, jy = 0;
+                for (int k = 0; k < 9; ++k) { R += s->rf[k * n + ind]; B += s->bf[k * n + ind]; }
+                s->rr[ind] = R; s->br[ind] = B; s->rho[ind] = R + B;
+                for (int k = 0; k < 9; ++k) {
+                    jx += (s->rf[k * n + ind] + s->bf[k * n + ind]) * CX9[k];
+                    jy += (s->rf[k * n + ind] + s->bf[k * n + ind]) * CY9[k];
+                }
+                s->u[ind] = jx / s->rho[ind];
+                s->v[ind] = jy / s->rho[ind];
+            }
+    }
+}
+
+void orc_cgcpu_get(const orc_cgcpu* s, double* r_f, double* b_f, double* r_rho, double* b_rho, double* rho,
+                   double* u, double* v)
+{
+    const size_t n = (size_t)s->n;
+    if (r_f) memcpy(r_f, s->rf, 9 * n * 8);
+    if (b_f) memcpy(b_f, s->bf, 9 * n * 8);
+    if (r_rho) memcpy(r_rho, s->rr, n * 8);
+    if (b_rho) memcpy(b_rho, s->br, n * 8);
+    if (rho) memcpy(rho, s->rho, n * 8);
+    if (u) memcpy(u, s->u, n * 8);
+    if (v) memcpy(v, s->v, n * 8);
+}
+
+/* the post-collision colours of the last step (r_fPert, b_fPert) and the gradient */
+void orc_cgcpu_get_coll(const orc_cgcpu* s, double* r_c, double* b_c, double* gx, double* gy)
+{
+    const size_t n = (size_t)s->n;
+    if (r_c) memcpy(r_c, s->rc, 9 * n * 8);
+    if (b_c) memcpy(b_c, s->bc, 9 * n * 8);
+    if (gx) memcpy(gx, s->gx, n * 8);
+    if (gy) memcpy(gy, s->gy, n * 8);
+}

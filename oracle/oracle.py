@@ -48,6 +48,14 @@ class OrcCgParams(C.Structure):
     ]
 
 
+class OrcCgCpuParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("r_alpha", C.c_double), ("b_alpha", C.c_double),
+        ("r_nu", C.c_double), ("b_nu", C.c_double), ("beta", C.c_double), ("A", C.c_double),
+        ("g_limit", C.c_double), ("test_case", C.c_int), ("u_wall", C.c_double),
+    ]
+
+
 _lib = None
 
 
@@ -273,6 +281,54 @@ class CgSim:
         if self.nz > 1:
             return tuple(a.reshape(self.shape) for a in g)
         return g[0].reshape(self.shape), g[1].reshape(self.shape)
+
+
+class CgCpuSim:
+    """The reference's CPU colour-gradient path (mp2DColl, streamMP,
+    peridicBoundaries, updateMacroMP of S/Multiphase.cu) on its full n x m
+    lattice: a second opinion beside the GPU formulation.  Arrays (ny, nx)."""
+
+    def __init__(self, nx, ny, *, r_alpha=4 / 9, b_alpha=4 / 9, r_nu=1 / 6, b_nu=1 / 6, beta=0.7, A=0.01125,
+                 g_limit=0.0, test_case=1, u_wall=0.0):
+        p = OrcCgCpuParams()
+        p.nx, p.ny, p.r_alpha, p.b_alpha, p.r_nu, p.b_nu = nx, ny, r_alpha, b_alpha, r_nu, b_nu
+        p.beta, p.A, p.g_limit, p.test_case, p.u_wall = beta, A, g_limit, test_case, u_wall
+        self.nx, self.ny, self.n, self.shape = nx, ny, nx * ny, (ny, nx)
+        L = lib()
+        L.orc_cgcpu_create.restype = C.c_void_p
+        L.orc_cgcpu_create.argtypes = [C.POINTER(OrcCgCpuParams)]
+        L.orc_cgcpu_destroy.argtypes = [C.c_void_p]
+        dp = C.POINTER(C.c_double)
+        L.orc_cgcpu_init.argtypes = [C.c_void_p, dp, dp]
+        L.orc_cgcpu_step.argtypes = [C.c_void_p, C.c_int]
+        L.orc_cgcpu_get.argtypes = [C.c_void_p] + [dp] * 7
+        L.orc_cgcpu_get_coll.argtypes = [C.c_void_p] + [dp] * 4
+        self.h = L.orc_cgcpu_create(C.byref(p))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_cgcpu_destroy(self.h)
+            self.h = None
+
+    def init(self, r_rho, b_rho):
+        lib().orc_cgcpu_init(self.h, _dp(_f64(r_rho)), _dp(_f64(b_rho)))
+
+    def step(self, n=1):
+        lib().orc_cgcpu_step(self.h, int(n))
+
+    def get(self):
+        n = self.n
+        arrs = [np.zeros(9 * n), np.zeros(9 * n)] + [np.zeros(n) for _ in range(5)]
+        lib().orc_cgcpu_get(self.h, *[_dp(a) for a in arrs])
+        keys = ["r_f", "b_f", "r_rho", "b_rho", "rho", "u", "v"]
+        return {k: a.reshape((9,) + self.shape if i < 2 else self.shape) for i, (k, a) in enumerate(zip(keys, arrs))}
+
+    def collided(self):
+        n = self.n
+        arrs = [np.zeros(9 * n), np.zeros(9 * n), np.zeros(n), np.zeros(n)]
+        lib().orc_cgcpu_get_coll(self.h, *[_dp(a) for a in arrs])
+        return (arrs[0].reshape((9,) + self.shape), arrs[1].reshape((9,) + self.shape),
+                arrs[2].reshape(self.shape), arrs[3].reshape(self.shape))
 
 
 def ref_mrt_tables(q, omega):
