@@ -2,6 +2,8 @@
 // periodic box) and their C ABI.
 #pragma once
 
+#include "lbm_internal.h"
+
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
@@ -15,6 +17,8 @@ struct CgParams {
     uint32_t plane;          // elements per plane (2Q planes: Q deviations h = f - w, Q red)
     int nz;                  // 1 for D2Q9
     int enhanced;            // 1: density-gradient terms in f_eq (enhanced model)
+    uint32_t xcd_shift;      // workgroup renumbering of the D3Q19 kernels (xcd_block)
+    uint32_t xchunk_shift;
 };
 
 constexpr int NRED = 12;     // reductions of k_cg_reduce

@@ -182,10 +182,17 @@ __device__ inline void gradient(const CgParams& cp, int x, int y, int z, Psi psi
     gx = ax; gy = ay; gz = az;
 }
 
+// The workgroup's (x, y, z) block from its dispatch order renumbered by
+// xcd_block, so that the row blocks an XCD runs together are neighbours in y
+// and share their neighbour rows' lines in that XCD's L2.
 __device__ inline bool node_of(const CgParams& cp, int& x, int& y, int& z) {
-    x = blockIdx.x * BX + threadIdx.x;
-    y = blockIdx.y * BY + threadIdx.y;
-    z = blockIdx.z;
+    const uint32_t gx = gridDim.x, gxy = gx * gridDim.y;
+    const uint32_t b = xcd_block(blockIdx.x + gx * blockIdx.y + gxy * blockIdx.z, gxy * gridDim.z, cp.xcd_shift,
+                                 cp.xchunk_shift);
+    const uint32_t bz = b / gxy, rxy = b - bz * gxy, by = rxy / gx;
+    x = (int)((rxy - by * gx) * BX + threadIdx.x);
+    y = (int)(by * BY + threadIdx.y);
+    z = (int)bz;
     return x < cp.nx && y < cp.ny;
 }
 

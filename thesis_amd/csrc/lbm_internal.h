@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <cstdlib>
 
 namespace lbm {
 
@@ -45,7 +46,8 @@ struct KParams {
     uint32_t nf;             // non-solid nodes (compact population count)
     uint64_t mx, my;         // invariant-divisor multipliers for nx, ny
     uint32_t sx, sy;
-    uint32_t nxcd;           // XCDs the workgroups are dealt over (1: no renumbering)
+    uint32_t xcd_shift;      // log2 of the XCDs the workgroups are dealt over (0: no renumbering)
+    uint32_t xchunk_shift;   // log2 of the consecutive workgroups an XCD takes per turn
     int tile;                // population layout: 0 SoA planes, 1 tiles of 64 nodes x Q
     const uint32_t* didx;    // compact -> dense node index
     const uint32_t* nodew;   // compact node words
@@ -119,5 +121,29 @@ hipError_t launch_l2_fused(int q, int storage, bool open, const KParams& kp, con
 // scratch (kL2ScratchDoubles doubles); looked up once per device.
 constexpr int kL2ScratchDoubles = 1024 + 8;
 hipError_t l2_scratch(double** out);
+
+// log2 of a power-of-two setting read from the environment (rounded down, at
+// most 256), for the workgroup renumbering below; out is kept when unset.
+inline void log2_env(const char* name, uint32_t& out) {
+    if (const char* e = std::getenv(name)) {
+        const unsigned long v = std::strtoul(e, nullptr, 10);
+        uint32_t sh = 0;
+        while ((2ul << sh) <= v && sh < 8) ++sh;
+        out = v ? sh : 0;
+    }
+}
+
+// Workgroups are dealt round-robin to the 2^xs XCDs; renumber them so that each
+// XCD's turn takes the next 2^cs consecutive workgroups of the lattice, and
+// neighbouring rows (and their cache lines of cidx and P) meet in the same L2.
+// The tail that fills no whole round keeps its numbers; xs = cs = 0 is the
+// identity.  Shifts only: the XCD count and the chunk are powers of two, so
+// no integer division runs in the kernel.
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb, uint32_t xs, uint32_t cs) {
+    const uint32_t rs = xs + cs;
+    if (b >= (nb >> rs) << rs) return b;
+    const uint32_t k = b >> xs;
+    return ((k >> cs) << rs) + ((b & ((1u << xs) - 1u)) << cs) + (k & ((1u << cs) - 1u));
+}
 
 }  // namespace lbm

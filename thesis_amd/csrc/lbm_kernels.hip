@@ -72,14 +72,6 @@ __device__ inline void store_node(typename Io<L, S>::raw* __restrict__ dst, cons
 
 // ---------------------------------------------------------------------------
 // kernels
-// Workgroups are dealt round-robin to the XCDs; renumber them so that each XCD
-// runs one contiguous range of the lattice and neighbouring rows (and their
-// cache lines of cidx and P) meet in the same L2.  Bijective for any count.
-__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb, uint32_t nxcd) {
-    if (nxcd <= 1) return b;
-    const uint32_t q = nb / nxcd, r = nb % nxcd, x = b % nxcd, k = b / nxcd;
-    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
-}
 
 // One thread per non-solid node, in compact order: reads of the node's own
 // data and all writes are contiguous and aligned across a wave.
@@ -91,7 +83,7 @@ __global__ __launch_bounds__(256) void k_step(const typename Io<L, S>::raw* __re
                                               const uint32_t* __restrict__ cidx, KParams kp,
                                               Fields fl, int write_macros) {
     using T = typename Io<L, S>::T;
-    const uint32_t own = xcd_block(blockIdx.x, gridDim.x, kp.nxcd) * blockDim.x + threadIdx.x;
+    const uint32_t own = xcd_block(blockIdx.x, gridDim.x, kp.xcd_shift, kp.xchunk_shift) * blockDim.x + threadIdx.x;
     if (own >= kp.nf) return;
     const uint32_t idx = didx[own];
     const uint32_t nd = nodew[own];
@@ -148,7 +140,7 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
     using T = typename Io<L, S>::T;
     extern __shared__ uint2 smeta[];
     const int W = kp.words, lg = kp.lgw, nx = kp.nx, ny = kp.ny, nz = kp.nz;
-    const uint32_t bid = xcd_block(blockIdx.x, gridDim.x, kp.nxcd);
+    const uint32_t bid = xcd_block(blockIdx.x, gridDim.x, kp.xcd_shift, kp.xchunk_shift);
     const uint2 bi = blk[bid];
     const uint32_t r0 = bi.x;
     const int R = (int)bi.y;
