@@ -269,7 +269,7 @@ const char* lbm_latticegen_last_error(void);
  * :709-779), gpuStreaming3DCG (S/GpuStream.cu:196-285), gpuUpdateMacro3DCG
  * (S/GpuUpdateMacro.cu:215-361) and the sums of S/Iterate3D.cu:637-645,
  * driven by S/Iterate3D.cu:495-800.  Colour parameters as in S/main.cu:52-72.
- * A D2Q9 box with walls or a body force runs the reference's order instead
+ * A box with walls or a body force runs the reference's order instead
  * (collide, stream, wall condition, macros; two launches per step). */
 typedef struct lbm_cg_ctx lbm_cg_ctx;
 typedef struct {
@@ -286,18 +286,25 @@ typedef struct {
     int32_t nz;               /* 0 or 1: D2Q9 box nx x ny; >= 3: D3Q19 box nx x ny x nz */
     int32_t enhanced;         /* 1: f_eq += nu (chi u.grad rho + psi G:cc), the reference's enhanced_distrib
                                  (gpuCollEnhancedBgkwGC2D/3D, S/GpuCollision.cu:624-707, :781-880) */
-    /* D2Q9 only (F64 or F32 storage): walls on the rows y = 0 and y = ny-1, x
-     * periodic (gpuBcPeriodic2D, S/GpuBoundaries.cu:1876-1968): 0 none; 1 per-colour
-     * Zou-He velocity walls with each wall node's stored velocity, the north row
-     * starting at (u_wall, 0) and the south row at rest (the reference's
-     * Couette test_case 2); 2 on-node bounce-back walls (its Rayleigh-Taylor
-     * test_case 6).  Wall rows take the mirrored gradient of
-     * S/GpuCollision.cu:135-156 and, with high_order, the next rows the 8-point one. */
+    /* F64 or F32 storage: walls on the rows (planes) y = 0 and y = ny-1, the
+     * other axes periodic; 0 none.  D2Q9 (gpuBcPeriodic2D,
+     * S/GpuBoundaries.cu:1876-1968): 1 per-colour Zou-He velocity walls with
+     * each wall node's stored velocity, the north row starting at (u_wall, 0)
+     * and the south row at rest (the reference's Couette test_case 2); 2
+     * on-node bounce-back walls (its Rayleigh-Taylor test_case 6).  Wall rows
+     * take the mirrored gradient of S/GpuCollision.cu:135-156 and, with
+     * high_order, the next rows the 8-point one.  D3Q19 (the single-phase
+     * faces per colour, S/Iterate3D.cu:576-598): 1 the 3D inlet condition
+     * (gpuBcInlet3D) with (u_wall, 0, 0) on the north face and rest on the
+     * south face (3D Couette test_case 4); 2 on-node bounce-back (gpuBcSimpleWall3D).
+     * Wall faces take the mirrored 18-point gradient (S/GpuCollision.cu:306-333)
+     * and, with high_order, the next planes the 18-point one (:505-518). */
     int32_t walls;
     double u_wall;
-    /* body force by the velocity shift of gpuUpdateMacro2DCG (S/GpuUpdateMacro.cu:118-119),
-     * D2Q9 only: external_force 1 along x (u += g / (rho omega_eff)), 0 along y
-     * (v += g / omega_eff), omega_eff from the densities of the previous step */
+    /* body force by the velocity shift of gpuUpdateMacro2DCG / 3DCG
+     * (S/GpuUpdateMacro.cu:118-119, :340-341): external_force 1 along x
+     * (u += g / (rho omega_eff)), 0 along y (v += g / omega_eff), omega_eff from
+     * the densities of the previous step */
     double g;
     int32_t external_force;
 } lbm_cg_params;

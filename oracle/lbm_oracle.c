@@ -1019,6 +1019,28 @@ static void cg_grad_node(const orc_cg* s, int x, int y, int z, double* g)
         g[3] = bx; g[4] = by; g[5] = 0.0;
         return;
     }
+    if (s->p.walls && s->q == 19 && (y == 0 || y == ny - 1 || (s->p.high_order && (y == 1 || y == ny - 2)))) {
+        /* y wall faces: the 18-point stencil with the plane beyond the wall
+         * mirrored onto the plane inside, x and z components only
+         * (calculateColorGradient3D cases 1 and 2, S/GpuCollision.cu:306-333);
+         * next to them the 104-point stencil falls back to the 18-point one
+         * (calculateHOColorGradient3D case 11, :505-518) */
+        const int wall = y == 0 || y == ny - 1;
+        const int into = y == 0 ? 1 : -1;
+        for (int k = 1; k < 19; ++k) {
+            const int dy = wall ? into * abs(s->cy[k]) : s->cy[k];
+            long j = ((long)cg_wrap(z + s->cz[k], nz) * ny + (y + dy)) * nx + cg_wrap(x + s->cx[k], nx);
+            double wk = k <= 6 ? 1.0 / 6.0 : 1.0 / 12.0;
+            double psi = wk * (s->rr[j] - s->br[j]) / s->rho[j];
+            double rw = wk * s->rho[j];
+            ax += psi * s->cx[k]; bx += rw * s->cx[k];
+            az += psi * s->cz[k]; bz += rw * s->cz[k];
+            if (!wall) { ay += psi * s->cy[k]; by += rw * s->cy[k]; }
+        }
+        g[0] = ax; g[1] = ay; g[2] = az;
+        g[3] = bx; g[4] = by; g[5] = bz;
+        return;
+    }
     for (int k = 0; k < s->nst; ++k) {
         int xx = cg_wrap(x + s->hx[k], nx), yy = cg_wrap(y + s->hy[k], ny), zz = cg_wrap(z + s->hz[k], nz);
         long j = ((long)zz * ny + yy) * nx + xx;
@@ -1069,6 +1091,23 @@ static void cg_bounce(double* f, int north)
     else { f[2] = f[4]; f[5] = f[7]; f[6] = f[8]; }
 }
 
+/* D3Q19 y wall face of one colour: the reference's 3D inlet condition with
+ * the face velocity (gpuBcInlet3D north / south, S/GpuBoundaries.cu:73-399:
+ * (u_wall, 0, 0) on the north face, rest on the south one), or on-node
+ * bounce-back, each unknown link taking its opposite's streamed value (the
+ * STRAIGHT case of gpuBcSimpleWall3D, S/GpuBoundaries.cu:482-484) */
+static void cg_wall3(double* f, int walls, int north, double u_wall)
+{
+    if (walls == 1) {
+        const double U[3] = {north ? u_wall : 0.0, 0.0, 0.0};
+        velocity_face(f, 1, north ? -1 : 1, U, 0);
+        return;
+    }
+    for (int k = 1; k < 19; ++k)
+        if (CY19[k] == (north ? -1 : 1))
+            f[k] = f[dir3(-CX19[k], -CY19[k], -CZ19[k])];
+}
+
 void orc_cg_step(orc_cg* s, int nsteps)
 {
     const long n = s->n;
@@ -1111,7 +1150,7 @@ void orc_cg_step(orc_cg* s, int nsteps)
                 s->bc[k * n + ind] = kb * fs - kk * cs * rest;
             }
         }
-        const int walls = q == 9 ? P->walls : 0;
+        const int walls = P->walls;
 #pragma omp parallel for
         for (long ind = 0; ind < n; ++ind) {
             int x = (int)(ind % nx), y = (int)(ind / nx % ny), z = (int)(ind / ((long)nx * ny));
@@ -1129,7 +1168,15 @@ void orc_cg_step(orc_cg* s, int nsteps)
         for (long ind = 0; ind < n; ++ind) {
             int y = (int)(ind / nx % ny);
             double rr0 = s->rr[ind], br0 = s->br[ind], r0 = s->rho[ind];
-            if (walls && (y == 0 || y == ny - 1)) {
+            if (walls && q == 19 && (y == 0 || y == ny - 1)) {
+                /* per-colour wall faces (S/Iterate3D.cu:576-598); the 3D
+                 * condition stores no densities */
+                double fr[19], fb[19];
+                for (int k = 0; k < 19; ++k) { fr[k] = s->r[k * n + ind]; fb[k] = s->b[k * n + ind]; }
+                cg_wall3(fr, walls, y == ny - 1, P->u_wall);
+                cg_wall3(fb, walls, y == ny - 1, P->u_wall);
+                for (int k = 0; k < 19; ++k) { s->r[k * n + ind] = fr[k]; s->b[k * n + ind] = fb[k]; }
+            } else if (walls && (y == 0 || y == ny - 1)) {
                 /* per-colour wall condition (gpuBcPeriodic2D, S/GpuBoundaries.cu:1885-1968) */
                 double fr[9], fb[9];
                 for (int k = 0; k < 9; ++k) { fr[k] = s->r[k * n + ind]; fb[k] = s->b[k * n + ind]; }

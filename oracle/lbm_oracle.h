@@ -127,13 +127,19 @@ typedef struct {
     int nthreads;
     int nz;                      /* > 1: D3Q19 box nx x ny x nz */
     int enhanced;                /* 1: density-gradient correction of f_eq (gpuCollEnhancedBgkwGC2D/3D) */
-    /* D2Q9 only: walls on the rows y = 0 (south) and y = ny-1 (north), x
-     * periodic.  1: per-colour Zou-He velocity walls with the node's stored
-     * velocity, north row initialised to (u_wall, 0), south to rest (test
-     * case 2, Couette); 2: on-node bounce-back of the incoming links (test
-     * case 6, Rayleigh-Taylor).  gpuBcPeriodic2D, S/GpuBoundaries.cu:1876-1968;
-     * wall rows take the mirrored gradient of S/GpuCollision.cu:135-156 and,
-     * with the 24-point stencil, the next rows the 8-point one (:262-275). */
+    /* Walls on the rows (planes) y = 0 (south) and y = ny-1 (north), the
+     * other axes periodic.  D2Q9: 1: per-colour Zou-He velocity walls with
+     * the node's stored velocity, north row initialised to (u_wall, 0), south
+     * to rest (test case 2, Couette); 2: on-node bounce-back of the incoming
+     * links (test case 6, Rayleigh-Taylor).  gpuBcPeriodic2D,
+     * S/GpuBoundaries.cu:1876-1968; wall rows take the mirrored gradient of
+     * S/GpuCollision.cu:135-156 and, with the 24-point stencil, the next rows
+     * the 8-point one (:262-275).  D3Q19: the single-phase faces applied per
+     * colour (S/Iterate3D.cu:576-598): 1: the 3D inlet condition with
+     * (u_wall, 0, 0) on the north face and rest on the south one; 2: on-node
+     * bounce-back; wall faces take the mirrored 18-point gradient
+     * (S/GpuCollision.cu:306-333), the planes next to them the 18-point one
+     * with the 104-point stencil (:505-518). */
     int walls;
     double u_wall;
 } orc_cg_params;

@@ -28,27 +28,6 @@ namespace cg3 {
 using cg::CgParams;
 
 constexpr int BX = 64, BY = 4;   // one wavefront per x row segment, 4 rows per block
-__device__ constexpr int CX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
-__device__ constexpr int CY[19] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
-__device__ constexpr int CZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
-__host__ __device__ constexpr double W19(int i) { return i == 0 ? 1.0 / 3.0 : (i <= 6 ? 1.0 / 18.0 : 1.0 / 36.0); }
-// phi_k, theta_k, B_k by shell (S/GpuInit.cu:900-915)
-__host__ __device__ constexpr double PHI(int i) { return i == 0 ? 0.0 : (i <= 6 ? 1.0 / 12.0 : 1.0 / 24.0); }
-__host__ __device__ constexpr double TETA(int i) { return i == 0 ? 1.0 : (i <= 6 ? -1.0 / 12.0 : -1.0 / 24.0); }
-__host__ __device__ constexpr double BPERT(int i) { return i == 0 ? -2.0 / 9.0 : (i <= 6 ? 1.0 / 54.0 : 1.0 / 27.0); }
-__host__ __device__ constexpr double CNORM(int i) { return i <= 6 ? 1.0 : 1.4142135623730950488; }
-// enhanced-model chi_k, psi_k (S/GpuInit.cu:916-926)
-__host__ __device__ constexpr double CHI(int i) { return i == 0 ? -2.5 : (i <= 6 ? -1.0 / 6.0 : 1.0 / 24.0); }
-__host__ __device__ constexpr double PSI(int i) { return i == 0 ? 0.0 : (i <= 6 ? 0.25 : 0.125); }
-// gradient weights by |c|^2: 18-neighbour stencil 1/6, 1/12 (S/GpuInit.cu:928);
-// 104-neighbour stencil (S/GpuInit.cu:950-990): shells 1, 2, 3, 4, 5, 6 and 9
-// (the (2,2,1) points), 0 for the points of [-2,2]^3 it leaves out
-__host__ __device__ constexpr double LOW(int c2) { return c2 == 1 ? 1.0 / 6.0 : c2 == 2 ? 1.0 / 12.0 : 0.0; }
-__host__ __device__ constexpr double HOW(int c2) {
-    return c2 == 1 ? 4.0 / 45 : c2 == 2 ? 1.0 / 21 : c2 == 3 ? 2.0 / 105 : c2 == 4 ? 5.0 / 504
-         : c2 == 5 ? 1.0 / 315 : c2 == 6 ? 1.0 / 630 : c2 == 9 ? 1.0 / 5040 : 0.0;
-}
-
 // storage: FP64 / FP32 raw; FP16 scaled by 1024/w (deviation) and 1/w (red)
 template <int S> struct CIo;
 template <> struct CIo<0> {

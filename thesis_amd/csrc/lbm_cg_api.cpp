@@ -31,7 +31,7 @@ struct lbm_cg_ctx {
     void* psi = nullptr;         // 2 N phase field and density (D3Q19)
     double* fout = nullptr;      // 2 q N: populations in and out
     double* partials = nullptr;
-    bool walls = false;          // D2Q9 walls or body force: the reference-order path (lbm_cgw.hip)
+    bool walls = false;          // walls or body force: the reference-order path (lbm_cgw.hip)
     lbm::cg::WallParams wp{};
     std::string err;
 };
@@ -69,7 +69,7 @@ lbm_status lbm_cg_create(const lbm_cg_params* p, lbm_cg_ctx** out) {
         !std::isfinite(p->u_wall))
         return LBM_EINVAL;
     const bool walls = p->walls != 0 || p->g != 0.0;
-    if (walls && (three_d || p->storage == LBM_STORE_F16)) return LBM_EUNSUPPORTED;   // D2Q9, F64 / F32 only
+    if (walls && p->storage == LBM_STORE_F16) return LBM_EUNSUPPORTED;   // F64 / F32 only
     if (p->walls && p->ny < 5) return LBM_EINVAL;
     const int nz = three_d ? p->nz : 1, q = three_d ? 19 : 9;
     const size_t n = (size_t)p->nx * p->ny * nz;
@@ -230,7 +230,7 @@ lbm_status lbm_cg_get_f(lbm_cg_ctx* c, double* rf, double* bf) {
     const size_t qn = (size_t)c->q * c->n;
     if (!c->fout && hipMalloc((void**)&c->fout, 2 * qn * 8) != hipSuccess) return LBM_ENOMEM;
     if (c->walls)
-        CGCHK(c, lbm::cgw_launch_out(c->p.storage, c->P[0], c->n, c->fout, c->fout + qn, c->stream));
+        CGCHK(c, lbm::cgw_launch_out(c->p.storage, c->q, c->P[0], c->n, c->fout, c->fout + qn, c->stream));
     else if (c->three_d)
         CGCHK(c, lbm::cg3_launch_recon(c->p.storage, c->P[c->cur ^ 1], c->cp, c->fout, c->fout + qn, c->stream));
     else

@@ -390,7 +390,7 @@ class Solver:
 class CgSolver:
     """Colour-gradient two-phase step (GPU): D2Q9 for CgSolver(nx, ny) with
     arrays (ny, nx), D3Q19 for CgSolver(nx, ny, nz=...) with arrays
-    (nz, ny, nx); periodic, or for D2Q9 walls on the y rows (walls=1 Zou-He
+    (nz, ny, nx); periodic, or walls on the y rows / planes (walls=1 Zou-He
     velocity walls, the north one at u_wall; walls=2 bounce-back) and a body
     force g.  Same keyword arguments as the oracle's CgSim."""
 
