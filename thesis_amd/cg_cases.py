@@ -8,9 +8,10 @@ layer over a light one with a cosine perturbation (Rayleigh-Taylor).  Each
 returns the red density field (ny, nx) of a unit-density start; blue is one
 minus it.
 
-Validators (S/Multiphase.cu:1104-1340) read colour densities (ny, nx); the
-"middle" row or column is index m/2 for even m and (m+1)/2 for odd m, as the
-reference takes it.
+Validators (S/Multiphase.cu:1104-1340) read colour densities (ny, nx), or
+(nz, ny, nx) for the 3D cases, where they look at the middle plane; the
+"middle" row, column or plane is index m/2 for even m and (m+1)/2 for odd m,
+as the reference takes it.
 
 Coordinates: the reference reads node coordinates from the mesh; here x
 runs over one period, x = i / nx (the box is periodic in x), and y uses the
@@ -107,27 +108,35 @@ def analytical_poiseuille(y, r_density, b_density, r_visc, b_visc, g):
 
 
 def mid_row_red_width(r_rho, b_rho):
-    """Sum of the positive phase field along the middle row: the red chord
-    through the centre, in nodes (deformingBubbleValid / validateCoalescenceCase)."""
+    """Sum of the positive phase field along the middle row (of the middle
+    plane in 3D): the red chord through the centre, in nodes
+    (deformingBubbleValid / validateCoalescenceCase)."""
     phi = _phase(r_rho, b_rho)
+    if phi.ndim == 3:
+        phi = phi[_mid(phi.shape[0])]
     row = phi[_mid(phi.shape[0])]
     return float(row[row > 0.0].sum())
 
 
 def deforming_radius(r_rho, b_rho):
-    """Case 3 (deformingBubbleValid, S/Multiphase.cu:1162-1197): measured and
-    predicted radius in nodes; the square of side n/2 becomes the disc of the
-    same area, radius n / (2 sqrt(pi))."""
-    nx = np.asarray(r_rho).shape[1]
-    return mid_row_red_width(r_rho, b_rho) / 2.0, nx / (2.0 * math.sqrt(math.pi))
+    """2D case 3 / 3D case 2 (deformingBubbleValid, S/Multiphase.cu:1162-1192):
+    measured and predicted radius in nodes; the square of side n/2 becomes
+    the disc of the same area, radius n / (2 sqrt(pi)), the cube the sphere
+    of the same volume, radius (n/2) / (4 pi / 3)^(1/3)."""
+    a = np.asarray(r_rho)
+    nx = a.shape[-1]
+    want = nx / 2.0 / (4.0 / 3.0 * math.pi) ** (1.0 / 3.0) if a.ndim == 3 else nx / (2.0 * math.sqrt(math.pi))
+    return mid_row_red_width(r_rho, b_rho) / 2.0, want
 
 
 def coalescence_radius(r_rho, b_rho, radius):
-    """Case 4 (validateCoalescenceCase, S/Multiphase.cu:1104-1134): measured
-    radius in units of the box width and the predicted radius sqrt(2) R of
-    the merged disc."""
-    nx = np.asarray(r_rho).shape[1]
-    return mid_row_red_width(r_rho, b_rho) / (2.0 * nx), radius * math.sqrt(2.0)
+    """2D case 4 / 3D case 3 (validateCoalescenceCase, S/Multiphase.cu:1104-1134):
+    measured radius in units of the box width and the predicted radius of the
+    merged drop, sqrt(2) R for two discs, 2^(1/3) R for two spheres."""
+    a = np.asarray(r_rho)
+    nx = a.shape[-1]
+    want = radius * (2.0 ** (1.0 / 3.0) if a.ndim == 3 else math.sqrt(2.0))
+    return mid_row_red_width(r_rho, b_rho) / (2.0 * nx), want
 
 
 def _crossing(phi_a, phi_b, y_a, y_b):

@@ -25,7 +25,11 @@
 //                                  bubble, 2 two layers between Zou-He walls, the north one at U
 //                                  (Couette; Poiseuille with Force != 0), 3 square, 4 two bubbles,
 //                                  5 ellipse, 6 Rayleigh-Taylor between bounce-back walls under
-//                                  Force; cg_external_force = 0 gravity along y, 1 along x)
+//                                  Force; cg_external_force = 0 gravity along y, 1 along x;
+//                                  3D, S/GpuInit.cu:536-720: 1 sphere, 2 cube, 3 two spheres,
+//                                  4 two layers between inlet faces on y (Couette; the lattice
+//                                  cyclic along x and z), 5 Rayleigh-Taylor between bounce-back
+//                                  y faces)
 // BCoutlet maps as the reference reads it (S/FilesReading.cu:180-190 with
 // S/include/Arguments.h:22-28): in 2D Vout -> velocity outlet, 1stE ->
 // second-order extrapolation, 2ndE -> first-order extrapolation, HEmodel ->
@@ -596,6 +600,65 @@ void multiphase_validation(const Args& a, int tcase, const int dims[3], const in
     (void)hr; (void)hb;
 }
 
+// End-of-run validation of the 3D test cases 2-4 (S/Iterate3D.cu:882-901):
+// deformingBubbleValid and validateCoalescenceCase on the middle row of the
+// middle plane with the sphere's predictions, analyticalCouette and
+// writeCouetteSolution on the middle column of the middle plane
+// (S/Multiphase.cu:1104-1192, S/FilesWriting.cu:359-382); kappa as in 2D.
+template <class Mid>
+void multiphase_validation3d(const Args& a, int tcase, const int dims[3], const int sim[3], double dx,
+                             const std::vector<double>& hr, const std::vector<double>& hb, lbm_cg_ctx* cg,
+                             double b_density, double rad, Mid mid) {
+    const int n = dims[0], m = dims[1], h = dims[2];
+    auto at = [&](int i, int j, int k) {
+        return ((size_t)(k % sim[2]) * sim[1] + (size_t)(j % sim[1])) * sim[0] + (size_t)(i % sim[0]);
+    };
+    auto red_chord = [&]() {
+        const int j = mid(m), k = mid(h);
+        double s2 = 0;
+        for (int i = 0; i < n; ++i) {
+            const size_t x = at(i, j, k);
+            const double ph = (hr[x] - hb[x]) / (hr[x] + hb[x]);
+            if (ph > 0.0) s2 += ph;
+        }
+        return s2;
+    };
+    switch (tcase) {
+    case 2: {   // deformingBubbleValid: the cube of side n/2 becomes the sphere of its volume
+        const double got = red_chord() / 2.0, want = n / 2.0 / std::cbrt(4.0 / 3.0 * M_PI);
+        std::printf("Predicted radius: %lf Final radius: %lf\n", want, got);
+        std::printf("Radius error %% = %lf \n", std::fabs(got - want) / want * 100.0);
+        break;
+    }
+    case 3: {   // validateCoalescenceCase: two spheres of radius R merge into one of 2^(1/3) R
+        const double got = red_chord() / (n * 2.0), want = rad * std::cbrt(2.0);
+        std::printf("Predicted radius: %lf Final radius: %lf  (physical units)\n", want, got);
+        std::printf("Radius error %% = %lf \n", std::fabs(got - want) / want * 100.0);
+        break;
+    }
+    case 4: {   // analyticalCouette + writeCouetteSolution
+        const size_t NS = (size_t)sim[0] * sim[1] * sim[2];
+        std::vector<double> u(NS);
+        lbm_cg_get_macros3d(cg, nullptr, nullptr, nullptr, u.data(), nullptr, nullptr);
+        const double kappa = (a.cg_r_density * a.cg_r_nu) / (b_density * a.cg_b_nu);
+        if (FILE* f = std::fopen((a.out_dir + "Profile_Couette.txt").c_str(), "w")) {
+            std::fprintf(f, "y numerical analytical\n");
+            for (int j = 0; j < m; ++j) {
+                const double y = j * dx;
+                const double an = j >= mid(m) ? (2.0 * y / (kappa + 1.0) + (kappa - 1.0) / (kappa + 1.0)) * a.u
+                                              : (2.0 * kappa * y / (kappa + 1.0)) * a.u;
+                std::fprintf(f, "%.4f %.10f %.10f\n", y, u[at(mid(n), j, mid(h))], an);
+            }
+            std::fclose(f);
+        }
+        std::printf("Couette profile written to Profile_Couette in Results/\n");
+        break;
+    }
+    default:
+        break;
+    }
+}
+
 int run_multiphase(const Args& a, const int dims[3], const int sim[3], double dx, Times& tm, double t_start) {
     const bool three_d = dims[2] > 1;
     const double b_density = a.cg_r_density / a.cg_gamma;
@@ -609,16 +672,16 @@ int run_multiphase(const Args& a, const int dims[3], const int sim[3], double dx
     p.high_order = a.cg_high_order; p.kappa = a.cg_kappa;
     p.storage = a.storage == LBM_STORE_BF16 ? LBM_STORE_F16 : a.storage;
     p.enhanced = a.cg_enhanced;
-    const int tcase = three_d ? 1 : a.cg_test_case;
-    if (!three_d) {
-        // walls on the first and last rows for the Couette and Rayleigh-Taylor
-        // cases (gpuBcPeriodic2D, S/GpuBoundaries.cu:1876-1968); the force of
-        // the macroscopic update in every 2D case (S/GpuUpdateMacro.cu:118-119)
-        p.walls = tcase == 2 ? 1 : tcase == 6 ? 2 : 0;
-        p.u_wall = a.u;
-        p.g = a.g;
-        p.external_force = a.cg_external_force;
-    }
+    const int tcase = a.cg_test_case;
+    // walls on the first and last rows for the 2D Couette and Rayleigh-Taylor
+    // cases (gpuBcPeriodic2D, S/GpuBoundaries.cu:1876-1968), on the y faces for
+    // the 3D ones (the single-phase faces per colour, S/Iterate3D.cu:576-598:
+    // the inlet condition for the Couette case, bounce-back for Rayleigh-Taylor);
+    // the force of the macroscopic update in every case (S/GpuUpdateMacro.cu:118-119, :340-341)
+    p.walls = three_d ? (tcase == 4 ? 1 : tcase == 5 ? 2 : 0) : (tcase == 2 ? 1 : tcase == 6 ? 2 : 0);
+    p.u_wall = a.u;
+    p.g = a.g;
+    p.external_force = a.cg_external_force;
     lbm_cg_ctx* cg = nullptr;
     lbm_status st = lbm_cg_create(&p, &cg);
     if (st != LBM_OK) { std::fprintf(stderr, "colour-gradient set-up failed (%d)\n", (int)st); return 2; }
@@ -635,6 +698,27 @@ int run_multiphase(const Args& a, const int dims[3], const int sim[3], double dx
         // coordinates and node indices of the full n x m lattice
         const double n = dims[0], m = dims[1];
         bool red = false;
+        if (three_d) {   // initCGBubble3D's shapes (S/GpuInit.cu:543-720)
+            const double h = dims[2];
+            const int iz = (int)(i / ((size_t)sim[0] * sim[1]));
+            switch (tcase) {
+            case 2:
+                red = ix < 0.75 * n && ix > 0.25 * n && iy < 0.75 * m && iy > 0.25 * m && iz < 0.75 * h && iz > 0.25 * h;
+                break;
+            case 3: {
+                const double a2 = (x - 0.5) * (x - 0.5) + (z - 0.5) * (z - 0.5);
+                red = std::sqrt(a2 + (y - 0.5 + rad) * (y - 0.5 + rad)) <= rad ||
+                      std::sqrt(a2 + (y - 0.5 - rad) * (y - 0.5 - rad)) <= rad;
+                break;
+            }
+            case 4: red = y > 0.5; break;
+            case 5: red = y > 1.0 + 0.1 * std::cos(2.0 * M_PI * x); break;
+            default: red = std::sqrt(d2) <= rad; break;
+            }
+            if (red) rr[i] = a.cg_r_density;
+            else rb[i] = b_density;
+            continue;
+        }
         switch (tcase) {
         case 2: red = y > 0.5; break;
         case 3: red = ix < 0.75 * n && ix > 0.25 * n && iy < 0.75 * m && iy > 0.25 * m; break;
@@ -705,7 +789,7 @@ int run_multiphase(const Args& a, const int dims[3], const int sim[3], double dx
         lbm_cg_reduce(cg, red);
         tm.resi += now_s() - t0;
         if (red[11] > 0) { std::fprintf(stderr, "\nDIVERGENCE!\n"); code = 1; break; }
-        if (tcase == 5 || tcase == 6) {   // the interface trace (S/Iterate.cu:505-515)
+        if (!three_d && (tcase == 5 || tcase == 6)) {   // the interface trace (S/Iterate.cu:505-515)
             lbm_cg_get_macros(cg, hr.data(), hb.data(), nullptr, nullptr, nullptr);
             st_err[it] = interface_y(tcase == 5);
         } else {
@@ -722,7 +806,8 @@ int run_multiphase(const Args& a, const int dims[3], const int sim[3], double dx
     if (a.iterations > 0 && tcase == 1) std::printf("Suface tension error: %lf\n", st_err[(size_t)a.iterations - 1]);
     if (tcase != 1) {
         lbm_cg_get_macros(cg, hr.data(), hb.data(), nullptr, nullptr, nullptr);
-        multiphase_validation(a, tcase, dims, sim, dx, hr, hb, cg, st_err, sigma, b_density, rad, mid, phase);
+        if (three_d) multiphase_validation3d(a, tcase, dims, sim, dx, hr, hb, cg, b_density, rad, mid);
+        else multiphase_validation(a, tcase, dims, sim, dx, hr, hb, cg, st_err, sigma, b_density, rad, mid, phase);
     } else if (three_d) {   // WriteArray (S/Iterate3D.cu:871-872)
         if (FILE* f = std::fopen((a.out_dir + "surface tension.txt").c_str(), "w")) {
             for (double e : st_err) std::fprintf(f, "%.10f\n", e);
@@ -855,7 +940,14 @@ int main(int argc, char** argv) {
             // the Couette and Rayleigh-Taylor cases keep every row: walls on the first and last
             sim[1] = a.cg_test_case == 2 || a.cg_test_case == 6 ? dims[1] : dims[1] - 1;
         }
-        else if (tc[1] + tc[2] + tc[3] + tc[5] > 0) {
+        else if (a.cg_test_case == 4 || a.cg_test_case == 5) {
+            // walls on the y faces: every y plane is simulated
+            if (!cyc[0] || !cyc[2] || cyc[1]) {
+                std::fprintf(stderr, "the 3D Couette and Rayleigh-Taylor cases need a lattice cyclic along x and z "
+                                     "with walls on the y faces\n");
+                return 2;
+            }
+        } else if (tc[1] + tc[2] + tc[3] + tc[5] > 0) {
             std::fprintf(stderr, "the colour-gradient run needs a periodic lattice (CYCL links only)\n");
             return 2;
         }

@@ -290,14 +290,18 @@ def bubble3d_populations(r_rho, b_rho):
     return phi[:, None, None, None] * r[None], phi[:, None, None, None] * b[None]
 
 
-def write_latticegen2(path_prefix, node_class, bb=None, delta=1.0, inlet_dirs=None, outlet_dirs=None, wall_q=None):
+def write_latticegen2(path_prefix, node_class, bb=None, delta=1.0, inlet_dirs=None, outlet_dirs=None, wall_q=None,
+                      cyclic_axes=()):
     """Write <prefix>_Node.dat and <prefix>_BC.dat in the latticegen2 layout
     (L/main.cpp:59-66, L/FluidTracking.cpp:229-236): 2D lines "i j x y fluid"
     and "i j dir type x y id", 3D with the k column.  WALL links come from
     `bb`; VEL_W / PRES_E / VEL_N nodes of a 2D lattice get INLE / OUTL / INLE
     links in the directions leaving the west / east / north face (wall links
     excepted).  A WALL line's point lies at the wall fraction wall_q[..., d]
-    of the link (1/2 when wall_q is None or 0 there)."""
+    of the link (1/2 when wall_q is None or 0 there).  With cyclic_axes
+    (0 x, 1 y, 2 z) every link that leaves the lattice through faces of those
+    axes only is a CYCL link, and one that also leaves through a face of
+    another axis an INLE link (the box's remaining faces)."""
     cls = np.asarray(node_class)
     two_d = cls.ndim == 2
     c3 = cls[None] if two_d else cls
@@ -348,3 +352,12 @@ def write_latticegen2(path_prefix, node_class, bb=None, delta=1.0, inlet_dirs=No
                         for d in outlet_dirs:
                             if not (int(b3[k, j, i]) >> d) & 1:
                                 link(i, j, k, d, 3)
+                    if cyclic_axes:
+                        pos, size = (i, j, k), (nx, ny, nz)
+                        for d in range(1, q):
+                            if (int(b3[k, j, i]) >> d) & 1:
+                                continue
+                            out = [a for a in range(3) if size[a] > 1 and
+                                   ((c[d, a] < 0 and pos[a] == 0) or (c[d, a] > 0 and pos[a] == size[a] - 1))]
+                            if out:
+                                link(i, j, k, d, 4 if all(a in cyclic_axes for a in out) else 2)
