@@ -500,6 +500,30 @@ def evaluate(net, test, log=print, single=16, tol=1e-6):
     return rep
 
 
+def start_field_ablation(net, test, log=print):
+    """Which predicted field limits the warm start: steps to the residual of
+    the batched solver from starts that mix the network's density and
+    velocity with the converged fields (a rest run continued to 1e-8), and
+    with the analytic density.  Returns {start: (mean steps, rest/start step ratio)}."""
+    rest, _, ref = solve(test, then=1e-8)
+    pred = predict_fields(net, test)
+    sr = np.array([r[3] for r in rest], float)
+    starts = {
+        "network rho, network u": pred,
+        "converged rho, network u": [(r[0], p[1], p[2]) for p, r in zip(pred, ref)],
+        "network rho, converged u": [(p[0], r[1], r[2]) for p, r in zip(pred, ref)],
+        "converged rho, converged u": [(r[0], r[1], r[2]) for r in ref],
+        "analytic rho, network u": [(LT.poiseuille_start(d)[0], p[1], p[2]) for d, p in zip(test, pred)],
+    }
+    out = {"rest": (float(sr.mean()), 1.0)}
+    for k, st in starts.items():
+        res, _ = solve(test, starts=st)
+        sw = np.array([r[3] for r in res], float)
+        out[k] = (float(sw.mean()), float(sr.sum() / sw.sum()))
+        log(f"{k}: mean steps {sw.mean():.0f}, {sr.sum() / sw.sum():.3f} x fewer than from rest")
+    return out
+
+
 def main(argv=None):
     import argparse
     import os
@@ -514,6 +538,8 @@ def main(argv=None):
     ap.add_argument("--rho-weight", type=float, default=1.0, help="loss weight of the density channel")
     ap.add_argument("--out", default="gpurun_out/surrogate")
     ap.add_argument("--weights", default=None, help="skip training, evaluate these weights")
+    ap.add_argument("--ablate", type=int, default=0,
+                    help="with --weights: only the start-field ablation on this many held-out channels")
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
     t0 = time.time()
@@ -522,6 +548,12 @@ def main(argv=None):
         print(f"[{time.time() - t0:7.1f} s] {msg}", flush=True)
 
     report = dict(args=vars(a))
+    if a.weights and a.ablate:
+        report["ablation"] = start_field_ablation(Net.load(a.weights), LT.surrogate_channels(a.ablate, a.test_seed),
+                                                  log=log)
+        with open(os.path.join(a.out, "ablation.json"), "w") as f:
+            json.dump(report, f, indent=1, default=float)
+        return
     if a.weights:
         net = Net.load(a.weights)
     else:
