@@ -349,8 +349,13 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     }
     // nodes per workgroup of the rank step (256, 512 or 1024): 512 measured 3 %
     // faster for 4- and 8-byte storage on the 256^3 porous lattice, 256 best for
-    // 16-bit storage
-    size_t rblock = (p->storage == LBM_STORE_F16 || p->storage == LBM_STORE_BF16) ? 256 : 512;
+    // 16-bit storage with one node per thread.  The 16-bit periodic BGK/TRT
+    // step runs two nodes per thread (k_stepr2) on 512-node workgroups:
+    // 181.7 / 187.2 vs 187.2 / 191.7 us per step at 256^3 on two boxes.
+    const bool narrow = p->storage == LBM_STORE_F16 || p->storage == LBM_STORE_BF16;
+    const bool periodic_all = p->periodic[0] && p->periodic[1] && (p->q == 9 || p->periodic[2]);
+    const int npt = narrow && simple && periodic_all && p->collision != LBM_MRT ? 2 : 1;
+    size_t rblock = narrow ? (npt == 2 ? 512 : 256) : 512;
     if (const char* e = std::getenv("LBM_RANK_BLOCK")) {
         const size_t v = std::strtoul(e, nullptr, 10);
         if (v == 256 || v == 512 || v == 1024) rblock = v;
@@ -512,6 +517,11 @@ lbm_status lbm_create(const lbm_params* p, const lbm_lattice* lat, lbm_ctx** out
     k.lgw = lgw;
     k.rblock = (int)rblock;
     k.rmax = rmax;
+    k.npt = npt;
+    if (const char* e = std::getenv("LBM_NPT")) {
+        const int v = std::atoi(e);
+        k.npt = (v == 2 || v == 4) && (int)rblock / v >= 64 ? v : 1;
+    }
     k.simple = simple ? 1 : 0;
     k.qslot = c->qslot;
     k.qv = c->qv;

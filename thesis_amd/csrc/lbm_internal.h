@@ -57,7 +57,8 @@ struct KParams {
     int simple;              // walls are exactly the links into solid nodes; all fluid plain
     int words;               // 32-node words per row
     int lgw;                 // log2 of the words per row rounded up to a power of two (LDS pitch)
-    int rblock;              // nodes (threads) per workgroup of the row-bitmask step
+    int rblock;              // nodes per workgroup of the row-bitmask step
+    int npt;                 // nodes per thread of its 16-bit periodic variant (0 or 1: one, 2, 4)
     int rmax;                // rows staged per workgroup (LDS sizing)
     const uint2* rowmeta;    // per row and word: (fluid mask, compact index of first fluid node)
     const uint2* blk;        // per rblock-node workgroup: (first row, rows spanned)

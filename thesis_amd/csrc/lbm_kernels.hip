@@ -433,6 +433,119 @@ __global__ __launch_bounds__(1024) void k_stepr(const typename Io<L, S>::raw* __
     store_node<L, S, kSoA ? 2 : 1>(dst, kp, own, out);
 }
 
+// NPT nodes per thread (own0 + t + h blockDim, h < NPT) for the 16-bit
+// periodic variant of the rank step with BGK/TRT: every node's loads are
+// issued before any node's arithmetic, so a wave keeps NPT times the gathers
+// in flight.  Per node the same staging, addresses, arithmetic, rounding and
+// stores as k_stepr<L, S, C, false, true>.
+template <class L, int S, int C, int NPT>
+__global__ __launch_bounds__(512) void k_stepr2(const typename Io<L, S>::raw* __restrict__ src,
+                                               typename Io<L, S>::raw* __restrict__ dst,
+                                               const uint2* __restrict__ rowmeta,
+                                               const uint2* __restrict__ blk, KParams kp, Fields fl,
+                                               int write_macros) {
+    using T = typename Io<L, S>::T;
+    using raw = typename Io<L, S>::raw;
+    static_assert(sizeof(raw) == 2 && C != 2, "16-bit storage, BGK or TRT");
+    extern __shared__ uint2 smeta[];
+    const int W = kp.words, lg = kp.lgw, nx = kp.nx, ny = kp.ny, nz = kp.nz;
+    const uint32_t bid = xcd_block(blockIdx.x, gridDim.x, kp.xcd_shift, kp.xchunk_shift);
+    const uint2 bi = blk[bid];
+    const uint32_t r0 = bi.x;
+    const int R = (int)bi.y;
+    constexpr uint32_t kSh = kRankShift;
+    const uint32_t nt = blockDim.x;
+    const uint32_t own0 = bid * (uint32_t)NPT * nt;
+    uint32_t dj[NPT];
+#pragma unroll
+    for (int h = 0; h < NPT; ++h) dj[h] = 0u;
+#pragma unroll
+    for (int h = 0; h < NPT; ++h)
+        if (own0 + (uint32_t)h * nt + threadIdx.x < kp.nf) dj[h] = kp.didx[own0 + (uint32_t)h * nt + threadIdx.x];
+    const int tot = (R * 9) << lg;
+    uint32_t* rowoff = (uint32_t*)(smeta + ((kp.rmax * 9) << lg)) + kp.rblock;
+    for (int t = threadIdx.x; t < R * 9; t += nt) {
+        const int kk = t / 9, nb = t - 9 * kk;
+        const uint32_t rr = r0 + (uint32_t)kk;
+        const uint32_t zq = fdiv(rr, kp.my, kp.sy);
+        int y = (int)(rr - zq * (uint32_t)ny) + nb % 3 - 1, z = (int)zq + nb / 3 - 1;
+        y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
+        z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
+        rowoff[t] = ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)W;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < tot; e += nt) {
+        const int w = e & ((1 << lg) - 1), slot = e >> lg;
+        const int kk = slot / 9, nb = slot - 9 * kk;
+        uint2 val = make_uint2(0u, 0u);
+        if (w < W) val = rowmeta[rowoff[slot] + (uint32_t)w];
+        val.y += kSh;
+        smeta[((kk << lg) + w) * 9 + nb] = val;
+    }
+    __syncthreads();
+    constexpr uint32_t rb = (uint32_t)sizeof(raw);
+    constexpr uint32_t kShBytes = (kSh / 64u) * 64u * L::Q * rb;
+    const char* const srcb = (const char*)src - kShBytes;
+    raw hr[NPT][L::Q];
+#pragma unroll
+    for (int h = 0; h < NPT; ++h) {
+        const uint32_t own = own0 + (uint32_t)h * nt + threadIdx.x;
+        if (own >= kp.nf) continue;
+        const uint32_t r = fdiv(dj[h], kp.mx, kp.sx);
+        const int x = (int)(dj[h] - r * (uint32_t)nx);
+        const int k = (int)(r - r0);
+        const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+        const uint32_t oslot = tslot<L>(0, own);
+        hr[h][0] = ldo(src, oslot);
+        uint32_t cb[3], clow[3], cpos[3];
+        const int sxs[3] = {xm, x, xp};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            cb[c] = (((uint32_t)k << lg) + ((uint32_t)sxs[c] >> 5)) * 9u;
+            cpos[c] = (uint32_t)sxs[c] & 31u;
+            clow[c] = (1u << cpos[c]) - 1u;
+        }
+        uint32_t cnt[L::Q], word[L::Q];
+#pragma unroll
+        for (int i = 1; i < L::Q; ++i) {
+            const int c = 1 - L::cx(i);
+            const uint2 e = smeta[cb[c] + (uint32_t)((1 - L::cz(i)) * 3 + (1 - L::cy(i)))];
+            cnt[i] = e.y + __popc(e.x & clow[c]);
+            word[i] = e.x;
+        }
+#pragma unroll
+        for (int i = 1; i < L::Q; ++i) asm volatile("" : "+v"(cnt[i]));
+        const uint32_t obyte = oslot * rb + kShBytes;
+#pragma unroll
+        for (int i = 1; i < L::Q; ++i) {
+            const int c = 1 - L::cx(i), o = L::opp(i);
+            const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int)word[i], cpos[c], 1);
+            const uint32_t there = (__umul24(cnt[i] >> 6, 64u * (L::Q - 1)) + cnt[i]) * rb;
+            const uint32_t self = obyte + (uint32_t)(64 * (int)rb * (o - i));
+            const uint32_t off = (m & there) | (~m & self);
+            hr[h][i] = ((const raw*)(srcb + off))[64 * i];
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < NPT; ++h) {
+        const uint32_t own = own0 + (uint32_t)h * nt + threadIdx.x;
+        if (own >= kp.nf) continue;
+        T xs[L::Q];
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i) xs[i] = Io<L, S>::x(hr[h][i]);
+        T drho, rho, u[3];
+        moments<L, T>(kp, (uint32_t)FLUID, xs, drho, rho, u, [](int i) { return Io<L, S>::a(i); });
+        if (write_macros) {
+            const uint32_t idx = dj[h];
+            fl.rho[idx] = (double)rho; fl.u[idx] = (double)u[0];
+            fl.v[idx] = (double)u[1]; fl.w[idx] = (double)u[2];
+        }
+        T out[L::Q];
+        collide_pairs<L, T, C>(kp, xs, drho, rho, u, out, [](int) { return Io<L, S>::kw; });
+        store_node<L, S, 1, true>(dst, kp, own, out);
+    }
+}
+
 // Post-boundary populations h of non-solid node idx (compact slot own) from
 // the post-collision state in src: the pull, walls and open faces of the step.
 template <class L, int S, bool OPEN>
@@ -754,9 +867,22 @@ static hipError_t step_t(const KParams& kp, const void* src, void* dst, const ui
             if constexpr (sizeof(raw) == 2 && !OPEN)
                 hipLaunchKernelGGL((k_stepr<L, S, C, OPEN, true, true>), dim3(rb), dim3(kp.rblock), lds, st,
                                    (const raw*)src, (raw*)dst, kp.rowmeta, kp.blk, kp.nodew, kp, fl, wm ? 1 : 0);
-        } else if (kp.simple)
+        } else if (kp.simple) {
+            if constexpr (sizeof(raw) == 2 && !OPEN && C != 2) {
+                if (kp.npt == 2 && kp.tile) {
+                    hipLaunchKernelGGL((k_stepr2<L, S, C, 2>), dim3(rb), dim3(kp.rblock / 2), lds, st,
+                                       (const raw*)src, (raw*)dst, kp.rowmeta, kp.blk, kp, fl, wm ? 1 : 0);
+                    return hipGetLastError();
+                }
+                if (kp.npt == 4 && kp.tile) {
+                    hipLaunchKernelGGL((k_stepr2<L, S, C, 4>), dim3(rb), dim3(kp.rblock / 4), lds, st,
+                                       (const raw*)src, (raw*)dst, kp.rowmeta, kp.blk, kp, fl, wm ? 1 : 0);
+                    return hipGetLastError();
+                }
+            }
             hipLaunchKernelGGL((k_stepr<L, S, C, OPEN, true>), dim3(rb), dim3(kp.rblock), lds, st, (const raw*)src,
                                (raw*)dst, kp.rowmeta, kp.blk, kp.nodew, kp, fl, wm ? 1 : 0);
+        }
         else
             hipLaunchKernelGGL((k_stepr<L, S, C, OPEN, false>), dim3(rb), dim3(kp.rblock), lds, st, (const raw*)src,
                                (raw*)dst, kp.rowmeta, kp.blk, kp.nodew, kp, fl, wm ? 1 : 0);
