@@ -451,7 +451,7 @@ def test_lbmsolver_multiphase_test_cases(tmp_path, case):
             assert "Theoretical" in r.stdout
 
 
-@pytest.mark.parametrize("case", [2, 4, 5])
+@pytest.mark.parametrize("case", [2, 3, 4, 5])
 def test_lbmsolver_multiphase_test_cases_3d(tmp_path, case):
     """[mi355x] cg_test_case on a 3D lattice: the reference's 3D two-phase
     cases (initCGBubble3D, S/GpuInit.cu:536-720) through lbmsolver equal the
@@ -459,36 +459,40 @@ def test_lbmsolver_multiphase_test_cases_3d(tmp_path, case):
     3D deformingBubbleValid print), two layers between inlet faces on y (4,
     Couette, Profile_Couette.txt of the middle column of the middle plane) and
     a heavy layer over a light one between bounce-back faces under gravity (5,
-    Rayleigh-Taylor).  The lattice is cyclic along x and z (and y for case 2)."""
+    Rayleigh-Taylor); two spheres merging (3, the 3D validateCoalescenceCase
+    print).  The lattice is cyclic along x and z (and y for cases 2 and 3)."""
     from thesis_amd import cg_cases as CC
     print(f"case: lbmsolver two-phase 3D test case {case}", flush=True)
     os.makedirs(tmp_path / "Mesh")
     n = 16
-    M = {2: n + 1, 4: 17, 5: 33}[case]
-    cyc = (0, 1, 2) if case == 2 else (0, 2)
+    M = {2: n + 1, 3: n + 1, 4: 17, 5: 33}[case]
+    cyc = (0, 1, 2) if case in (2, 3) else (0, 2)
     LT.write_latticegen2(str(tmp_path / "Mesh" / "box"), np.ones((n + 1, M, n + 1), np.uint8), delta=1.0 / n,
                          cyclic_axes=cyc)
     nu_r, nu_b, U, g, gamma, A = 1 / 6, 1 / 6, 0.0, 0.0, 1.0, 1e-3
-    iters = {2: 400, 4: 1500, 5: 600}[case]
+    iters = {2: 400, 3: 400, 4: 1500, 5: 600}[case]
     if case == 4:
         nu_b, U = 1 / 24, 0.01
     if case == 5:
         g, gamma = -5e-5, 3.0
     extra = (f"cg_r_viscosity = {nu_r!r}\ncg_b_viscosity = {nu_b!r}\ncg_A = {A!r}\ncg_beta = 0.7\n"
-             f"cg_gamma = {gamma!r}\ncg_high_order = 0\n")
+             f"cg_gamma = {gamma!r}\ncg_high_order = 0\ncg_bubble_radius = 5.0\n")
     (tmp_path / "SetUpData.ini").write_text(_mp_case_ini(case, U, iters, extra).replace("Force = 0", f"Force = {g!r}"))
     r = subprocess.run([CLI, "-f", "SetUpData.ini"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     print(r.stdout[-600:], r.stderr[-300:])
     assert r.returncode == 0
     # the same run through the C ABI: x and z periodic with period n, y with
     # period n (case 2) or every plane between the walls
-    ny = n if case == 2 else M
+    ny = n if case in (2, 3) else M
     b_alpha = 4 / 9
     kw = dict(r_alpha=1 - (1 - b_alpha) / gamma, b_alpha=b_alpha, r_nu=nu_r, b_nu=nu_b, beta=0.7, A=A,
               high_order=0, kappa=0.9, walls={4: 1, 5: 2}.get(case, 0), u_wall=U, g=g, external_force=0)
     z, y, x = np.mgrid[0:n, 0:ny, 0:n]
     N = n + 1
+    R = 5.0 / N                                # cg_bubble_radius 5 over the lattice width
+    c2 = (x / n - 0.5) ** 2 + (z / n - 0.5) ** 2
     red = {2: (x < 0.75 * N) & (x > 0.25 * N) & (y < 0.75 * M) & (y > 0.25 * M) & (z < 0.75 * N) & (z > 0.25 * N),
+           3: (np.sqrt(c2 + (y / n - 0.5 + R) ** 2) <= R) | (np.sqrt(c2 + (y / n - 0.5 - R) ** 2) <= R),
            4: y / n > 0.5,
            5: y / n > 1.0 + 0.1 * np.cos(2 * np.pi * x / n)}[case].astype(float)
     s = L.CgSolver(n, ny, nz=n, **kw)
@@ -505,6 +509,10 @@ def test_lbmsolver_multiphase_test_cases_3d(tmp_path, case):
         got, want = CC.deforming_radius(mm["r_rho"][full], mm["b_rho"][full])
         assert abs(want - N / 2 * 0.6203504908994) < 1e-9 * N  # (3 / (4 pi))^(1/3) of the lattice's half width
         assert f"Predicted radius: {want:.6f} Final radius: {got:.6f}" in r.stdout
+    elif case == 3:
+        got, want = CC.coalescence_radius(mm["r_rho"][full], mm["b_rho"][full], R)
+        assert abs(want - R * 2 ** (1 / 3)) < 1e-12
+        assert f"Predicted radius: {want:.6f} Final radius: {got:.6f}  (physical units)" in r.stdout
     elif case == 4:
         prof = np.loadtxt(tmp_path / "Results" / "Profile_Couette.txt", skiprows=1)
         assert prof.shape == (M, 3)
