@@ -139,6 +139,32 @@ def batch64():
             "steps_min": int(steps.min()), "steps_max": int(steps.max()), "max_residual": float(res.max())}
 
 
+def colour_gradient_3d(n=256, steps=20, settle=0.3):
+    """The D3Q19 colour-gradient step (two launches: phase-field pass, fused
+    step) on an n^3 periodic box with a red sphere of radius 40 in blue (rest
+    fractions 1/3, A = 0.01125, beta = 0.7, tau = 1, 18-point gradient), FP32
+    and FP16 storage.  Algorithmic bytes per step: one read and one write of
+    the 38 stored values per node."""
+    from thesis_amd import lattices as LT
+    from thesis_amd import lbm as L
+    r, b = LT.droplet3d(n, n, n, 40)
+    res = {}
+    for name, st, sb in (("f32", L.STORE_F32, 4), ("f16", L.STORE_F16, 2)):
+        g = L.CgSolver(n, n, nz=n, storage=st, A=0.01125, r_alpha=1 / 3, b_alpha=1 / 3, high_order=0)
+        g.init(r, b)
+        dev = 0.0
+        while dev < settle:
+            dev += g.time_steps(20) / 1e3
+        ms = g.time_steps(steps) / steps
+        red = g.reduce()
+        g.close()
+        alg = n ** 3 * 2 * 38 * sb
+        res[name] = {"ms_per_step": round(ms, 4), "mlups": round(n ** 3 / (ms / 1e3) / 1e6, 1),
+                     "gbs": round(alg / (ms / 1e3) / 1e9, 1), "frac": round(alg / (ms / 1e3) / 1e9 / PEAK_GBS, 4),
+                     "finite": bool(red["nonfinite"] == 0)}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,6 +262,7 @@ def main():
             mrt32 = line("f32", L.MRT)
             variants["mrt_f32"] = brief(mrt32)
             variants["mrt_f16_storage"] = brief(line("f16", L.MRT), mrt32["mlups"])
+            variants["colour_gradient_d3q19_256"] = colour_gradient_3d(n)
     if args.variants:
         variants["bgk_f32"] = brief(line("f32", L.BGK))
         variants["batch64"] = batch64()
