@@ -536,7 +536,7 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--rho-weight", type=float, default=1.0, help="loss weight of the density channel")
-    ap.add_argument("--out", default="gpurun_out/surrogate")
+    ap.add_argument("--out", default="surrogate_out")
     ap.add_argument("--weights", default=None, help="skip training, evaluate these weights")
     ap.add_argument("--ablate", type=int, default=0,
                     help="with --weights: only the start-field ablation on this many held-out channels")
