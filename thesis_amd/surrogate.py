@@ -366,9 +366,13 @@ def dataset(count, seed, log=print):
     return X, Y, Mk
 
 
-def train(X, Y, Mk, epochs=30, batch=16, lr=2e-3, seed=0, log=print, net=None, rho_weight=1.0):
+def train(X, Y, Mk, epochs=30, batch=16, lr=2e-3, seed=0, log=print, net=None, rho_weight=1.0, flip=False):
     """Adam with a cosine learning-rate decay over the epochs; rho_weight
-    weighs the density channel of the loss against the two velocity channels."""
+    weighs the density channel of the loss against the two velocity channels.
+    flip: each sample is mirrored across the channel's centre line with
+    probability 1/2 (the walls, the inlet profile and the analytic start are
+    symmetric, so the mirrored obstacles' steady state is the mirrored state
+    with v negated)."""
     dev = torch.device("cuda")
     net = net or Net()
     n, H, W, _ = X.shape
@@ -388,9 +392,16 @@ def train(X, Y, Mk, epochs=30, batch=16, lr=2e-3, seed=0, log=print, net=None, r
         acc = torch.zeros(1, dtype=torch.float32, device=dev)
         for s in range(steps_per_epoch):
             idx = perm[s * batch:(s + 1) * batch]
-            xb = Xd.index_select(0, idx).contiguous().view(-1)
-            yb = Yd.index_select(0, idx).contiguous().view(-1)
-            mb = Md.index_select(0, idx).contiguous().view(-1)
+            xs_, ys_, ms_ = Xd.index_select(0, idx), Yd.index_select(0, idx), Md.index_select(0, idx)
+            if flip:
+                f = (torch.rand(batch, generator=g) < 0.5).to(dev)
+                sgn = torch.tensor([1.0, -1.0, 1.0], dtype=torch.float32, device=dev)
+                xs_ = torch.where(f[:, None, None, None], torch.flip(xs_, dims=[1]), xs_)
+                ys_ = torch.where(f[:, None, None, None], torch.flip(ys_, dims=[1]) * sgn, ys_)
+                ms_ = torch.where(f[:, None, None], torch.flip(ms_, dims=[1]), ms_)
+            xb = xs_.contiguous().view(-1)
+            yb = ys_.contiguous().view(-1)
+            mb = ms_.contiguous().view(-1)
             lr_t = lr * 0.5 * (1.0 + math.cos(math.pi * it / total))
             acc += net.step(xb, yb, mb, batch, H, W, lr_t, wch)
             it += 1
@@ -536,6 +547,7 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--rho-weight", type=float, default=1.0, help="loss weight of the density channel")
+    ap.add_argument("--flip", action="store_true", help="mirror training samples across the centre line")
     ap.add_argument("--out", default="surrogate_out")
     ap.add_argument("--weights", default=None, help="skip training, evaluate these weights")
     ap.add_argument("--ablate", type=int, default=0,
@@ -560,12 +572,13 @@ def main(argv=None):
         X, Y, Mk = dataset(a.train, a.train_seed, log=log)
         report["data_s"] = time.time() - t0
         t1 = time.time()
-        net, hist = train(X, Y, Mk, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log, rho_weight=a.rho_weight)
+        net, hist = train(X, Y, Mk, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log, rho_weight=a.rho_weight,
+                          flip=a.flip)
         report["train_s"] = time.time() - t1
         report["loss_history"] = hist
         del X, Y, Mk
         net.save(os.path.join(a.out, "weights.npz"),
-                 meta=dict(train=a.train, seed=a.train_seed, epochs=a.epochs, rho_weight=a.rho_weight))
+                 meta=dict(train=a.train, seed=a.train_seed, epochs=a.epochs, rho_weight=a.rho_weight, flip=a.flip))
     if a.test == 0:
         # training only (e.g. to profile the kernels of the training steps)
         with open(os.path.join(a.out, "report.json"), "w") as f:
