@@ -1263,7 +1263,9 @@ void orc_cg_laplace(const orc_cg* s, double* out)
 struct orc_cgcpu {
     orc_cgcpu_params p;
     long n;
+    int nz, q;
     double *rf, *bf, *rc, *bc, *rr, *br, *rho, *u, *v, *gx, *gy;
+    double *w3, *f, *gz;   /* D3Q19 only: z velocity, the summed populations, z gradient */
     double rphi[9], bphi[9];
 };
 
@@ -1271,11 +1273,14 @@ orc_cgcpu* orc_cgcpu_create(const orc_cgcpu_params* p)
 {
     orc_cgcpu* s = (orc_cgcpu*)calloc(1, sizeof *s);
     s->p = *p;
-    s->n = (long)p->nx * p->ny;
-    const size_t n = (size_t)s->n, nq = n * 9;
+    s->nz = p->nz > 1 ? p->nz : 1;
+    s->q = s->nz > 1 ? 19 : 9;
+    s->n = (long)p->nx * p->ny * s->nz;
+    const size_t n = (size_t)s->n, nq = n * s->q;
     s->rf = calloc(nq, 8); s->bf = calloc(nq, 8); s->rc = calloc(nq, 8); s->bc = calloc(nq, 8);
     s->rr = calloc(n, 8); s->br = calloc(n, 8); s->rho = calloc(n, 8);
     s->u = calloc(n, 8); s->v = calloc(n, 8); s->gx = calloc(n, 8); s->gy = calloc(n, 8);
+    if (s->q == 19) { s->w3 = calloc(n, 8); s->f = calloc(nq, 8); s->gz = calloc(n, 8); }
     /* r_phi, b_phi (S/Iterate.cu:194-204) */
     for (int k = 0; k < 9; ++k) {
         s->rphi[k] = k == 0 ? p->r_alpha : (k < 5 ? (1.0 - p->r_alpha) / 5.0 : (1.0 - p->r_alpha) / 20.0);
@@ -1289,6 +1294,7 @@ void orc_cgcpu_destroy(orc_cgcpu* s)
     if (!s) return;
     free(s->rf); free(s->bf); free(s->rc); free(s->bc); free(s->rr); free(s->br); free(s->rho);
     free(s->u); free(s->v); free(s->gx); free(s->gy);
+    free(s->w3); free(s->f); free(s->gz);
     free(s);
 }
 
@@ -1297,6 +1303,21 @@ void orc_cgcpu_destroy(orc_cgcpu* s)
 void orc_cgcpu_init(orc_cgcpu* s, const double* r_rho, const double* b_rho)
 {
     const long n = s->n;
+    if (s->q == 19) {
+        /* createBubble3D (S/Multiphase.cu:415-449): rho_k phi_i for each
+         * colour, phi = (0, 1/12 x 6, 1/24 x 12) (S/Iterate3D.cu:254-273), at rest */
+        for (long i = 0; i < n; ++i) {
+            s->rr[i] = r_rho[i]; s->br[i] = b_rho[i]; s->rho[i] = r_rho[i] + b_rho[i];
+            s->u[i] = 0.0; s->v[i] = 0.0; s->w3[i] = 0.0;
+            for (int k = 0; k < 19; ++k) {
+                const double ph = k == 0 ? 0.0 : (k <= 6 ? 1.0 / 12.0 : 1.0 / 24.0);
+                s->rf[k * n + i] = r_rho[i] * ph;
+                s->bf[k * n + i] = b_rho[i] * ph;
+                s->f[k * n + i] = s->rf[k * n + i] + s->bf[k * n + i];
+            }
+        }
+        return;
+    }
     for (long i = 0; i < n; ++i) {
         s->rr[i] = r_rho[i]; s->br[i] = b_rho[i]; s->rho[i] = r_rho[i] + b_rho[i];
         s->u[i] = 0.0; s->v[i] = 0.0;
@@ -1375,10 +1396,16 @@ static void cgcpu_copy(orc_cgcpu* s, long to, long from, const int* ks, int nk)
     }
 }
 
+static void cgcpu3_step(orc_cgcpu* s);
+
 void orc_cgcpu_step(orc_cgcpu* s, int nsteps)
 {
     const int nx = s->p.nx, ny = s->p.ny;
     const long n = s->n;
+    if (s->q == 19) {
+        for (int it = 0; it < nsteps; ++it) cgcpu3_step(s);
+        return;
+    }
     for (int it = 0; it < nsteps; ++it) {
         cgcpu_collide(s);
         /* streamMP: pull the links whose source lies in the box */
@@ -1448,8 +1475,8 @@ void orc_cgcpu_get(const orc_cgcpu* s, double* r_f, double* b_f, double* r_rho, 
                    double* u, double* v)
 {
     const size_t n = (size_t)s->n;
-    if (r_f) memcpy(r_f, s->rf, 9 * n * 8);
-    if (b_f) memcpy(b_f, s->bf, 9 * n * 8);
+    if (r_f) memcpy(r_f, s->rf, s->q * n * 8);
+    if (b_f) memcpy(b_f, s->bf, s->q * n * 8);
     if (r_rho) memcpy(r_rho, s->rr, n * 8);
     if (b_rho) memcpy(b_rho, s->br, n * 8);
     if (rho) memcpy(rho, s->rho, n * 8);
@@ -1461,8 +1488,233 @@ void orc_cgcpu_get(const orc_cgcpu* s, double* r_f, double* b_f, double* r_rho, 
 void orc_cgcpu_get_coll(const orc_cgcpu* s, double* r_c, double* b_c, double* gx, double* gy)
 {
     const size_t n = (size_t)s->n;
-    if (r_c) memcpy(r_c, s->rc, 9 * n * 8);
-    if (b_c) memcpy(b_c, s->bc, 9 * n * 8);
+    if (r_c) memcpy(r_c, s->rc, s->q * n * 8);
+    if (b_c) memcpy(b_c, s->bc, s->q * n * 8);
     if (gx) memcpy(gx, s->gx, n * 8);
     if (gy) memcpy(gy, s->gy, n * 8);
+}
+
+/* ---- the reference's D3Q19 CPU colour-gradient path: mp3DColl
+ * (S/Multiphase.cu:140-323), streamMP3D (:1012-1033), peridicBoundaries3D
+ * (:741-849), updateMacroMP3D (:531-596), in the order of the #define CUDA 0
+ * branch of S/Iterate3D.cu (:500-503, :538-539, :571-572, :633-635; the
+ * shipped file defines CUDA 1, :28, so this branch is compiled out).
+ *
+ * Unlike its D2Q9 sibling it collides the summed populations f towards one
+ * equilibrium, as the GPU formulation does, with the gradient of the phase
+ * field (rho_r - rho_b) / rho, half the perturbation amplitude (0.5 A), the
+ * enhanced terms always on and weighted by the mean alpha, and both colours
+ * recoloured with the same sign (:317-318), so blue is pushed up the phase
+ * gradient as red is.
+ *
+ * Only interior nodes are streamed and updated; the faces keep the
+ * populations, densities and velocities of the start, and their collision
+ * (on those frozen values, with a gradient from the live interior) feeds the
+ * interior.  The periodic copies between opposite faces move face
+ * populations that nothing reads afterwards; they are kept, with the links
+ * the reference copies into the red array where the blue one was meant
+ * (:767, :811-812, :845-846).
+ *
+ * Two points the reference leaves undefined are fixed here:
+ * - the gradient weights: the host branch writes 1/6 and 1/12 into cg_w[0]
+ *   inside its loops over the axis and diagonal links
+ *   (S/Iterate3D.cu:259, :266, :274) and never sets cg_w[1..18]; the oracle takes
+ *   1/6 for the axis links and 1/12 for the diagonal ones, the values those
+ *   lines write and the GPU path's;
+ * - the x and y face branches (:192-235) read the planes k - 1 and k + 1
+ *   beyond the box on the front and back planes; the oracle leaves those
+ *   terms out.
+ * Where the gradient norm is not above g_limit the reference keeps the
+ * cosine of the link before (:272-288); the oracle takes 0 there. */
+static void cgcpu3_collide(orc_cgcpu* s)
+{
+    const int n = s->p.nx, m = s->p.ny, h = s->nz;
+    const long N = s->n;
+    for (int k = 0; k < h; ++k)
+        for (int j = 0; j < m; ++j)
+            for (int i = 0; i < n; ++i) {
+                const long ind = ((long)k * m + j) * n + i;
+                /* the reference's branches in its order: interior, north, south,
+                 * east, west, front, back; a node on a y face and an x face
+                 * matches none and gets no gradient */
+                int br = -1;
+                if (i && j && k && i != n - 1 && j != m - 1 && k != h - 1) br = 0;
+                else if (j == m - 1 && i && i != n - 1) br = 1;
+                else if (j == 0 && i && i != n - 1) br = 2;
+                else if (i == n - 1 && j && j != m - 1) br = 3;
+                else if (i == 0 && j && j != m - 1) br = 4;
+                else if (k == h - 1 && j && j != m - 1) br = 5;
+                else if (k == 0 && j && j != m - 1) br = 6;
+                double cg[3] = {0, 0, 0}, gr[3] = {0, 0, 0};
+                for (int d = 1; br >= 0 && d < 19; ++d) {
+                    const int cx = CX19[d], cy = CY19[d], cz = CZ19[d];
+                    int ti = i + cx, tj = j + cy, tk = k + cz;
+                    switch (br) {
+                    case 1: tj = j - abs(cy); break;
+                    case 2: tj = j + abs(cy); break;
+                    case 3: ti = i - abs(cx); break;
+                    case 4: ti = i + abs(cx); break;
+                    case 5: ti = i + abs(cx); tk = k - abs(cz); break;   /* i + |cx| as written */
+                    case 6: ti = i + abs(cx); tk = k + abs(cz); break;
+                    default: break;
+                    }
+                    if (tk < 0 || tk >= h) continue;   /* beyond the box: left out */
+                    const long t = ((long)tk * m + tj) * n + ti;
+                    const double wk = d <= 6 ? 1.0 / 6.0 : 1.0 / 12.0;
+                    const double ph = (s->rr[t] - s->br[t]) / s->rho[t];
+                    const int c[3] = {cx, cy, cz};
+                    for (int a = 0; a < 3; ++a) {
+                        gr[a] += s->rho[t] * c[a] * wk;
+                        cg[a] += ph * c[a] * wk;
+                    }
+                }
+                /* the face branches zero the normal components */
+                const int normal = br == 1 || br == 2 ? 1 : (br == 3 || br == 4 ? 0 : (br >= 5 ? 2 : -1));
+                if (normal >= 0) { cg[normal] = 0.0; gr[normal] = 0.0; }
+                s->gx[ind] = cg[0]; s->gy[ind] = cg[1]; s->gz[ind] = cg[2];
+
+                const double u = s->u[ind], v = s->v[ind], w = s->w3[ind];
+                const double G[9] = {2.0 * u * gr[0], u * gr[1] + v * gr[0], u * gr[2] + w * gr[0],
+                                     u * gr[1] + v * gr[0], 2.0 * v * gr[1], v * gr[2] + w * gr[1],
+                                     u * gr[2] + w * gr[0], v * gr[2] + w * gr[1], 2.0 * w * gr[2]};
+                const double ugr = u * gr[0] + v * gr[1] + w * gr[2];
+                const double cu1 = u * u + v * v + w * w;
+                const double gn = sqrt(cg[0] * cg[0] + cg[1] * cg[1] + cg[2] * cg[2]);
+                const double rr = s->rr[ind], bb = s->br[ind], rho = s->rho[ind];
+                const double kr = rr / rho, kb = bb / rho, kk = s->p.beta * rr * bb / rho;
+                const double mean_nu = 1.0 / (rr / (rho * s->p.r_nu) + bb / (rho * s->p.b_nu));
+                const double om = 1.0 / (3.0 * mean_nu + 0.5);
+                const double alpha = s->p.r_alpha * rr / rho + s->p.b_alpha * bb / rho;
+                for (int d = 0; d < 19; ++d) {
+                    const int cx = CX19[d], cy = CY19[d], cz = CZ19[d];
+                    const double wd = d == 0 ? 1.0 / 3.0 : (d <= 6 ? 1.0 / 18.0 : 1.0 / 36.0);
+                    const double wpert = d == 0 ? -2.0 / 9.0 : (d <= 6 ? 1.0 / 54.0 : 1.0 / 27.0);
+                    const double phi = d == 0 ? 0.0 : (d <= 6 ? 1.0 / 12.0 : 1.0 / 24.0);
+                    const double teta = d == 0 ? 1.0 : (d <= 6 ? -1.0 / 12.0 : -1.0 / 24.0);
+                    const double chi = d == 0 ? -2.5 : (d <= 6 ? -1.0 / 6.0 : 1.0 / 24.0);
+                    const double psi = d == 0 ? 0.0 : (d <= 6 ? 0.25 : 0.125);
+                    double cosin = 0.0, pert = 0.0;
+                    if (gn > s->p.g_limit) {
+                        const double pcg = cx * cg[0] + cy * cg[1] + cz * cg[2];
+                        if (d) cosin = pcg / (gn * sqrt((double)(cx * cx + cy * cy + cz * cz)));
+                        pert = 0.5 * s->p.A * gn * (wd * (pcg / gn) * (pcg / gn) - wpert);
+                    }
+                    const int c[3] = {cx, cy, cz};
+                    double tc = 0.0;
+                    for (int a = 0; a < 3; ++a)
+                        for (int b = 0; b < 3; ++b) tc += G[3 * a + b] * c[a] * c[b];
+                    const double cu2 = u * cx + v * cy + w * cz;
+                    const double feq = alpha * (chi * ugr + psi * tc) +
+                                       rho * (phi + teta * alpha + wd * (3 * cu2 + 4.5 * cu2 * cu2 - 1.5 * cu1));
+                    const long q = (long)d * N + ind;
+                    const double fs = om * feq + (1 - om) * s->f[q] + pert;
+                    const double rec = kk * cosin * (phi + teta * alpha);
+                    s->rc[q] = kr * fs + rec;
+                    s->bc[q] = kb * fs + rec;
+                }
+            }
+}
+
+/* one face-to-face copy of the listed links of the red and blue arrays;
+ * nb of them in the blue one, the rest into the red one (the reference's
+ * slips) */
+static void cgcpu3_copy(orc_cgcpu* s, long to, long from, const int* ks, int nk, int nb)
+{
+    for (int a = 0; a < nk; ++a) s->rf[ks[a] * s->n + to] = s->rf[ks[a] * s->n + from];
+    for (int a = 0; a < nk; ++a) {
+        double* dst = a < nb ? s->bf : s->rf;
+        const double* src = a < nb ? s->bf : s->rf;
+        dst[ks[a] * s->n + to] = src[ks[a] * s->n + from];
+    }
+}
+
+static void cgcpu3_step(orc_cgcpu* s)
+{
+    const int n = s->p.nx, m = s->p.ny, h = s->nz;
+    const long N = s->n;
+    cgcpu3_collide(s);
+    /* streamMP3D: interior nodes pull from x - c (every link streams in a
+     * box without solids) */
+    for (int k = 1; k < h - 1; ++k)
+        for (int j = 1; j < m - 1; ++j)
+            for (int i = 1; i < n - 1; ++i) {
+                const long ind = ((long)k * m + j) * n + i;
+                for (int d = 0; d < 19; ++d) {
+                    const long src = ((long)(k - CZ19[d]) * m + (j - CY19[d])) * n + (i - CX19[d]);
+                    s->rf[d * N + ind] = s->rc[d * N + src];
+                    s->bf[d * N + ind] = s->bc[d * N + src];
+                }
+            }
+    /* peridicBoundaries3D, in its order */
+    static const int north[5] = {4, 9, 10, 16, 18}, south[5] = {3, 7, 8, 15, 17};
+    static const int east[5] = {2, 8, 10, 12, 14}, west[5] = {1, 7, 9, 11, 13};
+    static const int front[5] = {6, 13, 14, 17, 18}, back[5] = {5, 11, 12, 15, 16};
+    for (int k = 0; k < h - 1; ++k) {
+        for (int i = 0; i < n; ++i) {
+            const long e = ((long)k * m + m - 1) * n + i, st = (long)k * m * n + i;
+            cgcpu3_copy(s, e, st, north, 5, 4);
+            cgcpu3_copy(s, st, e, south, 5, 5);
+        }
+        for (int j = 1; j < m - 1; ++j) {
+            const long e = ((long)k * m + j) * n + n - 1, w = ((long)k * m + j) * n;
+            cgcpu3_copy(s, e, w, east, 5, 5);
+            cgcpu3_copy(s, w, e, west, 5, 3);
+        }
+    }
+    for (int j = 0; j < m; ++j)
+        for (int i = 0; i < n; ++i) {
+            const long e = ((long)(h - 1) * m + j) * n + i, st = (long)j * n + i;
+            cgcpu3_copy(s, e, st, front, 5, 5);
+            cgcpu3_copy(s, st, e, back, 5, 3);
+        }
+    /* updateMacroMP3D: interior nodes only (the surface-tension estimate it
+     * also makes is a diagnostic and is left out) */
+    for (int k = 1; k < h - 1; ++k)
+        for (int j = 1; j < m - 1; ++j)
+            for (int i = 1; i < n - 1; ++i) {
+                const long ind = ((long)k * m + j) * n + i;
+                double R = 0, B = 0, jx = 0, jy = 0, jz = 0;
+                for (int d = 0; d < 19; ++d) {
+                    const long q = d * N + ind;
+                    R += s->rf[q]; B += s->bf[q];
+                    s->f[q] = s->rf[q] + s->bf[q];
+                }
+                s->rr[ind] = R; s->br[ind] = B; s->rho[ind] = R + B;
+                for (int d = 0; d < 19; ++d) {
+                    const double fq = s->rf[d * N + ind] + s->bf[d * N + ind];
+                    jx += fq * CX19[d]; jy += fq * CY19[d]; jz += fq * CZ19[d];
+                }
+                s->u[ind] = jx / s->rho[ind];
+                s->v[ind] = jy / s->rho[ind];
+                s->w3[ind] = jz / s->rho[ind];
+            }
+}
+
+void orc_cgcpu_init_f(orc_cgcpu* s, const double* r_f, const double* b_f)
+{
+    const long n = s->n;
+    const int q = s->q;
+    for (long ind = 0; ind < n; ++ind) {
+        double R = 0, B = 0, jx = 0, jy = 0, jz = 0;
+        for (int d = 0; d < q; ++d) {
+            const long j = (long)d * n + ind;
+            s->rf[j] = r_f[j]; s->bf[j] = b_f[j];
+            if (s->f) s->f[j] = r_f[j] + b_f[j];
+            R += r_f[j]; B += b_f[j];
+            const double fq = r_f[j] + b_f[j];
+            jx += fq * (q == 19 ? CX19[d] : CX9[d]);
+            jy += fq * (q == 19 ? CY19[d] : CY9[d]);
+            jz += q == 19 ? fq * CZ19[d] : 0.0;
+        }
+        s->rr[ind] = R; s->br[ind] = B; s->rho[ind] = R + B;
+        s->u[ind] = jx / (R + B); s->v[ind] = jy / (R + B);
+        if (s->w3) s->w3[ind] = jz / (R + B);
+    }
+}
+
+void orc_cgcpu_get_w(const orc_cgcpu* s, double* w, double* gz)
+{
+    const size_t n = (size_t)s->n;
+    if (w && s->w3) memcpy(w, s->w3, n * 8);
+    if (gz && s->gz) memcpy(gz, s->gz, n * 8);
 }

@@ -161,13 +161,14 @@ void orc_cg_laplace(const orc_cg* s, double* out);
 /* gradient field (cg_x, cg_y, cg_z) for tests of the stencil; cgz may be NULL */
 void orc_cg_gradient(const orc_cg* s, double* cgx, double* cgy, double* cgz);
 
-/* ---- the reference's CPU colour-gradient path (D2Q9; test infrastructure,
- * a second opinion beside the GPU formulation) ---- */
+/* ---- the reference's CPU colour-gradient path (D2Q9 and D3Q19; test
+ * infrastructure, a second opinion beside the GPU formulation) ---- */
 typedef struct {
     int nx, ny;                  /* the full lattice, edges included */
     double r_alpha, b_alpha, r_nu, b_nu, beta, A, g_limit;
-    int test_case;               /* 2: Zou-He walls on the first and last rows; else edge copies */
+    int test_case;               /* 2: Zou-He walls on the first and last rows; else edge copies (D2Q9) */
     double u_wall;               /* north-row velocity of test case 2 */
+    int nz;                      /* > 1: the D3Q19 path on an nx x ny x nz box */
 } orc_cgcpu_params;
 typedef struct orc_cgcpu orc_cgcpu;
 orc_cgcpu* orc_cgcpu_create(const orc_cgcpu_params* p);
@@ -177,6 +178,11 @@ void orc_cgcpu_step(orc_cgcpu* s, int nsteps);
 void orc_cgcpu_get(const orc_cgcpu* s, double* r_f, double* b_f, double* r_rho, double* b_rho, double* rho,
                    double* u, double* v);
 void orc_cgcpu_get_coll(const orc_cgcpu* s, double* r_c, double* b_c, double* gx, double* gy);
+/* colour populations r_f[q*N + i], b_f[q*N + i] as the state before the
+ * first collision; the macros of every node follow from them */
+void orc_cgcpu_init_f(orc_cgcpu* s, const double* r_f, const double* b_f);
+/* D3Q19: the z velocity and the z component of the last step's gradient */
+void orc_cgcpu_get_w(const orc_cgcpu* s, double* w, double* gz);
 
 #ifdef __cplusplus
 }

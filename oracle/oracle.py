@@ -52,7 +52,7 @@ class OrcCgCpuParams(C.Structure):
     _fields_ = [
         ("nx", C.c_int), ("ny", C.c_int), ("r_alpha", C.c_double), ("b_alpha", C.c_double),
         ("r_nu", C.c_double), ("b_nu", C.c_double), ("beta", C.c_double), ("A", C.c_double),
-        ("g_limit", C.c_double), ("test_case", C.c_int), ("u_wall", C.c_double),
+        ("g_limit", C.c_double), ("test_case", C.c_int), ("u_wall", C.c_double), ("nz", C.c_int),
     ]
 
 
@@ -284,16 +284,20 @@ class CgSim:
 
 
 class CgCpuSim:
-    """The reference's CPU colour-gradient path (mp2DColl, streamMP,
-    peridicBoundaries, updateMacroMP of S/Multiphase.cu) on its full n x m
-    lattice: a second opinion beside the GPU formulation.  Arrays (ny, nx)."""
+    """The reference's CPU colour-gradient path on its full lattice: a second
+    opinion beside the GPU formulation.  D2Q9 (mp2DColl, streamMP,
+    peridicBoundaries, updateMacroMP of S/Multiphase.cu), arrays (ny, nx); with
+    nz > 1 D3Q19 (mp3DColl, streamMP3D, peridicBoundaries3D, updateMacroMP3D),
+    arrays (nz, ny, nx)."""
 
-    def __init__(self, nx, ny, *, r_alpha=4 / 9, b_alpha=4 / 9, r_nu=1 / 6, b_nu=1 / 6, beta=0.7, A=0.01125,
+    def __init__(self, nx, ny, *, nz=1, r_alpha=4 / 9, b_alpha=4 / 9, r_nu=1 / 6, b_nu=1 / 6, beta=0.7, A=0.01125,
                  g_limit=0.0, test_case=1, u_wall=0.0):
         p = OrcCgCpuParams()
         p.nx, p.ny, p.r_alpha, p.b_alpha, p.r_nu, p.b_nu = nx, ny, r_alpha, b_alpha, r_nu, b_nu
-        p.beta, p.A, p.g_limit, p.test_case, p.u_wall = beta, A, g_limit, test_case, u_wall
-        self.nx, self.ny, self.n, self.shape = nx, ny, nx * ny, (ny, nx)
+        p.beta, p.A, p.g_limit, p.test_case, p.u_wall, p.nz = beta, A, g_limit, test_case, u_wall, nz
+        self.nz, self.q = nz, 19 if nz > 1 else 9
+        self.nx, self.ny, self.n = nx, ny, nx * ny * max(nz, 1)
+        self.shape = (nz, ny, nx) if nz > 1 else (ny, nx)
         L = lib()
         L.orc_cgcpu_create.restype = C.c_void_p
         L.orc_cgcpu_create.argtypes = [C.POINTER(OrcCgCpuParams)]
@@ -303,6 +307,8 @@ class CgCpuSim:
         L.orc_cgcpu_step.argtypes = [C.c_void_p, C.c_int]
         L.orc_cgcpu_get.argtypes = [C.c_void_p] + [dp] * 7
         L.orc_cgcpu_get_coll.argtypes = [C.c_void_p] + [dp] * 4
+        L.orc_cgcpu_init_f.argtypes = [C.c_void_p, dp, dp]
+        L.orc_cgcpu_get_w.argtypes = [C.c_void_p, dp, dp]
         self.h = L.orc_cgcpu_create(C.byref(p))
 
     def __del__(self):
@@ -316,19 +322,34 @@ class CgCpuSim:
     def step(self, n=1):
         lib().orc_cgcpu_step(self.h, int(n))
 
+    def init_f(self, r_f, b_f):
+        """colour populations (q, *shape) as the state before the first collision"""
+        lib().orc_cgcpu_init_f(self.h, _dp(_f64(r_f)), _dp(_f64(b_f)))
+
     def get(self):
-        n = self.n
-        arrs = [np.zeros(9 * n), np.zeros(9 * n)] + [np.zeros(n) for _ in range(5)]
+        n, q = self.n, self.q
+        arrs = [np.zeros(q * n), np.zeros(q * n)] + [np.zeros(n) for _ in range(5)]
         lib().orc_cgcpu_get(self.h, *[_dp(a) for a in arrs])
         keys = ["r_f", "b_f", "r_rho", "b_rho", "rho", "u", "v"]
-        return {k: a.reshape((9,) + self.shape if i < 2 else self.shape) for i, (k, a) in enumerate(zip(keys, arrs))}
+        out = {k: a.reshape((q,) + self.shape if i < 2 else self.shape) for i, (k, a) in enumerate(zip(keys, arrs))}
+        if q == 19:
+            w = np.zeros(n)
+            lib().orc_cgcpu_get_w(self.h, _dp(w), None)
+            out["w"] = w.reshape(self.shape)
+        return out
 
     def collided(self):
-        n = self.n
-        arrs = [np.zeros(9 * n), np.zeros(9 * n), np.zeros(n), np.zeros(n)]
+        """the last step's collided colours and colour gradient (gx, gy[, gz])"""
+        n, q = self.n, self.q
+        arrs = [np.zeros(q * n), np.zeros(q * n), np.zeros(n), np.zeros(n)]
         lib().orc_cgcpu_get_coll(self.h, *[_dp(a) for a in arrs])
-        return (arrs[0].reshape((9,) + self.shape), arrs[1].reshape((9,) + self.shape),
-                arrs[2].reshape(self.shape), arrs[3].reshape(self.shape))
+        out = [arrs[0].reshape((q,) + self.shape), arrs[1].reshape((q,) + self.shape),
+               arrs[2].reshape(self.shape), arrs[3].reshape(self.shape)]
+        if q == 19:
+            gz = np.zeros(n)
+            lib().orc_cgcpu_get_w(self.h, None, _dp(gz))
+            out.append(gz.reshape(self.shape))
+        return tuple(out)
 
 
 def ref_mrt_tables(q, omega):
