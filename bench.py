@@ -140,10 +140,10 @@ def batch64():
 
 
 def colour_gradient_3d(n=256, steps=20, settle=0.3):
-    """The D3Q19 colour-gradient step (two launches: phase-field pass, fused
-    step) on an n^3 periodic box with a red sphere of radius 40 in blue (rest
-    fractions 1/3, A = 0.01125, beta = 0.7, tau = 1, 18-point gradient), FP32
-    and FP16 storage.  Algorithmic bytes per step: one read and one write of
+    """The D3Q19 colour-gradient step (one pass over z-columns of row tiles,
+    phase field in LDS) on an n^3 periodic box with a red sphere of radius 40
+    in blue (rest fractions 1/3, A = 0.01125, beta = 0.7, tau = 1, 18-point
+    gradient), FP32 and FP16 storage.  Algorithmic bytes per step: one read and one write of
     the 38 stored values per node."""
     from thesis_amd import lattices as LT
     from thesis_amd import lbm as L

@@ -1,0 +1,90 @@
+"""The single-pass D3Q19 colour-gradient step (k_cg3_fz: z-columns of 64 x 8
+or 64 x 4 row tiles, phase field in an LDS ring of planes) on the GPU: FP64
+parity with the oracle for chunk heights that split the box unevenly, and
+agreement to rounding with the two-launch step (phase-field pass + step) in
+FP32 and FP16 storage (the two kernels contract multiply-adds differently).
+LBM_CG3_ZCHUNK sets the planes per workgroup (0: two launches), LBM_CG3_ROWS
+the tile rows; both are read when a solver is created."""
+import numpy as np
+import pytest
+
+from thesis_amd import lattices as LT
+from thesis_amd import lbm as L
+
+pytestmark = pytest.mark.gpu
+
+
+def solver(monkeypatch, zchunk, rows, nx, ny, nz, **kw):
+    monkeypatch.setenv("LBM_CG3_ZCHUNK", str(zchunk))
+    monkeypatch.setenv("LBM_CG3_ROWS", str(rows))
+    return L.CgSolver(nx, ny, nz=nz, **kw)
+
+
+@pytest.mark.parametrize("enh,rows,zchunk", [(0, 8, 32), (0, 8, 5), (0, 4, 5), (1, 8, 5), (1, 4, 1)])
+def test_single_pass_f64_matches_oracle(oracle, monkeypatch, enh, rows, zchunk):
+    nx, ny, nz = 64, 16, 12
+    print(f"case: cg3 single pass f64 enhanced={enh} rows={rows} zchunk={zchunk}", flush=True)
+    r, b = LT.droplet3d(nx, ny, nz, 4.6, cx=30.3, cy=6.7, cz=5.2)
+    b = 0.6 * b
+    kw = dict(beta=0.7, A=0.01, r_nu=1 / 6, b_nu=0.1, r_alpha=0.3, b_alpha=0.4, high_order=0, enhanced=enh)
+    g = solver(monkeypatch, zchunk, rows, nx, ny, nz, storage=L.STORE_F64, **kw)
+    o = oracle.CgSim(nx, ny, nz=nz, nthreads=4, **kw)
+    g.init(r, b)
+    o.init(r, b)
+    g.step(60)
+    o.step(60)
+    gm, om = g.macros(), o.get()
+    for k in ("r_rho", "b_rho", "rho"):
+        assert np.abs(gm[k] - om[k]).max() <= 1e-12 * np.abs(om[k]).max(), k
+    du = np.sqrt((gm["u"] - om["u"]) ** 2 + (gm["v"] - om["v"]) ** 2 + (gm["w"] - om["w"]) ** 2).max()
+    assert du <= 1e-12 * np.sqrt(om["u"] ** 2 + om["v"] ** 2 + om["w"] ** 2).max()
+    gr, gb = g.get_f()
+    assert np.abs(gr - om["r_f"]).max() <= 1e-12 and np.abs(gb - om["b_f"]).max() <= 1e-12
+    g.close()
+
+
+# FP16: a rounding difference can flip one stored half; the rest population's
+# red value (stored as r / w0) has a quantum of w0 2^-10 = 3.3e-4
+@pytest.mark.parametrize("storage,tol", [(L.STORE_F32, 2e-6), (L.STORE_F16, 1e-3)])
+@pytest.mark.parametrize("enh", [0, 1])
+def test_single_pass_matches_two_launches(monkeypatch, storage, tol, enh):
+    nx, ny, nz = 128, 64, 40           # chunks of 32 planes: 32 + 8
+    r, b = LT.droplet3d(nx, ny, nz, 11.0, cx=60.2, cy=30.7, cz=21.3)
+    kw = dict(beta=0.7, A=0.01125, r_alpha=1 / 3, b_alpha=1 / 3, r_nu=1 / 6, b_nu=0.1, high_order=0, enhanced=enh,
+              storage=storage)
+    out = []
+    for zchunk, rows in ((0, 8), (32, 8), (32, 4), (7, 8)):
+        print(f"case: cg3 single pass vs two launches storage={storage} enhanced={enh} zchunk={zchunk} rows={rows}",
+              flush=True)
+        g = solver(monkeypatch, zchunk, rows, nx, ny, nz, **kw)
+        g.init(r, 0.7 * b)
+        g.step(25)
+        out.append((g.get_f(), g.macros()))
+        g.close()
+    (f0, m0) = out[0]
+    for f, m in out[1:]:
+        df = max(np.abs(f[0] - f0[0]).max(), np.abs(f[1] - f0[1]).max())
+        dm = max(np.abs(m[k] - m0[k]).max() for k in m0)
+        print(f"max |df| {df:.2e}, max |d macro| {dm:.2e}", flush=True)
+        assert df <= tol and dm <= tol
+
+
+def test_single_pass_throughput_256(monkeypatch):
+    """Whole-step time at 256^3 (droplet R = 40, 18-point gradient), FP32 and
+    FP16, single pass against two launches."""
+    n = 256
+    r, b = LT.droplet3d(n, n, n, 40)
+    res = {}
+    for st in (L.STORE_F32, L.STORE_F16):
+        for zchunk, rows in ((0, 8), (32, 8), (64, 8), (128, 8), (16, 8)):
+            print(f"case: cg3 throughput storage={st} zchunk={zchunk} rows={rows}", flush=True)
+            g = solver(monkeypatch, zchunk, rows, n, n, n, storage=st, A=0.01125, r_alpha=1 / 3, b_alpha=1 / 3,
+                       high_order=0)
+            g.init(r, b)
+            g.step(5)
+            ms = g.time_steps(20) / 20
+            red = g.reduce()
+            g.close()
+            res[(st, zchunk, rows)] = ms
+            print(f"storage {st} zchunk {zchunk} rows {rows}: {ms:.3f} ms/step, {n ** 3 / ms / 1e6:.2f} GLUPS", flush=True)
+            assert red["nonfinite"] == 0

@@ -109,8 +109,8 @@ def test_cg3d_laplace_law_sphere():
 
 @pytest.mark.parametrize("ho", [0, 1])
 def test_cg3d_throughput_256(ho):
-    """Throughput of the two-launch step at 256^3 (FP32 and FP16 storage;
-    18- and 104-neighbour gradients)."""
+    """Throughput at 256^3 (FP32 and FP16 storage): the single-pass step with
+    the 18-neighbour gradient, the two-launch step with the 104-neighbour one."""
     n = 256
     r, b = LT.droplet3d(n, n, n, 40)
     for st in (L.STORE_F32, L.STORE_F16):

@@ -19,6 +19,9 @@ struct CgParams {
     int enhanced;            // 1: density-gradient terms in f_eq (enhanced model)
     uint32_t xcd_shift;      // workgroup renumbering of the D3Q19 kernels (xcd_block)
     uint32_t xchunk_shift;
+    int zchunk;              // D3Q19, 18-point gradient: planes per workgroup of the
+                             // single-pass step (k_cg3_fz); 0: phase-field pass + step
+    int zrows;               // rows of its 64-wide tiles (4 or 8)
 };
 
 constexpr int NRED = 12;     // reductions of k_cg_reduce

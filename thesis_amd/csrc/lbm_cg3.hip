@@ -11,12 +11,14 @@
 // post-collision colours, as in the single-phase step.
 //
 // Per link the storage holds h = f - w and the red population r (b = f - r),
-// 38 planes.  A step is two launches: k_cg3_psi pulls each node's
+// 38 planes.  With the 104-point gradient (and on boxes the single pass does
+// not tile) a step is two launches: k_cg3_psi pulls each node's
 // post-streaming colours and stores the phase field psi (and the density,
 // for the enhanced model's density gradient); k_cg3_step pulls
 // them again, takes the gradient of psi over its neighbours (served from
 // L1/L2: each psi value is read by 18 or 104 nodes), collides and recolours
-// in registers and stores the new post-collision state.
+// in registers and stores the new post-collision state.  With the 18-point
+// gradient k_cg3_fz does both in one pass over z-columns of row tiles.
 #include "lbm_cg.h"
 
 #include <hip/hip_runtime.h>
@@ -82,9 +84,9 @@ __device__ inline void densities(const T* h, const T* r, T& rho, T& rr) {
 // S/GpuCollision.cu:781-880, from the density gradient d): new deviation
 // h' = f* - w and red r'; mac = rho_r, rho_b, rho, u, v, w of the incoming
 // state.
-template <class T, bool E>
-__device__ inline void collide(const CgParams& cp, const T* h, const T* r, T gx, T gy, T gz, const T* d, T* ho, T* ro,
-                               T* mac) {
+template <class T, bool E, class Out>
+__device__ inline void collide_to(const CgParams& cp, const T* h, const T* r, T gx, T gy, T gz, const T* d, Out out,
+                                  T* mac) {
     T rho, rr;
     densities<T>(h, r, rho, rr);
     const T rb = rho - rr;
@@ -121,10 +123,16 @@ __device__ inline void collide(const CgParams& cp, const T* h, const T* r, T gx,
             feqd += mean_nu * ((T)CHI(k) * ugr + (T)PSI(k) * (T)2 * cu * cgr);
         }
         const T fsd = om * feqd + ((T)1 - om) * h[k] + pert;       // f* - w
-        ho[k] = fsd;
-        ro[k] = kr * (fsd + (T)W19(k)) + kk * cs * rest;
+        out(k, fsd, kr * (fsd + (T)W19(k)) + kk * cs * rest);
     }
     mac[0] = rr; mac[1] = rb; mac[2] = rho; mac[3] = u; mac[4] = v; mac[5] = w;
+}
+
+// the same into arrays: ho[k] = f*_k - w_k, ro[k] = red
+template <class T, bool E>
+__device__ inline void collide(const CgParams& cp, const T* h, const T* r, T gx, T gy, T gz, const T* d, T* ho, T* ro,
+                               T* mac) {
+    collide_to<T, E>(cp, h, r, gx, gy, gz, d, [&](int k, T hv, T rv) { ho[k] = hv; ro[k] = rv; }, mac);
 }
 
 // Gradient of the phase field at (x, y, z): psi(i) returns the phase field of
@@ -221,6 +229,126 @@ __global__ __launch_bounds__(BX * BY) void k_cg3_step(const typename CIo<S>::raw
     }
 }
 
+// Single-pass step for the 18-point gradient.  A workgroup owns a 64 x TY
+// column of rows and walks up zchunk planes of it.  For each plane it pulls
+// the post-streaming colours of its nodes and of the one-node ring around
+// them, puts their phase field (and with E their density) into a ring of four
+// LDS planes, and keeps its own nodes' populations in registers; once the
+// plane above is in, it finishes the plane below it: gradient from LDS,
+// collision, store.  Every population is read once, plus the ring nodes'
+// (2 (64 + TY) + 4 per plane) and two extra planes per chunk, instead of
+// twice.  The gradient sums in the order of gradient<1>.  Needs nx % 64 == 0
+// and ny % TY == 0.
+constexpr int FX = 64;
+
+template <int TY> struct Frame {
+    static constexpr int W = FX + 2, H = TY + 2, N = W * H, RING = N - FX * TY;
+    // LDS cell of ring node e: the first and last rows of the frame, then the
+    // first and last columns of the rows between
+    __device__ static int cell(int e, int& lx, int& ly) {
+        if (e < W) { lx = e; ly = 0; }
+        else if (e < 2 * W) { lx = e - W; ly = H - 1; }
+        else { const int c = e - 2 * W; ly = 1 + (c >> 1); lx = (c & 1) ? W - 1 : 0; }
+        return ly * W + lx;
+    }
+};
+
+// gradient<1, T, SKIP> over three LDS planes (below, at, above the node) of
+// row width FW
+template <int FW, class T, bool SKIP>
+__device__ inline void gradient_lds(const T* p0, const T* p1, const T* p2, int l, T& gx, T& gy, T& gz) {
+    const T* pl[3] = {p0, p1, p2};
+    T ax = 0, ay = 0, az = 0;
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int c2 = dx * dx + dy * dy + dz * dz;
+                const double wk = LOW(c2);
+                if (wk == 0.0) continue;
+                const T p = (T)wk * pl[dz + 1][l + dy * FW + dx];
+                if (!SKIP || dx) ax += (T)dx * p;
+                if (!SKIP || dy) ay += (T)dy * p;
+                if (!SKIP || dz) az += (T)dz * p;
+            }
+    gx = ax; gy = ay; gz = az;
+}
+
+template <int S, bool E, int TY>
+__global__ __launch_bounds__(FX * TY) void k_cg3_fz(const typename CIo<S>::raw* __restrict__ src,
+                                                    typename CIo<S>::raw* __restrict__ dst, CgParams cp,
+                                                    double* __restrict__ mac, int write_macros) {
+    using T = typename CIo<S>::T;
+    using F = Frame<TY>;
+    __shared__ T lpsi[4][F::N];
+    __shared__ T lrho[E ? 4 : 1][E ? F::N : 1];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * FX + tx;
+    const uint32_t tiles_x = (uint32_t)cp.nx / FX, txy = tiles_x * ((uint32_t)cp.ny / TY);
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x, cp.xcd_shift, cp.xchunk_shift);
+    const uint32_t chunk = b / txy, rxy = b - chunk * txy, tyi = rxy / tiles_x, txi = rxy - tyi * tiles_x;
+    const int x0 = (int)txi * FX, y0 = (int)tyi * TY, z0 = (int)chunk * cp.zchunk;
+    const int z1 = min(z0 + cp.zchunk, cp.nz);
+    const int x = x0 + tx, y = y0 + ty, own = (ty + 1) * F::W + tx + 1;
+    const bool ring = tid < F::RING;
+    int rx = 0, ry = 0, rl = 0;
+    if (ring) {
+        int lx, ly;
+        rl = F::cell(tid, lx, ly);
+        rx = wrap(x0 - 1 + lx, cp.nx);
+        ry = wrap(y0 - 1 + ly, cp.ny);
+    }
+    // plane z goes to LDS slot (z - z0 + 1) & 3
+    auto put = [&](int slot, int l, const T* h, const T* r) {
+        T rho, rr;
+        densities<T>(h, r, rho, rr);
+        lpsi[slot][l] = ((T)2 * rr - rho) / rho;
+        if (E) lrho[slot][l] = rho;
+    };
+    auto put_ring = [&](int slot, int z) {
+        if (!ring) return;
+        T h[19], r[19];
+        pull<S>(src, cp, rx, ry, z, h, r);
+        put(slot, rl, h, r);
+    };
+    T ch[19], cr[19];
+    {   // the plane below the chunk: phase field only
+        const int zb = wrap(z0 - 1, cp.nz);
+        put_ring(0, zb);
+        pull<S>(src, cp, x, y, zb, ch, cr);
+        put(0, own, ch, cr);
+    }
+    put_ring(1, z0);
+    pull<S>(src, cp, x, y, z0, ch, cr);
+    put(1, own, ch, cr);
+    const uint32_t xy = (uint32_t)y * (uint32_t)cp.nx + (uint32_t)x, nxy = (uint32_t)cp.nx * (uint32_t)cp.ny;
+    for (int z = z0; z < z1; ++z) {
+        const int za = z + 1 == cp.nz ? 0 : z + 1, s0 = (z - z0) & 3, s1 = (z - z0 + 1) & 3, s2 = (z - z0 + 2) & 3;
+        put_ring(s2, za);
+        T nh[19], nr[19];
+        pull<S>(src, cp, x, y, za, nh, nr);
+        put(s2, own, nh, nr);
+        // slot s2 was last read two planes ago, before the previous barrier
+        __syncthreads();
+        T gx, gy, gz, d[3] = {0, 0, 0};
+        gradient_lds<F::W, T, (S >= 2)>(lpsi[s0], lpsi[s1], lpsi[s2], own, gx, gy, gz);
+        if (E) gradient_lds<F::W, T, (S >= 2)>(lrho[s0], lrho[s1], lrho[s2], own, d[0], d[1], d[2]);
+        T m[6];
+        const uint32_t idx = (uint32_t)z * nxy + xy;
+        collide_to<T, E>(cp, ch, cr, gx, gy, gz, d, [&](int i, T hv, T rv) {
+            CIo<S>::st(dst, (uint32_t)i * cp.plane + idx, hv, i, false);
+            CIo<S>::st(dst, (uint32_t)(19 + i) * cp.plane + idx, rv, i, true);
+        }, m);
+        if (write_macros) {
+            const size_t n = (size_t)nxy * cp.nz;
+            for (int a = 0; a < 6; ++a) mac[a * n + idx] = (double)m[a];
+        }
+#pragma unroll
+        for (int i = 0; i < 19; ++i) { ch[i] = nh[i]; cr[i] = nr[i]; }
+    }
+}
+
 // Phase field and density of colour densities.
 template <class T>
 __global__ __launch_bounds__(BX * BY) void k_cg3_psi0(const double* __restrict__ rr0, const double* __restrict__ rb0,
@@ -304,6 +432,25 @@ hipError_t cg3_launch_step(int storage, int ho, const void* src, void* dst, void
                            double* mac, bool wm, hipStream_t st) {
     const dim3 grid = cg3::grid_of(cp), block(cg3::BX, cg3::BY);
     const int w = wm ? 1 : 0;
+    if (!ho && cp.zchunk > 0) {
+        const int ty = cp.zrows;
+        if ((ty != 4 && ty != 8) || cp.nx % cg3::FX || cp.ny % ty) return hipErrorInvalidValue;
+        const uint32_t nb = (uint32_t)(cp.nx / cg3::FX) * (uint32_t)(cp.ny / ty) *
+                            (uint32_t)((cp.nz + cp.zchunk - 1) / cp.zchunk);
+        const dim3 fgrid(nb), fblock(cg3::FX, ty);
+#define CG3_FZT_(S_, E_, TY_)                                                                                       \
+    hipLaunchKernelGGL((cg3::k_cg3_fz<S_, E_, TY_>), fgrid, fblock, 0, st,                                        \
+                       (const typename cg3::CIo<S_>::raw*)src, (typename cg3::CIo<S_>::raw*)dst, cp, mac, w)
+#define CG3_FZ_(S_, E_) do { if (ty == 8) CG3_FZT_(S_, E_, 8); else CG3_FZT_(S_, E_, 4); } while (0)
+#define CG3_FZ(S_) do { if (cp.enhanced) CG3_FZ_(S_, true); else CG3_FZ_(S_, false); } while (0)
+        if (storage == 0) CG3_FZ(0);
+        else if (storage == 1) CG3_FZ(1);
+        else CG3_FZ(2);
+#undef CG3_FZ
+#undef CG3_FZ_
+#undef CG3_FZT_
+        return hipGetLastError();
+    }
 #define CG3_STEP_(S_, H_, E_)                                                                                  \
     do {                                                                                                       \
         using R_ = typename cg3::CIo<S_>::raw;                                                                 \
