@@ -71,13 +71,17 @@ def test_single_pass_matches_two_launches(monkeypatch, storage, tol, enh):
 
 def test_single_pass_throughput_256(monkeypatch):
     """Whole-step time at 256^3 (droplet R = 40, 18-point gradient), FP32 and
-    FP16, single pass against two launches."""
+    FP16: two launches against the single pass over column heights, tile rows
+    and the XCD renumbering (LBM_XCD x LBM_XCHUNK)."""
     n = 256
     r, b = LT.droplet3d(n, n, n, 40)
-    res = {}
+    configs = [(0, 8, 8, 8), (32, 8, 8, 8), (64, 8, 8, 8), (128, 8, 8, 8), (16, 8, 8, 8), (32, 4, 8, 8),
+               (32, 8, 1, 1), (32, 8, 8, 1), (32, 8, 8, 2), (32, 8, 8, 4), (32, 8, 8, 16), (32, 8, 8, 32)]
     for st in (L.STORE_F32, L.STORE_F16):
-        for zchunk, rows in ((0, 8), (32, 8), (64, 8), (128, 8), (16, 8)):
-            print(f"case: cg3 throughput storage={st} zchunk={zchunk} rows={rows}", flush=True)
+        for zchunk, rows, xcd, xchunk in configs:
+            print(f"case: cg3 throughput storage={st} zchunk={zchunk} rows={rows} xcd={xcd}x{xchunk}", flush=True)
+            monkeypatch.setenv("LBM_XCD", str(xcd))
+            monkeypatch.setenv("LBM_XCHUNK", str(xchunk))
             g = solver(monkeypatch, zchunk, rows, n, n, n, storage=st, A=0.01125, r_alpha=1 / 3, b_alpha=1 / 3,
                        high_order=0)
             g.init(r, b)
@@ -85,6 +89,6 @@ def test_single_pass_throughput_256(monkeypatch):
             ms = g.time_steps(20) / 20
             red = g.reduce()
             g.close()
-            res[(st, zchunk, rows)] = ms
-            print(f"storage {st} zchunk {zchunk} rows {rows}: {ms:.3f} ms/step, {n ** 3 / ms / 1e6:.2f} GLUPS", flush=True)
+            print(f"storage {st} zchunk {zchunk} rows {rows} xcd {xcd}x{xchunk}: {ms:.4f} ms/step, "
+                  f"{n ** 3 / ms / 1e6:.2f} GLUPS", flush=True)
             assert red["nonfinite"] == 0
