@@ -220,7 +220,11 @@ lbm_status lbm_batch_create(const lbm_batch_instance* in, int32_t n, int32_t col
         b->err = "occupancy query for the batch kernel failed";
         return bad(LBM_EDEVICE);
     }
-    b->grid = b->ntiles < maxb ? b->ntiles : maxb;
+    // grid.sync() needs every workgroup resident at once: leave one
+    // workgroup per XCD of headroom below the occupancy query's co-residency,
+    // so that a CU the runtime keeps back cannot leave the grid waiting
+    const int head = maxb >= 64 ? 8 : (maxb > 1 ? 1 : 0);
+    b->grid = b->ntiles < maxb - head ? b->ntiles : maxb - head;
     if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
         hipEventCreate(&b->ev1) != hipSuccess) {
         b->err = "stream or event creation failed";
