@@ -76,7 +76,8 @@ def test_single_pass_throughput_256(monkeypatch):
     n = 256
     r, b = LT.droplet3d(n, n, n, 40)
     configs = [(0, 8, 8, 8), (32, 8, 8, 8), (64, 8, 8, 8), (128, 8, 8, 8), (16, 8, 8, 8), (32, 4, 8, 8),
-               (32, 8, 1, 1), (32, 8, 8, 1), (32, 8, 8, 2), (32, 8, 8, 4), (32, 8, 8, 16), (32, 8, 8, 32)]
+               (32, 8, 1, 1), (32, 8, 8, 1), (32, 8, 8, 2), (32, 8, 8, 4), (32, 8, 8, 16), (32, 8, 8, 32),
+               (64, 8, 1, 1)]
     for st in (L.STORE_F32, L.STORE_F16):
         for zchunk, rows, xcd, xchunk in configs:
             print(f"case: cg3 throughput storage={st} zchunk={zchunk} rows={rows} xcd={xcd}x{xchunk}", flush=True)

@@ -105,25 +105,49 @@ __device__ inline void collide_to(const CgParams& cp, const T* h, const T* r, T 
     const T gn = sqrt(gx * gx + gy * gy + gz * gz);
     const bool on = gn > (T)cp.g_limit;
     const T ign = on ? (T)1 / gn : (T)0;
+    const T agn = on ? (T)cp.A * gn : (T)0;
     const T kr = rr * ir, kk = (T)cp.beta * rr * rb * ir;
     const T uu = u * u + v * v + w * w;
     const T ugr = E ? u * d[0] + v * d[1] + w * d[2] : (T)0;
+    // The node's factors of each speed shell (rest, axis, diagonal), so that a
+    // link adds only what depends on its own c:
+    //   rest    = phi + theta alpha
+    //   base    = f_eq - w at u = 0 = (rho - 1) rest + (rest - w)
+    //   pa, pb  : perturbation A |g| (w (c.g)^2 / |g|^2 - B) = pa (c.g)^2 - pb
+    //   ic      : cos = (c.g) ic, ic = 1 / (|g| |c|)
+    //   kw, kq  : red = kr (f* - w) + kr w + kk cos rest = kr (f* - w) + kw + kq cos
+    T rest[3], base[3], rw[3], pa[3], pb[3], ic[3], kw[3], kq[3], ec[3], ep[3];
+#pragma unroll
+    for (int sh = 0; sh < 3; ++sh) {
+        const int k = sh == 0 ? 0 : (sh == 1 ? 1 : 7);
+        rest[sh] = (T)PHI(k) + (T)TETA(k) * alpha;
+        base[sh] = (rho - (T)1) * rest[sh] + (rest[sh] - (T)W19(k));
+        rw[sh] = rho * (T)W19(k);
+        pa[sh] = agn * ign * ign * (T)W19(k);
+        pb[sh] = agn * (T)BPERT(k);
+        ic[sh] = ign * (T)(1.0 / CNORM(k));
+        kw[sh] = kr * (T)W19(k);
+        kq[sh] = kk * rest[sh];
+        // enhanced model: nu (chi u.grad rho + psi G:cc), G:cc = 2 (c.u)(c.grad rho)
+        ec[sh] = E ? mean_nu * (T)CHI(k) * ugr : (T)0;
+        ep[sh] = E ? mean_nu * (T)PSI(k) * (T)2 : (T)0;
+    }
+    const T omc = (T)1 - om, m15uu = (T)-1.5 * uu;
 #pragma unroll
     for (int k = 0; k < 19; ++k) {
+        const int sh = k == 0 ? 0 : (k <= 6 ? 1 : 2);
         const T cg = cg::cdot<T>(CX[k], CY[k], CZ[k], gx, gy, gz);
-        const T cs = (on && k) ? cg * ign * (T)(1.0 / CNORM(k)) : (T)0;
-        const T pert = on ? (T)cp.A * gn * ((T)W19(k) * cg * cg * ign * ign - (T)BPERT(k)) : (T)0;
-        const T rest = (T)PHI(k) + (T)TETA(k) * alpha;
+        const T cs = k ? cg * ic[sh] : (T)0;
+        const T pert = pa[sh] * (cg * cg) - pb[sh];
         const T cu = cg::cdot<T>(CX[k], CY[k], CZ[k], u, v, w);
-        // f_eq - w = (rho - 1) rest + (rest - w) + rho w (3cu + 4.5cu^2 - 1.5uu)
-        T feqd = (rho - (T)1) * rest + (rest - (T)W19(k)) +
-                 rho * (T)W19(k) * ((T)3 * cu + (T)4.5 * cu * cu - (T)1.5 * uu);
-        if (E) {   // nu (chi u.grad rho + psi G:cc), G:cc = 2 (c.u)(c.grad rho)
+        // f_eq - w = base + rho w (3cu + 4.5cu^2 - 1.5uu)
+        T feqd = rw[sh] * (cu * ((T)3 + (T)4.5 * cu) + m15uu) + base[sh];
+        if (E) {
             const T cgr = cg::cdot<T>(CX[k], CY[k], CZ[k], d[0], d[1], d[2]);
-            feqd += mean_nu * ((T)CHI(k) * ugr + (T)PSI(k) * (T)2 * cu * cgr);
+            feqd += ec[sh] + ep[sh] * cu * cgr;
         }
-        const T fsd = om * feqd + ((T)1 - om) * h[k] + pert;       // f* - w
-        out(k, fsd, kr * (fsd + (T)W19(k)) + kk * cs * rest);
+        const T fsd = om * feqd + (omc * h[k] + pert);               // f* - w
+        out(k, fsd, kr * fsd + (kw[sh] + kq[sh] * cs));
     }
     mac[0] = rr; mac[1] = rb; mac[2] = rho; mac[3] = u; mac[4] = v; mac[5] = w;
 }
