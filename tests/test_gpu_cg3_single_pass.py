@@ -20,11 +20,15 @@ def solver(monkeypatch, zchunk, rows, nx, ny, nz, **kw):
     return L.CgSolver(nx, ny, nz=nz, **kw)
 
 
-@pytest.mark.parametrize("enh,rows,zchunk", [(0, 8, 32), (0, 8, 5), (0, 4, 5), (1, 8, 5), (1, 4, 1)])
-def test_single_pass_f64_matches_oracle(oracle, monkeypatch, enh, rows, zchunk):
-    nx, ny, nz = 64, 16, 12
-    print(f"case: cg3 single pass f64 enhanced={enh} rows={rows} zchunk={zchunk}", flush=True)
-    r, b = LT.droplet3d(nx, ny, nz, 4.6, cx=30.3, cy=6.7, cz=5.2)
+@pytest.mark.parametrize("enh,rows,zchunk,shape", [(0, 8, 32, (64, 16, 12)), (0, 8, 5, (64, 16, 12)),
+                                                   (0, 4, 5, (64, 16, 12)), (1, 8, 5, (64, 16, 12)),
+                                                   (1, 4, 1, (64, 16, 12)),
+                                                   # one tile: the ring wraps onto the tile itself; three planes
+                                                   (0, 8, 32, (64, 8, 3)), (1, 4, 2, (128, 4, 3))])
+def test_single_pass_f64_matches_oracle(oracle, monkeypatch, enh, rows, zchunk, shape):
+    nx, ny, nz = shape
+    print(f"case: cg3 single pass f64 enhanced={enh} rows={rows} zchunk={zchunk} {nx}x{ny}x{nz}", flush=True)
+    r, b = LT.droplet3d(nx, ny, nz, min(4.6, ny / 2 - 0.5, nz / 2), cx=nx * 0.47, cy=ny * 0.42, cz=nz * 0.43)
     b = 0.6 * b
     kw = dict(beta=0.7, A=0.01, r_nu=1 / 6, b_nu=0.1, r_alpha=0.3, b_alpha=0.4, high_order=0, enhanced=enh)
     g = solver(monkeypatch, zchunk, rows, nx, ny, nz, storage=L.STORE_F64, **kw)
