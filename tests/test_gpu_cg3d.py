@@ -64,11 +64,12 @@ def test_cg3d_reference_bubble_start_f64(oracle):
     g.close()
 
 
+# 32^3 runs the two-launch step, 64^3 the single pass (whole 64 x 8 row tiles)
 @pytest.mark.parametrize("storage,tol", [(L.STORE_F32, 1e-5), (L.STORE_F16, 2e-3)])
-def test_cg3d_reduced_storage_close_to_f64(storage, tol):
-    n = 32
+@pytest.mark.parametrize("n", [32, 64])
+def test_cg3d_reduced_storage_close_to_f64(storage, tol, n):
     print(f"case: cg3d storage {storage} {n}^3", flush=True)
-    r, b = LT.droplet3d(n, n, n, 8.3, cx=15.2, cy=16.1, cz=14.7)
+    r, b = LT.droplet3d(n, n, n, 8.3 * n / 32, cx=15.2 * n / 32, cy=16.1 * n / 32, cz=14.7 * n / 32)
     kw = dict(beta=0.7, A=0.01125, r_alpha=1 / 3, b_alpha=1 / 3, high_order=0)
     out = {}
     for st in (L.STORE_F64, storage):
