@@ -97,3 +97,29 @@ def test_single_pass_throughput_256(monkeypatch):
             print(f"storage {st} zchunk {zchunk} rows {rows} xcd {xcd}x{xchunk}: {ms:.4f} ms/step, "
                   f"{n ** 3 / ms / 1e6:.2f} GLUPS", flush=True)
             assert red["nonfinite"] == 0
+
+
+@pytest.mark.parametrize("R", [16, 24])
+def test_single_pass_laplace_law_128(R):
+    """Laplace law of a sphere with the single-pass step at 128^3 (FP32
+    storage, tau = 1, sigma = 4 A / 9 / omega = 0.005), 8000 steps (the
+    viscous time R^2 / nu is 3500 steps at R = 24): R Delta p / (2 sigma)
+    within 4 % of 1 (the diffuse interface adds a term in 1 / R^2, +4 % at
+    R = 12), spurious currents below 1e-3, and the red mass kept to 3e-4 (FP32
+    storage rounding moves it by about 5e-6 per 300 steps)."""
+    n, A = 128, 0.01125
+    print(f"case: cg3 single pass laplace {n}^3 R={R}", flush=True)
+    g = L.CgSolver(n, n, nz=n, beta=0.7, A=A, r_alpha=1 / 3, b_alpha=1 / 3, storage=L.STORE_F32)
+    r, b = LT.droplet3d(n, n, n, R)
+    g.init(r, b)
+    m0 = g.reduce()
+    g.step(8000)
+    red = g.reduce()
+    g.close()
+    dp = (red["rho_in_sum"] / red["rho_in_n"] - red["rho_out_sum"] / red["rho_out_n"]) / 3.0
+    ratio = dp * R / (2 * 4 * A / 9)
+    print(f"R dp/(2 sigma) {ratio:.4f}, umax {red['umax']:.2e}", flush=True)
+    print(f"red mass drift {abs(red['mass_r'] - m0['mass_r']) / m0['mass_r']:.2e}", flush=True)
+    assert abs(ratio - 1.0) < 0.04
+    assert red["umax"] < 1e-3
+    assert abs(red["mass_r"] - m0["mass_r"]) < 3e-4 * m0["mass_r"]
