@@ -110,3 +110,21 @@ def test_cg_init_f_f64_matches_oracle(oracle):
         assert np.abs(gm[k] - om[k]).max() <= 1e-12 * max(np.abs(om[k]).max(), 1e-3), k
     gr, gb = g.get_f()
     assert np.abs(gr - om["r_f"]).max() <= 1e-12 and np.abs(gb - om["b_f"]).max() <= 1e-12
+
+
+def test_cg_throughput():
+    """Whole-step time of the D2Q9 colour-gradient step (droplet, A = 0.01125,
+    beta = 0.7, tau = 1, 8-point gradient): 512^2 (the static-droplet box)
+    and 2048^2, FP64, FP32 and FP16 storage."""
+    for n in (512, 2048):
+        r, b = LT.droplet2d(n, n, n / 8)
+        for st in (L.STORE_F64, L.STORE_F32, L.STORE_F16):
+            print(f"case: cg throughput {n}^2 storage {st}", flush=True)
+            s = L.CgSolver(n, n, beta=0.7, A=0.01125, r_nu=1 / 6, b_nu=1 / 6, storage=st)
+            s.init(r, b)
+            s.step(20)
+            ms = min(s.time_steps(200) / 200 for _ in range(2))
+            red = s.reduce()
+            s.close()
+            print(f"{n}^2 storage {st}: {ms * 1e3:.1f} us/step, {n * n / ms / 1e6:.2f} GLUPS", flush=True)
+            assert red["nonfinite"] == 0
