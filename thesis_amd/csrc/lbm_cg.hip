@@ -188,15 +188,22 @@ __global__ __launch_bounds__(THREADS) void k_cg_step(const typename CIo<S>::raw*
     __shared__ T psi[RW * RH];
     __shared__ T rt[E ? RW * RH : 1];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    // tiles renumbered for the XCDs (xcd_block), so that the tiles an XCD
+    // runs together are neighbours in x and share their halo columns' lines
+    const uint32_t b = xcd_block(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y, cp.xcd_shift,
+                                 cp.xchunk_shift);
+    const uint32_t by = b / gridDim.x;
+    const int x0 = (int)(b - by * gridDim.x) * TX, y0 = (int)by * TY;
     const int nx = cp.nx, ny = cp.ny;
+    // periodic index; the division only where a coordinate leaves the box
+    auto pwrap = [](int a, int n) { return a >= 0 && a < n ? a : ((a % n) + n) % n; };
     T h[2][9], r[2][9];
     bool inside[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int lx = tx, ly = ty + 8 * k, x = x0 + lx, y = y0 + ly;
         inside[k] = x < nx && y < ny;
-        const int gxp = x % nx, gyp = y % ny;   // partial tiles wrap onto periodic neighbours
+        const int gxp = pwrap(x, nx), gyp = pwrap(y, ny);   // partial tiles wrap onto periodic neighbours
         pull<S>(src, cp, gxp, gyp, h[k], r[k]);
         T rho, rr;
         densities<T>(h[k], r[k], rho, rr);
@@ -211,9 +218,7 @@ __global__ __launch_bounds__(THREADS) void k_cg_step(const typename CIo<S>::raw*
         if (e < H * RW) { py = e / RW; px = e % RW; }
         else if (e < 2 * H * RW) { const int q = e - H * RW; py = TY + H + q / RW; px = q % RW; }
         else { const int q = e - 2 * H * RW; py = H + q / (2 * H); const int c = q % (2 * H); px = c < H ? c : TX + c; }
-        int x = x0 + px - H, y = y0 + py - H;
-        x = ((x % nx) + nx) % nx;
-        y = ((y % ny) + ny) % ny;
+        const int x = pwrap(x0 + px - H, nx), y = pwrap(y0 + py - H, ny);
         T hh[9], rr9[9];
         pull<S>(src, cp, x, y, hh, rr9);
         T rho, rr;
