@@ -121,7 +121,9 @@ __global__ __launch_bounds__(kBatchTile) void k_batch(const BInst* __restrict__ 
         if (remaining == 0) break;
         const bool check = (step + 1) % check_every == 0 || step + 1 == max_steps;
         int ci = 0;
-        for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        // the workgroups an XCD runs together take consecutive tiles (xcd_block,
+        // 8 XCDs, chunks of 8), so tiles that share boundary rows meet in one L2
+        for (int t = (int)xcd_block(blockIdx.x, gridDim.x, 3, 3); t < ntiles; t += gridDim.x) {
             while (ci + 1 < ninst && inst[ci + 1].tbase <= (uint32_t)t) ++ci;   // tiles ascend with t
             const int i = ci;
             if (!act[i]) continue;                          // uniform over the workgroup
